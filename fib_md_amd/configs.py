@@ -1,0 +1,92 @@
+"""Synthetic configurations for tests and benchmarks (host-side numpy).
+
+Diamond-cubic Si lattices, seeded perturbations, an amorphous (randomised)
+sphere, Maxwell-Boltzmann velocities and the reference's own initial state
+(pureepi, prms.f90:484-557; randvel, data.f90:169-204) are generated here.
+All lengths in m, velocities in m/s.
+"""
+from __future__ import annotations
+
+import math
+
+import numpy as np
+
+A_SI_REF = 5.431073e-10   # reference lattice constant (prms.f90:494)
+A_SI = 5.431e-10          # lattice constant of the benchmark configurations
+MASS_SI = 46.637063e-27
+MASS_GA = 1.157777417e-25
+KB = 1.381e-23
+EV = 1.60217646e-19
+
+# 8-site basis of the conventional cell in the reference's order
+# (prms.f90:498-505), fractions of the lattice constant.
+BASIS = np.array([
+    [0.00, 0.00, 0.00], [0.00, 0.50, 0.50], [0.25, 0.25, 0.75], [0.25, 0.75, 0.25],
+    [0.50, 0.00, 0.50], [0.50, 0.50, 0.00], [0.75, 0.25, 0.25], [0.75, 0.75, 0.75],
+])
+
+
+def diamond(n1: int, n2: int, n3: int, a: float = A_SI) -> np.ndarray:
+    """Lattice sites in the reference's loop order x, y, z, basis -> (N,3)."""
+    i, j, k = np.meshgrid(np.arange(n1), np.arange(n2), np.arange(n3), indexing="ij")
+    cells = np.stack([i.ravel(), j.ravel(), k.ravel()], axis=1).astype(np.float64)
+    pts = (cells[:, None, :] + BASIS[None, :, :]).reshape(-1, 3) * a
+    return pts
+
+
+def perturbed_crystal(n: int, sigma: float, seed: int, a: float = A_SI) -> np.ndarray:
+    """n^3-cell periodic crystal with N(0, sigma) displacement per component."""
+    rng = np.random.default_rng(seed)
+    X = diamond(n, n, n, a)
+    X = X + rng.normal(0.0, sigma, X.shape)
+    L = n * a
+    return np.mod(X, L)
+
+
+def amorphous_sphere(X: np.ndarray, L: float, radius: float, density: float, rmin: float,
+                     seed: int) -> np.ndarray:
+    """Replace the atoms inside a central sphere by uniform random points at
+    `density` (atoms/m^3) with rejection minimum distance `rmin`."""
+    rng = np.random.default_rng(seed)
+    c = np.array([L / 2, L / 2, L / 2])
+    d = X - c
+    inside = (d * d).sum(1) < radius * radius
+    outside = X[~inside]
+    n_new = int(round(density * 4.0 / 3.0 * math.pi * radius ** 3))
+    # neighbours of the sphere shell constrain the rejection test too
+    shell = outside[((outside - c) ** 2).sum(1) < (radius + rmin) ** 2]
+    pts = []
+    tries = 0
+    r2min = rmin * rmin
+    while len(pts) < n_new and tries < 200 * n_new:
+        tries += 1
+        p = c + radius * (2 * rng.random(3) - 1)
+        if ((p - c) ** 2).sum() >= radius * radius:
+            continue
+        if len(pts) and (((np.array(pts) - p) ** 2).sum(1) < r2min).any():
+            continue
+        if len(shell) and (((shell - p) ** 2).sum(1) < r2min).any():
+            continue
+        pts.append(p)
+    return np.vstack([outside, np.array(pts)]) if pts else outside
+
+
+def maxwell_boltzmann(masses: np.ndarray, T: float, seed: int) -> np.ndarray:
+    """Velocities at temperature T with zero net momentum."""
+    rng = np.random.default_rng(seed)
+    s = np.sqrt(KB * T / masses)[:, None]
+    V = rng.normal(0.0, 1.0, (len(masses), 3)) * s
+    p = (masses[:, None] * V).sum(0) / masses.sum()
+    return V - p[None, :]
+
+
+def force_microbench(n: int = 64, seed: int = 1002):
+    """The force microbenchmark: n^3-cell periodic Si crystal, N(0, 0.1 A)
+    noise, a central R = 20 A sphere randomised at 2.28 g/cc (rmin 2.0 A).
+    Returns (X, atype, Lb)."""
+    L = n * A_SI
+    X = perturbed_crystal(n, 0.1e-10, seed)
+    rho = 2.28e3 / (28.0855 * 1.66053907e-27)  # atoms/m^3 at 2.28 g/cc
+    X = amorphous_sphere(X, L, 20e-10, rho, 2.0e-10, seed + 1)
+    X = np.mod(X, L)
+    return X, np.ones(len(X), np.int32), (L, L, L)

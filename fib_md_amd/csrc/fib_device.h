@@ -1,0 +1,80 @@
+// fib_device.h -- data layout shared by the HIP kernels and the host runtime.
+//
+// All per-atom device arrays are indexed by the *sorted* position p (atoms
+// grouped by cell, instance by instance).  Each instance owns the contiguous
+// range [atom_off, atom_off + natm_pad); natm_pad is a multiple of BLOCK so
+// that every thread block belongs to exactly one instance.  Slots
+// [atom_off + natm, atom_off + natm_pad) hold padding atoms that never move.
+//
+// Positions are double4: (x, y, z, w) where w carries the 64-bit word
+// (orig << 32) | meta, so one 32-byte gather yields position, species, flags
+// and original index of a neighbour.
+#pragma once
+#include <stdint.h>
+
+#define FIB_BLOCK 256
+#define FIB_WAVE 64
+
+// meta bits
+#define META_TYPE_MASK 0x3u   // 1 = Si, 2 = Ge, 3 = Ga
+#define META_FROZEN 0x4u      // xsides > 0: pinned to Xi every step
+#define META_DEAD 0x8u        // removed from the simulation
+#define META_PAD 0x10u        // padding slot
+#define META_INST_SHIFT 8
+
+// Per-instance geometry and bookkeeping (device copy in an array).
+struct DInst {
+  double lb[3], ilb[3], h[3];  // box, 1/box (reference iLb), cell size Lb/N
+  int32_t nc[3];               // reference cell counts N (si.f90:57)
+  int32_t next[3];             // extended grid: N+1 when N <= 2 (phantom layer), else N
+  int32_t bucket_off;          // first global bucket of this instance
+  int32_t nbucket;             // next0*next1*next2 + 2 (dropped, dead)
+  int32_t atom_off, natm, natm_pad;
+  int32_t nsg, nlj, impact;    // ions with orig >= nsg + impact are unfired
+  int32_t pad_before;          // atom_off - sum of natm of earlier instances
+  int32_t cell_real_off;       // prefix of N0*N1*N2 over earlier instances
+  int32_t block_off;           // first thread block of this instance
+  int32_t nblocks;             // natm_pad / FIB_BLOCK
+};
+
+// Force-field constants common to all instances (kernel argument by value).
+struct FConst {
+  double isig[2][2], epssig[2][2], eps[2][2], leps[2][2][2];
+  double A, B, p, q, al, gamma;
+  double cut_sw;      // 0.999*al: reduced-distance gate of SW terms
+  double rv2, rv2lj;  // squared list cutoffs rc^2, rclj^2 (si.f90:115-116)
+  double rlj2;        // squared Moliere force cutoff (2.5 sigma)^2
+  double molc[3], molb[3];
+  double aamol[3][3], zmol[3];
+  double ie0ec2;      // ie0*ec*ec is NOT precomputed in the reference; kept
+                      // separate factors below for the FP64 path
+  double ie0, ec;
+  double umax;        // table abscissa range (si.f90:583)
+  double tabn1;       // REAL(Ntable-1)
+  int32_t ntable;
+  int32_t izero;      // first table index (1-based) whose T and successors are 0
+  double mixed_r64;   // mixed precision: Moliere pairs below this use FP64
+};
+
+// Pair-table entry for the reference's 3-point Lagrange interpolation
+// (si.f90:554-573): abscissas u_i, u_{i+1}, u_{i+2} and values T_i..T_{i+2}.
+struct TabEntry {
+  double u0, u1, u2, t0, t1, t2;
+};
+
+// Per-instance reduction outputs / control scalars (device).
+struct DCtl {
+  double utot;          // potential energy of the last evaluation
+  double virial[9];
+  double side_sum;      // sum m v^2 over side-thermostat atoms (after V update)
+  double side_cnt;      // their count
+  double glob_sum;      // sum m v^2 over atype < 3 (after V update)
+  double temp_eq;       // equilibration: T of the rescaled V (temp.f90:183-205)
+  double kemax;         // max 0.5 m v^2 / ec after the position update (eV)
+  double vmax;          // max |v| (m/s) after the V update
+  double ke_sum;        // sum 0.5 m v^2 (J) after the V update
+  int32_t maxcnt_sw, maxcnt_ion, maxcnt_ionbig, nonfinite;
+};
+
+// Number of doubles in a reduction partial record.
+#define FIB_NRED 16

@@ -1,0 +1,66 @@
+// fib_kernels.h -- host-side launchers of the gfx950 kernels (internal).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "fib_device.h"
+
+// Per-step arguments of the integrate kernels (onestep, timeint.f90:158-242).
+struct StepArgs {
+  double mass[3];            // kg by type 1..3
+  double ts, tau, ttar;      // timestep, Berendsen time constant, target T
+  double temp;               // equilibration: the reference's `temp` variable
+  double kB, ec;
+  double sidewidth;          // nm
+  double outz[4], atomz[2], ionz;
+  int32_t mode;              // 0 equilibrate, 1 impact (side shells), 2 free
+  int32_t frz;               // apply frzsptrdatms in this step
+};
+
+// All device buffers of a handle (sorted order, see fib_device.h).
+struct DevBufs {
+  int NA;                    // total padded atoms
+  int ninst;
+  int nbucket;               // total buckets
+  int total_cells;           // total real cells
+  int Wsw, Wion;             // ELL widths
+  int cur;                   // current buffer of the double-buffered arrays
+  DInst *I;                  // [ninst]
+  DCtl *ctl;                 // [ninst]
+  double4 *X[2];
+  double *V[2], *F[2], *Xi[2];
+  int *inv;                  // [NA] atom_off + orig -> sorted position
+  int *bkey, *brank, *bcnt, *cstart, *perm, *bsum;
+  int *cnt_sw, *nbr_sw, *cnt_ion, *nbr_ion;
+  uint8_t *rev;
+  void *G;                   // double4 or float4 [Wsw * NA]
+  uint64_t *amask;
+  double *Fpart, *upart;     // [3*NA], [NA]
+  double *Fout, *uout;       // force-only results [3*NA], [NA]
+  double *partials;          // [NA/64 * FIB_NRED]
+  unsigned *counters;        // [ninst]
+  TabEntry *tab;             // [ntable + 3]
+  double *xfer;              // staging [3*NA] for host transfers
+};
+
+namespace fibk {
+void launch_bin(const DevBufs &d, hipStream_t s);
+void launch_scan(const DevBufs &d, hipStream_t s);
+void launch_sort(const DevBufs &d, hipStream_t s);  // scatter + fixup + reorder (flips cur)
+void launch_nbuild(const DevBufs &d, const FConst &fc, hipStream_t s);
+void launch_cellid(const DevBufs &d, int inst, int32_t *dout, hipStream_t s);
+void launch_force(const DevBufs &d, const FConst &fc, bool mixed, bool energy, bool virial,
+                  int mode, const StepArgs &sa, hipStream_t s);
+void launch_pos(const DevBufs &d, const StepArgs &sa, hipStream_t s);
+void launch_sums(const DevBufs &d, const StepArgs &sa, hipStream_t s);
+void launch_import(const DevBufs &d, int inst, const DInst &in, const double *X,
+                   const double *V, const int32_t *atype, const int32_t *xsides,
+                   const double *Xi, hipStream_t s);
+void launch_export_x(const DevBufs &d, int inst, const DInst &in, double *out, hipStream_t s);
+void launch_export_vec(const DevBufs &d, int inst, const DInst &in, const double *src,
+                       double *out, hipStream_t s);
+void launch_export_scalar(const DevBufs &d, int inst, const DInst &in, const double *src,
+                          double *out, hipStream_t s);
+void launch_set_atom(const DevBufs &d, int p, const double x[3], const double v[3],
+                     hipStream_t s);
+}  // namespace fibk

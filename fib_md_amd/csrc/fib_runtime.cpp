@@ -1,0 +1,926 @@
+// fib_runtime.cpp -- host runtime behind include/fibgpu.h.
+//
+// Owns the device buffers of a handle, derives the reference's constants
+// (prms.f90 setupSiGe/setupMol, si.f90 init_nlist/mktables), sequences the
+// kernels of one neighbor rebuild / force evaluation / time step and mirrors
+// the reference's step-control state (Ts, Tau, cadences, lt, time) on the
+// host so that the Fortran or Python host can drive the reference's TI loop.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/fibgpu.h"
+#include "fib_device.h"
+#include "fib_kernels.h"
+
+namespace {
+
+struct Timer {
+  bool on = false;
+  hipStream_t s = nullptr;
+  std::vector<hipEvent_t> pool;
+  struct Rec { int phase; hipEvent_t a, b; };
+  std::vector<Rec> recs;
+  size_t used = 0;
+  double acc[5] = {0, 0, 0, 0, 0};
+  int nforce = 0, nnlist = 0;
+  hipEvent_t get() {
+    if (used == pool.size()) {
+      hipEvent_t e;
+      (void)hipEventCreate(&e);
+      pool.push_back(e);
+    }
+    return pool[used++];
+  }
+  int begin() {
+    if (!on) return -1;
+    hipEvent_t a = get();
+    (void)hipEventRecord(a, s);
+    recs.push_back({-1, a, nullptr});
+    return (int)recs.size() - 1;
+  }
+  void end(int id, int phase) {
+    if (!on || id < 0) return;
+    hipEvent_t b = get();
+    (void)hipEventRecord(b, s);
+    recs[id].phase = phase;
+    recs[id].b = b;
+    if (phase == 2) ++nforce;
+    if (phase == 1) ++nnlist;
+  }
+  void flush() {
+    if (recs.empty()) return;
+    (void)hipStreamSynchronize(s);
+    for (auto &r : recs) {
+      float ms = 0;
+      if (r.b && hipEventElapsedTime(&ms, r.a, r.b) == hipSuccess) acc[r.phase] += ms;
+    }
+    recs.clear();
+    used = 0;
+  }
+  void reset() {
+    flush();
+    for (double &a : acc) a = 0;
+    nforce = nnlist = 0;
+  }
+};
+
+struct RefCtl {
+  bool have_run = false;
+  fib_run_params rp{};
+  int64_t lt = 0;
+  double time = 0, temp = 0, ts = 0, tau = 0;
+  int regime = 1;
+  int atm_out = 0, ntlist = 1, atlist = 1, dslist = 1;
+  int impact = 0;
+  bool fast = true;
+  bool begun = false;
+  int whichctrl = 1;
+};
+
+}  // namespace
+
+struct fib_handle {
+  int dev = 0;
+  hipStream_t stream = nullptr;
+  int prec = FIB_PREC_FP64;
+  fib_params P{};
+  FConst fc{};
+  std::vector<TabEntry> tab;
+  std::vector<fib_instance_desc> desc;
+  std::vector<DInst> inst;
+  DevBufs d{};
+  int *maxcnt = nullptr;  // [ninst*4] device
+  bool have_state = false, have_lists = false, have_force = false;
+  int last_flags = 0;
+  std::string err;
+  RefCtl ref;
+  Timer tim;
+  StepArgs base{};
+  int n_builds = 0;
+};
+
+#define HIPCHK(h, x)                                                              \
+  do {                                                                            \
+    hipError_t e_ = (x);                                                          \
+    if (e_ != hipSuccess) {                                                       \
+      (h)->err = std::string(#x) + ": " + hipGetErrorString(e_);                  \
+      return FIB_ERR_HIP;                                                         \
+    }                                                                             \
+  } while (0)
+
+static int32_t fail(fib_handle *h, int32_t code, const std::string &msg) {
+  if (h) h->err = msg;
+  return code;
+}
+
+extern "C" void fib_default_params(fib_params *p) {
+  std::memset(p, 0, sizeof(*p));
+  p->sw_A = 7.049556277;
+  p->sw_B = 0.6022245584;
+  p->sw_p = 4.0;
+  p->sw_q = 0.0;
+  p->sw_a = 1.8;
+  p->sw_lambda_si = 21.0;
+  p->sw_lambda_ge = 31.0;
+  p->sw_gamma = 1.2;
+  p->eps_si = 3.473928E-19;
+  p->eps_ge = 3.085E-19;
+  p->sigma_si = 2.0951E-10;
+  p->sigma_ge = 2.0951E-10;
+  p->mass_si = 46.637063E-27;
+  p->mass_ge = 120.6049350E-27;
+  p->mass_ga = 1.157777417E-25;
+  p->z_si = 14.0;
+  p->z_ge = 32.0;
+  p->z_ga = 31.0;
+  p->mol_c[0] = 0.35; p->mol_c[1] = 0.55; p->mol_c[2] = 0.10;
+  p->mol_b[0] = 0.3; p->mol_b[1] = 1.2; p->mol_b[2] = 6.0;
+  p->bohr = 0.053E-09;
+  p->kB = 1.381E-23;
+  p->eps0 = 8.854187E-12;
+  p->ec = 1.60217646E-19;
+  p->listfac = 1.1;
+  p->ntable = 12000;
+  p->mixed_fp64_moliere_r = 0.5E-10;
+}
+
+// Mixing tables (setupSiGe, prms.f90:452-480; setupMol :433-449), cutoffs
+// (init_nlist, si.f90:56-61) and the pair-derivative table (mktables,
+// si.f90:575-597).  Expressions keep the reference's operation order.
+static void derive_constants(fib_handle *h) {
+  const fib_params &p = h->P;
+  FConst &c = h->fc;
+  std::memset(&c, 0, sizeof(c));
+  const double eSi = p.eps_si, eGe = p.eps_ge, sSi = p.sigma_si, sGe = p.sigma_ge;
+  const double lSi = p.sw_lambda_si, lGe = p.sw_lambda_ge;
+  c.eps[0][0] = eSi; c.eps[0][1] = 0.5 * (eSi + eGe); c.eps[1][1] = eGe; c.eps[1][0] = 0.5 * (eSi + eGe);
+  c.isig[0][0] = 1. / sSi; c.isig[0][1] = 2. / (sSi + sGe); c.isig[1][1] = 1. / sGe;
+  c.isig[1][0] = 2. / (sSi + sGe);
+  const double m1 = 2. * lSi * eSi / 3. + lGe * eGe / 3.;
+  const double m2 = lSi * eSi / 3. + 2. * lGe * eGe / 3.;
+  c.leps[0][0][0] = lSi * eSi;
+  c.leps[0][0][1] = m1; c.leps[0][1][0] = m1; c.leps[1][0][0] = m1;
+  c.leps[0][1][1] = m2; c.leps[1][1][0] = m2; c.leps[1][0][1] = m2;
+  c.leps[1][1][1] = lGe * eGe;
+  c.epssig[0][0] = eSi / sSi;
+  c.epssig[0][1] = (eSi + eGe) / (sSi + sGe);
+  c.epssig[1][0] = (eSi + eGe) / (sSi + sGe);
+  c.epssig[1][1] = eGe / sGe;
+  c.A = p.sw_A; c.B = p.sw_B; c.p = p.sw_p; c.q = p.sw_q; c.al = p.sw_a; c.gamma = p.sw_gamma;
+  c.cut_sw = 0.999 * p.sw_a;
+  const double smax = std::max(sGe, sSi);
+  const double rc = p.listfac * p.sw_a * smax;
+  double rlj = 2.5 * smax;
+  const double rclj = p.listfac * rlj;
+  c.rv2 = rc * rc;
+  c.rv2lj = rclj * rclj;
+  c.rlj2 = rlj * rlj;
+  for (int k = 0; k < 3; ++k) { c.molc[k] = p.mol_c[k]; c.molb[k] = p.mol_b[k]; }
+  const double Pi = 4. * std::atan(1.0);
+  const double Z[3] = {p.z_si, p.z_ge, p.z_ga};
+  for (int k = 0; k < 3; ++k) c.zmol[k] = Z[k];
+  c.aamol[0][2] = 0.885 * p.bohr * std::pow(std::sqrt(p.z_si) + std::sqrt(p.z_ga), -2. / 3.);
+  c.aamol[1][2] = 0.885 * p.bohr * std::pow(std::sqrt(p.z_ge) + std::sqrt(p.z_ga), -2. / 3.);
+  c.aamol[2][0] = c.aamol[0][2];
+  c.aamol[2][1] = c.aamol[1][2];
+  c.aamol[2][2] = 0.885 * p.bohr * std::pow(std::sqrt(p.z_ga) + std::sqrt(p.z_ga), -2. / 3.);
+  c.ie0 = 1 / (4 * Pi * p.eps0);
+  c.ec = p.ec;
+  c.ie0ec2 = c.ie0 * p.ec * p.ec;
+  c.mixed_r64 = p.mixed_fp64_moliere_r;
+  // table (1-based abscissas u(i) = umax*(i-1)/(Ntable-1))
+  const int nt = p.ntable;
+  c.ntable = nt;
+  c.umax = 2.5 * p.listfac * p.sw_a;
+  c.tabn1 = (double)(nt - 1);
+  const double du = c.umax / (double)(nt - 1);
+  std::vector<double> u(nt + 4, 0.0), T(nt + 4, 0.0);
+  int izero = nt + 1;
+  for (int i = 2; i <= nt; ++i) {
+    const double ui = c.umax * (double)(i - 1) / (double)(nt - 1);
+    u[i] = ui;
+    if (ui < .999 * p.sw_a) {
+      const double t1 = (p.sw_B * std::pow(ui, -p.sw_p) - std::pow(ui, -p.sw_q)) / ((ui - p.sw_a) * (ui - p.sw_a));
+      const double t2 = p.sw_p * p.sw_B * std::pow(ui, -p.sw_p - 1) - p.sw_q * std::pow(ui, -p.sw_q - 1);
+      T[i] = -(p.sw_A * std::exp(1. / (ui - p.sw_a)) * (t1 + t2) / (du * du));
+    } else {
+      T[i] = 0.;
+      if (izero > nt) izero = i;
+    }
+  }
+  c.izero = izero;
+  h->tab.assign(nt + 3, TabEntry{0, 0, 0, 0, 0, 0});
+  for (int i = 1; i <= nt; ++i) {
+    TabEntry &e = h->tab[i];
+    e.u0 = u[i]; e.u1 = u[i + 1]; e.u2 = u[i + 2];
+    e.t0 = T[i]; e.t1 = T[i + 1]; e.t2 = T[i + 2];
+  }
+  StepArgs &b = h->base;
+  std::memset(&b, 0, sizeof(b));
+  b.mass[0] = p.mass_si; b.mass[1] = p.mass_ge; b.mass[2] = p.mass_ga;
+  b.kB = p.kB;
+  b.ec = p.ec;
+}
+
+static int32_t free_bufs(fib_handle *h) {
+  DevBufs &d = h->d;
+  void *ptrs[] = {d.I, d.ctl, d.X[0], d.X[1], d.V[0], d.V[1], d.F[0], d.F[1], d.Xi[0], d.Xi[1],
+                  d.inv, d.bkey, d.brank, d.bcnt, d.cstart, d.perm, d.bsum, d.cnt_sw, d.nbr_sw,
+                  d.cnt_ion, d.nbr_ion, d.rev, d.G, d.amask, d.Fpart, d.upart, d.Fout, d.uout,
+                  d.partials, d.counters, d.tab, d.xfer, h->maxcnt};
+  for (void *p : ptrs)
+    if (p) (void)hipFree(p);
+  std::memset(&d, 0, sizeof(d));
+  h->maxcnt = nullptr;
+  return FIB_OK;
+}
+
+template <typename T>
+static hipError_t dalloc(T **p, size_t n) {
+  return hipMalloc((void **)p, std::max<size_t>(n, 1) * sizeof(T));
+}
+
+static int32_t alloc_lists(fib_handle *h, int Wsw, int Wion) {
+  DevBufs &d = h->d;
+  if (d.nbr_sw) (void)hipFree(d.nbr_sw);
+  if (d.nbr_ion) (void)hipFree(d.nbr_ion);
+  if (d.rev) (void)hipFree(d.rev);
+  if (d.G) (void)hipFree(d.G);
+  d.nbr_sw = nullptr; d.nbr_ion = nullptr; d.rev = nullptr; d.G = nullptr;
+  d.Wsw = Wsw;
+  d.Wion = Wion;
+  const size_t NA = (size_t)d.NA;
+  HIPCHK(h, dalloc(&d.nbr_sw, (size_t)Wsw * NA));
+  HIPCHK(h, dalloc(&d.nbr_ion, (size_t)Wion * NA));
+  HIPCHK(h, dalloc(&d.rev, (size_t)Wsw * NA));
+  const size_t gsz = h->prec == FIB_PREC_MIXED ? sizeof(float4) : sizeof(double4);
+  HIPCHK(h, hipMalloc(&d.G, gsz * (size_t)Wsw * NA));
+  return FIB_OK;
+}
+
+extern "C" int32_t fib_create(const fib_params *p, int32_t ninst, const fib_instance_desc *desc,
+                              int32_t precision, fib_handle **out) {
+  if (!out) return FIB_ERR_ARG;
+  *out = nullptr;
+  if (!p || ninst < 1 || !desc) return FIB_ERR_ARG;
+  if (precision != FIB_PREC_FP64 && precision != FIB_PREC_MIXED) return FIB_ERR_ARG;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return FIB_ERR_NODEVICE;
+  fib_handle *h = new fib_handle();
+  if (hipGetDevice(&h->dev) != hipSuccess) { delete h; return FIB_ERR_NODEVICE; }
+  hipDeviceProp_t prop;
+  if (hipGetDeviceProperties(&prop, h->dev) != hipSuccess ||
+      std::strncmp(prop.gcnArchName, "gfx950", 6) != 0) {
+    delete h;
+    return FIB_ERR_NODEVICE;
+  }
+  h->P = *p;
+  h->prec = precision;
+  derive_constants(h);
+  const double smax = std::max(p->sigma_ge, p->sigma_si);
+  const double rc = p->listfac * p->sw_a * smax;
+  h->desc.assign(desc, desc + ninst);
+  h->inst.resize(ninst);
+  long atom_off = 0, bucket_off = 0, cell_off = 0, natm_before = 0;
+  for (int k = 0; k < ninst; ++k) {
+    const fib_instance_desc &ds = desc[k];
+    if (ds.natm < 1 || ds.nsg < 0 || ds.nlj < 0 || ds.nsg + ds.nlj != ds.natm ||
+        !(ds.lb[0] > 0 && ds.lb[1] > 0 && ds.lb[2] > 0)) {
+      delete h;
+      return FIB_ERR_ARG;
+    }
+    DInst &in = h->inst[k];
+    std::memset(&in, 0, sizeof(in));
+    for (int a = 0; a < 3; ++a) {
+      in.lb[a] = ds.lb[a];
+      in.ilb[a] = 1. / ds.lb[a];
+      const int n = (int)(ds.lb[a] / rc);
+      in.nc[a] = n > 1 ? n : 1;
+      in.h[a] = ds.lb[a] / (double)in.nc[a];
+      in.next[a] = in.nc[a] <= 2 ? in.nc[a] + 1 : in.nc[a];
+    }
+    in.natm = ds.natm;
+    in.nsg = ds.nsg;
+    in.nlj = ds.nlj;
+    in.natm_pad = (ds.natm + FIB_BLOCK - 1) / FIB_BLOCK * FIB_BLOCK;
+    in.atom_off = (int)atom_off;
+    in.pad_before = (int)(atom_off - natm_before);
+    in.bucket_off = (int)bucket_off;
+    in.nbucket = in.next[0] * in.next[1] * in.next[2] + 2;
+    in.cell_real_off = (int)cell_off;
+    in.block_off = (int)(atom_off / FIB_BLOCK);
+    in.nblocks = in.natm_pad / FIB_BLOCK;
+    atom_off += in.natm_pad;
+    natm_before += ds.natm;
+    bucket_off += in.nbucket;
+    cell_off += (long)in.nc[0] * in.nc[1] * in.nc[2];
+    if (atom_off > (1L << 30) || bucket_off > (1L << 30)) { delete h; return FIB_ERR_ARG; }
+  }
+  DevBufs &d = h->d;
+  d.NA = (int)atom_off;
+  d.ninst = ninst;
+  d.nbucket = (int)bucket_off;
+  d.total_cells = (int)cell_off;
+  const size_t NA = (size_t)d.NA;
+  if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess) {
+    delete h;
+    return FIB_ERR_HIP;
+  }
+  h->tim.s = h->stream;
+  int32_t rc_ = FIB_OK;
+#define ALLOC(ptr, n)                                           \
+  if (rc_ == FIB_OK && dalloc(&(ptr), (n)) != hipSuccess) {     \
+    rc_ = FIB_ERR_HIP;                                          \
+    h->err = "hipMalloc failed for " #ptr;                      \
+  }
+  ALLOC(d.I, (size_t)ninst);
+  ALLOC(d.ctl, (size_t)ninst);
+  for (int c = 0; c < 2; ++c) {
+    ALLOC(d.X[c], NA);
+    ALLOC(d.V[c], 3 * NA);
+    ALLOC(d.F[c], 3 * NA);
+    ALLOC(d.Xi[c], 3 * NA);
+  }
+  ALLOC(d.inv, NA);
+  ALLOC(d.bkey, NA);
+  ALLOC(d.brank, NA);
+  ALLOC(d.bcnt, (size_t)d.nbucket);
+  ALLOC(d.cstart, (size_t)d.nbucket + 1);
+  ALLOC(d.perm, NA);
+  ALLOC(d.bsum, (size_t)d.nbucket / 2048 + 2);
+  ALLOC(d.cnt_sw, NA);
+  ALLOC(d.cnt_ion, NA);
+  ALLOC(d.amask, NA);
+  ALLOC(d.Fpart, 3 * NA);
+  ALLOC(d.upart, NA);
+  ALLOC(d.Fout, 3 * NA);
+  ALLOC(d.uout, NA);
+  ALLOC(d.partials, (NA / FIB_WAVE + 1) * FIB_NRED);
+  ALLOC(d.counters, (size_t)ninst);
+  ALLOC(d.tab, h->tab.size());
+  ALLOC(d.xfer, 3 * NA);
+  ALLOC(h->maxcnt, (size_t)ninst * 4);
+#undef ALLOC
+  if (rc_ == FIB_OK) rc_ = alloc_lists(h, 32, 64);
+  if (rc_ != FIB_OK) {
+    free_bufs(h);
+    (void)hipStreamDestroy(h->stream);
+    delete h;
+    return rc_;
+  }
+  hipError_t e = hipMemcpy(d.I, h->inst.data(), sizeof(DInst) * ninst, hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemset(d.ctl, 0, sizeof(DCtl) * ninst);
+  if (e == hipSuccess) e = hipMemset(d.counters, 0, sizeof(unsigned) * ninst);
+  if (e == hipSuccess) e = hipMemset(d.cnt_sw, 0, sizeof(int) * NA);
+  if (e == hipSuccess) e = hipMemset(d.cnt_ion, 0, sizeof(int) * NA);
+  if (e == hipSuccess) e = hipMemset(d.amask, 0, sizeof(uint64_t) * NA);
+  if (e == hipSuccess)
+    e = hipMemcpy(d.tab, h->tab.data(), sizeof(TabEntry) * h->tab.size(), hipMemcpyHostToDevice);
+  if (e != hipSuccess) {
+    free_bufs(h);
+    (void)hipStreamDestroy(h->stream);
+    delete h;
+    return FIB_ERR_HIP;
+  }
+  // every slot starts as padding of its instance so that all kernels see a
+  // consistent meta word before fib_set_state
+  for (int k = 0; k < ninst; ++k) {
+    DInst in = h->inst[k];
+    in.natm = 0;
+    fibk::launch_import(d, k, in, nullptr, nullptr, nullptr, nullptr, nullptr, h->stream);
+  }
+  if (hipStreamSynchronize(h->stream) != hipSuccess) {
+    free_bufs(h);
+    (void)hipStreamDestroy(h->stream);
+    delete h;
+    return FIB_ERR_HIP;
+  }
+  *out = h;
+  return FIB_OK;
+}
+
+extern "C" int32_t fib_destroy(fib_handle *h) {
+  if (!h) return FIB_ERR_ARG;
+  (void)hipStreamSynchronize(h->stream);
+  for (auto e : h->tim.pool) (void)hipEventDestroy(e);
+  free_bufs(h);
+  (void)hipStreamDestroy(h->stream);
+  delete h;
+  return FIB_OK;
+}
+
+extern "C" const char *fib_last_error(const fib_handle *h) {
+  return h ? h->err.c_str() : "null handle";
+}
+
+extern "C" int32_t fib_get_cells(const fib_handle *h, int32_t inst, int32_t nc[3]) {
+  if (!h || inst < 0 || inst >= (int)h->inst.size() || !nc) return FIB_ERR_ARG;
+  for (int a = 0; a < 3; ++a) nc[a] = h->inst[inst].nc[a];
+  return FIB_OK;
+}
+
+static bool bad_inst(const fib_handle *h, int32_t inst) {
+  return !h || inst < 0 || inst >= (int)h->inst.size();
+}
+
+extern "C" int32_t fib_set_state(fib_handle *h, int32_t inst, const double *X, const double *V,
+                                 const int32_t *atype, const int32_t *xsides, const double *Xi) {
+  if (bad_inst(h, inst) || !X || !atype) return fail(h, FIB_ERR_ARG, "fib_set_state: bad argument");
+  const DInst &in = h->inst[inst];
+  const size_t n = (size_t)in.natm;
+  for (size_t i = 0; i < n; ++i) {
+    const int t = atype[i];
+    const bool ion = i >= (size_t)in.nsg;
+    if (t < 1 || t > 3 || (ion != (t == 3)))
+      return fail(h, FIB_ERR_ARG, "fib_set_state: atype must be 1/2 for atoms 1..nsg and 3 for ions");
+  }
+  double *dx = nullptr, *dv = nullptr, *dxi = nullptr;
+  int32_t *dt = nullptr, *ds = nullptr;
+  HIPCHK(h, dalloc(&dx, 3 * n));
+  HIPCHK(h, hipMemcpy(dx, X, 3 * n * sizeof(double), hipMemcpyHostToDevice));
+  if (V) {
+    HIPCHK(h, dalloc(&dv, 3 * n));
+    HIPCHK(h, hipMemcpy(dv, V, 3 * n * sizeof(double), hipMemcpyHostToDevice));
+  }
+  if (Xi) {
+    HIPCHK(h, dalloc(&dxi, 3 * n));
+    HIPCHK(h, hipMemcpy(dxi, Xi, 3 * n * sizeof(double), hipMemcpyHostToDevice));
+  }
+  HIPCHK(h, dalloc(&dt, n));
+  HIPCHK(h, hipMemcpy(dt, atype, n * sizeof(int32_t), hipMemcpyHostToDevice));
+  if (xsides) {
+    HIPCHK(h, dalloc(&ds, n));
+    HIPCHK(h, hipMemcpy(ds, xsides, n * sizeof(int32_t), hipMemcpyHostToDevice));
+  }
+  fibk::launch_import(h->d, inst, in, dx, dv, dt, ds, dxi, h->stream);
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  (void)hipFree(dx);
+  if (dv) (void)hipFree(dv);
+  if (dxi) (void)hipFree(dxi);
+  (void)hipFree(dt);
+  if (ds) (void)hipFree(ds);
+  h->have_state = true;
+  h->have_lists = false;
+  h->have_force = false;
+  return FIB_OK;
+}
+
+extern "C" int32_t fib_get_state(fib_handle *h, int32_t inst, double *X, double *V, double *F) {
+  if (bad_inst(h, inst)) return fail(h, FIB_ERR_ARG, "fib_get_state: bad instance");
+  if (!h->have_state) return fail(h, FIB_ERR_STATE, "fib_get_state: no state");
+  const DInst &in = h->inst[inst];
+  const size_t n = (size_t)in.natm;
+  DevBufs &d = h->d;
+  if (X) {
+    fibk::launch_export_x(d, inst, in, d.xfer, h->stream);
+    HIPCHK(h, hipMemcpyAsync(X, d.xfer, 3 * n * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+  }
+  if (V) {
+    fibk::launch_export_vec(d, inst, in, d.V[d.cur], d.xfer, h->stream);
+    HIPCHK(h, hipMemcpyAsync(V, d.xfer, 3 * n * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+  }
+  if (F) {
+    fibk::launch_export_vec(d, inst, in, d.F[d.cur], d.xfer, h->stream);
+    HIPCHK(h, hipMemcpyAsync(F, d.xfer, 3 * n * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+  }
+  return FIB_OK;
+}
+
+extern "C" int32_t fib_set_impact(fib_handle *h, int32_t inst, int32_t impact) {
+  if (bad_inst(h, inst) || impact < 0) return fail(h, FIB_ERR_ARG, "fib_set_impact: bad argument");
+  h->inst[inst].impact = impact;
+  HIPCHK(h, hipMemcpyAsync(&h->d.I[inst], &h->inst[inst], sizeof(DInst), hipMemcpyHostToDevice,
+                           h->stream));
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  return FIB_OK;
+}
+
+extern "C" int32_t fib_bin_cells(fib_handle *h, int32_t inst, int32_t *icell) {
+  if (bad_inst(h, inst) || !icell) return fail(h, FIB_ERR_ARG, "fib_bin_cells: bad argument");
+  if (!h->have_state) return fail(h, FIB_ERR_STATE, "fib_bin_cells: no state");
+  const size_t n = (size_t)h->inst[inst].natm;
+  int32_t *dout = nullptr;
+  HIPCHK(h, dalloc(&dout, 3 * n));
+  fibk::launch_cellid(h->d, inst, dout, h->stream);
+  HIPCHK(h, hipMemcpyAsync(icell, dout, 3 * n * sizeof(int32_t), hipMemcpyDeviceToHost, h->stream));
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  (void)hipFree(dout);
+  return FIB_OK;
+}
+
+// Sort + list build.  Grows the ELL widths and rebuilds when a list
+// overflowed (the reference only prints, si.f90:264-272).
+static int32_t build_lists(fib_handle *h) {
+  DevBufs &d = h->d;
+  const int tid = h->tim.begin();
+  fibk::launch_bin(d, h->stream);
+  fibk::launch_scan(d, h->stream);
+  fibk::launch_sort(d, h->stream);
+  d.cur = 1 - d.cur;
+  for (int attempt = 0; attempt < 4; ++attempt) {
+    // k_nbuild reports max list lengths into DCtl::maxcnt_*
+    fibk::launch_nbuild(d, h->fc, h->stream);
+    std::vector<DCtl> c(d.ninst);
+    HIPCHK(h, hipMemcpyAsync(c.data(), d.ctl, sizeof(DCtl) * d.ninst, hipMemcpyDeviceToHost,
+                             h->stream));
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    int msw = 0, mion = 0, nonfin = 0;
+    for (auto &x : c) {
+      msw = std::max(msw, x.maxcnt_sw);
+      mion = std::max(mion, x.maxcnt_ion);
+      nonfin |= x.nonfinite;
+    }
+    // clear the counters for the next build (other DCtl fields unchanged)
+    for (int k = 0; k < d.ninst; ++k) {
+      int zero4[4] = {0, 0, 0, 0};
+      HIPCHK(h, hipMemcpyAsync(&d.ctl[k].maxcnt_sw, zero4, sizeof(zero4), hipMemcpyHostToDevice,
+                               h->stream));
+    }
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    if (nonfin) return fail(h, FIB_ERR_NONFINITE, "non-finite position detected while binning");
+    if (msw <= d.Wsw && mion <= d.Wion) {
+      h->tim.end(tid, 1);
+      h->have_lists = true;
+      h->have_force = false;
+      ++h->n_builds;
+      return FIB_OK;
+    }
+    int nsw = d.Wsw, nion = d.Wion;
+    while (nsw < msw) nsw *= 2;
+    while (nion < mion) nion *= 2;
+    if (nsw > 64) return fail(h, FIB_ERR_OVERFLOW, "SW neighbor list exceeds 64 entries per atom");
+    int32_t r = alloc_lists(h, nsw, nion);
+    if (r != FIB_OK) return r;
+  }
+  return fail(h, FIB_ERR_OVERFLOW, "neighbor list did not fit after regrowth");
+}
+
+extern "C" int32_t fib_build_nlist(fib_handle *h) {
+  if (!h) return FIB_ERR_ARG;
+  if (!h->have_state) return fail(h, FIB_ERR_STATE, "fib_build_nlist: no state");
+  return build_lists(h);
+}
+
+extern "C" int32_t fib_get_nlist(fib_handle *h, int32_t inst, int32_t *nss, int32_t *nlist2,
+                                 int32_t *len2, int32_t *nsslj, int32_t *nlistlj,
+                                 int32_t *lenlj) {
+  if (bad_inst(h, inst)) return fail(h, FIB_ERR_ARG, "fib_get_nlist: bad instance");
+  if (!h->have_lists) return fail(h, FIB_ERR_STATE, "fib_get_nlist: no lists");
+  const DInst &in = h->inst[inst];
+  const DevBufs &d = h->d;
+  const int n = in.natm, np = in.natm_pad, off = in.atom_off;
+  std::vector<double4> X(np);
+  std::vector<int> csw(np), cio(np);
+  HIPCHK(h, hipMemcpy(X.data(), d.X[d.cur] + off, sizeof(double4) * np, hipMemcpyDeviceToHost));
+  HIPCHK(h, hipMemcpy(csw.data(), d.cnt_sw + off, sizeof(int) * np, hipMemcpyDeviceToHost));
+  HIPCHK(h, hipMemcpy(cio.data(), d.cnt_ion + off, sizeof(int) * np, hipMemcpyDeviceToHost));
+  std::vector<int> orig(np, -1);
+  for (int q = 0; q < np; ++q) {
+    uint64_t w;
+    std::memcpy(&w, &X[q].w, 8);
+    if (!((uint32_t)w & META_PAD)) orig[q] = (int)(w >> 32);
+  }
+  auto fetch = [&](const int *lst, int W, const std::vector<int> &cnt, std::vector<std::vector<int>> &half) -> int32_t {
+    int mx = 0;
+    for (int q = 0; q < np; ++q) mx = std::max(mx, std::min(cnt[q], W));
+    std::vector<int> rows((size_t)mx * np);
+    for (int k = 0; k < mx; ++k) {
+      hipError_t e = hipMemcpy(rows.data() + (size_t)k * np, lst + (size_t)k * d.NA + off,
+                               sizeof(int) * np, hipMemcpyDeviceToHost);
+      if (e != hipSuccess) { h->err = hipGetErrorString(e); return FIB_ERR_HIP; }
+    }
+    half.assign(n, {});
+    for (int q = 0; q < np; ++q) {
+      if (orig[q] < 0) continue;
+      const int c = std::min(cnt[q], W);
+      for (int k = 0; k < c; ++k) {
+        const int j = rows[(size_t)k * np + q] - off;
+        if (j < 0 || j >= np || orig[j] < 0) continue;
+        if (orig[j] > orig[q]) half[orig[q]].push_back(orig[j] + 1);
+      }
+    }
+    return FIB_OK;
+  };
+  std::vector<std::vector<int>> h2, hl;
+  int32_t r = fetch(d.nbr_sw, d.Wsw, csw, h2);
+  if (r != FIB_OK) return r;
+  r = fetch(d.nbr_ion, d.Wion, cio, hl);
+  if (r != FIB_OK) return r;
+  auto emit = [&](const std::vector<std::vector<int>> &half, int32_t *ss, int32_t *lst, int32_t *len) {
+    int pos = 0;
+    for (int i = 0; i < n; ++i) {
+      if (ss) { ss[i] = pos + 1; ss[n + i] = pos + (int)half[i].size(); }
+      if (lst)
+        for (int v : half[i]) lst[pos++] = v;
+      else
+        pos += (int)half[i].size();
+    }
+    if (len) *len = pos;
+  };
+  emit(h2, nss, nlist2, len2);
+  emit(hl, nsslj, nlistlj, lenlj);
+  return FIB_OK;
+}
+
+static int32_t run_force(fib_handle *h, int flags, int mode, const StepArgs &sa) {
+  const bool en = (flags & FIB_EVAL_ENERGY) != 0, vir = (flags & FIB_EVAL_VIRIAL) != 0;
+  const int tid = h->tim.begin();
+  fibk::launch_force(h->d, h->fc, h->prec == FIB_PREC_MIXED, en, vir, mode, sa, h->stream);
+  h->tim.end(tid, 2);
+  return FIB_OK;
+}
+
+extern "C" int32_t fib_compute(fib_handle *h, int32_t flags) {
+  if (!h) return FIB_ERR_ARG;
+  if (!h->have_lists) return fail(h, FIB_ERR_STATE, "fib_compute: build the lists first");
+  int32_t r = run_force(h, flags, 0, h->base);
+  if (r != FIB_OK) return r;
+  HIPCHK(h, hipGetLastError());
+  h->have_force = true;
+  h->last_flags = flags;
+  return FIB_OK;
+}
+
+extern "C" int32_t fib_get_forces(fib_handle *h, int32_t inst, double *F, double *u, double *utot,
+                                  double virial[9]) {
+  if (bad_inst(h, inst)) return fail(h, FIB_ERR_ARG, "fib_get_forces: bad instance");
+  if (!h->have_force) return fail(h, FIB_ERR_STATE, "fib_get_forces: no force evaluation");
+  const DInst &in = h->inst[inst];
+  const size_t n = (size_t)in.natm;
+  DevBufs &d = h->d;
+  if (F) {
+    fibk::launch_export_vec(d, inst, in, d.Fout, d.xfer, h->stream);
+    HIPCHK(h, hipMemcpyAsync(F, d.xfer, 3 * n * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+  }
+  if (u) {
+    if (!(h->last_flags & FIB_EVAL_ENERGY)) return fail(h, FIB_ERR_STATE, "energy not evaluated");
+    fibk::launch_export_scalar(d, inst, in, d.uout, d.xfer, h->stream);
+    HIPCHK(h, hipMemcpyAsync(u, d.xfer, n * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+  }
+  DCtl c;
+  HIPCHK(h, hipMemcpyAsync(&c, &d.ctl[inst], sizeof(DCtl), hipMemcpyDeviceToHost, h->stream));
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  if (utot) *utot = c.utot;
+  if (virial)
+    for (int q = 0; q < 9; ++q) virial[q] = c.virial[q];
+  return FIB_OK;
+}
+
+extern "C" int32_t fib_force(fib_handle *h, int32_t inst, int32_t impact, double *F, double *u,
+                             double *utot, double virial[9]) {
+  int32_t r = fib_set_impact(h, inst, impact);
+  if (r != FIB_OK) return r;
+  int flags = FIB_EVAL_FORCE;
+  if (u || utot) flags |= FIB_EVAL_ENERGY;
+  if (virial) flags |= FIB_EVAL_VIRIAL;
+  r = fib_compute(h, flags);
+  if (r != FIB_OK) return r;
+  return fib_get_forces(h, inst, F, u, utot, virial);
+}
+
+// ---------------------------------------------------------------------------
+// reference step control (instance 0)
+
+static void ionrun(fib_handle *h, int tic) {
+  RefCtl &R = h->ref;
+  const fib_run_params &p = R.rp;
+  if (tic == 1) {
+    R.ts = p.ts_i; R.atm_out = p.atm_out_i; R.ntlist = p.ntlist_i; R.atlist = p.atlist_i;
+    R.tau = p.tau_i; R.dslist = p.dslist_i; R.regime = 1;
+  } else {
+    R.ts = p.ts_r; R.atm_out = p.atm_out_r; R.ntlist = p.ntlist_r; R.atlist = p.atlist_r;
+    R.tau = p.tau_r; R.dslist = p.dslist_r; R.regime = 2;
+  }
+}
+
+static StepArgs step_args(fib_handle *h, int mode) {
+  StepArgs sa = h->base;
+  const RefCtl &R = h->ref;
+  sa.ts = R.ts;
+  sa.tau = R.tau;
+  sa.ttar = R.rp.ttar;
+  sa.temp = R.temp;
+  sa.sidewidth = R.rp.sidewidth;
+  for (int k = 0; k < 4; ++k) sa.outz[k] = R.rp.outz[k];
+  sa.atomz[0] = R.rp.atomz[0];
+  sa.atomz[1] = R.rp.atomz[1];
+  sa.ionz = R.rp.ionz;
+  sa.mode = mode;
+  sa.frz = 0;
+  return sa;
+}
+
+extern "C" int32_t fib_set_run(fib_handle *h, int32_t inst, const fib_run_params *rp) {
+  if (bad_inst(h, inst) || inst != 0 || !rp) return fail(h, FIB_ERR_ARG, "fib_set_run: bad argument");
+  if (rp->ntlist_i < 1 || rp->ntlist_r < 1 || rp->atlist_i < 1 || rp->atlist_r < 1 ||
+      rp->dslist_i < 1 || rp->dslist_r < 1)
+    return fail(h, FIB_ERR_ARG, "fib_set_run: cadences must be >= 1");
+  h->ref.rp = *rp;
+  h->ref.have_run = true;
+  ionrun(h, 1);
+  return FIB_OK;
+}
+
+static int32_t read_ctl(fib_handle *h, int inst, DCtl *c) {
+  HIPCHK(h, hipMemcpyAsync(c, &h->d.ctl[inst], sizeof(DCtl), hipMemcpyDeviceToHost, h->stream));
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  return FIB_OK;
+}
+
+static double temp_of(const fib_handle *h, const DCtl &c) {
+  return c.glob_sum / 3. / (double)h->inst[0].nsg / h->P.kB;
+}
+
+extern "C" int32_t fib_temperature(fib_handle *h, int32_t inst, double *temp) {
+  if (bad_inst(h, inst) || !temp) return fail(h, FIB_ERR_ARG, "fib_temperature: bad argument");
+  if (!h->have_state) return fail(h, FIB_ERR_STATE, "fib_temperature: no state");
+  StepArgs sa = h->ref.have_run ? step_args(h, 1) : h->base;
+  fibk::launch_sums(h->d, sa, h->stream);
+  DCtl c;
+  int32_t r = read_ctl(h, inst, &c);
+  if (r != FIB_OK) return r;
+  *temp = c.glob_sum / 3. / (double)h->inst[inst].nsg / h->P.kB;
+  return FIB_OK;
+}
+
+extern "C" int32_t fib_tint_init(fib_handle *h, int64_t lt0, double time0) {
+  if (!h) return FIB_ERR_ARG;
+  if (!h->ref.have_run) return fail(h, FIB_ERR_STATE, "fib_tint_init: call fib_set_run first");
+  if (!h->have_lists) return fail(h, FIB_ERR_STATE, "fib_tint_init: build the lists first");
+  RefCtl &R = h->ref;
+  R.lt = lt0;
+  R.time = time0;
+  double t = 0;
+  int32_t r = fib_temperature(h, 0, &t);
+  if (r != FIB_OK) return r;
+  R.temp = t;
+  // initial force into F (timeint.f90:69): force-only pass, then copy
+  StepArgs sa = step_args(h, 1);
+  r = run_force(h, FIB_EVAL_FORCE, 0, sa);
+  if (r != FIB_OK) return r;
+  HIPCHK(h, hipMemcpyAsync(h->d.F[h->d.cur], h->d.Fout, 3 * sizeof(double) * (size_t)h->d.NA,
+                           hipMemcpyDeviceToDevice, h->stream));
+  fibk::launch_sums(h->d, sa, h->stream);
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  return FIB_OK;
+}
+
+extern "C" int32_t fib_ti_begin(fib_handle *h, int32_t whichctrl, int32_t impact) {
+  if (!h || whichctrl < 0 || whichctrl > 2) return FIB_ERR_ARG;
+  if (!h->ref.have_run) return fail(h, FIB_ERR_STATE, "fib_ti_begin: call fib_set_run first");
+  RefCtl &R = h->ref;
+  R.whichctrl = whichctrl;
+  R.begun = true;
+  if (whichctrl == 0) {
+    R.tau = 1.E-14;  // timeint.f90:85
+    double t = 0;
+    int32_t r = fib_temperature(h, 0, &t);
+    if (r != FIB_OK) return r;
+    R.temp = t;
+  } else if (whichctrl == 1) {
+    R.fast = true;
+    ionrun(h, 1);
+    R.impact = impact;
+    int32_t r = fib_set_impact(h, 0, impact);
+    if (r != FIB_OK) return r;
+  } else {
+    ionrun(h, 2);
+  }
+  return FIB_OK;
+}
+
+extern "C" int32_t fib_knock(fib_handle *h, int32_t knc, const double xion[3],
+                             const double vion[3]) {
+  if (!h || !xion || !vion) return FIB_ERR_ARG;
+  const DInst &in = h->inst[0];
+  if (knc < 1 || knc > in.nlj) return fail(h, FIB_ERR_ARG, "fib_knock: ion number out of range");
+  int p = -1;
+  HIPCHK(h, hipMemcpy(&p, h->d.inv + in.atom_off + in.nsg + knc - 1, sizeof(int),
+                      hipMemcpyDeviceToHost));
+  fibk::launch_set_atom(h->d, p, xion, vion, h->stream);
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  return FIB_OK;
+}
+
+extern "C" int32_t fib_step(fib_handle *h, const fib_step_ctl *ctl, fib_step_report *rep) {
+  if (!h || !ctl || !rep) return FIB_ERR_ARG;
+  if (!h->ref.have_run || !h->ref.begun) return fail(h, FIB_ERR_STATE, "fib_step: call fib_ti_begin first");
+  if (!h->have_lists) return fail(h, FIB_ERR_STATE, "fib_step: build the lists first");
+  if (h->inst.size() != 1) return fail(h, FIB_ERR_STATE, "fib_step drives a single instance");
+  RefCtl &R = h->ref;
+  const int w = ctl->whichctrl;
+  if (w != R.whichctrl) return fail(h, FIB_ERR_STATE, "fib_step: whichctrl differs from fib_ti_begin");
+  std::memset(rep, 0, sizeof(*rep));
+  const fib_run_params &P = R.rp;
+  const int builds0 = h->n_builds;
+  int steps = 0;
+  for (;;) {
+    if (ctl->max_steps > 0 && steps >= ctl->max_steps) break;
+    if (w == 0) {
+      if (!(std::fabs(R.temp - P.ttar) > P.teps)) { rep->ended = 1; break; }
+      R.lt += 1;
+    } else if (w == 1) {
+      R.lt += 1;
+      if (R.time > (double)R.impact * P.dti) { rep->ended = 1; break; }
+    } else {
+      R.lt += 1;
+    }
+    const int64_t lt = R.lt;
+    // onestep (timeint.f90:158-242)
+    R.time = R.time + R.ts;
+    StepArgs sa = step_args(h, w == 0 ? 0 : 1);
+    sa.frz = (lt % R.atlist == 0) ? 1 : 0;
+    int tid = h->tim.begin();
+    fibk::launch_pos(h->d, sa, h->stream);
+    h->tim.end(tid, 3);
+    if (w == 0) {
+      DCtl c;
+      int32_t r = read_ctl(h, 0, &c);
+      if (r != FIB_OK) return r;
+      R.temp = c.temp_eq;
+    }
+    if (lt % R.ntlist == 0) {
+      int32_t r = build_lists(h);
+      if (r != FIB_OK) return r;
+    }
+    if (lt % R.dslist == 0 && w == 1) {
+      DCtl c;
+      int32_t r = read_ctl(h, 0, &c);
+      if (r != FIB_OK) return r;
+      if (c.kemax < P.ke_limit_1 && R.fast) {
+        ionrun(h, 2);
+        R.fast = false;
+      }
+    }
+    StepArgs sb = step_args(h, 1);
+    int32_t r = run_force(h, FIB_EVAL_FORCE, 1, sb);
+    if (r != FIB_OK) return r;
+    ++steps;
+    int out = 0;
+    if (w > 0) {
+      out = (P.res_out > 0 && lt % P.res_out == 0) || (R.atm_out > 0 && lt % R.atm_out == 0);
+      if (out) {
+        DCtl c;
+        r = read_ctl(h, 0, &c);
+        if (r != FIB_OK) return r;
+        R.temp = temp_of(h, c);
+      }
+    } else {
+      out = (R.atm_out > 0 && lt % R.atm_out == 0);
+    }
+    if (out) {
+      rep->output_due = 1;
+      if (ctl->stop_on_output) break;
+    }
+  }
+  HIPCHK(h, hipGetLastError());
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  h->tim.flush();
+  rep->lt = R.lt;
+  rep->time = R.time;
+  rep->temp = R.temp;
+  rep->ts = R.ts;
+  rep->steps_done = steps;
+  rep->regime = R.regime;
+  rep->nlist_builds = h->n_builds - builds0;
+  h->have_force = false;
+  return FIB_OK;
+}
+
+extern "C" int32_t fib_timing_enable(fib_handle *h, int32_t on) {
+  if (!h) return FIB_ERR_ARG;
+  h->tim.reset();
+  h->tim.on = on != 0;
+  return FIB_OK;
+}
+
+extern "C" int32_t fib_get_timing(fib_handle *h, fib_timing *t) {
+  if (!h || !t) return FIB_ERR_ARG;
+  h->tim.flush();
+  std::memset(t, 0, sizeof(*t));
+  t->bin_ms = h->tim.acc[0];
+  t->nlist_ms = h->tim.acc[1];
+  t->force_ms = h->tim.acc[2];
+  t->integrate_ms = h->tim.acc[3];
+  t->total_ms = h->tim.acc[1] + h->tim.acc[2] + h->tim.acc[3];
+  t->n_force = h->tim.nforce;
+  t->n_nlist = h->tim.nnlist;
+  return FIB_OK;
+}
+
+extern "C" int32_t fib_synchronize(fib_handle *h) {
+  if (!h) return FIB_ERR_ARG;
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  HIPCHK(h, hipGetLastError());
+  return FIB_OK;
+}

@@ -1,0 +1,217 @@
+/*
+ * fibgpu.h -- C-ABI of the MI355X (gfx950) force-and-integrate library.
+ *
+ * This is the drop-in boundary for FIB_MD's per-step hot path.  The
+ * reference has no FFI: its boundary is four Fortran module procedures with
+ * implicit module state.  Each entry point below replaces one of them (cited
+ * as reference src/file:line) and is bound from Fortran through
+ * iso_c_binding (fortran/fibgpu_iface.f90) or from Python through ctypes.
+ *
+ * Conventions
+ *  - Every function returns an int32 status (FIB_OK == 0); fib_last_error()
+ *    gives a message.  No function prints or aborts.
+ *  - Host arrays are owned by the caller, passed by pointer, never retained.
+ *    Per-atom vectors are Fortran (natm,3) column-major arrays: x[0..natm),
+ *    then y, then z -- the reference's X(Natm,3).  Atom indices in exported
+ *    lists are 1-based, as in the reference.
+ *  - Device memory is owned by the library.  Calls on one handle must come
+ *    from one host thread; each handle owns one HIP stream.  Handles are
+ *    independent (no global state), so several may coexist.
+ *  - There is no CPU fallback: without a usable gfx950 device fib_create()
+ *    returns FIB_ERR_NODEVICE.
+ *  - A handle holds one or more independent instances (substrates).  The
+ *    reference corresponds to one instance; batched ensembles put many
+ *    instances of different sizes into the same launches.
+ */
+#ifndef FIBGPU_H
+#define FIBGPU_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define FIBGPU_ABI_VERSION 1
+
+enum fib_status {
+  FIB_OK = 0,
+  FIB_ERR_ARG = 1,       /* bad argument (size, index, null pointer)       */
+  FIB_ERR_HIP = 2,       /* a HIP runtime call failed                       */
+  FIB_ERR_OVERFLOW = 3,  /* a neighbor list exceeded capacity even after growth */
+  FIB_ERR_TABLE = 4,     /* pair-table range miss (reference si.f90:562-564) */
+  FIB_ERR_NONFINITE = 5, /* NaN/Inf detected in state                       */
+  FIB_ERR_STATE = 6,     /* call order violated (e.g. force before lists)   */
+  FIB_ERR_NODEVICE = 7   /* no gfx950 GPU visible                           */
+};
+
+enum fib_precision {
+  FIB_PREC_FP64 = 0,  /* all force arithmetic in FP64 (reference arithmetic) */
+  FIB_PREC_MIXED = 1  /* FP32 pair/triplet terms, compensated FP32 force sums,
+                         FP64 positions, velocities, integration, energies and
+                         close Moliere pairs */
+};
+
+/* Physical constants and numerical knobs.  fib_default_params() fills the
+ * reference's values (prms.f90:151-216). SI units throughout. */
+typedef struct fib_params {
+  /* Stillinger-Weber (prms.f90:189-201) */
+  double sw_A, sw_B, sw_p, sw_q, sw_a, sw_lambda_si, sw_lambda_ge, sw_gamma;
+  double eps_si, eps_ge, sigma_si, sigma_ge;
+  /* masses, kg (prms.f90:178-180) */
+  double mass_si, mass_ge, mass_ga;
+  /* Moliere (prms.f90:175-187, setupMol :433-449) */
+  double z_si, z_ge, z_ga;
+  double mol_c[3], mol_b[3];
+  double bohr;      /* ao = 0.053e-9 m (prms.f90:154)            */
+  double kB, eps0, ec;
+  /* numerics */
+  double listfac;   /* neighbor-list padding factor (prms.f90:212)   */
+  int32_t ntable;   /* pair-derivative table size (prms.f90:216)     */
+  /* mixed precision: Moliere pairs closer than this use FP64 (m) */
+  double mixed_fp64_moliere_r;
+} fib_params;
+
+void fib_default_params(fib_params *p);
+
+/* One substrate/instance.  Atoms 1..nsg are SW (Si/Ge) atoms, nsg+1..natm are
+ * Ga ions in firing order (prms.f90:541).  lb is the periodic box (m). */
+typedef struct fib_instance_desc {
+  int32_t natm, nsg, nlj;
+  int32_t reserved;
+  double lb[3];
+} fib_instance_desc;
+
+typedef struct fib_handle fib_handle;
+
+/* Create a handle on the current HIP device.  Computes the reference's
+ * derived quantities per instance: rc, Nc = MAX(1,INT(Lb/rc)), Moliere
+ * cutoffs (si.f90:48-62) and the SW pair table (si.f90:575-597). */
+int32_t fib_create(const fib_params *p, int32_t ninst,
+                   const fib_instance_desc *inst, int32_t precision,
+                   fib_handle **out);
+int32_t fib_destroy(fib_handle *h);
+const char *fib_last_error(const fib_handle *h);
+int32_t fib_get_cells(const fib_handle *h, int32_t inst, int32_t nc[3]);
+
+/* Upload state (reference ic / data.f90:48-112 results).  X,V (natm,3) m and
+ * m/s; atype (natm) 1=Si 2=Ge 3=Ga; xsides (natm, nullable) >0 marks side
+ * atoms pinned to Xi every step (tools.f90:38-51); Xi (natm,3, nullable)
+ * reference positions (defaults to X).  Forces are reset to zero. */
+int32_t fib_set_state(fib_handle *h, int32_t inst, const double *X,
+                      const double *V, const int32_t *atype,
+                      const int32_t *xsides, const double *Xi);
+/* Download state in the caller's original atom order (any pointer may be
+ * null).  F is the force of the last completed evaluation. */
+int32_t fib_get_state(fib_handle *h, int32_t inst, double *X, double *V,
+                      double *F);
+/* Number of fired ions; atoms with index > nsg+impact are skipped in the
+ * force (reference si.f90:415,428,457). */
+int32_t fib_set_impact(fib_handle *h, int32_t inst, int32_t impact);
+
+/* Cell assignment of every atom with the reference's arithmetic
+ * (replaces atomcell, nlist.f90:20-32 as called by chainlist :44-45):
+ *   Icell = INT((X - FLOOR(X*iLb)*Lb)/(Lb/N)) + 1,  1-based, (natm,3). */
+int32_t fib_bin_cells(fib_handle *h, int32_t inst, int32_t *icell);
+
+/* Rebuild the neighbor lists of every instance from current positions
+ * (replaces si_nlist, si.f90:87-276): counting-sort binning, spatial
+ * reordering, LDS-staged 27-cell scan.  SW pairs need both atoms non-Ga and
+ * r^2 < rc^2; ion pairs need a Ga atom and r^2 < rclj^2 (si.f90:156-170). */
+int32_t fib_build_nlist(fib_handle *h);
+
+/* Export the half lists in the reference's CSR layout (si.f90:234-260):
+ * nss (natm,2) 1-based [first,last] into nlist2 (a first > last range is
+ * empty); nlist2 holds 1-based j > i.  Same for the ion list.  Order inside
+ * one atom's range is unspecified (sets are what the reference defines).
+ * Call with nlist2 == NULL first to obtain the lengths. */
+int32_t fib_get_nlist(fib_handle *h, int32_t inst, int32_t *nss,
+                      int32_t *nlist2, int32_t *len2, int32_t *nsslj,
+                      int32_t *nlistlj, int32_t *lenlj);
+
+/* Forces on all instances at the current positions with the current lists
+ * (replaces si_force, si.f90:384-552; energies replace si_potential
+ * si.f90:278-368, restricted to the pairs and triplets the force includes;
+ * the virial W[a][b] = sum r_a f_b has no reference counterpart). */
+enum { FIB_EVAL_FORCE = 1, FIB_EVAL_ENERGY = 2, FIB_EVAL_VIRIAL = 4 };
+int32_t fib_compute(fib_handle *h, int32_t flags);
+/* Results of the last fib_compute for one instance (pointers nullable). */
+int32_t fib_get_forces(fib_handle *h, int32_t inst, double *F, double *u,
+                       double *utot, double virial[9]);
+/* Convenience: fib_set_impact + fib_compute + fib_get_forces. */
+int32_t fib_force(fib_handle *h, int32_t inst, int32_t impact, double *F,
+                  double *u, double *utot, double virial[9]);
+
+/* ---- time integration: the reference's onestep/TI loop ----------------
+ * Step control mirrors prms/timeint module state: Ts and the cadences switch
+ * from the *_i to the *_r set when the largest kinetic energy drops below
+ * ke_limit_1 (calc_atm, timeint.f90:277-298, ionrun prms.f90:563-584). */
+typedef struct fib_run_params {
+  double ts_i, ts_r;                 /* timesteps (s)                       */
+  double tau_i, tau_r;               /* Berendsen time constants (s)        */
+  double ttar;                       /* target temperature (K)              */
+  double teps;                       /* equilibration tolerance (K)         */
+  double ke_limit_1;                 /* eV                                  */
+  double dti;                        /* time between impacts (s)            */
+  double sidewidth;                  /* thermostat shell width (nm)         */
+  double outz[4], atomz[2], ionz;    /* storage windows, units of Lb(3)/10  */
+  int32_t ntlist_i, ntlist_r, atlist_i, atlist_r, dslist_i, dslist_r;
+  int32_t atm_out_i, atm_out_r, res_out;
+  int32_t reserved;
+} fib_run_params;
+
+int32_t fib_set_run(fib_handle *h, int32_t inst, const fib_run_params *rp);
+
+/* Phases of the reference's TI (timeint.f90:123-155). */
+enum { FIB_TI_EQUILIBRATE = 0, FIB_TI_IMPACT = 1, FIB_TI_FREE = 2 };
+
+typedef struct fib_step_ctl {
+  int32_t whichctrl;      /* FIB_TI_*                                     */
+  int32_t max_steps;      /* return after at most this many onestep calls  */
+  int32_t stop_on_output; /* return after a step on the atm_out/res_out cadence */
+  int32_t reserved;
+} fib_step_ctl;
+
+typedef struct fib_step_report {
+  int64_t lt;             /* reference step counter                        */
+  double time;            /* simulated time (s)                            */
+  double temp;            /* the reference's `temp` variable (K)           */
+  double ts;              /* current timestep (s)                          */
+  int32_t steps_done;     /* onestep calls in this fib_step                */
+  int32_t ended;          /* TI loop condition reached                     */
+  int32_t output_due;     /* last step is on the atm_out/res_out cadence   */
+  int32_t regime;         /* 1 = fast (Ts_i), 2 = slow (Ts_r)              */
+  int32_t nlist_builds;   /* rebuilds performed in this call               */
+  int32_t reserved;
+} fib_step_report;
+
+/* Begin the reference's run on instance 0: lt = lt0, time = time0, initial
+ * temperature and the initial force (tint, timeint.f90:47-69). */
+int32_t fib_tint_init(fib_handle *h, int64_t lt0, double time0);
+/* Start a TI phase: ionrun(1) for impacts, Tau = 1e-14 for equilibration
+ * (timeint.f90:85,101-103). `impact` = number of the ion being fired. */
+int32_t fib_ti_begin(fib_handle *h, int32_t whichctrl, int32_t impact);
+/* Place and launch ion `knc` (knock.f90:21-55 with the ranindex already
+ * drawn by the host): X, V of atom nsg+knc. */
+int32_t fib_knock(fib_handle *h, int32_t knc, const double xion[3],
+                  const double vion[3]);
+/* Run TI loop iterations (each: lt += 1, end test, onestep). */
+int32_t fib_step(fib_handle *h, const fib_step_ctl *ctl, fib_step_report *rep);
+/* Recompute the reference's temperature (temp.f90:183-205). */
+int32_t fib_temperature(fib_handle *h, int32_t inst, double *temp);
+
+/* Device-side phase timers (HIP events on the handle's stream), accumulated
+ * since fib_timing_enable(h, 1); for benchmarks.  bin_ms is unused (binning is
+ * counted in nlist_ms). */
+typedef struct fib_timing {
+  double bin_ms, nlist_ms, force_ms, integrate_ms, total_ms;
+  int32_t n_force, n_nlist;
+} fib_timing;
+int32_t fib_timing_enable(fib_handle *h, int32_t on);
+int32_t fib_get_timing(fib_handle *h, fib_timing *t);
+int32_t fib_synchronize(fib_handle *h);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* FIBGPU_H */
