@@ -1,0 +1,196 @@
+#!/usr/bin/env python3
+"""Throughput of the FIB_MD force path on one MI355X.
+
+Workload (the force microbenchmark): a 64x64x64-cell periodic Si crystal
+(2,097,152 sites), every atom displaced by N(0, 0.1 A), a central 20 A sphere
+randomised at 2.28 g/cc.  One "step" is one neighbour-list rebuild plus one
+SW force + energy + virial evaluation of every atom.  Reported value: atom
+evaluations per second of the mixed-precision path (FP32 terms, compensated
+FP32 force sums, FP64 positions and energies); the plain FP64 path and the
+force error of mixed vs FP64 are reported beside it.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
+       python bench.py --profile fp64|mixed   (few steps + a 256 MiB copy, for rocprofv3)
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+# Algorithmic bytes per atom of one evaluation (build 88 B: read X 24 B, write
+# 16 neighbour indices 64 B; force + energy 124 B: read X 24 B, type 4 B,
+# list 64 B, write F 24 B, u 8 B).
+BYTES_BUILD = 88.0
+BYTES_FORCE = 124.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def make_workload(ncell: int, seed: int = 1002):
+    from fib_md_amd import configs
+    X, at, L = configs.force_microbench(ncell, seed=seed)
+    return np.asfortranarray(X), at, L
+
+
+def run_gpu(X, at, L, precision, steps, warmup, timing_steps=3):
+    import torch
+    from fib_md_amd import FibGPU
+    n = len(X)
+    g = FibGPU([(n, n, 0, L)], precision=precision)
+    g.set_state(X, atype=at)
+    for _ in range(warmup):
+        g.build_nlist()
+        g.compute(energy=True, virial=True)
+    g.synchronize()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        g.build_nlist()
+        g.compute(energy=True, virial=True)
+    g.synchronize()
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    # per-kernel device time with HIP events on the library's stream
+    g.timing(enable=True)
+    for _ in range(timing_steps):
+        g.build_nlist()
+        g.compute(energy=True, virial=True)
+    g.synchronize()
+    tm = g.timing()
+    res = g.forces(energy=False, virial=True)
+    g.timing(enable=False)
+    out = dict(seconds=t1 - t0, ms_per_step=(t1 - t0) * 1e3 / steps,
+               rate=n * steps / (t1 - t0), timing=tm, F=res["F"], utot=res["utot"],
+               virial=res["virial"])
+    g.close()
+    return out
+
+
+def rel_err(Fg, Fo, floor_frac=1e-3):
+    d = np.sqrt(((Fg - Fo) ** 2).sum(1))
+    m = np.sqrt((Fo ** 2).sum(1))
+    frms = np.sqrt((m ** 2).mean())
+    e = d / np.maximum(m, floor_frac * frms)
+    return float(e.max()), float(np.sqrt((e ** 2).mean())), float(np.linalg.norm(Fg - Fo) / np.linalg.norm(Fo))
+
+
+def cpu_baseline(ncell=24, reps=5):
+    """The oracle (scalar C restatement of si_nlist/si_force/si_potential) on a
+    bounded sample of the same workload, timed on one host core."""
+    from oracle.oracle import Oracle, build
+    build()
+    X, at, L = make_workload(ncell, seed=1002)
+    n = len(X)
+    o = Oracle()
+    o.set_geometry(n, n, 0, L)
+    o.set_state(X=X, V=np.zeros((n, 3)), atype=at, mass=np.full(n, 46.637063e-27))
+    o.init_nlist()
+    o.mktables()
+    ts = []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        o.si_nlist(X)
+        o.si_force(X)
+        o.si_potential(X)
+        ts.append(time.perf_counter() - t0)
+    t = float(np.median(ts))
+    return dict(value=n / t, unit="atom-evals/s", cores=1, kind="port",
+                sample=f"{ncell}^3-cell crystal ({n} atoms, same noise and 20 A amorphous "
+                       f"sphere), si_nlist+si_force+si_potential, median of {reps}, "
+                       f"{t:.2f} s per evaluation")
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--ncell", type=int, default=64)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--profile", choices=["fp64", "mixed"], default=None)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); "
+              "multi-GPU sharding is a design in DESIGN.md, not a measured path.", flush=True)
+        sys.exit(2)
+    import torch
+    torch.cuda.init()
+    from fib_md_amd import _lib
+    _lib.lib()
+    t0 = time.perf_counter()
+    X, at, L = make_workload(args.ncell)
+    n = len(X)
+    log(f"workload: {n} atoms, box {L[0]*1e10:.2f} A, generated in {time.perf_counter()-t0:.1f} s")
+    if args.profile:
+        # a plain copy of known size calibrates the FETCH/WRITE counters
+        a = torch.empty(256 * 1024 * 1024 // 4, dtype=torch.float32, device="cuda").fill_(1.0)
+        b = torch.empty_like(a)
+        b.copy_(a)
+        torch.cuda.synchronize()
+        r = run_gpu(X, at, L, args.profile, 2, 1, timing_steps=1)
+        log(json.dumps({k: v for k, v in r.items() if k in ("ms_per_step", "timing")}))
+        return
+    res = {}
+    for prec in ("fp64", "mixed"):
+        log(f"running {prec} ...")
+        res[prec] = run_gpu(X, at, L, prec, args.steps, args.warmup)
+        log(f"  {prec}: {res[prec]['ms_per_step']:.3f} ms/step, {res[prec]['rate']:.4g} atom-evals/s, "
+            f"kernels {res[prec]['timing']}")
+    emax, erms, enorm = rel_err(res["mixed"]["F"], res["fp64"]["F"])
+    m = res["mixed"]
+    tm = m["timing"]
+    kern = {
+        "k_force_a+k_force_b": ((tm["force_a_ms"] + tm["force_b_ms"]) / max(1, tm["n_force_a"]), BYTES_FORCE),
+        "k_nbuild+k_rev": (tm["nbuild_ms"] / max(1, tm["n_nbuild"]), BYTES_BUILD),
+    }
+    dom = max(kern, key=lambda k: kern[k][0])
+    dur_ms, bpa = kern[dom]
+    achieved = bpa * n / (dur_ms * 1e-3) / 1e9
+    cpu = None
+    if not args.no_cpu_baseline:
+        log("cpu baseline ...")
+        cpu = cpu_baseline()
+    line = {
+        "metric": "atom-steps/s on 1 MI355X, mixed vs FP64 (SW+Moliere); Ga impacts/hour",
+        "value": m["rate"],
+        "unit": "atom-evals/s (one list rebuild + SW force/energy/virial per atom)",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": m["ms_per_step"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 terms + f64 positions/energies (mixed)",
+        "data": "synthetic",
+        "config": {"workload": "force microbench: 64^3-cell Si crystal, N(0,0.1 A) noise, "
+                               "20 A sphere at 2.28 g/cc; list rebuild + force/energy/virial",
+                   "atoms": n, "box_A": L[0] * 1e10, "parallelism": "single GPU"},
+        "fp64": {"value": res["fp64"]["rate"], "ms_per_step": res["fp64"]["ms_per_step"],
+                 "kernels_ms": res["fp64"]["timing"]},
+        "mixed_over_fp64": m["rate"] / res["fp64"]["rate"],
+        "force_error_mixed_vs_fp64": {"max_rel": emax, "rms_rel": erms, "norm_rel": enorm,
+                                      "floor": "1e-3 F_rms"},
+        "energy_rel_diff": abs(m["utot"] - res["fp64"]["utot"]) / abs(res["fp64"]["utot"]),
+        "kernels_ms": tm,
+        "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "bytes_per_atom": bpa, "avg_launch_ms": dur_ms},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -68,9 +68,10 @@ class fib_step_report(C.Structure):
 
 
 class fib_timing(C.Structure):
-    _fields_ = [("bin_ms", C.c_double), ("nlist_ms", C.c_double), ("force_ms", C.c_double),
-                ("integrate_ms", C.c_double), ("total_ms", C.c_double),
-                ("n_force", C.c_int32), ("n_nlist", C.c_int32)]
+    _fields_ = [("sort_ms", C.c_double), ("nbuild_ms", C.c_double), ("force_a_ms", C.c_double),
+                ("force_b_ms", C.c_double), ("integrate_ms", C.c_double), ("total_ms", C.c_double),
+                ("n_sort", C.c_int32), ("n_nbuild", C.c_int32), ("n_force_a", C.c_int32),
+                ("n_force_b", C.c_int32), ("n_integrate", C.c_int32), ("reserved", C.c_int32)]
 
 
 vp, i32, i64, f64 = C.c_void_p, C.c_int32, C.c_int64, C.c_double

@@ -10,6 +10,7 @@
 // (orig << 32) | meta, so one 32-byte gather yields position, species, flags
 // and original index of a neighbour.
 #pragma once
+#include <stddef.h>
 #include <stdint.h>
 
 #define FIB_BLOCK 256
@@ -62,7 +63,8 @@ struct TabEntry {
   double u0, u1, u2, t0, t1, t2;
 };
 
-// Per-instance reduction outputs / control scalars (device).
+// Per-instance reduction outputs / control scalars (device).  All reduced
+// fields are doubles so the finalize kernel can address them by index.
 struct DCtl {
   double utot;          // potential energy of the last evaluation
   double virial[9];
@@ -73,8 +75,24 @@ struct DCtl {
   double kemax;         // max 0.5 m v^2 / ec after the position update (eV)
   double vmax;          // max |v| (m/s) after the V update
   double ke_sum;        // sum 0.5 m v^2 (J) after the V update
-  int32_t maxcnt_sw, maxcnt_ion, maxcnt_ionbig, nonfinite;
+  double maxcnt_sw;     // longest SW / ion neighbour list of the last build
+  double maxcnt_ion;
+  double eq_sum;        // sum m v^2 of the rescaled V (equilibration)
+  int32_t nonfinite;    // set by k_bin on a NaN/Inf position
+  int32_t pad;
 };
+
+// Double index of a DCtl field (for the reduction finalize kernel).
+#define DCTL_IDX(field) ((int)(offsetof(DCtl, field) / sizeof(double)))
 
 // Number of doubles in a reduction partial record.
 #define FIB_NRED 16
+
+// Reduction finalize description: slot k of the producer's partials is
+// combined with op[k] (0 sum, 1 max) and stored at DCtl double index dst[k].
+struct RedSpec {
+  int32_t nv;
+  int32_t bs;            // block size of the producing kernel
+  int32_t op[FIB_NRED];
+  int32_t dst[FIB_NRED];
+};

@@ -37,8 +37,8 @@ struct DevBufs {
   uint64_t *amask;
   double *Fpart, *upart;     // [3*NA], [NA]
   double *Fout, *uout;       // force-only results [3*NA], [NA]
-  double *partials;          // [NA/64 * FIB_NRED]
-  unsigned *counters;        // [ninst]
+  double *partials;          // [FIB_NRED * pstride], layout [slot][block]
+  int pstride;               // NA/64 + 1
   TabEntry *tab;             // [ntable + 3]
   double *xfer;              // staging [3*NA] for host transfers
 };
@@ -49,8 +49,12 @@ void launch_scan(const DevBufs &d, hipStream_t s);
 void launch_sort(const DevBufs &d, hipStream_t s);  // scatter + fixup + reorder (flips cur)
 void launch_nbuild(const DevBufs &d, const FConst &fc, hipStream_t s);
 void launch_cellid(const DevBufs &d, int inst, int32_t *dout, hipStream_t s);
-void launch_force(const DevBufs &d, const FConst &fc, bool mixed, bool energy, bool virial,
-                  int mode, const StepArgs &sa, hipStream_t s);
+void launch_force_a(const DevBufs &d, const FConst &fc, bool mixed, bool energy, bool virial,
+                    hipStream_t s);
+// mode 0: store forces into Fout; mode 1: velocity update with Fout the
+// persistent force array F (timeint.f90:231-236)
+void launch_force_b(const DevBufs &d, const FConst &fc, bool mixed, bool energy, int mode,
+                    const StepArgs &sa, double *Fout, hipStream_t s);
 void launch_pos(const DevBufs &d, const StepArgs &sa, hipStream_t s);
 void launch_sums(const DevBufs &d, const StepArgs &sa, hipStream_t s);
 void launch_import(const DevBufs &d, int inst, const DInst &in, const double *X,

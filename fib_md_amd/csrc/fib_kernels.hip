@@ -108,15 +108,13 @@ __device__ __forceinline__ void kadd(float &s, float &c, float v) {
 }
 
 // ---------------------------------------------------------------------------
-// Block reduction of NV doubles (sum or max per slot) with a deterministic
-// last-block finalisation per instance.  op[k] = 0 sum, 1 max.  Returns true
-// in the block that finalised; there thread k < NV holds slot k in `res`.
+// Block reduction of NV doubles (sum or max per slot) into per-block
+// partials, layout partials[k * pstride + block].  k_finalize combines them
+// per instance in a fixed order, so results are deterministic.
 template <int NV, int BS>
-__device__ bool block_reduce_finalize(double (&v)[NV], const int (&op)[NV], int lblock, int blk0,
-                                      int nblk, int inst, double *partials, unsigned *counters,
-                                      double &res) {
+__device__ void block_partials(double (&v)[NV], const int (&op)[NV], int lblock, int pstride,
+                               double *__restrict__ partials) {
   __shared__ double red[BS / FIB_WAVE][NV];
-  __shared__ int s_last;
   const int lane = threadIdx.x % FIB_WAVE, wid = threadIdx.x / FIB_WAVE;
 #pragma unroll
   for (int k = 0; k < NV; ++k) {
@@ -132,33 +130,43 @@ __device__ bool block_reduce_finalize(double (&v)[NV], const int (&op)[NV], int 
     for (int k = 0; k < NV; ++k) red[wid][k] = v[k];
   }
   __syncthreads();
-  if (threadIdx.x == 0) {
-    for (int k = 0; k < NV; ++k) {
-      double x = red[0][k];
-      for (int w = 1; w < BS / FIB_WAVE; ++w) x = op[k] ? fmax(x, red[w][k]) : x + red[w][k];
-      partials[(size_t)lblock * FIB_NRED + k] = x;
-    }
-    __threadfence();
-    unsigned prev = atomicAdd(&counters[inst], 1u);
-    s_last = (prev == (unsigned)(nblk - 1));
+  if (threadIdx.x < NV) {
+    const int k = threadIdx.x;
+    double x = red[0][k];
+    for (int w = 1; w < BS / FIB_WAVE; ++w) x = op[k] ? fmax(x, red[w][k]) : x + red[w][k];
+    partials[(size_t)k * pstride + lblock] = x;
   }
-  __syncthreads();
-  const bool last = s_last != 0;
-  if (last) {
-    __threadfence();
-    if (threadIdx.x < NV) {
-      const int k = threadIdx.x;
-      double x = 0.0;
-      for (int b = 0; b < nblk; ++b) {
-        double y = __hip_atomic_load(&partials[(size_t)(blk0 + b) * FIB_NRED + k],
-                                     __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        x = (b == 0) ? y : (op[k] ? fmax(x, y) : x + y);
-      }
-      res = x;
+}
+
+#define FIN_T 256
+// One block per instance: combine the instance's partials in a fixed order.
+__global__ __launch_bounds__(FIN_T) void k_finalize(const DInst *__restrict__ I,
+                                                    const double *__restrict__ partials,
+                                                    int pstride, RedSpec rs,
+                                                    DCtl *__restrict__ ctl) {
+  __shared__ double red[FIN_T / FIB_WAVE];
+  const int inst = blockIdx.x;
+  const int blk0 = I[inst].atom_off / rs.bs, nblk = I[inst].natm_pad / rs.bs;
+  const int lane = threadIdx.x % FIB_WAVE, wid = threadIdx.x / FIB_WAVE;
+  double *out = reinterpret_cast<double *>(&ctl[inst]);
+  for (int k = 0; k < rs.nv; ++k) {
+    const bool mx = rs.op[k] != 0;
+    const double *p = partials + (size_t)k * pstride + blk0;
+    double x = mx ? -1.0e300 : 0.0;
+    for (int b = threadIdx.x; b < nblk; b += FIN_T) x = mx ? fmax(x, p[b]) : x + p[b];
+    for (int o = FIB_WAVE / 2; o > 0; o >>= 1) {
+      double y = __shfl_xor(x, o);
+      x = mx ? fmax(x, y) : x + y;
     }
-    if (threadIdx.x == 0) atomicExch(&counters[inst], 0u);
+    if (lane == 0) red[wid] = x;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      double t = red[0];
+      for (int w = 1; w < FIN_T / FIB_WAVE; ++w) t = mx ? fmax(t, red[w]) : t + red[w];
+      if (rs.dst[k] >= 0) out[rs.dst[k]] = t;
+    }
+    __syncthreads();
   }
-  return last;
 }
 
 // ---------------------------------------------------------------------------
@@ -439,8 +447,6 @@ __global__ __launch_bounds__(FIB_WAVE) void k_nbuild(
     if (lane == 0) {
       cnt_sw[i] = nsw;
       cnt_ion[i] = nion;
-      atomicMax(&ctl[k].maxcnt_sw, nsw);
-      atomicMax(&ctl[k].maxcnt_ion, nion);
     }
   }
 }
@@ -454,11 +460,15 @@ __global__ void k_clear_counts(int NA, int *__restrict__ cnt_sw, int *__restrict
   cnt_ion[p] = 0;
 }
 
-__global__ void k_rev(int NA, int Wsw, const int *__restrict__ cnt_sw,
-                      const int *__restrict__ nbr_sw, uint8_t *__restrict__ rev) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= NA) return;
-  int c = min(cnt_sw[i], Wsw);
+// Reverse slots plus the longest lists of the build (overflow check).
+__global__ __launch_bounds__(FIB_BLOCK) void k_rev(int NA, int Wsw, const int *__restrict__ cnt_sw,
+                                                   const int *__restrict__ cnt_ion,
+                                                   const int *__restrict__ nbr_sw,
+                                                   uint8_t *__restrict__ rev, int pstride,
+                                                   double *__restrict__ partials) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int cs = cnt_sw[i];
+  const int c = min(cs, Wsw);
   for (int k = 0; k < c; ++k) {
     int j = nbr_sw[(size_t)k * NA + i];
     int cj = min(cnt_sw[j], Wsw);
@@ -467,6 +477,9 @@ __global__ void k_rev(int NA, int Wsw, const int *__restrict__ cnt_sw,
       if (nbr_sw[(size_t)q * NA + j] == i) { r = q; break; }
     rev[(size_t)k * NA + i] = (uint8_t)r;
   }
+  double v[2] = {(double)cs, (double)cnt_ion[i]};
+  const int op[2] = {1, 1};
+  block_partials<2, FIB_BLOCK>(v, op, blockIdx.x, pstride, partials);
 }
 
 // ---------------------------------------------------------------------------
@@ -518,7 +531,7 @@ __global__ __launch_bounds__(FIB_WAVE) void k_force_a(
     const int *__restrict__ nbr_sw, const int *__restrict__ cnt_ion,
     const int *__restrict__ nbr_ion, typename Prec<MIXED>::T4 *__restrict__ G,
     uint64_t *__restrict__ amask, double *__restrict__ Fpart, double *__restrict__ upart,
-    double *__restrict__ partials, unsigned *__restrict__ counters, DCtl *__restrict__ ctl) {
+    int pstride, double *__restrict__ partials, DCtl *__restrict__ ctl) {
   typedef typename Prec<MIXED>::T T;
   typedef typename Prec<MIXED>::T4 T4;
   constexpr int MAXS = Prec<MIXED>::MAXS;
@@ -831,13 +844,7 @@ __global__ __launch_bounds__(FIB_WAVE) void k_force_a(
 #pragma unroll
     for (int q = 0; q < 9; ++q) v[1 + q] = W[q];
     const int op[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
-    const int blk0 = in.atom_off / FIB_WAVE, nblk = in.natm_pad / FIB_WAVE;
-    double res = 0;
-    if (block_reduce_finalize<10, FIB_WAVE>(v, op, lb, blk0, nblk, kin, partials, counters, res) &&
-        threadIdx.x < 10) {
-      if (threadIdx.x == 0) ctl[kin].utot = res;
-      else ctl[kin].virial[threadIdx.x - 1] = res;
-    }
+    block_partials<10, FIB_WAVE>(v, op, lb, pstride, partials);
   }
 }
 
@@ -851,8 +858,7 @@ __global__ __launch_bounds__(FIB_BLOCK) void k_force_b(
     const typename Prec<MIXED>::T4 *__restrict__ G, const uint64_t *__restrict__ amask,
     const double *__restrict__ Fpart, const double *__restrict__ upart, double fscale,
     double escale, double *__restrict__ Fout, double *__restrict__ uout, double *__restrict__ V,
-    StepArgs sa, double *__restrict__ partials, unsigned *__restrict__ counters,
-    DCtl *__restrict__ ctl) {
+    StepArgs sa, int pstride, double *__restrict__ partials, DCtl *__restrict__ ctl) {
   typedef typename Prec<MIXED>::T4 T4;
   const int nb = gridDim.x;
   const int lb = xcd_remap(blockIdx.x, nb);
@@ -916,16 +922,7 @@ __global__ __launch_bounds__(FIB_BLOCK) void k_force_b(
     v[4] = sqrt(vv[0] * vv[0] + vv[1] * vv[1] + vv[2] * vv[2]);
   }
   const int op[5] = {0, 0, 0, 0, 1};
-  const int blk0 = in.atom_off / FIB_BLOCK, nblk = in.natm_pad / FIB_BLOCK;
-  double res = 0;
-  if (block_reduce_finalize<5, FIB_BLOCK>(v, op, lb, blk0, nblk, kin, partials, counters, res)) {
-    DCtl &c = ctl[kin];
-    if (threadIdx.x == 0) c.glob_sum = res;
-    if (threadIdx.x == 1) c.side_sum = res;
-    if (threadIdx.x == 2) c.side_cnt = res;
-    if (threadIdx.x == 3) c.ke_sum = res;
-    if (threadIdx.x == 4) c.vmax = res;
-  }
+  block_partials<5, FIB_BLOCK>(v, op, lb, pstride, partials);
 }
 
 
@@ -939,8 +936,7 @@ __global__ __launch_bounds__(FIB_BLOCK) void k_pos(double4 *__restrict__ X,
                                                    const DInst *__restrict__ I, int NA,
                                                    double *__restrict__ V, double *__restrict__ F,
                                                    const double *__restrict__ Xi, StepArgs sa,
-                                                   double *__restrict__ partials,
-                                                   unsigned *__restrict__ counters,
+                                                   int pstride, double *__restrict__ partials,
                                                    DCtl *__restrict__ ctl) {
 #pragma clang fp contract(off)
   const int nb = gridDim.x;
@@ -1017,12 +1013,7 @@ __global__ __launch_bounds__(FIB_BLOCK) void k_pos(double4 *__restrict__ X,
     }
   }
   const int op[2] = {0, 1};
-  const int blk0 = in.atom_off / FIB_BLOCK, nblk = in.natm_pad / FIB_BLOCK;
-  double res = 0;
-  if (block_reduce_finalize<2, FIB_BLOCK>(red, op, lb, blk0, nblk, kin, partials, counters, res)) {
-    if (threadIdx.x == 0 && sa.mode == 0) ctl[kin].temp_eq = res / 3. / (double)in.nsg / sa.kB;
-    if (threadIdx.x == 1) ctl[kin].kemax = res;
-  }
+  block_partials<2, FIB_BLOCK>(red, op, lb, pstride, partials);
 }
 
 // Thermostat / temperature sums of the current state (same as k_force_b's
@@ -1030,8 +1021,7 @@ __global__ __launch_bounds__(FIB_BLOCK) void k_pos(double4 *__restrict__ X,
 __global__ __launch_bounds__(FIB_BLOCK) void k_sums(const double4 *__restrict__ X,
                                                     const DInst *__restrict__ I, int NA,
                                                     const double *__restrict__ V, StepArgs sa,
-                                                    double *__restrict__ partials,
-                                                    unsigned *__restrict__ counters,
+                                                    int pstride, double *__restrict__ partials,
                                                     DCtl *__restrict__ ctl) {
   const int nb = gridDim.x;
   const int lb = xcd_remap(blockIdx.x, nb);
@@ -1060,16 +1050,7 @@ __global__ __launch_bounds__(FIB_BLOCK) void k_sums(const double4 *__restrict__ 
     v[4] = sqrt(vv);
   }
   const int op[5] = {0, 0, 0, 0, 1};
-  const int blk0 = in.atom_off / FIB_BLOCK, nblk = in.natm_pad / FIB_BLOCK;
-  double res = 0;
-  if (block_reduce_finalize<5, FIB_BLOCK>(v, op, lb, blk0, nblk, kin, partials, counters, res)) {
-    DCtl &c = ctl[kin];
-    if (threadIdx.x == 0) c.glob_sum = res;
-    if (threadIdx.x == 1) c.side_sum = res;
-    if (threadIdx.x == 2) c.side_cnt = res;
-    if (threadIdx.x == 3) c.ke_sum = res;
-    if (threadIdx.x == 4) c.vmax = res;
-  }
+  block_partials<5, FIB_BLOCK>(v, op, lb, pstride, partials);
 }
 
 // ---------------------------------------------------------------------------
@@ -1156,6 +1137,26 @@ __global__ void k_set_atom(int NA, double4 *__restrict__ X, double *__restrict__
 // launchers
 static inline int nblk(long n, int b) { return (int)((n + b - 1) / b); }
 
+static void finalize(const DevBufs &d, int nv, int bs, const int *op, const int *dst,
+                     hipStream_t s) {
+  RedSpec rs;
+  rs.nv = nv;
+  rs.bs = bs;
+  for (int k = 0; k < FIB_NRED; ++k) {
+    rs.op[k] = k < nv ? op[k] : 0;
+    rs.dst[k] = k < nv ? dst[k] : -1;
+  }
+  hipLaunchKernelGGL(k_finalize, dim3(d.ninst), dim3(FIN_T), 0, s, d.I, d.partials, d.pstride,
+                     rs, d.ctl);
+}
+
+static void finalize_thermo(const DevBufs &d, hipStream_t s) {
+  const int op[5] = {0, 0, 0, 0, 1};
+  const int dst[5] = {DCTL_IDX(glob_sum), DCTL_IDX(side_sum), DCTL_IDX(side_cnt),
+                      DCTL_IDX(ke_sum), DCTL_IDX(vmax)};
+  finalize(d, 5, FIB_BLOCK, op, dst, s);
+}
+
 void launch_bin(const DevBufs &d, hipStream_t s) {
   (void)hipMemsetAsync(d.bcnt, 0, sizeof(int) * (size_t)d.nbucket, s);
   hipLaunchKernelGGL(k_bin, dim3(nblk(d.NA, FIB_BLOCK)), dim3(FIB_BLOCK), 0, s, d.X[d.cur], d.NA,
@@ -1188,8 +1189,11 @@ void launch_nbuild(const DevBufs &d, const FConst &fc, hipStream_t s) {
     hipLaunchKernelGGL(k_nbuild, dim3(d.total_cells), dim3(FIB_WAVE), 0, s, d.X[d.cur], d.I,
                        d.ninst, d.total_cells, d.cstart, d.bcnt, fc.rv2, fc.rv2lj, d.NA, d.Wsw,
                        d.Wion, d.cnt_sw, d.nbr_sw, d.cnt_ion, d.nbr_ion, d.ctl);
-  hipLaunchKernelGGL(k_rev, dim3(nblk(d.NA, FIB_BLOCK)), dim3(FIB_BLOCK), 0, s, d.NA, d.Wsw,
-                     d.cnt_sw, d.nbr_sw, d.rev);
+  hipLaunchKernelGGL(k_rev, dim3(d.NA / FIB_BLOCK), dim3(FIB_BLOCK), 0, s, d.NA, d.Wsw,
+                     d.cnt_sw, d.cnt_ion, d.nbr_sw, d.rev, d.pstride, d.partials);
+  const int op[2] = {1, 1};
+  const int dst[2] = {DCTL_IDX(maxcnt_sw), DCTL_IDX(maxcnt_ion)};
+  finalize(d, 2, FIB_BLOCK, op, dst, s);
 }
 
 void launch_cellid(const DevBufs &d, int inst, int32_t *dout, hipStream_t s) {
@@ -1202,45 +1206,62 @@ static void force_a(const DevBufs &d, const FConst &fc, hipStream_t s) {
   typedef typename Prec<MIXED>::T4 T4;
   hipLaunchKernelGGL((k_force_a<MIXED, EN, VIR>), dim3(d.NA / FIB_WAVE), dim3(FIB_WAVE), 0, s,
                      d.X[d.cur], d.I, fc, d.tab, d.NA, d.Wsw, d.Wion, d.cnt_sw, d.nbr_sw,
-                     d.cnt_ion, d.nbr_ion, (T4 *)d.G, d.amask, d.Fpart, d.upart, d.partials,
-                     d.counters, d.ctl);
+                     d.cnt_ion, d.nbr_ion, (T4 *)d.G, d.amask, d.Fpart, d.upart, d.pstride,
+                     d.partials, d.ctl);
+  if (EN || VIR) {
+    int op[10], dst[10];
+    for (int k = 0; k < 10; ++k) { op[k] = 0; dst[k] = DCTL_IDX(utot) + k; }
+    finalize(d, 10, FIB_WAVE, op, dst, s);
+  }
 }
 
 template <bool MIXED, bool EN, int MODE>
-static void force_b(const DevBufs &d, const FConst &fc, const StepArgs &sa, hipStream_t s) {
+static void force_b(const DevBufs &d, const FConst &fc, const StepArgs &sa, double *Fo,
+                    hipStream_t s) {
   typedef typename Prec<MIXED>::T4 T4;
   const double fscale = MIXED ? fc.eps[0][0] * fc.isig[0][0] : 1.0;
   const double escale = MIXED ? fc.eps[0][0] : 1.0;
-  double *Fo = MODE == 0 ? d.Fout : d.F[d.cur];
   hipLaunchKernelGGL((k_force_b<MIXED, EN, MODE>), dim3(d.NA / FIB_BLOCK), dim3(FIB_BLOCK), 0, s,
                      d.X[d.cur], d.I, d.NA, d.Wsw, d.nbr_sw, d.rev, (const T4 *)d.G, d.amask,
-                     d.Fpart, d.upart, fscale, escale, Fo, d.uout, d.V[d.cur], sa, d.partials,
-                     d.counters, d.ctl);
+                     d.Fpart, d.upart, fscale, escale, Fo, d.uout, d.V[d.cur], sa, d.pstride,
+                     d.partials, d.ctl);
+  if (MODE == 1) finalize_thermo(d, s);
 }
 
-void launch_force(const DevBufs &d, const FConst &fc, bool mixed, bool energy, bool virial,
-                  int mode, const StepArgs &sa, hipStream_t s) {
+void launch_force_a(const DevBufs &d, const FConst &fc, bool mixed, bool energy, bool virial,
+                    hipStream_t s) {
   if (mixed) {
     if (energy) { if (virial) force_a<true, true, true>(d, fc, s); else force_a<true, true, false>(d, fc, s); }
     else { if (virial) force_a<true, false, true>(d, fc, s); else force_a<true, false, false>(d, fc, s); }
-    if (mode == 0) { if (energy) force_b<true, true, 0>(d, fc, sa, s); else force_b<true, false, 0>(d, fc, sa, s); }
-    else force_b<true, false, 1>(d, fc, sa, s);
   } else {
     if (energy) { if (virial) force_a<false, true, true>(d, fc, s); else force_a<false, true, false>(d, fc, s); }
     else { if (virial) force_a<false, false, true>(d, fc, s); else force_a<false, false, false>(d, fc, s); }
-    if (mode == 0) { if (energy) force_b<false, true, 0>(d, fc, sa, s); else force_b<false, false, 0>(d, fc, sa, s); }
-    else force_b<false, false, 1>(d, fc, sa, s);
+  }
+}
+
+void launch_force_b(const DevBufs &d, const FConst &fc, bool mixed, bool energy, int mode,
+                    const StepArgs &sa, double *Fout, hipStream_t s) {
+  if (mixed) {
+    if (mode == 0) { if (energy) force_b<true, true, 0>(d, fc, sa, Fout, s); else force_b<true, false, 0>(d, fc, sa, Fout, s); }
+    else force_b<true, false, 1>(d, fc, sa, Fout, s);
+  } else {
+    if (mode == 0) { if (energy) force_b<false, true, 0>(d, fc, sa, Fout, s); else force_b<false, false, 0>(d, fc, sa, Fout, s); }
+    else force_b<false, false, 1>(d, fc, sa, Fout, s);
   }
 }
 
 void launch_pos(const DevBufs &d, const StepArgs &sa, hipStream_t s) {
   hipLaunchKernelGGL(k_pos, dim3(d.NA / FIB_BLOCK), dim3(FIB_BLOCK), 0, s, d.X[d.cur], d.I, d.NA,
-                     d.V[d.cur], d.F[d.cur], d.Xi[d.cur], sa, d.partials, d.counters, d.ctl);
+                     d.V[d.cur], d.F[d.cur], d.Xi[d.cur], sa, d.pstride, d.partials, d.ctl);
+  const int op[2] = {0, 1};
+  const int dst[2] = {DCTL_IDX(eq_sum), DCTL_IDX(kemax)};
+  finalize(d, 2, FIB_BLOCK, op, dst, s);
 }
 
 void launch_sums(const DevBufs &d, const StepArgs &sa, hipStream_t s) {
   hipLaunchKernelGGL(k_sums, dim3(d.NA / FIB_BLOCK), dim3(FIB_BLOCK), 0, s, d.X[d.cur], d.I, d.NA,
-                     d.V[d.cur], sa, d.partials, d.counters, d.ctl);
+                     d.V[d.cur], sa, d.pstride, d.partials, d.ctl);
+  finalize_thermo(d, s);
 }
 
 void launch_import(const DevBufs &d, int inst, const DInst &in, const double *X,

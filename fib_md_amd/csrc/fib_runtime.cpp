@@ -21,15 +21,17 @@
 
 namespace {
 
+// Phase timers: 0 sort, 1 nbuild, 2 force_a, 3 force_b, 4 integrate.
 struct Timer {
+  static constexpr int NP = 5;
   bool on = false;
   hipStream_t s = nullptr;
   std::vector<hipEvent_t> pool;
   struct Rec { int phase; hipEvent_t a, b; };
   std::vector<Rec> recs;
   size_t used = 0;
-  double acc[5] = {0, 0, 0, 0, 0};
-  int nforce = 0, nnlist = 0;
+  double acc[NP] = {0, 0, 0, 0, 0};
+  int cnt[NP] = {0, 0, 0, 0, 0};
   hipEvent_t get() {
     if (used == pool.size()) {
       hipEvent_t e;
@@ -51,8 +53,8 @@ struct Timer {
     (void)hipEventRecord(b, s);
     recs[id].phase = phase;
     recs[id].b = b;
-    if (phase == 2) ++nforce;
-    if (phase == 1) ++nnlist;
+    ++cnt[phase];
+    if (recs.size() > 4096) flush();
   }
   void flush() {
     if (recs.empty()) return;
@@ -66,8 +68,7 @@ struct Timer {
   }
   void reset() {
     flush();
-    for (double &a : acc) a = 0;
-    nforce = nnlist = 0;
+    for (int k = 0; k < NP; ++k) { acc[k] = 0; cnt[k] = 0; }
   }
 };
 
@@ -96,7 +97,6 @@ struct fib_handle {
   std::vector<fib_instance_desc> desc;
   std::vector<DInst> inst;
   DevBufs d{};
-  int *maxcnt = nullptr;  // [ninst*4] device
   bool have_state = false, have_lists = false, have_force = false;
   int last_flags = 0;
   std::string err;
@@ -234,11 +234,10 @@ static int32_t free_bufs(fib_handle *h) {
   void *ptrs[] = {d.I, d.ctl, d.X[0], d.X[1], d.V[0], d.V[1], d.F[0], d.F[1], d.Xi[0], d.Xi[1],
                   d.inv, d.bkey, d.brank, d.bcnt, d.cstart, d.perm, d.bsum, d.cnt_sw, d.nbr_sw,
                   d.cnt_ion, d.nbr_ion, d.rev, d.G, d.amask, d.Fpart, d.upart, d.Fout, d.uout,
-                  d.partials, d.counters, d.tab, d.xfer, h->maxcnt};
+                  d.partials, d.tab, d.xfer};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
   std::memset(&d, 0, sizeof(d));
-  h->maxcnt = nullptr;
   return FIB_OK;
 }
 
@@ -362,11 +361,10 @@ extern "C" int32_t fib_create(const fib_params *p, int32_t ninst, const fib_inst
   ALLOC(d.upart, NA);
   ALLOC(d.Fout, 3 * NA);
   ALLOC(d.uout, NA);
-  ALLOC(d.partials, (NA / FIB_WAVE + 1) * FIB_NRED);
-  ALLOC(d.counters, (size_t)ninst);
+  d.pstride = d.NA / FIB_WAVE + 1;
+  ALLOC(d.partials, (size_t)d.pstride * FIB_NRED);
   ALLOC(d.tab, h->tab.size());
   ALLOC(d.xfer, 3 * NA);
-  ALLOC(h->maxcnt, (size_t)ninst * 4);
 #undef ALLOC
   if (rc_ == FIB_OK) rc_ = alloc_lists(h, 32, 64);
   if (rc_ != FIB_OK) {
@@ -377,7 +375,6 @@ extern "C" int32_t fib_create(const fib_params *p, int32_t ninst, const fib_inst
   }
   hipError_t e = hipMemcpy(d.I, h->inst.data(), sizeof(DInst) * ninst, hipMemcpyHostToDevice);
   if (e == hipSuccess) e = hipMemset(d.ctl, 0, sizeof(DCtl) * ninst);
-  if (e == hipSuccess) e = hipMemset(d.counters, 0, sizeof(unsigned) * ninst);
   if (e == hipSuccess) e = hipMemset(d.cnt_sw, 0, sizeof(int) * NA);
   if (e == hipSuccess) e = hipMemset(d.cnt_ion, 0, sizeof(int) * NA);
   if (e == hipSuccess) e = hipMemset(d.amask, 0, sizeof(uint64_t) * NA);
@@ -522,34 +519,35 @@ extern "C" int32_t fib_bin_cells(fib_handle *h, int32_t inst, int32_t *icell) {
 // overflowed (the reference only prints, si.f90:264-272).
 static int32_t build_lists(fib_handle *h) {
   DevBufs &d = h->d;
-  const int tid = h->tim.begin();
+  int tid = h->tim.begin();
   fibk::launch_bin(d, h->stream);
   fibk::launch_scan(d, h->stream);
   fibk::launch_sort(d, h->stream);
+  h->tim.end(tid, 0);
   d.cur = 1 - d.cur;
   for (int attempt = 0; attempt < 4; ++attempt) {
     // k_nbuild reports max list lengths into DCtl::maxcnt_*
+    tid = h->tim.begin();
     fibk::launch_nbuild(d, h->fc, h->stream);
+    h->tim.end(tid, 1);
     std::vector<DCtl> c(d.ninst);
     HIPCHK(h, hipMemcpyAsync(c.data(), d.ctl, sizeof(DCtl) * d.ninst, hipMemcpyDeviceToHost,
                              h->stream));
     HIPCHK(h, hipStreamSynchronize(h->stream));
     int msw = 0, mion = 0, nonfin = 0;
     for (auto &x : c) {
-      msw = std::max(msw, x.maxcnt_sw);
-      mion = std::max(mion, x.maxcnt_ion);
+      msw = std::max(msw, (int)x.maxcnt_sw);
+      mion = std::max(mion, (int)x.maxcnt_ion);
       nonfin |= x.nonfinite;
     }
-    // clear the counters for the next build (other DCtl fields unchanged)
-    for (int k = 0; k < d.ninst; ++k) {
-      int zero4[4] = {0, 0, 0, 0};
-      HIPCHK(h, hipMemcpyAsync(&d.ctl[k].maxcnt_sw, zero4, sizeof(zero4), hipMemcpyHostToDevice,
-                               h->stream));
+    if (nonfin) {
+      for (int k = 0; k < d.ninst; ++k) {
+        int zero = 0;
+        HIPCHK(h, hipMemcpy(&d.ctl[k].nonfinite, &zero, sizeof(int), hipMemcpyHostToDevice));
+      }
     }
-    HIPCHK(h, hipStreamSynchronize(h->stream));
     if (nonfin) return fail(h, FIB_ERR_NONFINITE, "non-finite position detected while binning");
     if (msw <= d.Wsw && mion <= d.Wion) {
-      h->tim.end(tid, 1);
       h->have_lists = true;
       h->have_force = false;
       ++h->n_builds;
@@ -632,11 +630,19 @@ extern "C" int32_t fib_get_nlist(fib_handle *h, int32_t inst, int32_t *nss, int3
   return FIB_OK;
 }
 
-static int32_t run_force(fib_handle *h, int flags, int mode, const StepArgs &sa) {
+// mode 0: forces into dst (default d.Fout); mode 1: MD velocity update.
+static int32_t run_force(fib_handle *h, int flags, int mode, const StepArgs &sa,
+                         double *dst = nullptr) {
   const bool en = (flags & FIB_EVAL_ENERGY) != 0, vir = (flags & FIB_EVAL_VIRIAL) != 0;
-  const int tid = h->tim.begin();
-  fibk::launch_force(h->d, h->fc, h->prec == FIB_PREC_MIXED, en, vir, mode, sa, h->stream);
+  const bool mixed = h->prec == FIB_PREC_MIXED;
+  DevBufs &d = h->d;
+  double *Fo = mode == 0 ? (dst ? dst : d.Fout) : d.F[d.cur];
+  int tid = h->tim.begin();
+  fibk::launch_force_a(d, h->fc, mixed, en, vir, h->stream);
   h->tim.end(tid, 2);
+  tid = h->tim.begin();
+  fibk::launch_force_b(d, h->fc, mixed, en && mode == 0, mode, sa, Fo, h->stream);
+  h->tim.end(tid, 3);
   return FIB_OK;
 }
 
@@ -766,10 +772,8 @@ extern "C" int32_t fib_tint_init(fib_handle *h, int64_t lt0, double time0) {
   R.temp = t;
   // initial force into F (timeint.f90:69): force-only pass, then copy
   StepArgs sa = step_args(h, 1);
-  r = run_force(h, FIB_EVAL_FORCE, 0, sa);
+  r = run_force(h, FIB_EVAL_FORCE, 0, sa, h->d.F[h->d.cur]);
   if (r != FIB_OK) return r;
-  HIPCHK(h, hipMemcpyAsync(h->d.F[h->d.cur], h->d.Fout, 3 * sizeof(double) * (size_t)h->d.NA,
-                           hipMemcpyDeviceToDevice, h->stream));
   fibk::launch_sums(h->d, sa, h->stream);
   HIPCHK(h, hipStreamSynchronize(h->stream));
   return FIB_OK;
@@ -842,12 +846,12 @@ extern "C" int32_t fib_step(fib_handle *h, const fib_step_ctl *ctl, fib_step_rep
     sa.frz = (lt % R.atlist == 0) ? 1 : 0;
     int tid = h->tim.begin();
     fibk::launch_pos(h->d, sa, h->stream);
-    h->tim.end(tid, 3);
+    h->tim.end(tid, 4);
     if (w == 0) {
       DCtl c;
       int32_t r = read_ctl(h, 0, &c);
       if (r != FIB_OK) return r;
-      R.temp = c.temp_eq;
+      R.temp = c.eq_sum / 3. / (double)h->inst[0].nsg / h->P.kB;
     }
     if (lt % R.ntlist == 0) {
       int32_t r = build_lists(h);
@@ -908,13 +912,12 @@ extern "C" int32_t fib_get_timing(fib_handle *h, fib_timing *t) {
   if (!h || !t) return FIB_ERR_ARG;
   h->tim.flush();
   std::memset(t, 0, sizeof(*t));
-  t->bin_ms = h->tim.acc[0];
-  t->nlist_ms = h->tim.acc[1];
-  t->force_ms = h->tim.acc[2];
-  t->integrate_ms = h->tim.acc[3];
-  t->total_ms = h->tim.acc[1] + h->tim.acc[2] + h->tim.acc[3];
-  t->n_force = h->tim.nforce;
-  t->n_nlist = h->tim.nnlist;
+  const Timer &T = h->tim;
+  t->sort_ms = T.acc[0]; t->nbuild_ms = T.acc[1]; t->force_a_ms = T.acc[2];
+  t->force_b_ms = T.acc[3]; t->integrate_ms = T.acc[4];
+  t->total_ms = T.acc[0] + T.acc[1] + T.acc[2] + T.acc[3] + T.acc[4];
+  t->n_sort = T.cnt[0]; t->n_nbuild = T.cnt[1]; t->n_force_a = T.cnt[2];
+  t->n_force_b = T.cnt[3]; t->n_integrate = T.cnt[4];
   return FIB_OK;
 }
 

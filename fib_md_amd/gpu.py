@@ -206,8 +206,7 @@ class FibGPU:
             return None
         t = L.fib_timing()
         self._chk("fib_get_timing", self.L.fib_get_timing(self.h, C.byref(t)))
-        return dict(nlist_ms=t.nlist_ms, force_ms=t.force_ms, integrate_ms=t.integrate_ms,
-                    total_ms=t.total_ms, n_force=t.n_force, n_nlist=t.n_nlist)
+        return {k: getattr(t, k) for k, _ in L.fib_timing._fields_ if k != "reserved"}
 
     def synchronize(self):
         self._chk("fib_synchronize", self.L.fib_synchronize(self.h))

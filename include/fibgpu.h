@@ -201,11 +201,12 @@ int32_t fib_step(fib_handle *h, const fib_step_ctl *ctl, fib_step_report *rep);
 int32_t fib_temperature(fib_handle *h, int32_t inst, double *temp);
 
 /* Device-side phase timers (HIP events on the handle's stream), accumulated
- * since fib_timing_enable(h, 1); for benchmarks.  bin_ms is unused (binning is
- * counted in nlist_ms). */
+ * since fib_timing_enable(h, 1); for benchmarks.  sort = binning + counting
+ * sort + reorder, nbuild = list build + reverse-slot map, force_a/force_b =
+ * the two force kernels, integrate = the position-update kernel. */
 typedef struct fib_timing {
-  double bin_ms, nlist_ms, force_ms, integrate_ms, total_ms;
-  int32_t n_force, n_nlist;
+  double sort_ms, nbuild_ms, force_a_ms, force_b_ms, integrate_ms, total_ms;
+  int32_t n_sort, n_nbuild, n_force_a, n_force_b, n_integrate, reserved;
 } fib_timing;
 int32_t fib_timing_enable(fib_handle *h, int32_t on);
 int32_t fib_get_timing(fib_handle *h, fib_timing *t);

@@ -167,6 +167,8 @@ void orc_ionrun(orc *o, int32_t tic) {
 static void alloc_atoms(orc *o) {
   size_t n = (size_t)o->natm;
   free_arrays(o);
+  free(o->nss); free(o->nsslj);
+  o->nss = o->nsslj = NULL;
   o->X = (double *)calloc(3 * n, sizeof(double));
   o->Xi = (double *)calloc(3 * n, sizeof(double));
   o->V = (double *)calloc(3 * n, sizeof(double));
@@ -244,8 +246,25 @@ static void derived(orc *o) {
   for (int l = 0; l < 3; ++l) o->ilb[l] = 1. / o->lb[l];
 }
 
+static void copy_siga(orc *o, const orc_siga *s);
+
 /* readprms after the reads, prms.f90:279-357 */
 int orc_readprms(orc *o, const orc_siga *s, int32_t ranseed) {
+  copy_siga(o, s);
+  if (o->amorcrys != 0) return -1; /* amorphous input file: out of scope */
+  pureepi(o, s->nc[0], s->nc[1], s->nc[2]);
+  o->ranseed = ranseed;
+  derived(o);
+  return 0;
+}
+
+/* Run parameters only (no lattice): for synthetic geometries. */
+void orc_set_run(orc *o, const orc_siga *s) {
+  copy_siga(o, s);
+  orc_ionrun(o, 1);
+}
+
+static void copy_siga(orc *o, const orc_siga *s) {
   o->amorcrys = s->amorcrys; o->whichic = s->whichic;
   o->startimpact = s->startimpact; o->eV = s->eV;
   for (int l = 0; l < 3; ++l) { o->npc[l] = s->npc[l]; o->nc_lat[l] = s->nc[l]; }
@@ -263,11 +282,6 @@ int orc_readprms(orc *o, const orc_siga *s, int32_t ranseed) {
   o->atomz[0] = s->atomz[0]; o->atomz[1] = s->atomz[1];
   o->ionz = s->ionz; o->knockz = s->knockz; o->sidewidth = s->sidewidth;
   o->dti = s->dti; o->fwhm = s->fwhm; o->phiz = s->phiz; o->phixy = s->phixy;
-  if (o->amorcrys != 0) return -1; /* amorphous input file: out of scope */
-  pureepi(o, s->nc[0], s->nc[1], s->nc[2]);
-  o->ranseed = ranseed;
-  derived(o);
-  return 0;
 }
 
 int orc_set_geometry(orc *o, int32_t natm, int32_t nsg, int32_t nlj,
@@ -504,9 +518,15 @@ void orc_init_nlist(orc *o) {
   o->rlj2 = 2.5 * smax;
   o->rclj = o->listfac * o->rlj2;
   o->rlj2 = o->rlj2 * o->rlj2;
-  free(o->nss); free(o->nsslj);
-  o->nss = (int32_t *)calloc(2 * (size_t)o->natm, sizeof(int32_t));
-  o->nsslj = (int32_t *)calloc(2 * (size_t)o->natm, sizeof(int32_t));
+  /* nss/nsslj are allocated once (si.f90:70-72).  The reference re-allocates
+   * nlist2/nlist3/... here, leaving them undefined until the next si_nlist;
+   * every atlist step is also an ntlist step in the reference's inputs.  The
+   * restatement keeps the previous lists instead (well defined either way). */
+  if (!o->nss) {
+    o->nss = (int32_t *)calloc(2 * (size_t)o->natm, sizeof(int32_t));
+    o->nsslj = (int32_t *)calloc(2 * (size_t)o->natm, sizeof(int32_t));
+    for (int i = 0; i < o->natm; ++i) { o->nss[o->natm + i] = -1; o->nsslj[o->natm + i] = -1; }
+  }
 }
 
 /* mktables, si.f90:575-597 */

@@ -64,6 +64,8 @@ int orc_readprms(orc *o, const orc_siga *s, int32_t ranseed);
 int orc_set_geometry(orc *o, int32_t natm, int32_t nsg, int32_t nlj,
                      const double lb[3]);
 void orc_ionrun(orc *o, int32_t tic);
+/* Copy the run parameters of a siga record without building the lattice. */
+void orc_set_run(orc *o, const orc_siga *s);
 
 /* --- state I/O -------------------------------------------------------- */
 void orc_set_state(orc *o, const double *X, const double *V,

@@ -64,6 +64,7 @@ def lib():
             "orc_readprms": ([vp, P(Siga), i32], C.c_int),
             "orc_set_geometry": ([vp, i32, i32, i32, vp], C.c_int),
             "orc_ionrun": ([vp, i32], None),
+            "orc_set_run": ([vp, P(Siga)], None),
             "orc_set_state": ([vp, vp, vp, vp, vp], None),
             "orc_get_state": ([vp, vp, vp, vp], None),
             "orc_set_force": ([vp, vp], None),
@@ -130,6 +131,9 @@ class Oracle:
         rc = self.L.orc_readprms(self.h, C.byref(siga), int(ranseed))
         if rc != 0:
             raise ValueError("orc_readprms failed")
+
+    def set_run(self, siga: Siga) -> None:
+        self.L.orc_set_run(self.h, C.byref(siga))
 
     def set_geometry(self, natm, nsg, nlj, lb) -> None:
         lbv = np.asarray(lb, dtype=np.float64)

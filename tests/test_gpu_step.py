@@ -1,0 +1,151 @@
+"""GPU parity of the reference time integration (onestep/TI, timeint.f90)
+against the oracle, through fib_step.
+
+- Free flight (no neighbours, F == 0 exactly): positions and velocities are
+  bit-identical after every step, including side freezing, rebox across the
+  periodic boundary and the sputter-layer relocation.
+- Reference run (pureepi slab, uniform velocities, equilibration then one
+  impact with a regime switch): step counter, regime, timestep and simulated
+  time identical; X, V within 1e-9 relative (force summation order differs).
+"""
+import numpy as np
+import pytest
+
+from oracle.oracle import REFERENCE_SIGA, siga_from_values
+from tests import gpu_common as gc
+
+pytestmark = pytest.mark.gpu
+
+
+def _run_params(v):
+    from fib_md_amd.gpu import run_params_from_siga
+    return run_params_from_siga(v)
+
+
+def test_free_flight_bit_exact(oracle_lib):
+    from oracle.oracle import Oracle
+    from fib_md_amd import FibGPU
+    rng = np.random.default_rng(5)
+    L = (60e-10, 60e-10, 600e-10)
+    # 5 x 5 x 5 grid spaced 12 A in x/y; z layers outside every sputter window
+    # (T = Lb(3)/10 = 60 A: outz 78-210 and 450-588 A, atomz/ionz +-18 A
+    # around 270, 390, 330 A) and >= 27 A apart: no pair within rclj, F == 0
+    g3 = np.stack(np.meshgrid(np.arange(5), np.arange(5), np.arange(5), indexing="ij"), -1).reshape(-1, 3)
+    zl = np.array([20e-10, 60e-10, 230e-10, 300e-10, 430e-10])
+    X = np.column_stack([g3[:, 0] * 12e-10 + 3e-10, g3[:, 1] * 12e-10 + 3e-10, zl[g3[:, 2]]])
+    n = len(X)
+    # a few atoms inside the sputter windows (units of Lb(3)/10 = 60 A)
+    X[7, 2] = 1.8 * 60e-10     # outz(1..2) window -> relocated to atomz(1)
+    X[8, 2] = 8.0 * 60e-10     # outz(3..4) window -> relocated to atomz(2)
+    X[9, 2] = 5.45 * 60e-10    # ionz buffer -> V, F zeroed
+    V = rng.normal(0, 3e3, (n, 3))   # < 1 A drift over the 30 steps
+    V[3] = [0.0, -1.5e4, 0.0]        # crosses y = 0 (periodic rebox)
+    at = np.ones(n, np.int32)
+    xs = np.zeros(n, np.int32)
+    xs[[0, 11, 17]] = [1, 12, 18]
+    v = dict(REFERENCE_SIGA)
+    v.update(ts_i=1e-15, ts_r=1e-15, atlist_i=3, atlist_r=3, ntlist_i=7, ntlist_r=7,
+             dslist_i=1000, dslist_r=1000, atm_out_i=5, atm_out_r=5, nlj=0, dti=1.0,
+             sidewidth=0.0)
+    o = Oracle()
+    o.set_geometry(n, n, 0, L)
+    o.set_state(X=X, V=V, atype=at, mass=gc.masses(at))
+    o.set_xsides(xs)
+    o.set_xi(X)
+    o.set_run(siga_from_values(v))
+    o.init_nlist(); o.mktables(); o.si_nlist(X)
+    o.tint_init()
+    o.seti("impact", 1)  # impact*dti = 1 s: the TI(1) loop never ends here
+    g = FibGPU([(n, n, 0, L)])
+    g.set_state(X, V=V, atype=at, xsides=xs, Xi=X)
+    g.build_nlist()
+    g.set_run(_run_params(v))
+    g.tint_init(0, 0.0)
+    g.ti_begin(1, 1)
+    o.seti("atom_is_fast", 1)
+    for k in range(6):
+        for _ in range(5):
+            assert o.ti_iter(1, 300.0) == 0
+        rep = g.step(1, max_steps=5)
+        Xo, Vo, _ = o.state()
+        Xg, Vg, _ = g.state()
+        assert rep.lt == o.geti("lt")
+        assert rep.time == o.getd("time")
+        _, _, Fo = o.state()
+        assert np.all(Fo == 0.0)
+        np.testing.assert_array_equal(Xg, Xo)
+        np.testing.assert_array_equal(Vg, Vo)
+
+
+def _small_reference(eV=500.0, seed=-12345):
+    from oracle.oracle import Oracle
+    v = dict(REFERENCE_SIGA)
+    v.update(nc=(4, 4, 2), nlj=2, eV=eV, dti=6.2e-14, nrand=200, dslist_i=40, dslist_r=40,
+             atm_out_i=25, atm_out_r=25, ntlist_i=10, ntlist_r=10, atlist_i=10, atlist_r=10,
+             teps=30.0, ke_limit_1=600.0, knockz=1.05)
+    o = Oracle()
+    o.readprms(siga_from_values(v), seed)
+    o.initranions()
+    o.initdata()
+    o.init_nlist(); o.mktables()
+    X, V, _ = o.state()
+    o.si_nlist(X)
+    return o, v
+
+
+def test_reference_equilibrate_and_impact(oracle_lib):
+    from fib_md_amd import FibGPU
+    o, v = _small_reference()
+    X, V, _ = o.state()
+    at, _ = o.types()
+    n = o.natm
+    nsg, nlj = o.geti("nsg"), o.geti("nlj")
+    L = (o.getd("lb1"), o.getd("lb2"), o.getd("lb3"))
+    g = FibGPU([(n, nsg, nlj, L)])
+    g.set_state(X, V=V, atype=at, xsides=o.xsides(), Xi=o.xi())
+    g.build_nlist()
+    g.set_run(_run_params(v))
+    o.tint_init()
+    g.tint_init(0, 0.0)
+    # equilibration, TI(0) with Tau = 1e-14
+    o.setd("Tau", 1e-14)
+    o.ti_begin(0)
+    g.ti_begin(0)
+    while not o.ti_iter(0, v["ttar1"]):
+        pass
+    rep = g.step(0)
+    assert rep.ended
+    assert rep.lt == o.geti("lt")
+    assert abs(rep.temp - o.getd("temp")) <= 1e-9 * o.getd("temp")
+    Xo, Vo, _ = o.state()
+    Xg, Vg, _ = g.state()
+    assert np.abs(Xg - Xo).max() <= 1e-19
+    assert np.abs(Vg - Vo).max() <= 1e-9 * np.abs(Vo).max()
+    # impact 1 (timeint.f90:98-113): ionrun(1), knock, TI(1)
+    o.seti("impact", 1); o.seti("atom_is_fast", 1); o.seti("atom_is_slow", 0)
+    o.ionrun(1)
+    o.knock(1)
+    Xo, Vo, _ = o.state()
+    g.ti_begin(1, 1)
+    g.knock(1, Xo[nsg], Vo[nsg])
+    done_o = False
+    while True:
+        k = 0
+        while k < 25 and not done_o:
+            done_o = bool(o.ti_iter(1, v["ttar1"]))
+            if not done_o:
+                k += 1
+        rep = g.step(1, max_steps=25)
+        assert rep.steps_done == k
+        assert rep.lt == o.geti("lt")
+        assert rep.ts == o.getd("ts")
+        assert rep.time == o.getd("time")
+        Xo, Vo, _ = o.state()
+        Xg, Vg, _ = g.state()
+        assert np.abs(Xg - Xo).max() <= 1e-19, np.abs(Xg - Xo).max()
+        assert np.abs(Vg - Vo).max() <= 1e-9 * np.abs(Vo).max()
+        if done_o:
+            assert rep.ended
+            break
+    # the 500 eV ion is below ke_limit_1 at the first check: regime 2
+    assert rep.regime == 2 and o.getd("ts") == v["ts_r"]
