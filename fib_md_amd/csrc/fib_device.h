@@ -43,6 +43,9 @@ struct FConst {
   double isig[2][2], epssig[2][2], eps[2][2], leps[2][2][2];
   double A, B, p, q, al, gamma;
   double cut_sw;      // 0.999*al: reduced-distance gate of SW terms
+  double cut_sw2;     // cut_sw^2 (1 + 1e-6): FP32 pre-gate on r^2
+  double r2gate;      // (1.001 u_izero)^2: pairs beyond it have no table force
+                      // and no SW leg (pre-gate on r^2 / sigma^2)
   double rv2, rv2lj;  // squared list cutoffs rc^2, rclj^2 (si.f90:115-116)
   double rlj2;        // squared Moliere force cutoff (2.5 sigma)^2
   double molc[3], molb[3];

@@ -47,7 +47,10 @@ namespace fibk {
 void launch_bin(const DevBufs &d, hipStream_t s);
 void launch_scan(const DevBufs &d, hipStream_t s);
 void launch_sort(const DevBufs &d, hipStream_t s);  // scatter + fixup + reorder (flips cur)
+// list build + longest-list reduction (DCtl::maxcnt_*)
 void launch_nbuild(const DevBufs &d, const FConst &fc, hipStream_t s);
+// reverse-slot map, after a successful build
+void launch_rev(const DevBufs &d, hipStream_t s);
 void launch_cellid(const DevBufs &d, int inst, int32_t *dout, hipStream_t s);
 void launch_force_a(const DevBufs &d, const FConst &fc, bool mixed, bool energy, bool virial,
                     hipStream_t s);

@@ -138,34 +138,43 @@ __device__ void block_partials(double (&v)[NV], const int (&op)[NV], int lblock,
   }
 }
 
-#define FIN_T 256
-// One block per instance: combine the instance's partials in a fixed order.
+#define FIN_T 1024
+// One block per instance: combine the instance's partials in a fixed order
+// (thread t sums blocks t, t+FIN_T, ... then a fixed tree), all NV slots at
+// once so the loads of every slot are in flight together.
+template <int NV>
 __global__ __launch_bounds__(FIN_T) void k_finalize(const DInst *__restrict__ I,
                                                     const double *__restrict__ partials,
                                                     int pstride, RedSpec rs,
                                                     DCtl *__restrict__ ctl) {
-  __shared__ double red[FIN_T / FIB_WAVE];
+  __shared__ double red[FIN_T / FIB_WAVE][NV];
   const int inst = blockIdx.x;
   const int blk0 = I[inst].atom_off / rs.bs, nblk = I[inst].natm_pad / rs.bs;
   const int lane = threadIdx.x % FIB_WAVE, wid = threadIdx.x / FIB_WAVE;
-  double *out = reinterpret_cast<double *>(&ctl[inst]);
-  for (int k = 0; k < rs.nv; ++k) {
-    const bool mx = rs.op[k] != 0;
-    const double *p = partials + (size_t)k * pstride + blk0;
-    double x = mx ? -1.0e300 : 0.0;
-    for (int b = threadIdx.x; b < nblk; b += FIN_T) x = mx ? fmax(x, p[b]) : x + p[b];
+  double x[NV];
+#pragma unroll
+  for (int k = 0; k < NV; ++k) x[k] = rs.op[k] ? -1.0e300 : 0.0;
+  for (int b = threadIdx.x; b < nblk; b += FIN_T) {
+#pragma unroll
+    for (int k = 0; k < NV; ++k) {
+      const double y = partials[(size_t)k * pstride + blk0 + b];
+      x[k] = rs.op[k] ? fmax(x[k], y) : x[k] + y;
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < NV; ++k) {
     for (int o = FIB_WAVE / 2; o > 0; o >>= 1) {
-      double y = __shfl_xor(x, o);
-      x = mx ? fmax(x, y) : x + y;
+      const double y = __shfl_xor(x[k], o);
+      x[k] = rs.op[k] ? fmax(x[k], y) : x[k] + y;
     }
-    if (lane == 0) red[wid] = x;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-      double t = red[0];
-      for (int w = 1; w < FIN_T / FIB_WAVE; ++w) t = mx ? fmax(t, red[w]) : t + red[w];
-      if (rs.dst[k] >= 0) out[rs.dst[k]] = t;
-    }
-    __syncthreads();
+    if (lane == 0) red[wid][k] = x[k];
+  }
+  __syncthreads();
+  if (threadIdx.x < NV) {
+    const int k = threadIdx.x;
+    double t = red[0][k];
+    for (int w = 1; w < FIN_T / FIB_WAVE; ++w) t = rs.op[k] ? fmax(t, red[w][k]) : t + red[w][k];
+    if (rs.dst[k] >= 0) reinterpret_cast<double *>(&ctl[inst])[rs.dst[k]] = t;
   }
 }
 
@@ -341,7 +350,7 @@ __global__ void k_reorder(int NA, const int *__restrict__ perm, const DInst *__r
 
 // ---------------------------------------------------------------------------
 // neighbour build: one 64-lane block per real cell
-#define NB_CHUNK 512
+#define NB_CHUNK 128
 
 __global__ __launch_bounds__(FIB_WAVE) void k_nbuild(
     const double4 *__restrict__ X, const DInst *__restrict__ I, int ninst, int total_cells,
@@ -460,72 +469,150 @@ __global__ void k_clear_counts(int NA, int *__restrict__ cnt_sw, int *__restrict
   cnt_ion[p] = 0;
 }
 
-// Reverse slots plus the longest lists of the build (overflow check).
+// Reverse slots: rev[k][i] = slot of i in the list of its k-th neighbour.
+// A block handles 16 atoms with one thread per (slot, atom): the 16 atoms of
+// one slot row are contiguous, and each thread scans its neighbour's list
+// eight entries at a time so the loads overlap.
+#define REV_ATOMS 16
 __global__ __launch_bounds__(FIB_BLOCK) void k_rev(int NA, int Wsw, const int *__restrict__ cnt_sw,
-                                                   const int *__restrict__ cnt_ion,
                                                    const int *__restrict__ nbr_sw,
-                                                   uint8_t *__restrict__ rev, int pstride,
-                                                   double *__restrict__ partials) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  const int cs = cnt_sw[i];
-  const int c = min(cs, Wsw);
-  for (int k = 0; k < c; ++k) {
-    int j = nbr_sw[(size_t)k * NA + i];
-    int cj = min(cnt_sw[j], Wsw);
+                                                   uint8_t *__restrict__ rev) {
+  const int i = blockIdx.x * REV_ATOMS + (threadIdx.x % REV_ATOMS);
+  const int c = min(cnt_sw[i], Wsw);
+  for (int k = threadIdx.x / REV_ATOMS; k < c; k += FIB_BLOCK / REV_ATOMS) {
+    const int j = nbr_sw[(size_t)k * NA + i];
+    const int cj = min(cnt_sw[j], Wsw);
     int r = 255;
-    for (int q = 0; q < cj; ++q)
-      if (nbr_sw[(size_t)q * NA + j] == i) { r = q; break; }
+    for (int q0 = 0; q0 < cj && r == 255; q0 += 8) {
+      int e[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) e[u] = (q0 + u < cj) ? nbr_sw[(size_t)(q0 + u) * NA + j] : -1;
+#pragma unroll
+      for (int u = 7; u >= 0; --u)
+        if (e[u] == i) r = q0 + u;
+    }
     rev[(size_t)k * NA + i] = (uint8_t)r;
   }
-  double v[2] = {(double)cs, (double)cnt_ion[i]};
+}
+
+// Longest SW / ion lists of the build (overflow check).
+__global__ __launch_bounds__(FIB_BLOCK) void k_maxcnt(const int *__restrict__ cnt_sw,
+                                                      const int *__restrict__ cnt_ion,
+                                                      int pstride, double *__restrict__ partials) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  double v[2] = {(double)cnt_sw[i], (double)cnt_ion[i]};
   const int op[2] = {1, 1};
   block_partials<2, FIB_BLOCK>(v, op, blockIdx.x, pstride, partials);
 }
 
 // ---------------------------------------------------------------------------
-// force phase A
-template <typename T>
-struct Slot {  // one active SW leg of the centre atom
-  T e[3];      // unit vector from neighbour to centre
-  T rinv;      // 1/r (FP64: m^-1; mixed: sigma^-1 units)
-  T E;         // exp(gamma/(r/sigma - a))
-  T g;         // d ln E / dr
-  int k;       // slot in the buffered list | type << 8
-};
-
+// force phase A: four lanes per atom
+//
+// A group of four lanes owns one atom.  Phase 1: the lanes split the atom's
+// buffered SW list (slot k goes to lane k % 4), compute the reference pair
+// force and flag the legs inside the SW range; the flags are OR-ed over the
+// quad into the atom's active mask.  Phase 2: active leg L (in slot order) is
+// owned by lane L % 4, register slot L / 4; the lane computes its geometry
+// (unit vector, 1/r, E = exp(gamma/(r-a)), d ln E/dr).  Phase 3: every lane
+// loops over the legs b of its atom, fetches them with wavefront shuffles and
+// accumulates the three-body terms centred on the atom that act on its own
+// legs a; the share acting on neighbour a goes to G[slot(a)], the reaction on
+// the centre stays in registers.  Phase 4: quad reduction, lane 0 stores.
+// No LDS, no atomics; legs beyond 8 per atom are recomputed on the fly.
 template <bool MIXED>
 struct Prec;
 template <>
 struct Prec<false> {
   typedef double T;
   typedef double4 T4;
-  static constexpr int MAXS = 8;
 };
 template <>
 struct Prec<true> {
   typedef float T;
   typedef float4 T4;
-  static constexpr int MAXS = 12;
+};
+
+template <typename T>
+struct Leg {
+  T e0, e1, e2;  // unit vector from neighbour to centre
+  T rinv;        // 1/r (FP64: m^-1; mixed: sigma^-1)
+  T E;           // exp(gamma/(r/sigma - a))
+  T g;           // d ln E / dr
+  int t;         // species of the neighbour
 };
 
 // One ordered three-body term: centre c, legs a and b.  Returns the force on
 // a (F_a = -dh/dx_a) and h.  Derivation: h = leps E(ra) E(rb) (cos+1/3)^2,
 // d cos/dx_a = -(e_b - cos e_a)/r_a, d r_a/dx_a = -e_a.
 template <typename T>
-__device__ __forceinline__ void vterm(T leps, const T *ea, T rinva, T Ea, T ga, const T *eb,
-                                      T Eb, T *Fa, T &hh) {
-  T cs = ea[0] * eb[0] + ea[1] * eb[1] + ea[2] * eb[2];
-  T D = cs + T(1.0 / 3.0);
-  T Eab = leps * Ea * Eb;
+__device__ __forceinline__ void vterm(T leps, const Leg<T> &a, const Leg<T> &b, T *Fa, T &hh) {
+  const T cs = a.e0 * b.e0 + a.e1 * b.e1 + a.e2 * b.e2;
+  const T D = cs + T(1.0 / 3.0);
+  const T Eab = leps * a.E * b.E;
   hh = Eab * D * D;
-  T c1 = hh * ga;
-  T c2 = T(2) * Eab * D * rinva;
-#pragma unroll
-  for (int c = 0; c < 3; ++c) Fa[c] = c1 * ea[c] + c2 * (eb[c] - cs * ea[c]);
+  const T c1 = hh * a.g;
+  const T c2 = T(2) * Eab * D * a.rinv;
+  Fa[0] = c1 * a.e0 + c2 * (b.e0 - cs * a.e0);
+  Fa[1] = c1 * a.e1 + c2 * (b.e1 - cs * a.e1);
+  Fa[2] = c1 * a.e2 + c2 * (b.e2 - cs * a.e2);
+}
+
+template <bool MIXED>
+__device__ __forceinline__ void leg_geom(const double4 &xi, const double4 &xj, const DInst &in,
+                                         const FConst &fc, int ti,
+                                         Leg<typename Prec<MIXED>::T> &L) {
+  const int tj = type_of(meta_of(xj));
+  double d0, d1, d2;
+  min_image(xi, xj, in, d0, d1, d2);
+  const double isg = fc.isig[(ti == 2) ? 1 : 0][(tj == 2) ? 1 : 0];
+  L.t = tj;
+  if (!MIXED) {
+    const double r = sqrt(d0 * d0 + d1 * d1 + d2 * d2);
+    const double ri = 1.0 / r;
+    const double rs = r * isg;
+    const double inv_ra = 1.0 / (rs - fc.al);
+    L.e0 = d0 * ri; L.e1 = d1 * ri; L.e2 = d2 * ri;
+    L.rinv = ri;
+    L.E = exp(fc.gamma * inv_ra);
+    L.g = -fc.gamma * inv_ra * inv_ra * isg;
+  } else {
+    const float r0 = (float)(d0 * isg), r1 = (float)(d1 * isg), r2 = (float)(d2 * isg);
+    const float rs = sqrtf(r0 * r0 + r1 * r1 + r2 * r2);
+    const float ri = 1.0f / rs;
+    const float inv_ra = 1.0f / (rs - (float)fc.al);
+    L.e0 = r0 * ri; L.e1 = r1 * ri; L.e2 = r2 * ri;
+    L.rinv = ri;
+    L.E = __expf((float)fc.gamma * inv_ra);
+    L.g = -(float)fc.gamma * inv_ra * inv_ra;
+  }
+}
+
+// Slot of the n-th set bit of m (n is small).
+__device__ __forceinline__ int nth_bit(uint64_t m, int n) {
+  for (int r = 0; r < n; ++r) m &= m - 1;
+  return __ffsll((unsigned long long)m) - 1;
+}
+
+template <typename T>
+__device__ __forceinline__ T shfl_any(T v, int src) {
+  return __shfl(v, src);
+}
+
+#define FA_ATOMS (FIB_BLOCK / 4)  // atoms per block
+
+// Minimum image with v_rndne (ties to even).  Neighbour pairs are far from
+// the half-box tie, so this equals the reference's NINT image there.
+__device__ __forceinline__ void min_image_fast(const double4 &a, const double4 &b,
+                                               const DInst &in, double &x0, double &x1,
+                                               double &x2) {
+  const double d0 = a.x - b.x, d1 = a.y - b.y, d2 = a.z - b.z;
+  x0 = d0 - rint(d0 * in.ilb[0]) * in.lb[0];
+  x1 = d1 - rint(d1 * in.ilb[1]) * in.lb[1];
+  x2 = d2 - rint(d2 * in.ilb[2]) * in.lb[2];
 }
 
 template <bool MIXED, bool EN, bool VIR>
-__global__ __launch_bounds__(FIB_WAVE) void k_force_a(
+__global__ __launch_bounds__(FIB_BLOCK) void k_force_a(
     const double4 *__restrict__ X, const DInst *__restrict__ I, const FConst fc,
     const TabEntry *__restrict__ tab, int NA, int Wsw, int Wion, const int *__restrict__ cnt_sw,
     const int *__restrict__ nbr_sw, const int *__restrict__ cnt_ion,
@@ -534,135 +621,191 @@ __global__ __launch_bounds__(FIB_WAVE) void k_force_a(
     int pstride, double *__restrict__ partials, DCtl *__restrict__ ctl) {
   typedef typename Prec<MIXED>::T T;
   typedef typename Prec<MIXED>::T4 T4;
-  constexpr int MAXS = Prec<MIXED>::MAXS;
-  __shared__ Slot<T> s_sl[MAXS][FIB_WAVE];
 
   const int nb = gridDim.x;
   const int lb = xcd_remap(blockIdx.x, nb);
-  const int lane = threadIdx.x;
-  const int i = lb * FIB_WAVE + lane;
+  const int lane = threadIdx.x % FIB_WAVE;
+  const int q = lane & 3;
+  const int gbase = lane & ~3;
+  const int i = lb * FA_ATOMS + (int)(threadIdx.x >> 2);
   const double4 xi = X[i];
   const uint32_t mi = meta_of(xi);
-  const int kin = inst_of(mi);
+  // every block lies inside one instance: scalar loads of the instance record
+  const int kin = __builtin_amdgcn_readfirstlane(inst_of(mi));
   const DInst &in = I[kin];
   const bool live = !(mi & (META_PAD | META_DEAD));
   const int ti = type_of(mi);
+  const bool sw_atom = live && ti < 3;
+  const int tr = (ti == 2) ? 1 : 0;
+  const double isgA = fc.isig[tr][0], isgB = fc.isig[tr][1];
+  const double esgA = fc.epssig[tr][0], esgB = fc.epssig[tr][1];
+  const double epsA = fc.eps[tr][0], epsB = fc.eps[tr][1];
+  // leps in the unit of the three-body pass (J, or eps_Si in mixed)
+  const double lu = MIXED ? 1.0 / fc.eps[0][0] : 1.0;
+  const T lp11 = (T)(fc.leps[tr][0][0] * lu), lp12 = (T)(fc.leps[tr][0][1] * lu),
+          lp22 = (T)(fc.leps[tr][1][1] * lu);
+  // unit factors: FP64 works in m and N; mixed in sigma and eps/sigma
+  const double fscale = MIXED ? fc.eps[0][0] * fc.isig[0][0] : 1.0;
+  const double escale = MIXED ? fc.eps[0][0] : 1.0;
+  const double lscale = MIXED ? 1.0 / fc.isig[0][0] : 1.0;
 
-  double fx = 0, fy = 0, fz = 0;     // FP64 accumulation (FP64 path)
-  float sx = 0, sy = 0, sz = 0, cx = 0, cy = 0, cz = 0;  // compensated FP32 (mixed)
+  // force accumulators: FP64, or FP32 value + compensation (mixed)
+  double fx = 0, fy = 0, fz = 0;
+  float sx = 0, sy = 0, sz = 0, cx = 0, cy = 0, cz = 0;
   double ui = 0, etot = 0;
-  double W[9];
+  T W[9];
 #pragma unroll
-  for (int q = 0; q < 9; ++q) W[q] = 0;
-  uint64_t am = 0;
-  int na = 0;
-
-  auto addF = [&](double vx, double vy, double vz) {
-    if (MIXED) { kadd(sx, cx, (float)vx); kadd(sy, cy, (float)vy); kadd(sz, cz, (float)vz); }
-    else { fx += vx; fy += vy; fz += vz; }
+  for (int w = 0; w < 9; ++w) W[w] = 0;
+  auto addW = [&](T d0, T d1, T d2, T f0, T f1, T f2) {
+    W[0] += d0 * f0; W[1] += d0 * f1; W[2] += d0 * f2;
+    W[3] += d1 * f0; W[4] += d1 * f1; W[5] += d1 * f2;
+    W[6] += d2 * f0; W[7] += d2 * f1; W[8] += d2 * f2;
   };
 
-  if (live && ti < 3) {
-    const int cs = min(cnt_sw[i], Wsw);
-    for (int k = 0; k < cs; ++k) {
-      const int j = nbr_sw[(size_t)k * NA + i];
-      const double4 xj = X[j];
-      const int tj = type_of(meta_of(xj));
-      double d0, d1, d2;
-      min_image(xi, xj, in, d0, d1, d2);
-      const double isg = fc.isig[ti - 1][tj - 1];
-      if (!MIXED) {
-        const double r2 = d0 * d0 + d1 * d1 + d2 * d2;
-        const double r = sqrt(r2);
-        const double rinv = 1.0 / r;
-        const double rs = r * isg;
-        // reference pair force: 3-point Lagrange on the derivative table
-        const int t1 = (int)(rs / fc.umax * fc.tabn1) + 1;
-        if (t1 < fc.izero) {
-          const TabEntry te = tab[t1];
-          const double e1 = rs - te.u0, e2 = rs - te.u1, e3 = rs - te.u2;
-          const double df = 0.5 * e2 * e3 * te.t0 - e1 * e3 * te.t1 + 0.5 * e1 * e2 * te.t2;
-          const double fs = -df * fc.epssig[ti - 1][tj - 1] * rinv;
-          const double f0 = fs * d0, f1 = fs * d1, f2 = fs * d2;
-          fx += f0; fy += f1; fz += f2;
-          if (VIR) {
-            W[0] += 0.5 * d0 * f0; W[1] += 0.5 * d0 * f1; W[2] += 0.5 * d0 * f2;
-            W[3] += 0.5 * d1 * f0; W[4] += 0.5 * d1 * f1; W[5] += 0.5 * d1 * f2;
-            W[6] += 0.5 * d2 * f0; W[7] += 0.5 * d2 * f1; W[8] += 0.5 * d2 * f2;
-          }
-        }
-        if (rs < fc.cut_sw) {
-          const double inv_ra = 1.0 / (rs - fc.al);
-          const double E = exp(fc.gamma * inv_ra);
-          if (EN) {
-            const double rs2 = rs * rs;
-            const double ul = 0.5 * fc.eps[ti - 1][tj - 1] * fc.A * (fc.B / (rs2 * rs2) - 1.0) *
-                              exp(inv_ra);
-            ui += ul;
-            etot += ul;
-          }
-          if (na < MAXS) {
-            Slot<T> &s = s_sl[na][lane];
-            s.e[0] = d0 * rinv; s.e[1] = d1 * rinv; s.e[2] = d2 * rinv;
-            s.rinv = rinv; s.E = E; s.g = -fc.gamma * inv_ra * inv_ra * isg;
-            s.k = k | (tj << 8);
-          }
-          am |= 1ull << k;
-          ++na;
-        }
-      } else {
-        // mixed: FP64 difference, reduced units (sigma) in FP32
-        const float r0 = (float)(d0 * isg), r1 = (float)(d1 * isg), r2f = (float)(d2 * isg);
-        const float rr = r0 * r0 + r1 * r1 + r2f * r2f;
-        const float rs = sqrtf(rr);
-        const float rinv = 1.0f / rs;
-        if (rs < (float)fc.cut_sw) {
-          const float al = (float)fc.al;
-          const float inv_ra = 1.0f / (rs - al);
-          const float ex = __expf(inv_ra);
-          const float B = (float)fc.B;
-          const float A = (float)fc.A;
-          const float ir2 = rinv * rinv;
-          const float ir4 = ir2 * ir2;
-          // phi2(r) = A (B r^-4 - 1) e^{1/(r-a)};  dphi/dr
-          const float dphi = A * ex * (-4.0f * B * ir4 * rinv - (B * ir4 - 1.0f) * inv_ra * inv_ra);
-          const float es = (float)fc.epssig[ti - 1][tj - 1];
-          // force on i along +d: -dphi * eps/sigma * e
-          const float fs = -dphi * rinv;  // units eps/sigma per sigma
-          const float f0 = fs * r0, f1 = fs * r1, f2 = fs * r2f;
-          const double fsc = (double)es;
-          addF(f0 * fsc, f1 * fsc, f2 * fsc);
-          if (VIR) {
-            const double sg = 1.0 / isg;
-            const double w0 = 0.5 * sg * fsc;
-            W[0] += w0 * r0 * f0; W[1] += w0 * r0 * f1; W[2] += w0 * r0 * f2;
-            W[3] += w0 * r1 * f0; W[4] += w0 * r1 * f1; W[5] += w0 * r1 * f2;
-            W[6] += w0 * r2f * f0; W[7] += w0 * r2f * f1; W[8] += w0 * r2f * f2;
-          }
-          if (EN) {
-            const double ul = 0.5 * fc.eps[ti - 1][tj - 1] * (double)(A * (B * ir4 - 1.0f) * ex);
-            ui += ul;
-            etot += ul;
-          }
-          const float E = __expf((float)fc.gamma * inv_ra);
-          if (na < MAXS) {
-            Slot<T> &s = s_sl[na][lane];
-            s.e[0] = r0 * rinv; s.e[1] = r1 * rinv; s.e[2] = r2f * rinv;
-            s.rinv = rinv; s.E = E; s.g = -(float)fc.gamma * inv_ra * inv_ra;
-            s.k = k | (tj << 8);
-          }
-          am |= 1ull << k;
-          ++na;
-        }
+  // ---- phase 1: cheap distance test of every buffered SW slot -------------
+  // lane q tests slots q, q+4, q+8, ...; pm collects the pairs that carry a
+  // pair force (FP64: table range; mixed: SW range)
+  uint64_t pm = 0;
+  int cs = 0;
+  if (sw_atom) {
+    int jr[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) jr[u] = nbr_sw[(size_t)(q + 4 * u) * NA + i];
+    cs = min(cnt_sw[i], Wsw);
+    const double gate = MIXED ? fc.cut_sw2 : fc.r2gate;
+    for (int k0 = q; k0 < cs; k0 += 16) {
+      double4 xq[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const int k = k0 + 4 * u;
+        int j = (k0 == q) ? jr[u] : ((k < cs) ? nbr_sw[(size_t)k * NA + i] : i);
+        j = (k < cs) ? j : i;
+        xq[u] = X[j];
+      }
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const int k = k0 + 4 * u;
+        double d0, d1, d2;
+        min_image_fast(xi, xq[u], in, d0, d1, d2);
+        const int tj = type_of(meta_of(xq[u]));
+        const double isg = (tj != 2) ? isgA : isgB;
+        const double r2s = (d0 * d0 + d1 * d1 + d2 * d2) * (isg * isg);
+        if (k < cs && r2s < gate) pm |= 1ull << k;
       }
     }
   }
+  {
+    uint32_t lo = (uint32_t)pm, hi = (uint32_t)(pm >> 32);
+    lo |= __shfl_xor(lo, 1); hi |= __shfl_xor(hi, 1);
+    lo |= __shfl_xor(lo, 2); hi |= __shfl_xor(hi, 2);
+    pm = ((uint64_t)hi << 32) | lo;
+  }
+  const int np = __popcll(pm);
 
-  // Moliere pairs (si.f90:426-446): both atoms must be fired.
+  // ---- phase 2: pair force and leg geometry, entry n of pm on lane n % 4 ----
+  // register slot n / 4 (0 or 1); entries n >= 8 are recomputed in phase 3
+  Leg<T> L0, L1;
+  L0.e0 = L0.e1 = L0.e2 = L0.rinv = L0.E = L0.g = T(0); L0.t = 1;
+  L1 = L0;
+  uint64_t am = 0;  // legs inside the SW range found by this lane
+  auto pair_leg = [&](int k, Leg<T> &L) -> bool {
+    const int j = nbr_sw[(size_t)k * NA + i];
+    const double4 xj = X[j];
+    const int tj = type_of(meta_of(xj));
+    const bool jA = tj != 2;
+    const double isg = jA ? isgA : isgB;
+    double d0, d1, d2;
+    min_image_fast(xi, xj, in, d0, d1, d2);
+    L.t = tj;
+    L.E = T(0); L.g = T(0);
+    bool leg = false;
+    if (!MIXED) {
+      const double r = sqrt(d0 * d0 + d1 * d1 + d2 * d2);
+      const double rinv = 1.0 / r;
+      const double rs = r * isg;
+      // reference pair force: 3-point Lagrange on the derivative table
+      const int t1 = (int)(rs / fc.umax * fc.tabn1) + 1;
+      if (t1 < fc.izero) {
+        const TabEntry te = tab[t1];
+        const double e1 = rs - te.u0, e2 = rs - te.u1, e3 = rs - te.u2;
+        const double df = 0.5 * e2 * e3 * te.t0 - e1 * e3 * te.t1 + 0.5 * e1 * e2 * te.t2;
+        const double fs = -df * (jA ? esgA : esgB) * rinv;
+        fx += fs * d0; fy += fs * d1; fz += fs * d2;
+        if (VIR) addW(d0, d1, d2, 0.5 * fs * d0, 0.5 * fs * d1, 0.5 * fs * d2);
+      }
+      L.e0 = d0 * rinv; L.e1 = d1 * rinv; L.e2 = d2 * rinv;
+      L.rinv = rinv;
+      if (rs < fc.cut_sw) {
+        const double inv_ra = 1.0 / (rs - fc.al);
+        L.E = exp(fc.gamma * inv_ra);
+        L.g = -fc.gamma * inv_ra * inv_ra * isg;
+        if (EN) {
+          const double rs2 = rs * rs;
+          const double ul = 0.5 * (jA ? epsA : epsB) * fc.A * (fc.B / (rs2 * rs2) - 1.0) *
+                            exp(inv_ra);
+          ui += ul;
+          etot += ul;
+        }
+        leg = true;
+      }
+    } else {
+      // FP64 difference in sigma units, everything after in FP32
+      const float r0 = (float)(d0 * isg), r1 = (float)(d1 * isg), r2 = (float)(d2 * isg);
+      const float rr = r0 * r0 + r1 * r1 + r2 * r2;
+      const float rinv = __builtin_amdgcn_rsqf(rr);
+      const float rs = rr * rinv;
+      L.e0 = r0 * rinv; L.e1 = r1 * rinv; L.e2 = r2 * rinv;
+      L.rinv = rinv;
+      if (rs < (float)fc.cut_sw) {
+        const float inv_ra = __builtin_amdgcn_rcpf(rs - (float)fc.al);
+        const float ex = __expf(inv_ra);
+        const float B = (float)fc.B, A = (float)fc.A;
+        const float ir2 = rinv * rinv;
+        const float ir4 = ir2 * ir2;
+        // phi2(r) = A (B r^-4 - 1) e^{1/(r-a)};  force on i = -dphi/dr e
+        const float dphi = A * ex * (-4.0f * B * ir4 * rinv - (B * ir4 - 1.0f) * inv_ra * inv_ra);
+        const float fs = -dphi * rinv * (float)(jA ? esgA : esgB) * (float)(1.0 / fscale);
+        // fs in eps/sigma units of the Si pair: scaled by fscale at the end
+        const float f0 = fs * r0, f1 = fs * r1, f2 = fs * r2;
+        kadd(sx, cx, f0); kadd(sy, cy, f1); kadd(sz, cz, f2);
+        if (VIR) addW(r0, r1, r2, 0.5f * f0, 0.5f * f1, 0.5f * f2);
+        L.E = __expf((float)fc.gamma * inv_ra);
+        L.g = -(float)fc.gamma * inv_ra * inv_ra;
+        if (EN) {
+          const double ul = 0.5 * (jA ? epsA : epsB) * (double)(A * (B * ir4 - 1.0f) * ex);
+          ui += ul;
+          etot += ul;
+        }
+        leg = true;
+      }
+    }
+    return leg;
+  };
+  if (q < np) {
+    const int k = nth_bit(pm, q);
+    if (pair_leg(k, L0)) am |= 1ull << k;
+  }
+  if (q + 4 < np) {
+    const int k = nth_bit(pm, q + 4);
+    if (pair_leg(k, L1)) am |= 1ull << k;
+  }
+  for (int n = q + 8; n < np; n += 4) {  // dense atoms: pair force only here
+    Leg<T> Lx;
+    const int k = nth_bit(pm, n);
+    if (pair_leg(k, Lx)) am |= 1ull << k;
+  }
+  {
+    uint32_t lo = (uint32_t)am, hi = (uint32_t)(am >> 32);
+    lo |= __shfl_xor(lo, 1); hi |= __shfl_xor(hi, 1);
+    lo |= __shfl_xor(lo, 2); hi |= __shfl_xor(hi, 2);
+    am = ((uint64_t)hi << 32) | lo;
+  }
+
+  // ---- Moliere pairs (si.f90:426-446): both atoms must be fired ----------
   if (live) {
     const int ci = min(cnt_ion[i], Wion);
     const bool ifired = !(ti == 3 && orig_of(xi) >= in.nsg + in.impact);
-    for (int k = 0; ifired && k < ci; ++k) {
+    for (int k = q; ifired && k < ci; k += 4) {
       const int j = nbr_ion[(size_t)k * NA + i];
       const double4 xj = X[j];
       const uint32_t mj = meta_of(xj);
@@ -670,7 +813,7 @@ __global__ __launch_bounds__(FIB_WAVE) void k_force_a(
       if (tj == 3 && orig_of(xj) >= in.nsg + in.impact) continue;
       if (mj & META_DEAD) continue;
       double d0, d1, d2;
-      min_image(xi, xj, in, d0, d1, d2);
+      min_image_fast(xi, xj, in, d0, d1, d2);
       const double r2 = d0 * d0 + d1 * d1 + d2 * d2;
       if (!(r2 < fc.rlj2)) continue;
       const double r = sqrt(r2);
@@ -686,15 +829,18 @@ __global__ __launch_bounds__(FIB_WAVE) void k_force_a(
         const double fcm = zz / r * (sa / r + sb);
         const double rinv = 1.0 / r;
         const double f0 = fcm * d0 * rinv, f1 = fcm * d1 * rinv, f2 = fcm * d2 * rinv;
-        addF(f0, f1, f2);
-        if (EN) { const double ul = 0.5 * zz / r * sa; ui += ul; etot += ul; }
-        if (VIR) {
-          W[0] += 0.5 * d0 * f0; W[1] += 0.5 * d0 * f1; W[2] += 0.5 * d0 * f2;
-          W[3] += 0.5 * d1 * f0; W[4] += 0.5 * d1 * f1; W[5] += 0.5 * d1 * f2;
-          W[6] += 0.5 * d2 * f0; W[7] += 0.5 * d2 * f1; W[8] += 0.5 * d2 * f2;
+        if (MIXED) {
+          kadd(sx, cx, (float)(f0 / fscale)); kadd(sy, cy, (float)(f1 / fscale));
+          kadd(sz, cz, (float)(f2 / fscale));
+          if (VIR) addW((T)(d0 / lscale), (T)(d1 / lscale), (T)(d2 / lscale),
+                        (T)(0.5 * f0 / fscale), (T)(0.5 * f1 / fscale), (T)(0.5 * f2 / fscale));
+        } else {
+          fx += f0; fy += f1; fz += f2;
+          if (VIR) addW(d0, d1, d2, 0.5 * f0, 0.5 * f1, 0.5 * f2);
         }
+        if (EN) { const double ul = 0.5 * zz / r * sa; ui += ul; etot += ul; }
       } else {
-        // FP32: lengths in units of aa, force scaled by zz/aa^2
+        // FP32 with lengths in units of aa; F = (zz/aa^2) (1/x)(sa/x + sb) d/(aa x)
         const float x = (float)(r / aa);
         const float e1 = __expf(-(float)fc.molb[0] * x), e2 = __expf(-(float)fc.molb[1] * x),
                     e3 = __expf(-(float)fc.molb[2] * x);
@@ -702,149 +848,176 @@ __global__ __launch_bounds__(FIB_WAVE) void k_force_a(
         const float sa = c0 * e1 + c1 * e2 + c2 * e3;
         const float sb = c0 * (float)fc.molb[0] * e1 + c1 * (float)fc.molb[1] * e2 +
                          c2 * (float)fc.molb[2] * e3;
-        const float ix = 1.0f / x;
-        const float fr = ix * (sa * ix + sb) * ix;  // (zz/aa^2) units, divided by x for d
-        const double scale = zz / (aa * aa) / aa;   // times d (m) gives N
-        const double f0 = (double)(fr * (float)(d0 / aa)) * scale * aa;
-        const double f1 = (double)(fr * (float)(d1 / aa)) * scale * aa;
-        const double f2 = (double)(fr * (float)(d2 / aa)) * scale * aa;
-        addF(f0, f1, f2);
+        const float ix = __builtin_amdgcn_rcpf(x);
+        const float fr = ix * (sa * ix + sb) * ix;
+        const float scl = (float)(zz / (aa * aa) / fscale);
+        const float f0 = fr * (float)(d0 / aa) * scl, f1 = fr * (float)(d1 / aa) * scl,
+                    f2 = fr * (float)(d2 / aa) * scl;
+        kadd(sx, cx, f0); kadd(sy, cy, f1); kadd(sz, cz, f2);
+        if (VIR) addW((T)(d0 / lscale), (T)(d1 / lscale), (T)(d2 / lscale), 0.5f * f0,
+                      0.5f * f1, 0.5f * f2);
         if (EN) { const double ul = 0.5 * zz / aa * (double)(sa * ix); ui += ul; etot += ul; }
-        if (VIR) {
-          W[0] += 0.5 * d0 * f0; W[1] += 0.5 * d0 * f1; W[2] += 0.5 * d0 * f2;
-          W[3] += 0.5 * d1 * f0; W[4] += 0.5 * d1 * f1; W[5] += 0.5 * d1 * f2;
-          W[6] += 0.5 * d2 * f0; W[7] += 0.5 * d2 * f1; W[8] += 0.5 * d2 * f2;
-        }
       }
     }
   }
 
-  // Three-body terms centred on i (si.f90:451-549 restated per vertex).
-  // na == 1 still writes a zero G entry: the neighbour gathers it.
-  if (live && ti < 3 && na >= 1) {
-    const int cs = min(cnt_sw[i], Wsw);
-    // unit factor: FP64 path works in m and N; mixed path in sigma and eps/sigma
-    const double fscale = MIXED ? fc.eps[0][0] * fc.isig[0][0] : 1.0;
-    T fcx = 0, fcy = 0, fcz = 0;
-    if (na <= MAXS) {
-      for (int a = 0; a < na; ++a) {
-        const Slot<T> sa = s_sl[a][lane];
-        const int ta = sa.k >> 8;
-        T Ga[3] = {0, 0, 0};
-        T Ua = 0;
-        for (int b = 0; b < na; ++b) {
-          if (b == a) continue;
-          const Slot<T> sb = s_sl[b][lane];
-          const int tb = sb.k >> 8;
-          const T lep = MIXED ? (T)(fc.leps[ti - 1][ta - 1][tb - 1] / fc.eps[0][0])
-                              : (T)fc.leps[ti - 1][ta - 1][tb - 1];
-          T Fa[3], hh;
-          vterm<T>(lep, sa.e, sa.rinv, sa.E, sa.g, sb.e, sb.E, Fa, hh);
-          Ga[0] += Fa[0]; Ga[1] += Fa[1]; Ga[2] += Fa[2];
-          if (EN) Ua += hh * T(1.0 / 3.0);
-          if (EN) { ui += (double)hh * (1.0 / 6.0) * (MIXED ? fc.eps[0][0] : 1.0);
-                    etot += (double)hh * 0.5 * (MIXED ? fc.eps[0][0] : 1.0); }
-        }
-        fcx -= Ga[0]; fcy -= Ga[1]; fcz -= Ga[2];
-        if (VIR) {
-          // r_ca = x_a - x_c = -e_a / rinv_a
-          const double rl = -1.0 / (double)sa.rinv * (MIXED ? 1.0 / fc.isig[0][0] : 1.0);
-          const double g0 = (double)Ga[0] * fscale, g1 = (double)Ga[1] * fscale,
-                       g2 = (double)Ga[2] * fscale;
-          const double e0 = rl * sa.e[0], e1 = rl * sa.e[1], e2 = rl * sa.e[2];
-          W[0] += e0 * g0; W[1] += e0 * g1; W[2] += e0 * g2;
-          W[3] += e1 * g0; W[4] += e1 * g1; W[5] += e1 * g2;
-          W[6] += e2 * g0; W[7] += e2 * g1; W[8] += e2 * g2;
-        }
-        const int ka = sa.k & 0xff;
-        T4 gv;
-        gv.x = Ga[0]; gv.y = Ga[1]; gv.z = Ga[2];
-        gv.w = EN ? Ua : T(0);
-        G[(size_t)ka * NA + i] = gv;
-      }
-    } else {
-      // more active legs than LDS slots: recompute leg geometry on the fly
-      auto leg = [&](int k, T *e, T &rinv, T &E, T &g, int &tj) {
-        const int j = nbr_sw[(size_t)k * NA + i];
-        const double4 xj = X[j];
-        tj = type_of(meta_of(xj));
-        double d0, d1, d2;
-        min_image(xi, xj, in, d0, d1, d2);
-        const double isg = fc.isig[ti - 1][tj - 1];
-        if (!MIXED) {
-          const double r = sqrt(d0 * d0 + d1 * d1 + d2 * d2);
-          const double ri = 1.0 / r;
-          const double rs = r * isg;
-          const double inv_ra = 1.0 / (rs - fc.al);
-          e[0] = d0 * ri; e[1] = d1 * ri; e[2] = d2 * ri;
-          rinv = ri; E = exp(fc.gamma * inv_ra); g = -fc.gamma * inv_ra * inv_ra * isg;
+  // ---- phase 3: three-body terms centred on i ------------------------------
+  // pm entry n lives on lane n % 4, slot n / 4; entries that are not legs
+  // carry E = 0 and contribute exactly zero
+  int npmax = np;
+  for (int o = 32; o >= 4; o >>= 1) npmax = max(npmax, __shfl_xor(npmax, o));
+  const int na = __popcll(am);
+  if (npmax > 0 && __any(na >= 2)) {
+    T G0[4] = {0, 0, 0, 0}, G1[4] = {0, 0, 0, 0};  // force on own legs + energy share
+    T fcx = 0, fcy = 0, fcz = 0;                   // reaction on the centre
+    T uc = 0;
+    const int nreg = min(npmax, 8);
+    for (int bi = 0; bi < nreg; ++bi) {
+      const int src = gbase | (bi & 3);
+      const bool s0 = bi < 4;
+      Leg<T> Lb;
+      Lb.e0 = __shfl(s0 ? L0.e0 : L1.e0, src);
+      Lb.e1 = __shfl(s0 ? L0.e1 : L1.e1, src);
+      Lb.e2 = __shfl(s0 ? L0.e2 : L1.e2, src);
+      Lb.E = __shfl(s0 ? L0.E : L1.E, src);
+      Lb.t = __shfl(s0 ? L0.t : L1.t, src);
+      if (bi >= np || Lb.E == T(0)) continue;
+      // own entries a = q (slot 0) and q + 4 (slot 1)
+#pragma unroll
+      for (int sa = 0; sa < 2; ++sa) {
+        const int a = q + 4 * sa;
+        const Leg<T> &La = sa == 0 ? L0 : L1;
+        if (a >= np || a == bi || La.E == T(0)) continue;
+        const T lep = (La.t == 1 && Lb.t == 1) ? lp11 : ((La.t == 2 && Lb.t == 2) ? lp22 : lp12);
+        T Fa[3], hh;
+        vterm<T>(lep, La, Lb, Fa, hh);
+        if (sa == 0) {
+          G0[0] += Fa[0]; G0[1] += Fa[1]; G0[2] += Fa[2];
+          if (EN) G0[3] += hh * T(1.0 / 3.0);
         } else {
-          const float r0 = (float)(d0 * isg), r1 = (float)(d1 * isg), r2f = (float)(d2 * isg);
-          const float rs = sqrtf(r0 * r0 + r1 * r1 + r2f * r2f);
-          const float ri = 1.0f / rs;
-          const float inv_ra = 1.0f / (rs - (float)fc.al);
-          e[0] = r0 * ri; e[1] = r1 * ri; e[2] = r2f * ri;
-          rinv = ri; E = __expf((float)fc.gamma * inv_ra); g = -(float)fc.gamma * inv_ra * inv_ra;
+          G1[0] += Fa[0]; G1[1] += Fa[1]; G1[2] += Fa[2];
+          if (EN) G1[3] += hh * T(1.0 / 3.0);
         }
-      };
-      for (int ka = 0; ka < cs; ++ka) {
-        if (!((am >> ka) & 1ull)) continue;
-        T ea[3], ra, Ea, ga;
-        int ta;
-        leg(ka, ea, ra, Ea, ga, ta);
-        T Ga[3] = {0, 0, 0};
-        T Ua = 0;
-        for (int kb = 0; kb < cs; ++kb) {
-          if (kb == ka || !((am >> kb) & 1ull)) continue;
-          T eb[3], rb, Eb, gb;
-          int tb;
-          leg(kb, eb, rb, Eb, gb, tb);
-          const T lep = MIXED ? (T)(fc.leps[ti - 1][ta - 1][tb - 1] / fc.eps[0][0])
-                              : (T)fc.leps[ti - 1][ta - 1][tb - 1];
-          T Fa[3], hh;
-          vterm<T>(lep, ea, ra, Ea, ga, eb, Eb, Fa, hh);
-          Ga[0] += Fa[0]; Ga[1] += Fa[1]; Ga[2] += Fa[2];
-          if (EN) Ua += hh * T(1.0 / 3.0);
-          if (EN) { ui += (double)hh * (1.0 / 6.0) * (MIXED ? fc.eps[0][0] : 1.0);
-                    etot += (double)hh * 0.5 * (MIXED ? fc.eps[0][0] : 1.0); }
-        }
-        fcx -= Ga[0]; fcy -= Ga[1]; fcz -= Ga[2];
-        if (VIR) {
-          const double rl = -1.0 / (double)ra * (MIXED ? 1.0 / fc.isig[0][0] : 1.0);
-          const double g0 = (double)Ga[0] * fscale, g1 = (double)Ga[1] * fscale,
-                       g2 = (double)Ga[2] * fscale;
-          const double e0 = rl * ea[0], e1 = rl * ea[1], e2 = rl * ea[2];
-          W[0] += e0 * g0; W[1] += e0 * g1; W[2] += e0 * g2;
-          W[3] += e1 * g0; W[4] += e1 * g1; W[5] += e1 * g2;
-          W[6] += e2 * g0; W[7] += e2 * g1; W[8] += e2 * g2;
-        }
-        T4 gv;
-        gv.x = Ga[0]; gv.y = Ga[1]; gv.z = Ga[2];
-        gv.w = EN ? Ua : T(0);
-        G[(size_t)ka * NA + i] = gv;
+        fcx -= Fa[0]; fcy -= Fa[1]; fcz -= Fa[2];
+        if (EN) uc += hh;
       }
     }
-    addF((double)fcx * fscale, (double)fcy * fscale, (double)fcz * fscale);
+    // entries beyond the register slots (dense atoms): recompute on the fly
+    auto leg_at = [&](int n, Leg<T> &L) {
+      const int k = nth_bit(pm, n);
+      const int j = nbr_sw[(size_t)k * NA + i];
+      leg_geom<MIXED>(xi, X[j], in, fc, ti, L);
+      if (!((am >> k) & 1ull)) L.E = T(0);
+    };
+    if (npmax > 8) {
+      // terms (a < 8, b >= 8)
+      for (int bi = 8; bi < np; ++bi) {
+        Leg<T> Lb;
+        leg_at(bi, Lb);
+        if (Lb.E == T(0)) continue;
+#pragma unroll
+        for (int sa = 0; sa < 2; ++sa) {
+          const int a = q + 4 * sa;
+          const Leg<T> &La = sa == 0 ? L0 : L1;
+          if (a >= np || La.E == T(0)) continue;
+          const T lep = (La.t == 1 && Lb.t == 1) ? lp11 : ((La.t == 2 && Lb.t == 2) ? lp22 : lp12);
+          T Fa[3], hh;
+          vterm<T>(lep, La, Lb, Fa, hh);
+          if (sa == 0) {
+            G0[0] += Fa[0]; G0[1] += Fa[1]; G0[2] += Fa[2];
+            if (EN) G0[3] += hh * T(1.0 / 3.0);
+          } else {
+            G1[0] += Fa[0]; G1[1] += Fa[1]; G1[2] += Fa[2];
+            if (EN) G1[3] += hh * T(1.0 / 3.0);
+          }
+          fcx -= Fa[0]; fcy -= Fa[1]; fcz -= Fa[2];
+          if (EN) uc += hh;
+        }
+      }
+      // own entries a >= 8: full sum over all b
+      for (int a = q + 8; a < np; a += 4) {
+        Leg<T> La;
+        leg_at(a, La);
+        T Ga[4] = {0, 0, 0, 0};
+        // (a leg whose E underflowed still stores its zero share below)
+        for (int bi = 0; bi < np && La.E != T(0); ++bi) {
+          if (bi == a) continue;
+          Leg<T> Lb;
+          leg_at(bi, Lb);
+          if (Lb.E == T(0)) continue;
+          const T lep = (La.t == 1 && Lb.t == 1) ? lp11 : ((La.t == 2 && Lb.t == 2) ? lp22 : lp12);
+          T Fa[3], hh;
+          vterm<T>(lep, La, Lb, Fa, hh);
+          Ga[0] += Fa[0]; Ga[1] += Fa[1]; Ga[2] += Fa[2];
+          if (EN) Ga[3] += hh * T(1.0 / 3.0);
+          fcx -= Fa[0]; fcy -= Fa[1]; fcz -= Fa[2];
+          if (EN) uc += hh;
+        }
+        const int ka = nth_bit(pm, a);
+        T4 gv;
+        gv.x = Ga[0]; gv.y = Ga[1]; gv.z = Ga[2]; gv.w = Ga[3];
+        G[(size_t)ka * NA + i] = gv;
+        if (VIR) {
+          const T rl = -T(1) / La.rinv;
+          addW(rl * La.e0, rl * La.e1, rl * La.e2, Ga[0], Ga[1], Ga[2]);
+        }
+      }
+    }
+    // store the shares of the register-held legs
+    auto store_leg = [&](int n, const Leg<T> &La, const T (&Gs)[4]) {
+      const int ka = nth_bit(pm, n);
+      T4 gv;
+      gv.x = Gs[0]; gv.y = Gs[1]; gv.z = Gs[2]; gv.w = Gs[3];
+      G[(size_t)ka * NA + i] = gv;
+      if (VIR && La.E != T(0)) {
+        const T rl = -T(1) / La.rinv;
+        addW(rl * La.e0, rl * La.e1, rl * La.e2, Gs[0], Gs[1], Gs[2]);
+      }
+    };
+    if (q < np) store_leg(q, L0, G0);
+    if (q + 4 < np) store_leg(q + 4, L1, G1);
+    if (MIXED) {
+      kadd(sx, cx, (float)fcx); kadd(sy, cy, (float)fcy); kadd(sz, cz, (float)fcz);
+    } else {
+      fx += fcx; fy += fcy; fz += fcz;
+    }
+    if (EN) { ui += (double)uc * (1.0 / 6.0) * escale; etot += (double)uc * 0.5 * escale; }
+  } else {
+    // no three-body terms in this wave: every leg still owes its neighbour a
+    // (zero) G entry -- including legs whose E underflowed to zero
+    T4 gv;
+    gv.x = gv.y = gv.z = gv.w = T(0);
+    for (int n = q; n < np; n += 4) {
+      const int k = nth_bit(pm, n);
+      if ((am >> k) & 1ull) G[(size_t)k * NA + i] = gv;
+    }
   }
 
+  // ---- phase 4: quad reduction, lane 0 of the group stores ----------------
   if (MIXED) {
-    fx = (double)sx + (double)cx;
-    fy = (double)sy + (double)cy;
-    fz = (double)sz + (double)cz;
+    fx = ((double)sx + (double)cx) * fscale;
+    fy = ((double)sy + (double)cy) * fscale;
+    fz = ((double)sz + (double)cz) * fscale;
   }
-  Fpart[i] = fx;
-  Fpart[(size_t)NA + i] = fy;
-  Fpart[2 * (size_t)NA + i] = fz;
-  amask[i] = am;
-  if (EN) upart[i] = ui;
+  fx += __shfl_xor(fx, 1); fy += __shfl_xor(fy, 1); fz += __shfl_xor(fz, 1);
+  fx += __shfl_xor(fx, 2); fy += __shfl_xor(fy, 2); fz += __shfl_xor(fz, 2);
+  if (EN) { ui += __shfl_xor(ui, 1); ui += __shfl_xor(ui, 2); }
+  if (q == 0) {
+    Fpart[i] = fx;
+    Fpart[(size_t)NA + i] = fy;
+    Fpart[2 * (size_t)NA + i] = fz;
+    amask[i] = am;
+    if (EN) upart[i] = ui;
+  }
 
   if (EN || VIR) {
     double v[10];
     v[0] = etot;
+    const double ws = MIXED ? fscale * lscale : 1.0;
 #pragma unroll
-    for (int q = 0; q < 9; ++q) v[1 + q] = W[q];
+    for (int w = 0; w < 9; ++w) v[1 + w] = (double)W[w] * ws;
     const int op[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
-    block_partials<10, FIB_WAVE>(v, op, lb, pstride, partials);
+    block_partials<10, FIB_BLOCK>(v, op, lb, pstride, partials);
   }
 }
 
@@ -865,7 +1038,9 @@ __global__ __launch_bounds__(FIB_BLOCK) void k_force_b(
   const int i = lb * FIB_BLOCK + threadIdx.x;
   const double4 xi = X[i];
   const uint32_t mi = meta_of(xi);
-  const int kin = inst_of(mi);
+  // every block lies inside one instance: make the index wave-uniform so the
+  // instance record is read with scalar loads
+  const int kin = __builtin_amdgcn_readfirstlane(inst_of(mi));
   const DInst &in = I[kin];
   const bool live = !(mi & (META_PAD | META_DEAD));
   double f0 = 0, f1 = 0, f2 = 0, u = 0;
@@ -944,7 +1119,9 @@ __global__ __launch_bounds__(FIB_BLOCK) void k_pos(double4 *__restrict__ X,
   const int i = lb * FIB_BLOCK + threadIdx.x;
   double4 x = X[i];
   const uint32_t mi = meta_of(x);
-  const int kin = inst_of(mi);
+  // every block lies inside one instance: make the index wave-uniform so the
+  // instance record is read with scalar loads
+  const int kin = __builtin_amdgcn_readfirstlane(inst_of(mi));
   const DInst &in = I[kin];
   const bool live = !(mi & (META_PAD | META_DEAD));
   double red[2] = {0.0, -1.0};
@@ -1028,7 +1205,9 @@ __global__ __launch_bounds__(FIB_BLOCK) void k_sums(const double4 *__restrict__ 
   const int i = lb * FIB_BLOCK + threadIdx.x;
   const double4 xi = X[i];
   const uint32_t mi = meta_of(xi);
-  const int kin = inst_of(mi);
+  // every block lies inside one instance: make the index wave-uniform so the
+  // instance record is read with scalar loads
+  const int kin = __builtin_amdgcn_readfirstlane(inst_of(mi));
   const DInst &in = I[kin];
   const bool live = !(mi & (META_PAD | META_DEAD));
   double v[5] = {0, 0, 0, 0, 0};
@@ -1146,8 +1325,20 @@ static void finalize(const DevBufs &d, int nv, int bs, const int *op, const int 
     rs.op[k] = k < nv ? op[k] : 0;
     rs.dst[k] = k < nv ? dst[k] : -1;
   }
-  hipLaunchKernelGGL(k_finalize, dim3(d.ninst), dim3(FIN_T), 0, s, d.I, d.partials, d.pstride,
-                     rs, d.ctl);
+  switch (nv) {
+    case 2:
+      hipLaunchKernelGGL(k_finalize<2>, dim3(d.ninst), dim3(FIN_T), 0, s, d.I, d.partials,
+                         d.pstride, rs, d.ctl);
+      break;
+    case 5:
+      hipLaunchKernelGGL(k_finalize<5>, dim3(d.ninst), dim3(FIN_T), 0, s, d.I, d.partials,
+                         d.pstride, rs, d.ctl);
+      break;
+    default:
+      hipLaunchKernelGGL(k_finalize<10>, dim3(d.ninst), dim3(FIN_T), 0, s, d.I, d.partials,
+                         d.pstride, rs, d.ctl);
+      break;
+  }
 }
 
 static void finalize_thermo(const DevBufs &d, hipStream_t s) {
@@ -1189,11 +1380,16 @@ void launch_nbuild(const DevBufs &d, const FConst &fc, hipStream_t s) {
     hipLaunchKernelGGL(k_nbuild, dim3(d.total_cells), dim3(FIB_WAVE), 0, s, d.X[d.cur], d.I,
                        d.ninst, d.total_cells, d.cstart, d.bcnt, fc.rv2, fc.rv2lj, d.NA, d.Wsw,
                        d.Wion, d.cnt_sw, d.nbr_sw, d.cnt_ion, d.nbr_ion, d.ctl);
-  hipLaunchKernelGGL(k_rev, dim3(d.NA / FIB_BLOCK), dim3(FIB_BLOCK), 0, s, d.NA, d.Wsw,
-                     d.cnt_sw, d.cnt_ion, d.nbr_sw, d.rev, d.pstride, d.partials);
+  hipLaunchKernelGGL(k_maxcnt, dim3(d.NA / FIB_BLOCK), dim3(FIB_BLOCK), 0, s, d.cnt_sw,
+                     d.cnt_ion, d.pstride, d.partials);
   const int op[2] = {1, 1};
   const int dst[2] = {DCTL_IDX(maxcnt_sw), DCTL_IDX(maxcnt_ion)};
   finalize(d, 2, FIB_BLOCK, op, dst, s);
+}
+
+void launch_rev(const DevBufs &d, hipStream_t s) {
+  hipLaunchKernelGGL(k_rev, dim3(d.NA / REV_ATOMS), dim3(FIB_BLOCK), 0, s, d.NA, d.Wsw, d.cnt_sw,
+                     d.nbr_sw, d.rev);
 }
 
 void launch_cellid(const DevBufs &d, int inst, int32_t *dout, hipStream_t s) {
@@ -1204,14 +1400,14 @@ void launch_cellid(const DevBufs &d, int inst, int32_t *dout, hipStream_t s) {
 template <bool MIXED, bool EN, bool VIR>
 static void force_a(const DevBufs &d, const FConst &fc, hipStream_t s) {
   typedef typename Prec<MIXED>::T4 T4;
-  hipLaunchKernelGGL((k_force_a<MIXED, EN, VIR>), dim3(d.NA / FIB_WAVE), dim3(FIB_WAVE), 0, s,
+  hipLaunchKernelGGL((k_force_a<MIXED, EN, VIR>), dim3(d.NA / FA_ATOMS), dim3(FIB_BLOCK), 0, s,
                      d.X[d.cur], d.I, fc, d.tab, d.NA, d.Wsw, d.Wion, d.cnt_sw, d.nbr_sw,
                      d.cnt_ion, d.nbr_ion, (T4 *)d.G, d.amask, d.Fpart, d.upart, d.pstride,
                      d.partials, d.ctl);
   if (EN || VIR) {
     int op[10], dst[10];
     for (int k = 0; k < 10; ++k) { op[k] = 0; dst[k] = DCTL_IDX(utot) + k; }
-    finalize(d, 10, FIB_WAVE, op, dst, s);
+    finalize(d, 10, FA_ATOMS, op, dst, s);
   }
 }
 

@@ -216,6 +216,11 @@ static void derive_constants(fib_handle *h) {
     }
   }
   c.izero = izero;
+  c.cut_sw2 = c.cut_sw * c.cut_sw * (1.0 + 1e-6);
+  {
+    const double ug = std::max(c.umax * (double)(izero - 1) / (double)(nt - 1), c.cut_sw);
+    c.r2gate = (1.001 * ug) * (1.001 * ug);
+  }
   h->tab.assign(nt + 3, TabEntry{0, 0, 0, 0, 0, 0});
   for (int i = 1; i <= nt; ++i) {
     TabEntry &e = h->tab[i];
@@ -548,6 +553,9 @@ static int32_t build_lists(fib_handle *h) {
     }
     if (nonfin) return fail(h, FIB_ERR_NONFINITE, "non-finite position detected while binning");
     if (msw <= d.Wsw && mion <= d.Wion) {
+      tid = h->tim.begin();
+      fibk::launch_rev(d, h->stream);
+      h->tim.end(tid, 1);
       h->have_lists = true;
       h->have_force = false;
       ++h->n_builds;
