@@ -58,6 +58,8 @@ struct FConst {
   int32_t ntable;
   int32_t izero;      // first table index (1-based) whose T and successors are 0
   double mixed_r64;   // mixed precision: Moliere pairs below this use FP64
+  double mixed_xleg;  // mixed precision: three-body legs need r/sigma below
+                      // this, i.e. E = exp(gamma/(r/sigma - a)) > e^-30
 };
 
 // Pair-table entry for the reference's 3-point Lagrange interpolation
@@ -96,6 +98,8 @@ struct DCtl {
 struct RedSpec {
   int32_t nv;
   int32_t bs;            // block size of the producing kernel
+  int32_t nreg;          // 1, or 2: two producers wrote partials of the same
+                         // blocks at partials and partials + FIB_NRED * pstride
   int32_t op[FIB_NRED];
   int32_t dst[FIB_NRED];
 };

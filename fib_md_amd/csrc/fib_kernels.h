@@ -17,7 +17,8 @@ struct StepArgs {
   int32_t frz;               // apply frzsptrdatms in this step
 };
 
-// All device buffers of a handle (sorted order, see fib_device.h).
+// All device buffers of a handle (sorted order, see fib_device.h).  Neighbour
+// lists, reverse slots, leg rows and G are atom-major: row i = [i * W, i * W + W).
 struct DevBufs {
   int NA;                    // total padded atoms
   int ninst;
@@ -32,12 +33,14 @@ struct DevBufs {
   int *inv;                  // [NA] atom_off + orig -> sorted position
   int *bkey, *brank, *bcnt, *cstart, *perm, *bsum;
   int *cnt_sw, *nbr_sw, *cnt_ion, *nbr_ion;
-  uint8_t *rev;
-  void *G;                   // double4 or float4 [Wsw * NA]
-  uint64_t *amask;
-  double *Fpart, *upart;     // [3*NA], [NA]
+  uint8_t *rev;              // [NA * Wsw]: slot of i in its k-th neighbour's list
+  void *LA, *LB;             // leg scratch rows, double4 or float4 [NA * Wsw] each
+  void *G;                   // centre-term force on each leg, T4 [NA * Wsw]
+  uint64_t *amask;           // legs of the last evaluation, by list slot [NA]
+  int *cnt_leg;              // legs per atom of the last evaluation [NA]
+  double *Fpart, *upart;     // own-term force and energy [3*NA], [NA]
   double *Fout, *uout;       // force-only results [3*NA], [NA]
-  double *partials;          // [FIB_NRED * pstride], layout [slot][block]
+  double *partials;          // [2 * FIB_NRED * pstride], layout [region][slot][block]
   int pstride;               // NA/64 + 1
   TabEntry *tab;             // [ntable + 3]
   double *xfer;              // staging [3*NA] for host transfers
@@ -49,15 +52,18 @@ void launch_scan(const DevBufs &d, hipStream_t s);
 void launch_sort(const DevBufs &d, hipStream_t s);  // scatter + fixup + reorder (flips cur)
 // list build + longest-list reduction (DCtl::maxcnt_*)
 void launch_nbuild(const DevBufs &d, const FConst &fc, hipStream_t s);
+void launch_cellid(const DevBufs &d, int inst, int32_t *dout, hipStream_t s);
 // reverse-slot map, after a successful build
 void launch_rev(const DevBufs &d, hipStream_t s);
-void launch_cellid(const DevBufs &d, int inst, int32_t *dout, hipStream_t s);
+// passes 1-2 (k_legs, k_tri): own pair terms and legs, centre-term shares
+// into G, energy and virial reductions
 void launch_force_a(const DevBufs &d, const FConst &fc, bool mixed, bool energy, bool virial,
                     hipStream_t s);
-// mode 0: store forces into Fout; mode 1: velocity update with Fout the
-// persistent force array F (timeint.f90:231-236)
+// pass 3 (k_force3): gather of G (through rev, or by search when rev is
+// stale); mode 0: store forces into Fout; mode 1: velocity update with Fout
+// the persistent force array F (timeint.f90:231-236)
 void launch_force_b(const DevBufs &d, const FConst &fc, bool mixed, bool energy, int mode,
-                    const StepArgs &sa, double *Fout, hipStream_t s);
+                    bool search, const StepArgs &sa, double *Fout, hipStream_t s);
 void launch_pos(const DevBufs &d, const StepArgs &sa, hipStream_t s);
 void launch_sums(const DevBufs &d, const StepArgs &sa, hipStream_t s);
 void launch_import(const DevBufs &d, int inst, const DInst &in, const double *X,

@@ -12,12 +12,12 @@
 //               lane-parallel and survivors are compacted with ballots
 //   k_rev       slot of i inside each neighbour's list (for the force gather)
 // Per force evaluation (replaces si_force, si.f90:384-552):
-//   k_force_a   owner-computes: each lane owns one atom, sums its SW pair and
-//               Moliere forces and every three-body term centred on it in
-//               registers; the share of a centre term acting on neighbour a
-//               is written to G[slot(a)] -- no atomics anywhere
-//   k_force_b   gathers the G entries that neighbours wrote for this atom and
-//               (in MD mode) applies the velocity-Verlet velocity update
+//   k_legs      one thread per atom: SW pair and Moliere forces in registers,
+//               the atom's three-body legs to the leg arrays
+//   k_tri       one lane per leg: the vertex terms centred on the leg's atom,
+//               force on the leg's neighbour summed into G -- no atomics
+//   k_force3    one thread per atom: own terms + gather of the G entries its
+//               neighbours wrote for it; in MD mode also the velocity update
 // Per step (replaces the inline Verlet of onestep, timeint.f90:165-236):
 //   k_pos       thermostat scaling, position update, side freeze, rebox and
 //               the sputter-layer rules in one pass
@@ -157,7 +157,11 @@ __global__ __launch_bounds__(FIN_T) void k_finalize(const DInst *__restrict__ I,
   for (int b = threadIdx.x; b < nblk; b += FIN_T) {
 #pragma unroll
     for (int k = 0; k < NV; ++k) {
-      const double y = partials[(size_t)k * pstride + blk0 + b];
+      double y = partials[(size_t)k * pstride + blk0 + b];
+      if (rs.nreg > 1) {
+        const double y2 = partials[(size_t)(FIB_NRED + k) * pstride + blk0 + b];
+        y = rs.op[k] ? fmax(y, y2) : y + y2;
+      }
       x[k] = rs.op[k] ? fmax(x[k], y) : x[k] + y;
     }
   }
@@ -443,11 +447,11 @@ __global__ __launch_bounds__(FIB_WAVE) void k_nbuild(
         uint64_t below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
         if (sw) {
           int pos = nsw + __popcll(msw & below);
-          if (pos < Wsw) nbr_sw[(size_t)pos * NA + i] = j;
+          if (pos < Wsw) nbr_sw[(size_t)i * Wsw + pos] = j;
         }
         if (io) {
           int pos = nion + __popcll(mio & below);
-          if (pos < Wion) nbr_ion[(size_t)pos * NA + i] = j;
+          if (pos < Wion) nbr_ion[(size_t)i * Wion + pos] = j;
         }
         nsw += __popcll(msw);
         nion += __popcll(mio);
@@ -469,30 +473,37 @@ __global__ void k_clear_counts(int NA, int *__restrict__ cnt_sw, int *__restrict
   cnt_ion[p] = 0;
 }
 
-// Reverse slots: rev[k][i] = slot of i in the list of its k-th neighbour.
-// A block handles 16 atoms with one thread per (slot, atom): the 16 atoms of
-// one slot row are contiguous, and each thread scans its neighbour's list
-// eight entries at a time so the loads overlap.
-#define REV_ATOMS 16
-__global__ __launch_bounds__(FIB_BLOCK) void k_rev(int NA, int Wsw, const int *__restrict__ cnt_sw,
+// Reverse slots: rev[i][k] = slot of i inside the list of its k-th
+// neighbour j.  One thread per (atom, slot); lists are atom-major, so j's
+// list is one contiguous row read as int4.
+__global__ __launch_bounds__(FIB_BLOCK) void k_rev(int NA, int wshift, const int *__restrict__ cnt_sw,
                                                    const int *__restrict__ nbr_sw,
                                                    uint8_t *__restrict__ rev) {
-  const int i = blockIdx.x * REV_ATOMS + (threadIdx.x % REV_ATOMS);
+  const int Wsw = 1 << wshift;
+  const size_t t = (size_t)blockIdx.x * FIB_BLOCK + threadIdx.x;
+  const int i = (int)(t >> wshift), k = (int)(t & (size_t)(Wsw - 1));
+  if (i >= NA) return;
   const int c = min(cnt_sw[i], Wsw);
-  for (int k = threadIdx.x / REV_ATOMS; k < c; k += FIB_BLOCK / REV_ATOMS) {
-    const int j = nbr_sw[(size_t)k * NA + i];
-    const int cj = min(cnt_sw[j], Wsw);
-    int r = 255;
-    for (int q0 = 0; q0 < cj && r == 255; q0 += 8) {
-      int e[8];
+  if (k >= c) return;
+  const int j = nbr_sw[t];
+  const int cj = min(cnt_sw[j], Wsw);
+  const int4 *rj = reinterpret_cast<const int4 *>(nbr_sw + (size_t)j * Wsw);
+  int r = 255;
+  // the first 32 slots in one batch of loads, the rest (Wsw = 64) after
+  for (int q0 = 0; q0 < cj && r == 255; q0 += 32) {
+    int4 e[8];
 #pragma unroll
-      for (int u = 0; u < 8; ++u) e[u] = (q0 + u < cj) ? nbr_sw[(size_t)(q0 + u) * NA + j] : -1;
+    for (int u = 0; u < 8; ++u) e[u] = rj[(q0 >> 2) + u];
 #pragma unroll
-      for (int u = 7; u >= 0; --u)
-        if (e[u] == i) r = q0 + u;
+    for (int u = 7; u >= 0; --u) {
+      const int q = q0 + 4 * u;
+      if (q + 3 < cj && e[u].w == i) r = q + 3;
+      if (q + 2 < cj && e[u].z == i) r = q + 2;
+      if (q + 1 < cj && e[u].y == i) r = q + 1;
+      if (q < cj && e[u].x == i) r = q;
     }
-    rev[(size_t)k * NA + i] = (uint8_t)r;
   }
+  rev[t] = (uint8_t)r;
 }
 
 // Longest SW / ion lists of the build (overflow check).
@@ -506,19 +517,31 @@ __global__ __launch_bounds__(FIB_BLOCK) void k_maxcnt(const int *__restrict__ cn
 }
 
 // ---------------------------------------------------------------------------
-// force phase A: four lanes per atom
+// Force evaluation in three passes, no atomics.
 //
-// A group of four lanes owns one atom.  Phase 1: the lanes split the atom's
-// buffered SW list (slot k goes to lane k % 4), compute the reference pair
-// force and flag the legs inside the SW range; the flags are OR-ed over the
-// quad into the atom's active mask.  Phase 2: active leg L (in slot order) is
-// owned by lane L % 4, register slot L / 4; the lane computes its geometry
-// (unit vector, 1/r, E = exp(gamma/(r-a)), d ln E/dr).  Phase 3: every lane
-// loops over the legs b of its atom, fetches them with wavefront shuffles and
-// accumulates the three-body terms centred on the atom that act on its own
-// legs a; the share acting on neighbour a goes to G[slot(a)], the reaction on
-// the centre stays in registers.  Phase 4: quad reduction, lane 0 stores.
-// No LDS, no atomics; legs beyond 8 per atom are recomputed on the fly.
+// k_legs (one thread per atom): the atom's buffered SW list is scanned once.
+// Pairs inside the pair-force range add the SW pair force (and the ion list
+// the Moliere pairs) to the atom in registers.  Pairs inside the three-body
+// range are the atom's "legs": unit vector, E = exp(gamma/(r/sigma - a)),
+// d ln E/dr, 1/r and the neighbour's index go to the leg arrays, slot-major
+// by compact leg index n ([n * NA + i]), legs in list-slot order.
+//
+// k_tri (one lane per leg): every vertex term centred on atom i,
+// h(a, i, b) = leps E(r_a) E(r_b) (cos + 1/3)^2, is a pair of i's legs.  The
+// lane owning leg a sums the force on a's atom over all partners b into
+// G[n(a)][i] (w: a's third of the energy).  A wavefront takes the legs of 64
+// atoms in turn (a prefix sum over the atoms' leg counts in LDS maps lanes to
+// legs), so atoms with many legs do not stall the others.
+//
+// k_force2 (one thread per atom): F_i = own pair/Moliere force - sum of its
+// own G row (the reaction on the centre) + the G entries its neighbours wrote
+// for it.  Neighbour c's entry for i is the leg of c in the list slot r that
+// holds i; its compact index is the popcount of c's leg mask below r.  r
+// comes from rev (computed once per list build) or, for a single evaluation
+// after a build, from scanning c's list row.  This
+// turns the reference's scatter F(i) += fi, F(j) += fj, F(k) += fk
+// (si.f90:544-546) into a gather.  In MD mode the same pass applies the
+// velocity update of onestep (timeint.f90:231-236).
 template <bool MIXED>
 struct Prec;
 template <>
@@ -534,16 +557,45 @@ struct Prec<true> {
 
 template <typename T>
 struct Leg {
-  T e0, e1, e2;  // unit vector from neighbour to centre
-  T rinv;        // 1/r (FP64: m^-1; mixed: sigma^-1)
+  T e0, e1, e2;  // unit vector from the neighbour to the centre
+  T rinv;        // 1/r (FP64: m^-1; mixed: sigma_Si^-1)
   T E;           // exp(gamma/(r/sigma - a))
-  T g;           // d ln E / dr
-  int t;         // species of the neighbour
+  T g;           // d ln E / dr (same length unit as rinv)
+  uint32_t ks;   // neighbour index | (species == Ge) << 31
 };
 
-// One ordered three-body term: centre c, legs a and b.  Returns the force on
-// a (F_a = -dh/dx_a) and h.  Derivation: h = leps E(ra) E(rb) (cos+1/3)^2,
-// d cos/dx_a = -(e_b - cos e_a)/r_a, d r_a/dx_a = -e_a.
+__device__ __forceinline__ double ks_pack(double, uint32_t ks) {
+  return __longlong_as_double((long long)ks);
+}
+__device__ __forceinline__ float ks_pack(float, uint32_t ks) { return __uint_as_float(ks); }
+__device__ __forceinline__ uint32_t ks_unpack(double v) {
+  return (uint32_t)__double_as_longlong(v);
+}
+__device__ __forceinline__ uint32_t ks_unpack(float v) { return __float_as_uint(v); }
+__device__ __forceinline__ int ks_index(uint32_t ks) { return (int)(ks & 0x7fffffffu); }
+__device__ __forceinline__ int ks_ge(uint32_t ks) { return (int)(ks >> 31); }
+
+// Leg scratch: two T4 per leg, slot-major by compact leg index n
+// ([n * NA + i]), so a wave's threads touching their n-th legs coalesce.
+template <typename T, typename T4>
+__device__ __forceinline__ void store_leg(T4 *__restrict__ LA, T4 *__restrict__ LB, size_t at,
+                                          const Leg<T> &L) {
+  T4 a, b;
+  a.x = L.e0; a.y = L.e1; a.z = L.e2; a.w = L.E;
+  b.x = L.g; b.y = L.rinv; b.z = ks_pack(T(0), L.ks); b.w = T(0);
+  LA[at] = a;
+  LB[at] = b;
+}
+template <typename T, typename T4>
+__device__ __forceinline__ void load_leg(const T4 *LA, const T4 *LB, size_t at, Leg<T> &L) {
+  const T4 a = LA[at], b = LB[at];
+  L.e0 = a.x; L.e1 = a.y; L.e2 = a.z; L.E = a.w;
+  L.g = b.x; L.rinv = b.y; L.ks = ks_unpack(b.z);
+}
+
+// Force on leg a of one vertex term (centre c, legs a and b), and h.
+// h = leps E(ra) E(rb) (cos+1/3)^2 with d cos/dx_a = -(e_b - cos e_a)/r_a and
+// d r_a/dx_a = -e_a, so F_a = -dh/dx_a = h g_a e_a + 2 leps Ea Eb D (e_b - cos e_a)/r_a.
 template <typename T>
 __device__ __forceinline__ void vterm(T leps, const Leg<T> &a, const Leg<T> &b, T *Fa, T &hh) {
   const T cs = a.e0 * b.e0 + a.e1 * b.e1 + a.e2 * b.e2;
@@ -557,49 +609,6 @@ __device__ __forceinline__ void vterm(T leps, const Leg<T> &a, const Leg<T> &b, 
   Fa[2] = c1 * a.e2 + c2 * (b.e2 - cs * a.e2);
 }
 
-template <bool MIXED>
-__device__ __forceinline__ void leg_geom(const double4 &xi, const double4 &xj, const DInst &in,
-                                         const FConst &fc, int ti,
-                                         Leg<typename Prec<MIXED>::T> &L) {
-  const int tj = type_of(meta_of(xj));
-  double d0, d1, d2;
-  min_image(xi, xj, in, d0, d1, d2);
-  const double isg = fc.isig[(ti == 2) ? 1 : 0][(tj == 2) ? 1 : 0];
-  L.t = tj;
-  if (!MIXED) {
-    const double r = sqrt(d0 * d0 + d1 * d1 + d2 * d2);
-    const double ri = 1.0 / r;
-    const double rs = r * isg;
-    const double inv_ra = 1.0 / (rs - fc.al);
-    L.e0 = d0 * ri; L.e1 = d1 * ri; L.e2 = d2 * ri;
-    L.rinv = ri;
-    L.E = exp(fc.gamma * inv_ra);
-    L.g = -fc.gamma * inv_ra * inv_ra * isg;
-  } else {
-    const float r0 = (float)(d0 * isg), r1 = (float)(d1 * isg), r2 = (float)(d2 * isg);
-    const float rs = sqrtf(r0 * r0 + r1 * r1 + r2 * r2);
-    const float ri = 1.0f / rs;
-    const float inv_ra = 1.0f / (rs - (float)fc.al);
-    L.e0 = r0 * ri; L.e1 = r1 * ri; L.e2 = r2 * ri;
-    L.rinv = ri;
-    L.E = __expf((float)fc.gamma * inv_ra);
-    L.g = -(float)fc.gamma * inv_ra * inv_ra;
-  }
-}
-
-// Slot of the n-th set bit of m (n is small).
-__device__ __forceinline__ int nth_bit(uint64_t m, int n) {
-  for (int r = 0; r < n; ++r) m &= m - 1;
-  return __ffsll((unsigned long long)m) - 1;
-}
-
-template <typename T>
-__device__ __forceinline__ T shfl_any(T v, int src) {
-  return __shfl(v, src);
-}
-
-#define FA_ATOMS (FIB_BLOCK / 4)  // atoms per block
-
 // Minimum image with v_rndne (ties to even).  Neighbour pairs are far from
 // the half-box tie, so this equals the reference's NINT image there.
 __device__ __forceinline__ void min_image_fast(const double4 &a, const double4 &b,
@@ -611,23 +620,41 @@ __device__ __forceinline__ void min_image_fast(const double4 &a, const double4 &
   x2 = d2 - rint(d2 * in.ilb[2]) * in.lb[2];
 }
 
+// Register accumulator of one force vector: FP64, or FP32 with a TwoSum
+// compensation term (mixed).
+template <bool MIXED>
+struct FAcc;
+template <>
+struct FAcc<false> {
+  double x = 0, y = 0, z = 0;
+  __device__ void add(double a, double b, double c) { x += a; y += b; z += c; }
+  __device__ void get(double s, double &a, double &b, double &c) const {
+    a = x * s; b = y * s; c = z * s;
+  }
+};
+template <>
+struct FAcc<true> {
+  float x = 0, y = 0, z = 0, cx = 0, cy = 0, cz = 0;
+  __device__ void add(float a, float b, float c) { kadd(x, cx, a); kadd(y, cy, b); kadd(z, cz, c); }
+  __device__ void get(double s, double &a, double &b, double &c) const {
+    a = ((double)x + (double)cx) * s;
+    b = ((double)y + (double)cy) * s;
+    c = ((double)z + (double)cz) * s;
+  }
+};
+
 template <bool MIXED, bool EN, bool VIR>
-__global__ __launch_bounds__(FIB_BLOCK) void k_force_a(
+__global__ __launch_bounds__(FIB_BLOCK) void k_legs(
     const double4 *__restrict__ X, const DInst *__restrict__ I, const FConst fc,
     const TabEntry *__restrict__ tab, int NA, int Wsw, int Wion, const int *__restrict__ cnt_sw,
     const int *__restrict__ nbr_sw, const int *__restrict__ cnt_ion,
-    const int *__restrict__ nbr_ion, typename Prec<MIXED>::T4 *__restrict__ G,
-    uint64_t *__restrict__ amask, double *__restrict__ Fpart, double *__restrict__ upart,
-    int pstride, double *__restrict__ partials, DCtl *__restrict__ ctl) {
+    const int *__restrict__ nbr_ion, typename Prec<MIXED>::T4 *LA, typename Prec<MIXED>::T4 *LB,
+    int *__restrict__ cnt_leg, uint64_t *__restrict__ amask,
+    double *__restrict__ Fpart, double *__restrict__ upart, int pstride,
+    double *__restrict__ partials) {
   typedef typename Prec<MIXED>::T T;
-  typedef typename Prec<MIXED>::T4 T4;
-
-  const int nb = gridDim.x;
-  const int lb = xcd_remap(blockIdx.x, nb);
-  const int lane = threadIdx.x % FIB_WAVE;
-  const int q = lane & 3;
-  const int gbase = lane & ~3;
-  const int i = lb * FA_ATOMS + (int)(threadIdx.x >> 2);
+  const int lb = xcd_remap(blockIdx.x, gridDim.x);
+  const int i = lb * FIB_BLOCK + threadIdx.x;
   const double4 xi = X[i];
   const uint32_t mi = meta_of(xi);
   // every block lies inside one instance: scalar loads of the instance record
@@ -640,18 +667,12 @@ __global__ __launch_bounds__(FIB_BLOCK) void k_force_a(
   const double isgA = fc.isig[tr][0], isgB = fc.isig[tr][1];
   const double esgA = fc.epssig[tr][0], esgB = fc.epssig[tr][1];
   const double epsA = fc.eps[tr][0], epsB = fc.eps[tr][1];
-  // leps in the unit of the three-body pass (J, or eps_Si in mixed)
-  const double lu = MIXED ? 1.0 / fc.eps[0][0] : 1.0;
-  const T lp11 = (T)(fc.leps[tr][0][0] * lu), lp12 = (T)(fc.leps[tr][0][1] * lu),
-          lp22 = (T)(fc.leps[tr][1][1] * lu);
-  // unit factors: FP64 works in m and N; mixed in sigma and eps/sigma
-  const double fscale = MIXED ? fc.eps[0][0] * fc.isig[0][0] : 1.0;
-  const double escale = MIXED ? fc.eps[0][0] : 1.0;
-  const double lscale = MIXED ? 1.0 / fc.isig[0][0] : 1.0;
+  const double isgS = fc.isig[0][0];
+  const double fscale = MIXED ? fc.eps[0][0] * isgS : 1.0;
+  const double lscale = MIXED ? 1.0 / isgS : 1.0;
+  const size_t row = (size_t)i * Wsw;
 
-  // force accumulators: FP64, or FP32 value + compensation (mixed)
-  double fx = 0, fy = 0, fz = 0;
-  float sx = 0, sy = 0, sz = 0, cx = 0, cy = 0, cz = 0;
+  FAcc<MIXED> f;
   double ui = 0, etot = 0;
   T W[9];
 #pragma unroll
@@ -662,151 +683,119 @@ __global__ __launch_bounds__(FIB_BLOCK) void k_force_a(
     W[6] += d2 * f0; W[7] += d2 * f1; W[8] += d2 * f2;
   };
 
-  // ---- phase 1: cheap distance test of every buffered SW slot -------------
-  // lane q tests slots q, q+4, q+8, ...; pm collects the pairs that carry a
-  // pair force (FP64: table range; mixed: SW range)
-  uint64_t pm = 0;
-  int cs = 0;
-  if (sw_atom) {
-    int jr[4];
-#pragma unroll
-    for (int u = 0; u < 4; ++u) jr[u] = nbr_sw[(size_t)(q + 4 * u) * NA + i];
-    cs = min(cnt_sw[i], Wsw);
-    const double gate = MIXED ? fc.cut_sw2 : fc.r2gate;
-    for (int k0 = q; k0 < cs; k0 += 16) {
-      double4 xq[4];
-#pragma unroll
-      for (int u = 0; u < 4; ++u) {
-        const int k = k0 + 4 * u;
-        int j = (k0 == q) ? jr[u] : ((k < cs) ? nbr_sw[(size_t)k * NA + i] : i);
-        j = (k < cs) ? j : i;
-        xq[u] = X[j];
-      }
-#pragma unroll
-      for (int u = 0; u < 4; ++u) {
-        const int k = k0 + 4 * u;
-        double d0, d1, d2;
-        min_image_fast(xi, xq[u], in, d0, d1, d2);
-        const int tj = type_of(meta_of(xq[u]));
-        const double isg = (tj != 2) ? isgA : isgB;
-        const double r2s = (d0 * d0 + d1 * d1 + d2 * d2) * (isg * isg);
-        if (k < cs && r2s < gate) pm |= 1ull << k;
-      }
-    }
-  }
-  {
-    uint32_t lo = (uint32_t)pm, hi = (uint32_t)(pm >> 32);
-    lo |= __shfl_xor(lo, 1); hi |= __shfl_xor(hi, 1);
-    lo |= __shfl_xor(lo, 2); hi |= __shfl_xor(hi, 2);
-    pm = ((uint64_t)hi << 32) | lo;
-  }
-  const int np = __popcll(pm);
-
-  // ---- phase 2: pair force and leg geometry, entry n of pm on lane n % 4 ----
-  // register slot n / 4 (0 or 1); entries n >= 8 are recomputed in phase 3
-  Leg<T> L0, L1;
-  L0.e0 = L0.e1 = L0.e2 = L0.rinv = L0.E = L0.g = T(0); L0.t = 1;
-  L1 = L0;
-  uint64_t am = 0;  // legs inside the SW range found by this lane
-  auto pair_leg = [&](int k, Leg<T> &L) -> bool {
-    const int j = nbr_sw[(size_t)k * NA + i];
-    const double4 xj = X[j];
+  int na = 0;
+  uint64_t am = 0;
+  // one buffered SW pair in list slot k: pair force, pair energy, leg
+  auto pair = [&](int k, int jn, const double4 &xj) {
     const int tj = type_of(meta_of(xj));
     const bool jA = tj != 2;
     const double isg = jA ? isgA : isgB;
     double d0, d1, d2;
     min_image_fast(xi, xj, in, d0, d1, d2);
-    L.t = tj;
-    L.E = T(0); L.g = T(0);
-    bool leg = false;
+    Leg<T> L;
+    L.ks = (uint32_t)jn | (jA ? 0u : 0x80000000u);
     if (!MIXED) {
-      const double r = sqrt(d0 * d0 + d1 * d1 + d2 * d2);
+      const double rr = d0 * d0 + d1 * d1 + d2 * d2;
+      if (!(rr * (isg * isg) < fc.r2gate)) return;
+      const double r = sqrt(rr);
       const double rinv = 1.0 / r;
       const double rs = r * isg;
       // reference pair force: 3-point Lagrange on the derivative table
+      // (si.f90:416-424, 554-573)
       const int t1 = (int)(rs / fc.umax * fc.tabn1) + 1;
       if (t1 < fc.izero) {
         const TabEntry te = tab[t1];
         const double e1 = rs - te.u0, e2 = rs - te.u1, e3 = rs - te.u2;
         const double df = 0.5 * e2 * e3 * te.t0 - e1 * e3 * te.t1 + 0.5 * e1 * e2 * te.t2;
         const double fs = -df * (jA ? esgA : esgB) * rinv;
-        fx += fs * d0; fy += fs * d1; fz += fs * d2;
+        f.add(fs * d0, fs * d1, fs * d2);
         if (VIR) addW(d0, d1, d2, 0.5 * fs * d0, 0.5 * fs * d1, 0.5 * fs * d2);
       }
-      L.e0 = d0 * rinv; L.e1 = d1 * rinv; L.e2 = d2 * rinv;
-      L.rinv = rinv;
       if (rs < fc.cut_sw) {
         const double inv_ra = 1.0 / (rs - fc.al);
+        L.e0 = d0 * rinv; L.e1 = d1 * rinv; L.e2 = d2 * rinv;
+        L.rinv = rinv;
         L.E = exp(fc.gamma * inv_ra);
         L.g = -fc.gamma * inv_ra * inv_ra * isg;
         if (EN) {
           const double rs2 = rs * rs;
-          const double ul = 0.5 * (jA ? epsA : epsB) * fc.A * (fc.B / (rs2 * rs2) - 1.0) *
-                            exp(inv_ra);
+          const double ul =
+              0.5 * (jA ? epsA : epsB) * fc.A * (fc.B / (rs2 * rs2) - 1.0) * exp(inv_ra);
           ui += ul;
           etot += ul;
         }
-        leg = true;
+        store_leg(LA, LB, (size_t)na * NA + i, L);
+        am |= 1ull << k;
+        ++na;
       }
     } else {
-      // FP64 difference in sigma units, everything after in FP32
+      // FP64 difference, everything after in FP32 (pair units: sigma_ij)
       const float r0 = (float)(d0 * isg), r1 = (float)(d1 * isg), r2 = (float)(d2 * isg);
       const float rr = r0 * r0 + r1 * r1 + r2 * r2;
+      if (!(rr < (float)fc.cut_sw2)) return;
       const float rinv = __builtin_amdgcn_rsqf(rr);
       const float rs = rr * rinv;
-      L.e0 = r0 * rinv; L.e1 = r1 * rinv; L.e2 = r2 * rinv;
-      L.rinv = rinv;
-      if (rs < (float)fc.cut_sw) {
-        const float inv_ra = __builtin_amdgcn_rcpf(rs - (float)fc.al);
-        const float ex = __expf(inv_ra);
-        const float B = (float)fc.B, A = (float)fc.A;
-        const float ir2 = rinv * rinv;
-        const float ir4 = ir2 * ir2;
-        // phi2(r) = A (B r^-4 - 1) e^{1/(r-a)};  force on i = -dphi/dr e
-        const float dphi = A * ex * (-4.0f * B * ir4 * rinv - (B * ir4 - 1.0f) * inv_ra * inv_ra);
-        const float fs = -dphi * rinv * (float)(jA ? esgA : esgB) * (float)(1.0 / fscale);
-        // fs in eps/sigma units of the Si pair: scaled by fscale at the end
-        const float f0 = fs * r0, f1 = fs * r1, f2 = fs * r2;
-        kadd(sx, cx, f0); kadd(sy, cy, f1); kadd(sz, cz, f2);
-        if (VIR) addW(r0, r1, r2, 0.5f * f0, 0.5f * f1, 0.5f * f2);
+      if (!(rs < (float)fc.cut_sw)) return;
+      const float inv_ra = __builtin_amdgcn_rcpf(rs - (float)fc.al);
+      const float ex = __expf(inv_ra);
+      const float B = (float)fc.B, A = (float)fc.A;
+      const float ir2 = rinv * rinv;
+      const float ir4 = ir2 * ir2;
+      // phi2(r) = A (B r^-4 - 1) e^{1/(r-a)};  force on i = -dphi/dr e
+      const float dphi = A * ex * (-4.0f * B * ir4 * rinv - (B * ir4 - 1.0f) * inv_ra * inv_ra);
+      const float fs = -dphi * rinv * (float)((jA ? esgA : esgB) / fscale);
+      const float f0 = fs * r0, f1 = fs * r1, f2 = fs * r2;
+      f.add(f0, f1, f2);
+      const float rho = (float)(isg / isgS);  // 1 for Si-Si
+      if (VIR) {
+        const float q = 1.0f / rho;
+        addW(r0 * q, r1 * q, r2 * q, 0.5f * f0, 0.5f * f1, 0.5f * f2);
+      }
+      if (EN) {
+        const double ul = 0.5 * (jA ? epsA : epsB) * (double)(A * (B * ir4 - 1.0f) * ex);
+        ui += ul;
+        etot += ul;
+      }
+      // legs with E = exp(gamma/(r-a)) < e^-30 are left out of the
+      // three-body sum: each of their terms is below 1e-9 eps/sigma
+      if (rs < (float)fc.mixed_xleg) {
+        L.e0 = r0 * rinv; L.e1 = r1 * rinv; L.e2 = r2 * rinv;
+        L.rinv = rinv * rho;
         L.E = __expf((float)fc.gamma * inv_ra);
-        L.g = -(float)fc.gamma * inv_ra * inv_ra;
-        if (EN) {
-          const double ul = 0.5 * (jA ? epsA : epsB) * (double)(A * (B * ir4 - 1.0f) * ex);
-          ui += ul;
-          etot += ul;
-        }
-        leg = true;
+        L.g = -(float)fc.gamma * inv_ra * inv_ra * rho;
+        store_leg(LA, LB, (size_t)na * NA + i, L);
+        am |= 1ull << k;
+        ++na;
       }
     }
-    return leg;
   };
-  if (q < np) {
-    const int k = nth_bit(pm, q);
-    if (pair_leg(k, L0)) am |= 1ull << k;
+
+  if (sw_atom) {
+    const int cs = min(cnt_sw[i], Wsw);
+    const int4 *r4 = reinterpret_cast<const int4 *>(nbr_sw + row);
+    for (int k0 = 0; k0 < cs; k0 += 4) {
+      const int4 q = r4[k0 >> 2];
+      const int jq[4] = {q.x, (k0 + 1 < cs) ? q.y : i, (k0 + 2 < cs) ? q.z : i,
+                         (k0 + 3 < cs) ? q.w : i};
+      double4 xq[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) xq[u] = X[jq[u]];
+#pragma unroll
+      for (int u = 0; u < 4; ++u)
+        if (k0 + u < cs) pair(k0 + u, jq[u], xq[u]);
+    }
   }
-  if (q + 4 < np) {
-    const int k = nth_bit(pm, q + 4);
-    if (pair_leg(k, L1)) am |= 1ull << k;
-  }
-  for (int n = q + 8; n < np; n += 4) {  // dense atoms: pair force only here
-    Leg<T> Lx;
-    const int k = nth_bit(pm, n);
-    if (pair_leg(k, Lx)) am |= 1ull << k;
-  }
-  {
-    uint32_t lo = (uint32_t)am, hi = (uint32_t)(am >> 32);
-    lo |= __shfl_xor(lo, 1); hi |= __shfl_xor(hi, 1);
-    lo |= __shfl_xor(lo, 2); hi |= __shfl_xor(hi, 2);
-    am = ((uint64_t)hi << 32) | lo;
-  }
+
+  cnt_leg[i] = na;
+  amask[i] = am;
 
   // ---- Moliere pairs (si.f90:426-446): both atoms must be fired ----------
   if (live) {
     const int ci = min(cnt_ion[i], Wion);
     const bool ifired = !(ti == 3 && orig_of(xi) >= in.nsg + in.impact);
-    for (int k = q; ifired && k < ci; k += 4) {
-      const int j = nbr_ion[(size_t)k * NA + i];
+    const int *ri = nbr_ion + (size_t)i * Wion;
+    for (int k = 0; ifired && k < ci; ++k) {
+      const int j = ri[k];
       const double4 xj = X[j];
       const uint32_t mj = meta_of(xj);
       const int tj = type_of(mj);
@@ -829,15 +818,9 @@ __global__ __launch_bounds__(FIB_BLOCK) void k_force_a(
         const double fcm = zz / r * (sa / r + sb);
         const double rinv = 1.0 / r;
         const double f0 = fcm * d0 * rinv, f1 = fcm * d1 * rinv, f2 = fcm * d2 * rinv;
-        if (MIXED) {
-          kadd(sx, cx, (float)(f0 / fscale)); kadd(sy, cy, (float)(f1 / fscale));
-          kadd(sz, cz, (float)(f2 / fscale));
-          if (VIR) addW((T)(d0 / lscale), (T)(d1 / lscale), (T)(d2 / lscale),
-                        (T)(0.5 * f0 / fscale), (T)(0.5 * f1 / fscale), (T)(0.5 * f2 / fscale));
-        } else {
-          fx += f0; fy += f1; fz += f2;
-          if (VIR) addW(d0, d1, d2, 0.5 * f0, 0.5 * f1, 0.5 * f2);
-        }
+        f.add((T)(f0 / fscale), (T)(f1 / fscale), (T)(f2 / fscale));
+        if (VIR) addW((T)(d0 / lscale), (T)(d1 / lscale), (T)(d2 / lscale),
+                      (T)(0.5 * f0 / fscale), (T)(0.5 * f1 / fscale), (T)(0.5 * f2 / fscale));
         if (EN) { const double ul = 0.5 * zz / r * sa; ui += ul; etot += ul; }
       } else {
         // FP32 with lengths in units of aa; F = (zz/aa^2) (1/x)(sa/x + sb) d/(aa x)
@@ -853,163 +836,20 @@ __global__ __launch_bounds__(FIB_BLOCK) void k_force_a(
         const float scl = (float)(zz / (aa * aa) / fscale);
         const float f0 = fr * (float)(d0 / aa) * scl, f1 = fr * (float)(d1 / aa) * scl,
                     f2 = fr * (float)(d2 / aa) * scl;
-        kadd(sx, cx, f0); kadd(sy, cy, f1); kadd(sz, cz, f2);
-        if (VIR) addW((T)(d0 / lscale), (T)(d1 / lscale), (T)(d2 / lscale), 0.5f * f0,
-                      0.5f * f1, 0.5f * f2);
+        f.add((T)f0, (T)f1, (T)f2);
+        if (VIR) addW((T)(d0 / lscale), (T)(d1 / lscale), (T)(d2 / lscale), (T)(0.5f * f0),
+                      (T)(0.5f * f1), (T)(0.5f * f2));
         if (EN) { const double ul = 0.5 * zz / aa * (double)(sa * ix); ui += ul; etot += ul; }
       }
     }
   }
 
-  // ---- phase 3: three-body terms centred on i ------------------------------
-  // pm entry n lives on lane n % 4, slot n / 4; entries that are not legs
-  // carry E = 0 and contribute exactly zero
-  int npmax = np;
-  for (int o = 32; o >= 4; o >>= 1) npmax = max(npmax, __shfl_xor(npmax, o));
-  const int na = __popcll(am);
-  if (npmax > 0 && __any(na >= 2)) {
-    T G0[4] = {0, 0, 0, 0}, G1[4] = {0, 0, 0, 0};  // force on own legs + energy share
-    T fcx = 0, fcy = 0, fcz = 0;                   // reaction on the centre
-    T uc = 0;
-    const int nreg = min(npmax, 8);
-    for (int bi = 0; bi < nreg; ++bi) {
-      const int src = gbase | (bi & 3);
-      const bool s0 = bi < 4;
-      Leg<T> Lb;
-      Lb.e0 = __shfl(s0 ? L0.e0 : L1.e0, src);
-      Lb.e1 = __shfl(s0 ? L0.e1 : L1.e1, src);
-      Lb.e2 = __shfl(s0 ? L0.e2 : L1.e2, src);
-      Lb.E = __shfl(s0 ? L0.E : L1.E, src);
-      Lb.t = __shfl(s0 ? L0.t : L1.t, src);
-      if (bi >= np || Lb.E == T(0)) continue;
-      // own entries a = q (slot 0) and q + 4 (slot 1)
-#pragma unroll
-      for (int sa = 0; sa < 2; ++sa) {
-        const int a = q + 4 * sa;
-        const Leg<T> &La = sa == 0 ? L0 : L1;
-        if (a >= np || a == bi || La.E == T(0)) continue;
-        const T lep = (La.t == 1 && Lb.t == 1) ? lp11 : ((La.t == 2 && Lb.t == 2) ? lp22 : lp12);
-        T Fa[3], hh;
-        vterm<T>(lep, La, Lb, Fa, hh);
-        if (sa == 0) {
-          G0[0] += Fa[0]; G0[1] += Fa[1]; G0[2] += Fa[2];
-          if (EN) G0[3] += hh * T(1.0 / 3.0);
-        } else {
-          G1[0] += Fa[0]; G1[1] += Fa[1]; G1[2] += Fa[2];
-          if (EN) G1[3] += hh * T(1.0 / 3.0);
-        }
-        fcx -= Fa[0]; fcy -= Fa[1]; fcz -= Fa[2];
-        if (EN) uc += hh;
-      }
-    }
-    // entries beyond the register slots (dense atoms): recompute on the fly
-    auto leg_at = [&](int n, Leg<T> &L) {
-      const int k = nth_bit(pm, n);
-      const int j = nbr_sw[(size_t)k * NA + i];
-      leg_geom<MIXED>(xi, X[j], in, fc, ti, L);
-      if (!((am >> k) & 1ull)) L.E = T(0);
-    };
-    if (npmax > 8) {
-      // terms (a < 8, b >= 8)
-      for (int bi = 8; bi < np; ++bi) {
-        Leg<T> Lb;
-        leg_at(bi, Lb);
-        if (Lb.E == T(0)) continue;
-#pragma unroll
-        for (int sa = 0; sa < 2; ++sa) {
-          const int a = q + 4 * sa;
-          const Leg<T> &La = sa == 0 ? L0 : L1;
-          if (a >= np || La.E == T(0)) continue;
-          const T lep = (La.t == 1 && Lb.t == 1) ? lp11 : ((La.t == 2 && Lb.t == 2) ? lp22 : lp12);
-          T Fa[3], hh;
-          vterm<T>(lep, La, Lb, Fa, hh);
-          if (sa == 0) {
-            G0[0] += Fa[0]; G0[1] += Fa[1]; G0[2] += Fa[2];
-            if (EN) G0[3] += hh * T(1.0 / 3.0);
-          } else {
-            G1[0] += Fa[0]; G1[1] += Fa[1]; G1[2] += Fa[2];
-            if (EN) G1[3] += hh * T(1.0 / 3.0);
-          }
-          fcx -= Fa[0]; fcy -= Fa[1]; fcz -= Fa[2];
-          if (EN) uc += hh;
-        }
-      }
-      // own entries a >= 8: full sum over all b
-      for (int a = q + 8; a < np; a += 4) {
-        Leg<T> La;
-        leg_at(a, La);
-        T Ga[4] = {0, 0, 0, 0};
-        // (a leg whose E underflowed still stores its zero share below)
-        for (int bi = 0; bi < np && La.E != T(0); ++bi) {
-          if (bi == a) continue;
-          Leg<T> Lb;
-          leg_at(bi, Lb);
-          if (Lb.E == T(0)) continue;
-          const T lep = (La.t == 1 && Lb.t == 1) ? lp11 : ((La.t == 2 && Lb.t == 2) ? lp22 : lp12);
-          T Fa[3], hh;
-          vterm<T>(lep, La, Lb, Fa, hh);
-          Ga[0] += Fa[0]; Ga[1] += Fa[1]; Ga[2] += Fa[2];
-          if (EN) Ga[3] += hh * T(1.0 / 3.0);
-          fcx -= Fa[0]; fcy -= Fa[1]; fcz -= Fa[2];
-          if (EN) uc += hh;
-        }
-        const int ka = nth_bit(pm, a);
-        T4 gv;
-        gv.x = Ga[0]; gv.y = Ga[1]; gv.z = Ga[2]; gv.w = Ga[3];
-        G[(size_t)ka * NA + i] = gv;
-        if (VIR) {
-          const T rl = -T(1) / La.rinv;
-          addW(rl * La.e0, rl * La.e1, rl * La.e2, Ga[0], Ga[1], Ga[2]);
-        }
-      }
-    }
-    // store the shares of the register-held legs
-    auto store_leg = [&](int n, const Leg<T> &La, const T (&Gs)[4]) {
-      const int ka = nth_bit(pm, n);
-      T4 gv;
-      gv.x = Gs[0]; gv.y = Gs[1]; gv.z = Gs[2]; gv.w = Gs[3];
-      G[(size_t)ka * NA + i] = gv;
-      if (VIR && La.E != T(0)) {
-        const T rl = -T(1) / La.rinv;
-        addW(rl * La.e0, rl * La.e1, rl * La.e2, Gs[0], Gs[1], Gs[2]);
-      }
-    };
-    if (q < np) store_leg(q, L0, G0);
-    if (q + 4 < np) store_leg(q + 4, L1, G1);
-    if (MIXED) {
-      kadd(sx, cx, (float)fcx); kadd(sy, cy, (float)fcy); kadd(sz, cz, (float)fcz);
-    } else {
-      fx += fcx; fy += fcy; fz += fcz;
-    }
-    if (EN) { ui += (double)uc * (1.0 / 6.0) * escale; etot += (double)uc * 0.5 * escale; }
-  } else {
-    // no three-body terms in this wave: every leg still owes its neighbour a
-    // (zero) G entry -- including legs whose E underflowed to zero
-    T4 gv;
-    gv.x = gv.y = gv.z = gv.w = T(0);
-    for (int n = q; n < np; n += 4) {
-      const int k = nth_bit(pm, n);
-      if ((am >> k) & 1ull) G[(size_t)k * NA + i] = gv;
-    }
-  }
-
-  // ---- phase 4: quad reduction, lane 0 of the group stores ----------------
-  if (MIXED) {
-    fx = ((double)sx + (double)cx) * fscale;
-    fy = ((double)sy + (double)cy) * fscale;
-    fz = ((double)sz + (double)cz) * fscale;
-  }
-  fx += __shfl_xor(fx, 1); fy += __shfl_xor(fy, 1); fz += __shfl_xor(fz, 1);
-  fx += __shfl_xor(fx, 2); fy += __shfl_xor(fy, 2); fz += __shfl_xor(fz, 2);
-  if (EN) { ui += __shfl_xor(ui, 1); ui += __shfl_xor(ui, 2); }
-  if (q == 0) {
-    Fpart[i] = fx;
-    Fpart[(size_t)NA + i] = fy;
-    Fpart[2 * (size_t)NA + i] = fz;
-    amask[i] = am;
-    if (EN) upart[i] = ui;
-  }
-
+  double fx, fy, fz;
+  f.get(fscale, fx, fy, fz);
+  Fpart[i] = fx;
+  Fpart[(size_t)NA + i] = fy;
+  Fpart[2 * (size_t)NA + i] = fz;
+  if (EN) upart[i] = ui;
   if (EN || VIR) {
     double v[10];
     v[0] = etot;
@@ -1021,25 +861,123 @@ __global__ __launch_bounds__(FIB_BLOCK) void k_force_a(
   }
 }
 
+
+// Lane-per-leg three-body pass.  Block = 4 wavefronts, each owning 64
+// consecutive atoms.
+#define TRI_MAXLEG 2048  // 64 atoms x 32 legs; more legs are taken in batches
+template <bool MIXED, bool EN, bool VIR>
+__global__ __launch_bounds__(FIB_BLOCK) void k_tri(
+    const double4 *__restrict__ X, const FConst fc, int NA, const int *__restrict__ cnt_leg,
+    const typename Prec<MIXED>::T4 *__restrict__ LA,
+    const typename Prec<MIXED>::T4 *__restrict__ LB, typename Prec<MIXED>::T4 *__restrict__ G,
+    int pstride, double *__restrict__ partials) {
+  typedef typename Prec<MIXED>::T T;
+  typedef typename Prec<MIXED>::T4 T4;
+  __shared__ uint8_t owner[FIB_BLOCK / FIB_WAVE][TRI_MAXLEG];
+  __shared__ int soff[FIB_BLOCK / FIB_WAVE][FIB_WAVE + 1];
+  const int lb = xcd_remap(blockIdx.x, gridDim.x);
+  const int lane = threadIdx.x % FIB_WAVE, wid = threadIdx.x / FIB_WAVE;
+  const int i0 = lb * FIB_BLOCK + wid * FIB_WAVE;
+  const double escale = MIXED ? fc.eps[0][0] : 1.0;
+  const double fscale = MIXED ? fc.eps[0][0] * fc.isig[0][0] : 1.0;
+  const double lscale = MIXED ? 1.0 / fc.isig[0][0] : 1.0;
+  // lambda*eps by number of Ge atoms in the term (symmetric in the three
+  // species, prms.f90:464-471), in J or eps_Si
+  const double lu = MIXED ? 1.0 / fc.eps[0][0] : 1.0;
+  const T lp[4] = {(T)(fc.leps[0][0][0] * lu), (T)(fc.leps[0][0][1] * lu),
+                   (T)(fc.leps[0][1][1] * lu), (T)(fc.leps[1][1][1] * lu)};
+
+  // legs of the wave's atoms: exclusive prefix sum, owner table in LDS
+  const int nl = cnt_leg[i0 + lane];
+  int inc = nl;
+  for (int o = 1; o < FIB_WAVE; o <<= 1) {
+    const int y = __shfl_up(inc, o);
+    if (lane >= o) inc += y;
+  }
+  const int total = __shfl(inc, FIB_WAVE - 1);
+  soff[wid][lane] = inc - nl;
+  if (lane == FIB_WAVE - 1) soff[wid][FIB_WAVE] = total;
+
+  T uc = 0;
+  T W[9];
+#pragma unroll
+  for (int w = 0; w < 9; ++w) W[w] = 0;
+
+  // batches of up to TRI_MAXLEG legs (one batch unless the atoms are dense)
+  for (int g0 = 0; g0 < total; g0 += TRI_MAXLEG) {
+    const int g1 = min(total, g0 + TRI_MAXLEG);
+    __builtin_amdgcn_wave_barrier();
+    {
+      const int s0 = max(inc - nl, g0), s1 = min(inc, g1);
+      for (int g = s0; g < s1; ++g) owner[wid][g - g0] = (uint8_t)lane;
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    for (int g = g0 + lane; g < g1; g += FIB_WAVE) {
+      const int o = owner[wid][g - g0];
+      const int i = i0 + o;
+      const int na = soff[wid][o + 1] - soff[wid][o];
+      const int a = g - soff[wid][o];
+      Leg<T> La;
+      load_leg(LA, LB, (size_t)a * NA + i, La);
+      const int gi = (type_of(meta_of(X[i])) == 2) ? 1 : 0;
+      const int ga = gi + ks_ge(La.ks);
+      T Ga0 = 0, Ga1 = 0, Ga2 = 0, Gu = 0;
+      for (int b = 0; b < na; ++b) {
+        if (b == a) continue;
+        Leg<T> Lb;
+        load_leg(LA, LB, (size_t)b * NA + i, Lb);
+        T Fa[3], hh;
+        vterm<T>(lp[ga + ks_ge(Lb.ks)], La, Lb, Fa, hh);
+        Ga0 += Fa[0]; Ga1 += Fa[1]; Ga2 += Fa[2];
+        Gu += hh;
+      }
+      T4 gv;
+      gv.x = Ga0; gv.y = Ga1; gv.z = Ga2; gv.w = Gu * T(1.0 / 3.0);
+      G[(size_t)a * NA + i] = gv;
+      uc += Gu;
+      if (VIR) {
+        // r_a = x_a - x_i = -e_a / rinv_a
+        const T ra = -T(1) / La.rinv;
+        const T d0 = ra * La.e0, d1 = ra * La.e1, d2 = ra * La.e2;
+        W[0] += d0 * Ga0; W[1] += d0 * Ga1; W[2] += d0 * Ga2;
+        W[3] += d1 * Ga0; W[4] += d1 * Ga1; W[5] += d1 * Ga2;
+        W[6] += d2 * Ga0; W[7] += d2 * Ga1; W[8] += d2 * Ga2;
+      }
+    }
+  }
+  if (EN || VIR) {
+    double v[10];
+    // every unordered term was summed twice, as (a, b) and (b, a)
+    v[0] = (double)uc * 0.5 * escale;
+    const double ws = MIXED ? fscale * lscale : 1.0;
+#pragma unroll
+    for (int w = 0; w < 9; ++w) v[1 + w] = (double)W[w] * ws;
+    const int op[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+    block_partials<10, FIB_BLOCK>(v, op, lb, pstride, partials);
+  }
+}
+
 // ---------------------------------------------------------------------------
-// force phase B: gather neighbour contributions; MODE 0 = store forces,
-// MODE 1 = reference velocity update (timeint.f90:231-236) + thermostat sums.
-template <bool MIXED, bool EN, int MODE>
-__global__ __launch_bounds__(FIB_BLOCK) void k_force_b(
+// force pass 3: own terms + the G entries neighbours wrote for this atom;
+// MODE 0 = store forces, MODE 1 = reference velocity update
+// (timeint.f90:231-236) + thermostat sums.  SEARCH: scan the neighbour's list
+// row for the atom instead of reading rev.
+template <bool MIXED, bool EN, int MODE, bool SEARCH>
+__global__ __launch_bounds__(FIB_BLOCK) void k_force3(
     const double4 *__restrict__ X, const DInst *__restrict__ I, int NA, int Wsw,
     const int *__restrict__ nbr_sw, const uint8_t *__restrict__ rev,
     const typename Prec<MIXED>::T4 *__restrict__ G, const uint64_t *__restrict__ amask,
+    const int *__restrict__ cnt_leg, const int *__restrict__ cnt_sw,
     const double *__restrict__ Fpart, const double *__restrict__ upart, double fscale,
     double escale, double *__restrict__ Fout, double *__restrict__ uout, double *__restrict__ V,
-    StepArgs sa, int pstride, double *__restrict__ partials, DCtl *__restrict__ ctl) {
+    StepArgs sa, int pstride, double *__restrict__ partials) {
   typedef typename Prec<MIXED>::T4 T4;
-  const int nb = gridDim.x;
-  const int lb = xcd_remap(blockIdx.x, nb);
+  const int lb = xcd_remap(blockIdx.x, gridDim.x);
   const int i = lb * FIB_BLOCK + threadIdx.x;
   const double4 xi = X[i];
   const uint32_t mi = meta_of(xi);
-  // every block lies inside one instance: make the index wave-uniform so the
-  // instance record is read with scalar loads
   const int kin = __builtin_amdgcn_readfirstlane(inst_of(mi));
   const DInst &in = I[kin];
   const bool live = !(mi & (META_PAD | META_DEAD));
@@ -1048,15 +986,46 @@ __global__ __launch_bounds__(FIB_BLOCK) void k_force_b(
     f0 = Fpart[i]; f1 = Fpart[(size_t)NA + i]; f2 = Fpart[2 * (size_t)NA + i];
     if (EN) u = upart[i];
     uint64_t m = amask[i];
+    const size_t row = (size_t)i * Wsw;
     double g0 = 0, g1 = 0, g2 = 0, gu = 0;
+    // reaction on the centre: minus the forces its own terms put on its legs
+    const int na = cnt_leg[i];
+    for (int n = 0; n < na; ++n) {
+      const T4 g = G[(size_t)n * NA + i];
+      g0 -= (double)g.x; g1 -= (double)g.y; g2 -= (double)g.z;
+      if (EN) gu += 0.5 * (double)g.w;
+    }
     while (m) {
       const int k = __ffsll((unsigned long long)m) - 1;
       m &= m - 1;
-      const int j = nbr_sw[(size_t)k * NA + i];
-      const int r = rev[(size_t)k * NA + i];
-      // r == 255: j never lists i (a phantom-layer atom, see k_nbuild)
-      if (r >= Wsw) continue;
-      const T4 g = G[(size_t)r * NA + j];
+      const int j = nbr_sw[row + k];
+      int n = -1;
+      if (SEARCH) {
+        // slot of i in j's list row (contiguous, atom-major), as k_rev does
+        const int cj = min(cnt_sw[j], Wsw);
+        const int4 *rj = reinterpret_cast<const int4 *>(nbr_sw + (size_t)j * Wsw);
+        int r = 255;
+        for (int q0 = 0; q0 < cj && r == 255; q0 += 32) {
+          int4 e[8];
+#pragma unroll
+          for (int u = 0; u < 8; ++u) e[u] = rj[(q0 >> 2) + u];
+#pragma unroll
+          for (int u = 7; u >= 0; --u) {
+            const int q = q0 + 4 * u;
+            if (q + 3 < cj && e[u].w == i) r = q + 3;
+            if (q + 2 < cj && e[u].z == i) r = q + 2;
+            if (q + 1 < cj && e[u].y == i) r = q + 1;
+            if (q < cj && e[u].x == i) r = q;
+          }
+        }
+        if (r < Wsw) n = __popcll(amask[j] & ((1ull << r) - 1ull));
+      } else {
+        const int r = rev[row + k];
+        // r == 255: j never lists i (a phantom-layer atom, see k_nbuild)
+        if (r < Wsw) n = __popcll(amask[j] & ((1ull << r) - 1ull));
+      }
+      if (n < 0) continue;
+      const T4 g = G[(size_t)n * NA + j];
       g0 += (double)g.x; g1 += (double)g.y; g2 += (double)g.z;
       if (EN) gu += (double)g.w;
     }
@@ -1317,10 +1286,11 @@ __global__ void k_set_atom(int NA, double4 *__restrict__ X, double *__restrict__
 static inline int nblk(long n, int b) { return (int)((n + b - 1) / b); }
 
 static void finalize(const DevBufs &d, int nv, int bs, const int *op, const int *dst,
-                     hipStream_t s) {
+                     hipStream_t s, int nreg = 1) {
   RedSpec rs;
   rs.nv = nv;
   rs.bs = bs;
+  rs.nreg = nreg;
   for (int k = 0; k < FIB_NRED; ++k) {
     rs.op[k] = k < nv ? op[k] : 0;
     rs.dst[k] = k < nv ? dst[k] : -1;
@@ -1387,63 +1357,78 @@ void launch_nbuild(const DevBufs &d, const FConst &fc, hipStream_t s) {
   finalize(d, 2, FIB_BLOCK, op, dst, s);
 }
 
-void launch_rev(const DevBufs &d, hipStream_t s) {
-  hipLaunchKernelGGL(k_rev, dim3(d.NA / REV_ATOMS), dim3(FIB_BLOCK), 0, s, d.NA, d.Wsw, d.cnt_sw,
-                     d.nbr_sw, d.rev);
-}
-
 void launch_cellid(const DevBufs &d, int inst, int32_t *dout, hipStream_t s) {
   hipLaunchKernelGGL(k_cellid, dim3(nblk(d.NA, FIB_BLOCK)), dim3(FIB_BLOCK), 0, s, d.X[d.cur],
                      d.NA, d.I, inst, dout);
 }
 
+void launch_rev(const DevBufs &d, hipStream_t s) {
+  int sh = 0;
+  while ((1 << sh) < d.Wsw) ++sh;
+  hipLaunchKernelGGL(k_rev, dim3((unsigned)(((size_t)d.NA << sh) / FIB_BLOCK)), dim3(FIB_BLOCK), 0,
+                     s, d.NA, sh, d.cnt_sw, d.nbr_sw, d.rev);
+}
+
 template <bool MIXED, bool EN, bool VIR>
-static void force_a(const DevBufs &d, const FConst &fc, hipStream_t s) {
+static void force_ab(const DevBufs &d, const FConst &fc, hipStream_t s) {
   typedef typename Prec<MIXED>::T4 T4;
-  hipLaunchKernelGGL((k_force_a<MIXED, EN, VIR>), dim3(d.NA / FA_ATOMS), dim3(FIB_BLOCK), 0, s,
+  hipLaunchKernelGGL((k_legs<MIXED, EN, VIR>), dim3(d.NA / FIB_BLOCK), dim3(FIB_BLOCK), 0, s,
                      d.X[d.cur], d.I, fc, d.tab, d.NA, d.Wsw, d.Wion, d.cnt_sw, d.nbr_sw,
-                     d.cnt_ion, d.nbr_ion, (T4 *)d.G, d.amask, d.Fpart, d.upart, d.pstride,
-                     d.partials, d.ctl);
+                     d.cnt_ion, d.nbr_ion, (T4 *)d.LA, (T4 *)d.LB, d.cnt_leg, d.amask, d.Fpart,
+                     d.upart, d.pstride, d.partials);
+  hipLaunchKernelGGL((k_tri<MIXED, EN, VIR>), dim3(d.NA / FIB_BLOCK), dim3(FIB_BLOCK), 0, s,
+                     d.X[d.cur], fc, d.NA, d.cnt_leg, (const T4 *)d.LA, (const T4 *)d.LB,
+                     (T4 *)d.G, d.pstride, d.partials + (size_t)FIB_NRED * d.pstride);
   if (EN || VIR) {
     int op[10], dst[10];
     for (int k = 0; k < 10; ++k) { op[k] = 0; dst[k] = DCTL_IDX(utot) + k; }
-    finalize(d, 10, FA_ATOMS, op, dst, s);
+    finalize(d, 10, FIB_BLOCK, op, dst, s, 2);
   }
 }
 
-template <bool MIXED, bool EN, int MODE>
-static void force_b(const DevBufs &d, const FConst &fc, const StepArgs &sa, double *Fo,
+template <bool MIXED, bool EN, int MODE, bool SEARCH>
+static void force_c(const DevBufs &d, const FConst &fc, const StepArgs &sa, double *Fo,
                     hipStream_t s) {
   typedef typename Prec<MIXED>::T4 T4;
   const double fscale = MIXED ? fc.eps[0][0] * fc.isig[0][0] : 1.0;
   const double escale = MIXED ? fc.eps[0][0] : 1.0;
-  hipLaunchKernelGGL((k_force_b<MIXED, EN, MODE>), dim3(d.NA / FIB_BLOCK), dim3(FIB_BLOCK), 0, s,
-                     d.X[d.cur], d.I, d.NA, d.Wsw, d.nbr_sw, d.rev, (const T4 *)d.G, d.amask,
-                     d.Fpart, d.upart, fscale, escale, Fo, d.uout, d.V[d.cur], sa, d.pstride,
-                     d.partials, d.ctl);
+  hipLaunchKernelGGL((k_force3<MIXED, EN, MODE, SEARCH>), dim3(d.NA / FIB_BLOCK), dim3(FIB_BLOCK),
+                     0, s, d.X[d.cur], d.I, d.NA, d.Wsw, d.nbr_sw, d.rev, (const T4 *)d.G,
+                     d.amask, d.cnt_leg, d.cnt_sw, d.Fpart, d.upart, fscale, escale, Fo,
+                     d.uout, d.V[d.cur], sa, d.pstride, d.partials);
   if (MODE == 1) finalize_thermo(d, s);
 }
 
 void launch_force_a(const DevBufs &d, const FConst &fc, bool mixed, bool energy, bool virial,
                     hipStream_t s) {
   if (mixed) {
-    if (energy) { if (virial) force_a<true, true, true>(d, fc, s); else force_a<true, true, false>(d, fc, s); }
-    else { if (virial) force_a<true, false, true>(d, fc, s); else force_a<true, false, false>(d, fc, s); }
+    if (energy) { if (virial) force_ab<true, true, true>(d, fc, s); else force_ab<true, true, false>(d, fc, s); }
+    else { if (virial) force_ab<true, false, true>(d, fc, s); else force_ab<true, false, false>(d, fc, s); }
   } else {
-    if (energy) { if (virial) force_a<false, true, true>(d, fc, s); else force_a<false, true, false>(d, fc, s); }
-    else { if (virial) force_a<false, false, true>(d, fc, s); else force_a<false, false, false>(d, fc, s); }
+    if (energy) { if (virial) force_ab<false, true, true>(d, fc, s); else force_ab<false, true, false>(d, fc, s); }
+    else { if (virial) force_ab<false, false, true>(d, fc, s); else force_ab<false, false, false>(d, fc, s); }
+  }
+}
+
+template <bool MIXED>
+static void force_c_mode(const DevBufs &d, const FConst &fc, bool energy, int mode, bool search,
+                         const StepArgs &sa, double *Fout, hipStream_t s) {
+  if (mode == 1) {
+    if (search) force_c<MIXED, false, 1, true>(d, fc, sa, Fout, s);
+    else force_c<MIXED, false, 1, false>(d, fc, sa, Fout, s);
+  } else if (energy) {
+    if (search) force_c<MIXED, true, 0, true>(d, fc, sa, Fout, s);
+    else force_c<MIXED, true, 0, false>(d, fc, sa, Fout, s);
+  } else {
+    if (search) force_c<MIXED, false, 0, true>(d, fc, sa, Fout, s);
+    else force_c<MIXED, false, 0, false>(d, fc, sa, Fout, s);
   }
 }
 
 void launch_force_b(const DevBufs &d, const FConst &fc, bool mixed, bool energy, int mode,
-                    const StepArgs &sa, double *Fout, hipStream_t s) {
-  if (mixed) {
-    if (mode == 0) { if (energy) force_b<true, true, 0>(d, fc, sa, Fout, s); else force_b<true, false, 0>(d, fc, sa, Fout, s); }
-    else force_b<true, false, 1>(d, fc, sa, Fout, s);
-  } else {
-    if (mode == 0) { if (energy) force_b<false, true, 0>(d, fc, sa, Fout, s); else force_b<false, false, 0>(d, fc, sa, Fout, s); }
-    else force_b<false, false, 1>(d, fc, sa, Fout, s);
-  }
+                    bool search, const StepArgs &sa, double *Fout, hipStream_t s) {
+  if (mixed) force_c_mode<true>(d, fc, energy, mode, search, sa, Fout, s);
+  else force_c_mode<false>(d, fc, energy, mode, search, sa, Fout, s);
 }
 
 void launch_pos(const DevBufs &d, const StepArgs &sa, hipStream_t s) {

@@ -98,6 +98,7 @@ struct fib_handle {
   std::vector<DInst> inst;
   DevBufs d{};
   bool have_state = false, have_lists = false, have_force = false;
+  bool rev_valid = false;  // rev matches the current lists
   int last_flags = 0;
   std::string err;
   RefCtl ref;
@@ -217,6 +218,7 @@ static void derive_constants(fib_handle *h) {
   }
   c.izero = izero;
   c.cut_sw2 = c.cut_sw * c.cut_sw * (1.0 + 1e-6);
+  c.mixed_xleg = std::min(c.cut_sw, c.al - c.gamma / 30.0);
   {
     const double ug = std::max(c.umax * (double)(izero - 1) / (double)(nt - 1), c.cut_sw);
     c.r2gate = (1.001 * ug) * (1.001 * ug);
@@ -238,7 +240,7 @@ static int32_t free_bufs(fib_handle *h) {
   DevBufs &d = h->d;
   void *ptrs[] = {d.I, d.ctl, d.X[0], d.X[1], d.V[0], d.V[1], d.F[0], d.F[1], d.Xi[0], d.Xi[1],
                   d.inv, d.bkey, d.brank, d.bcnt, d.cstart, d.perm, d.bsum, d.cnt_sw, d.nbr_sw,
-                  d.cnt_ion, d.nbr_ion, d.rev, d.G, d.amask, d.Fpart, d.upart, d.Fout, d.uout,
+                  d.cnt_ion, d.nbr_ion, d.rev, d.LA, d.LB, d.G, d.amask, d.cnt_leg, d.Fpart, d.upart, d.Fout, d.uout,
                   d.partials, d.tab, d.xfer};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
@@ -255,16 +257,21 @@ static int32_t alloc_lists(fib_handle *h, int Wsw, int Wion) {
   DevBufs &d = h->d;
   if (d.nbr_sw) (void)hipFree(d.nbr_sw);
   if (d.nbr_ion) (void)hipFree(d.nbr_ion);
-  if (d.rev) (void)hipFree(d.rev);
-  if (d.G) (void)hipFree(d.G);
-  d.nbr_sw = nullptr; d.nbr_ion = nullptr; d.rev = nullptr; d.G = nullptr;
+  void *old[] = {d.rev, d.LA, d.LB, d.G};
+  for (void *q : old)
+    if (q) (void)hipFree(q);
+  d.nbr_sw = nullptr; d.nbr_ion = nullptr; d.rev = nullptr;
+  d.LA = nullptr; d.LB = nullptr; d.G = nullptr;
   d.Wsw = Wsw;
   d.Wion = Wion;
   const size_t NA = (size_t)d.NA;
   HIPCHK(h, dalloc(&d.nbr_sw, (size_t)Wsw * NA));
   HIPCHK(h, dalloc(&d.nbr_ion, (size_t)Wion * NA));
   HIPCHK(h, dalloc(&d.rev, (size_t)Wsw * NA));
+  // a leg is a listed pair, so Wsw slots always hold every leg of an atom
   const size_t gsz = h->prec == FIB_PREC_MIXED ? sizeof(float4) : sizeof(double4);
+  HIPCHK(h, hipMalloc(&d.LA, gsz * (size_t)Wsw * NA));
+  HIPCHK(h, hipMalloc(&d.LB, gsz * (size_t)Wsw * NA));
   HIPCHK(h, hipMalloc(&d.G, gsz * (size_t)Wsw * NA));
   return FIB_OK;
 }
@@ -362,12 +369,13 @@ extern "C" int32_t fib_create(const fib_params *p, int32_t ninst, const fib_inst
   ALLOC(d.cnt_sw, NA);
   ALLOC(d.cnt_ion, NA);
   ALLOC(d.amask, NA);
+  ALLOC(d.cnt_leg, NA);
   ALLOC(d.Fpart, 3 * NA);
   ALLOC(d.upart, NA);
   ALLOC(d.Fout, 3 * NA);
   ALLOC(d.uout, NA);
   d.pstride = d.NA / FIB_WAVE + 1;
-  ALLOC(d.partials, (size_t)d.pstride * FIB_NRED);
+  ALLOC(d.partials, (size_t)d.pstride * FIB_NRED * 2);
   ALLOC(d.tab, h->tab.size());
   ALLOC(d.xfer, 3 * NA);
 #undef ALLOC
@@ -383,6 +391,7 @@ extern "C" int32_t fib_create(const fib_params *p, int32_t ninst, const fib_inst
   if (e == hipSuccess) e = hipMemset(d.cnt_sw, 0, sizeof(int) * NA);
   if (e == hipSuccess) e = hipMemset(d.cnt_ion, 0, sizeof(int) * NA);
   if (e == hipSuccess) e = hipMemset(d.amask, 0, sizeof(uint64_t) * NA);
+  if (e == hipSuccess) e = hipMemset(d.cnt_leg, 0, sizeof(int) * NA);
   if (e == hipSuccess)
     e = hipMemcpy(d.tab, h->tab.data(), sizeof(TabEntry) * h->tab.size(), hipMemcpyHostToDevice);
   if (e != hipSuccess) {
@@ -553,9 +562,7 @@ static int32_t build_lists(fib_handle *h) {
     }
     if (nonfin) return fail(h, FIB_ERR_NONFINITE, "non-finite position detected while binning");
     if (msw <= d.Wsw && mion <= d.Wion) {
-      tid = h->tim.begin();
-      fibk::launch_rev(d, h->stream);
-      h->tim.end(tid, 1);
+      h->rev_valid = false;
       h->have_lists = true;
       h->have_force = false;
       ++h->n_builds;
@@ -597,20 +604,16 @@ extern "C" int32_t fib_get_nlist(fib_handle *h, int32_t inst, int32_t *nss, int3
     if (!((uint32_t)w & META_PAD)) orig[q] = (int)(w >> 32);
   }
   auto fetch = [&](const int *lst, int W, const std::vector<int> &cnt, std::vector<std::vector<int>> &half) -> int32_t {
-    int mx = 0;
-    for (int q = 0; q < np; ++q) mx = std::max(mx, std::min(cnt[q], W));
-    std::vector<int> rows((size_t)mx * np);
-    for (int k = 0; k < mx; ++k) {
-      hipError_t e = hipMemcpy(rows.data() + (size_t)k * np, lst + (size_t)k * d.NA + off,
-                               sizeof(int) * np, hipMemcpyDeviceToHost);
-      if (e != hipSuccess) { h->err = hipGetErrorString(e); return FIB_ERR_HIP; }
-    }
+    std::vector<int> rows((size_t)W * np);
+    hipError_t e = hipMemcpy(rows.data(), lst + (size_t)off * W, sizeof(int) * (size_t)W * np,
+                             hipMemcpyDeviceToHost);
+    if (e != hipSuccess) { h->err = hipGetErrorString(e); return FIB_ERR_HIP; }
     half.assign(n, {});
     for (int q = 0; q < np; ++q) {
       if (orig[q] < 0) continue;
       const int c = std::min(cnt[q], W);
       for (int k = 0; k < c; ++k) {
-        const int j = rows[(size_t)k * np + q] - off;
+        const int j = rows[(size_t)q * W + k] - off;
         if (j < 0 || j >= np || orig[j] < 0) continue;
         if (orig[j] > orig[q]) half[orig[q]].push_back(orig[j] + 1);
       }
@@ -638,6 +641,16 @@ extern "C" int32_t fib_get_nlist(fib_handle *h, int32_t inst, int32_t *nss, int3
   return FIB_OK;
 }
 
+// The reverse-slot map pays off when several force evaluations share one
+// list build (MD steps); a single evaluation searches instead.
+static void ensure_rev(fib_handle *h) {
+  if (h->rev_valid || !h->have_lists) return;
+  int tid = h->tim.begin();
+  fibk::launch_rev(h->d, h->stream);
+  h->tim.end(tid, 1);
+  h->rev_valid = true;
+}
+
 // mode 0: forces into dst (default d.Fout); mode 1: MD velocity update.
 static int32_t run_force(fib_handle *h, int flags, int mode, const StepArgs &sa,
                          double *dst = nullptr) {
@@ -649,7 +662,8 @@ static int32_t run_force(fib_handle *h, int flags, int mode, const StepArgs &sa,
   fibk::launch_force_a(d, h->fc, mixed, en, vir, h->stream);
   h->tim.end(tid, 2);
   tid = h->tim.begin();
-  fibk::launch_force_b(d, h->fc, mixed, en && mode == 0, mode, sa, Fo, h->stream);
+  fibk::launch_force_b(d, h->fc, mixed, en && mode == 0, mode, !h->rev_valid, sa, Fo,
+                       h->stream);
   h->tim.end(tid, 3);
   return FIB_OK;
 }
@@ -836,6 +850,7 @@ extern "C" int32_t fib_step(fib_handle *h, const fib_step_ctl *ctl, fib_step_rep
   const fib_run_params &P = R.rp;
   const int builds0 = h->n_builds;
   int steps = 0;
+  ensure_rev(h);
   for (;;) {
     if (ctl->max_steps > 0 && steps >= ctl->max_steps) break;
     if (w == 0) {
@@ -864,6 +879,7 @@ extern "C" int32_t fib_step(fib_handle *h, const fib_step_ctl *ctl, fib_step_rep
     if (lt % R.ntlist == 0) {
       int32_t r = build_lists(h);
       if (r != FIB_OK) return r;
+      ensure_rev(h);
     }
     if (lt % R.dslist == 0 && w == 1) {
       DCtl c;
