@@ -11,6 +11,9 @@ force error of mixed vs FP64 are reported beside it.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
        python bench.py --profile fp64|mixed   (few steps + a 256 MiB copy, for rocprofv3)
+       python bench.py --summarize-stats DIR OUT.txt CMD   (rocprofv3 --stats csv -> text)
+       python bench.py --summarize-pmc FETCH_DIR WRITE_DIR OUT.txt CMD
+                       (rocprofv3 --pmc csv -> text + profiles/pmc_traffic.json)
 """
 from __future__ import annotations
 
@@ -26,11 +29,85 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
-# Algorithmic bytes per atom of one evaluation (build 88 B: read X 24 B, write
-# 16 neighbour indices 64 B; force + energy 124 B: read X 24 B, type 4 B,
-# list 64 B, write F 24 B, u 8 B).
-BYTES_BUILD = 88.0
-BYTES_FORCE = 124.0
+# Algorithmic bytes per atom of one evaluation: build 88 B (read X 24 B, write
+# 16 neighbour indices 64 B); force + energy 124 B (read X 24 B, type 4 B,
+# list 64 B; write F 24 B, u 8 B).  Per kernel: k_nbuild carries the build's
+# 88 B; of the force, k_legs reads X, type and list (92 B) and k_force3 writes
+# F and u (32 B); k_tri only moves intermediate data (0 B).
+KERNEL_BYTES = {"k_nbuild": 88.0, "k_legs": 92.0, "k_force3": 32.0, "k_tri": 0.0}
+PMC_FILE = os.path.join(ROOT, "profiles", "pmc_traffic.json")
+
+
+COPY_KIB = 256 * 1024  # the plain copy of --profile mode: 256 MiB read, 256 MiB written
+
+
+def short_name(n: str) -> str:
+    import re
+    n = re.sub(r"^void ", "", n)
+    n = re.sub(r"\(.*$", "", n)
+    n = n.replace("fibk::", "")
+    return n
+
+
+def summarize_stats(d, out, cmd):
+    """Kernel table of a rocprofv3 --kernel-trace --stats --output-format csv run."""
+    import csv
+    rows = list(csv.DictReader(open(os.path.join(d, "run_kernel_stats.csv"))))
+    lines = [f"# rocprofv3 kernel statistics", f"# command: {cmd}", "",
+             f"{'kernel':44s} {'calls':>6s} {'avg_us':>10s} {'min_us':>10s} {'max_us':>10s} {'pct':>6s}"]
+    for r in rows:
+        lines.append(f"{short_name(r['Name'])[:44]:44s} {int(r['Calls']):6d} "
+                     f"{float(r['AverageNs']) / 1e3:10.1f} {float(r['MinNs']) / 1e3:10.1f} "
+                     f"{float(r['MaxNs']) / 1e3:10.1f} {float(r['Percentage']):6.2f}")
+    open(out, "w").write("\n".join(lines) + "\n")
+    print("\n".join(lines))
+
+
+def summarize_pmc(dfetch, dwrite, out, cmd):
+    """Per-dispatch FETCH_SIZE / WRITE_SIZE (KiB) of two rocprofv3 --pmc runs,
+    scaled by what the counters read for the 256 MiB copy."""
+    import csv
+    from collections import defaultdict
+
+    def per_kernel(d, counter):
+        rows = list(csv.DictReader(open(os.path.join(d, "run_counter_collection.csv"))))
+        disp = defaultdict(float)
+        name = {}
+        for r in rows:
+            if r["Counter_Name"] != counter:
+                continue
+            k = int(r["Dispatch_Id"])
+            disp[k] += float(r["Counter_Value"])
+            name[k] = short_name(r["Kernel_Name"])
+        agg = defaultdict(list)
+        for k, v in disp.items():
+            agg[name[k]].append(v)
+        copy = max(v for k, v in disp.items() if name[k] == "__amd_rocclr_copyBuffer")
+        return {k: sum(v) / len(v) for k, v in agg.items()}, copy
+
+    fetch, fcopy = per_kernel(dfetch, "FETCH_SIZE")
+    write, wcopy = per_kernel(dwrite, "WRITE_SIZE")
+    ff, wf = COPY_KIB / fcopy, COPY_KIB / wcopy
+    X, at, L = make_workload(64)
+    res = {"command": cmd, "atoms": len(X), "copy_fetch_kib": fcopy, "copy_write_kib": wcopy,
+           "fetch_factor": ff, "write_factor": wf, "kernels": {}}
+    lines = ["# HBM traffic per dispatch from rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE (KiB)",
+             f"# command: {cmd}",
+             f"# calibration: 256 MiB copy reads FETCH_SIZE {fcopy:.0f} KiB (factor {ff:.3f}), "
+             f"WRITE_SIZE {wcopy:.0f} KiB (factor {wf:.3f})", "",
+             f"{'kernel':36s} {'fetch_KiB':>12s} {'write_KiB':>12s} {'traffic_MB':>12s} {'B/atom':>8s}"]
+    for k in sorted(fetch, key=lambda k: -(fetch[k] + write.get(k, 0))):
+        if k not in write:
+            continue
+        tb = (fetch[k] * ff + write[k] * wf) * 1024.0
+        res["kernels"][k] = {"fetch_kib": fetch[k], "write_kib": write[k], "traffic_bytes": tb,
+                             "bytes_per_atom": tb / len(X)}
+        lines.append(f"{k[:36]:36s} {fetch[k]:12.0f} {write[k]:12.0f} {tb / 1e6:12.1f} "
+                     f"{tb / len(X):8.1f}")
+    open(out, "w").write("\n".join(lines) + "\n")
+    with open(PMC_FILE, "w") as f:
+        json.dump(res, f, indent=1)
+    print("\n".join(lines))
 
 
 def log(*a):
@@ -43,7 +120,7 @@ def make_workload(ncell: int, seed: int = 1002):
     return np.asfortranarray(X), at, L
 
 
-def run_gpu(X, at, L, precision, steps, warmup, timing_steps=3):
+def run_gpu(X, at, L, precision, steps, warmup):
     import torch
     from fib_md_amd import FibGPU
     n = len(X)
@@ -54,6 +131,9 @@ def run_gpu(X, at, L, precision, steps, warmup, timing_steps=3):
         g.compute(energy=True, virial=True)
     g.synchronize()
     torch.cuda.synchronize()
+    # kernel timers (HIP events on the library's stream) over the timed steps,
+    # on the build and force kernels only
+    g.timing(enable=True, only=("k_nbuild", "k_legs", "k_tri", "k_force3"))
     t0 = time.perf_counter()
     for _ in range(steps):
         g.build_nlist()
@@ -61,13 +141,10 @@ def run_gpu(X, at, L, precision, steps, warmup, timing_steps=3):
     g.synchronize()
     torch.cuda.synchronize()
     t1 = time.perf_counter()
-    # per-kernel device time with HIP events on the library's stream
-    g.timing(enable=True)
-    for _ in range(timing_steps):
-        g.build_nlist()
-        g.compute(energy=True, virial=True)
-    g.synchronize()
     tm = g.timing()
+    timing_steps = steps
+    for v in tm.values():
+        v["calls_per_step"] = v["calls"] / timing_steps
     res = g.forces(energy=False, virial=True)
     g.timing(enable=False)
     out = dict(seconds=t1 - t0, ms_per_step=(t1 - t0) * 1e3 / steps,
@@ -119,7 +196,15 @@ def main():
     ap.add_argument("--ncell", type=int, default=64)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--profile", choices=["fp64", "mixed"], default=None)
+    ap.add_argument("--summarize-stats", nargs=3, metavar=("DIR", "OUT", "CMD"))
+    ap.add_argument("--summarize-pmc", nargs=4, metavar=("FETCH_DIR", "WRITE_DIR", "OUT", "CMD"))
     args = ap.parse_args()
+    if args.summarize_stats:
+        summarize_stats(*args.summarize_stats)
+        return
+    if args.summarize_pmc:
+        summarize_pmc(*args.summarize_pmc)
+        return
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); "
               "multi-GPU sharding is a design in DESIGN.md, not a measured path.", flush=True)
@@ -138,25 +223,32 @@ def main():
         b = torch.empty_like(a)
         b.copy_(a)
         torch.cuda.synchronize()
-        r = run_gpu(X, at, L, args.profile, 2, 1, timing_steps=1)
+        r = run_gpu(X, at, L, args.profile, 2, 1)
         log(json.dumps({k: v for k, v in r.items() if k in ("ms_per_step", "timing")}))
         return
     res = {}
     for prec in ("fp64", "mixed"):
         log(f"running {prec} ...")
         res[prec] = run_gpu(X, at, L, prec, args.steps, args.warmup)
-        log(f"  {prec}: {res[prec]['ms_per_step']:.3f} ms/step, {res[prec]['rate']:.4g} atom-evals/s, "
-            f"kernels {res[prec]['timing']}")
+        log(f"  {prec}: {res[prec]['ms_per_step']:.3f} ms/step, {res[prec]['rate']:.4g} atom-evals/s")
+        for k, v in sorted(res[prec]["timing"].items(), key=lambda kv: -kv[1]["ms"]):
+            log(f"    {k:10s} {v['avg_ms']:.4f} ms x {v['calls_per_step']:.0f}/step")
     emax, erms, enorm = rel_err(res["mixed"]["F"], res["fp64"]["F"])
     m = res["mixed"]
     tm = m["timing"]
-    kern = {
-        "k_force_a+k_force_b": ((tm["force_a_ms"] + tm["force_b_ms"]) / max(1, tm["n_force_a"]), BYTES_FORCE),
-        "k_nbuild+k_rev": (tm["nbuild_ms"] / max(1, tm["n_nbuild"]), BYTES_BUILD),
-    }
-    dom = max(kern, key=lambda k: kern[k][0])
-    dur_ms, bpa = kern[dom]
+    # roofline of the longest single kernel of the mixed step
+    dom = max((k for k in tm if k in KERNEL_BYTES and KERNEL_BYTES[k] > 0),
+              key=lambda k: tm[k]["avg_ms"])
+    dur_ms = tm[dom]["avg_ms"]
+    bpa = KERNEL_BYTES[dom]
     achieved = bpa * n / (dur_ms * 1e-3) / 1e9
+    traffic = None
+    if os.path.exists(PMC_FILE):
+        with open(PMC_FILE) as f:
+            pmc = json.load(f)
+        for name, k in pmc.get("kernels", {}).items():
+            if name.split("<")[0] == dom and pmc.get("atoms") == n:
+                traffic = k["traffic_bytes"]
     cpu = None
     if not args.no_cpu_baseline:
         log("cpu baseline ...")
@@ -178,14 +270,14 @@ def main():
                                "20 A sphere at 2.28 g/cc; list rebuild + force/energy/virial",
                    "atoms": n, "box_A": L[0] * 1e10, "parallelism": "single GPU"},
         "fp64": {"value": res["fp64"]["rate"], "ms_per_step": res["fp64"]["ms_per_step"],
-                 "kernels_ms": res["fp64"]["timing"]},
+                 "kernels_avg_ms": {k: v["avg_ms"] for k, v in res["fp64"]["timing"].items()}},
         "mixed_over_fp64": m["rate"] / res["fp64"]["rate"],
         "force_error_mixed_vs_fp64": {"max_rel": emax, "rms_rel": erms, "norm_rel": enorm,
                                       "floor": "1e-3 F_rms"},
         "energy_rel_diff": abs(m["utot"] - res["fp64"]["utot"]) / abs(res["fp64"]["utot"]),
-        "kernels_ms": tm,
+        "kernels_avg_ms": {k: v["avg_ms"] for k, v in tm.items()},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "bytes_per_atom": bpa, "avg_launch_ms": dur_ms},
         "cpu_baseline": cpu,
     }

@@ -67,11 +67,11 @@ class fib_step_report(C.Structure):
                 ("nlist_builds", C.c_int32), ("reserved", C.c_int32)]
 
 
+FIB_NTIMERS = 16
+
+
 class fib_timing(C.Structure):
-    _fields_ = [("sort_ms", C.c_double), ("nbuild_ms", C.c_double), ("force_a_ms", C.c_double),
-                ("force_b_ms", C.c_double), ("integrate_ms", C.c_double), ("total_ms", C.c_double),
-                ("n_sort", C.c_int32), ("n_nbuild", C.c_int32), ("n_force_a", C.c_int32),
-                ("n_force_b", C.c_int32), ("n_integrate", C.c_int32), ("reserved", C.c_int32)]
+    _fields_ = [("ms", C.c_double * FIB_NTIMERS), ("calls", C.c_int32 * FIB_NTIMERS)]
 
 
 vp, i32, i64, f64 = C.c_void_p, C.c_int32, C.c_int64, C.c_double
@@ -99,6 +99,7 @@ SIGNATURES = {
     "fib_temperature": ([vp, i32, vp], i32),
     "fib_timing_enable": ([vp, i32], i32),
     "fib_get_timing": ([vp, P(fib_timing)], i32),
+    "fib_timer_name": ([i32], C.c_char_p),
     "fib_synchronize": ([vp], i32),
 }
 

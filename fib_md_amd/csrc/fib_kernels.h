@@ -17,6 +17,29 @@ struct StepArgs {
   int32_t frz;               // apply frzsptrdatms in this step
 };
 
+// Per-kernel timing keys (index into fib_timing.ms).
+enum KKey {
+  KT_BIN = 0,   // k_bin
+  KT_SORT,      // scan, scatter, fixup, reorder
+  KT_NBUILD,    // k_nbuild
+  KT_LISTAUX,   // k_clear_counts, k_maxcnt
+  KT_REV,       // k_rev
+  KT_LEGS,      // k_legs
+  KT_TRI,       // k_tri
+  KT_GATHER,    // k_force3
+  KT_POS,       // k_pos
+  KT_REDUCE,    // k_finalize, k_sums
+  KT_IO,        // import / export / set-atom / cell-id kernels
+  KT_N
+};
+
+// Optional timing hook: the runtime brackets launches with events.
+struct KTimer {
+  void *ctx;
+  int (*begin)(void *ctx, int key);  // -1: not timed
+  void (*end)(void *ctx, int id, int key);
+};
+
 // All device buffers of a handle (sorted order, see fib_device.h).  Neighbour
 // lists, reverse slots, leg rows and G are atom-major: row i = [i * W, i * W + W).
 struct DevBufs {
@@ -44,6 +67,7 @@ struct DevBufs {
   int pstride;               // NA/64 + 1
   TabEntry *tab;             // [ntable + 3]
   double *xfer;              // staging [3*NA] for host transfers
+  const KTimer *kt;          // timing hook (may be null)
 };
 
 namespace fibk {

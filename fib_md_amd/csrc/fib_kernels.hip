@@ -216,6 +216,12 @@ __global__ void k_bin(const double4 *__restrict__ X, int NA, const DInst *__rest
   brank[p] = atomicAdd(&bcnt[b], 1);
 }
 
+// Zero fill (used instead of a memset node so the step can be captured).
+__global__ void k_zero_i32(int *__restrict__ p, int n) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < n) p[k] = 0;
+}
+
 // Export of the reference cell ids in original atom order.
 __global__ void k_cellid(const double4 *__restrict__ X, int NA, const DInst *__restrict__ I,
                          int inst, int32_t *__restrict__ out) {
@@ -365,7 +371,8 @@ __global__ __launch_bounds__(FIB_WAVE) void k_nbuild(
   __shared__ int s_j[NB_CHUNK];
   __shared__ int s_rs[27], s_off[28];
   const int lane = threadIdx.x;
-  const int gcell = blockIdx.x;
+  // each XCD walks a contiguous run of cells, so halos stay in its L2
+  const int gcell = xcd_remap(blockIdx.x, gridDim.x);
   if (gcell >= total_cells) return;
   const int k = find_inst_by_cell(I, ninst, gcell);
   const DInst &in = I[k];
@@ -1285,6 +1292,18 @@ __global__ void k_set_atom(int NA, double4 *__restrict__ X, double *__restrict__
 // launchers
 static inline int nblk(long n, int b) { return (int)((n + b - 1) / b); }
 
+// Times the launches issued in its scope under one key.
+struct KScope {
+  const KTimer *t;
+  int id, key;
+  KScope(const DevBufs &d, int k) : t(d.kt), id(-1), key(k) {
+    if (t && t->begin) id = t->begin(t->ctx, k);
+  }
+  ~KScope() {
+    if (id >= 0) t->end(t->ctx, id, key);
+  }
+};
+
 static void finalize(const DevBufs &d, int nv, int bs, const int *op, const int *dst,
                      hipStream_t s, int nreg = 1) {
   RedSpec rs;
@@ -1297,16 +1316,25 @@ static void finalize(const DevBufs &d, int nv, int bs, const int *op, const int 
   }
   switch (nv) {
     case 2:
-      hipLaunchKernelGGL(k_finalize<2>, dim3(d.ninst), dim3(FIN_T), 0, s, d.I, d.partials,
-                         d.pstride, rs, d.ctl);
+      {
+        KScope ks_(d, KT_REDUCE);
+        hipLaunchKernelGGL(k_finalize<2>, dim3(d.ninst), dim3(FIN_T), 0, s, d.I, d.partials,
+                           d.pstride, rs, d.ctl);
+      }
       break;
     case 5:
-      hipLaunchKernelGGL(k_finalize<5>, dim3(d.ninst), dim3(FIN_T), 0, s, d.I, d.partials,
-                         d.pstride, rs, d.ctl);
+      {
+        KScope ks_(d, KT_REDUCE);
+        hipLaunchKernelGGL(k_finalize<5>, dim3(d.ninst), dim3(FIN_T), 0, s, d.I, d.partials,
+                           d.pstride, rs, d.ctl);
+      }
       break;
     default:
-      hipLaunchKernelGGL(k_finalize<10>, dim3(d.ninst), dim3(FIN_T), 0, s, d.I, d.partials,
-                         d.pstride, rs, d.ctl);
+      {
+        KScope ks_(d, KT_REDUCE);
+        hipLaunchKernelGGL(k_finalize<10>, dim3(d.ninst), dim3(FIN_T), 0, s, d.I, d.partials,
+                           d.pstride, rs, d.ctl);
+      }
       break;
   }
 }
@@ -1319,66 +1347,111 @@ static void finalize_thermo(const DevBufs &d, hipStream_t s) {
 }
 
 void launch_bin(const DevBufs &d, hipStream_t s) {
-  (void)hipMemsetAsync(d.bcnt, 0, sizeof(int) * (size_t)d.nbucket, s);
-  hipLaunchKernelGGL(k_bin, dim3(nblk(d.NA, FIB_BLOCK)), dim3(FIB_BLOCK), 0, s, d.X[d.cur], d.NA,
-                     d.I, d.bkey, d.brank, d.bcnt, d.ctl);
+  {
+    KScope ks_(d, KT_BIN);
+    hipLaunchKernelGGL(k_zero_i32, dim3(nblk(d.nbucket, FIB_BLOCK)), dim3(FIB_BLOCK), 0, s, d.bcnt,
+                       d.nbucket);
+  }
+  {
+    KScope ks_(d, KT_BIN);
+    hipLaunchKernelGGL(k_bin, dim3(nblk(d.NA, FIB_BLOCK)), dim3(FIB_BLOCK), 0, s, d.X[d.cur], d.NA,
+                       d.I, d.bkey, d.brank, d.bcnt, d.ctl);
+  }
 }
 
 void launch_scan(const DevBufs &d, hipStream_t s) {
   const int nb = nblk(d.nbucket, SCAN_CH);
-  hipLaunchKernelGGL(k_scan_reduce, dim3(nb), dim3(SCAN_T), 0, s, d.bcnt, d.nbucket, d.bsum);
-  hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(SCAN_T), 0, s, d.bsum, nb);
-  hipLaunchKernelGGL(k_scan_down, dim3(nb), dim3(SCAN_T), 0, s, d.bcnt, d.nbucket, d.bsum, d.I,
-                     d.ninst, d.cstart);
+  {
+    KScope ks_(d, KT_SORT);
+    hipLaunchKernelGGL(k_scan_reduce, dim3(nb), dim3(SCAN_T), 0, s, d.bcnt, d.nbucket, d.bsum);
+  }
+  {
+    KScope ks_(d, KT_SORT);
+    hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(SCAN_T), 0, s, d.bsum, nb);
+  }
+  {
+    KScope ks_(d, KT_SORT);
+    hipLaunchKernelGGL(k_scan_down, dim3(nb), dim3(SCAN_T), 0, s, d.bcnt, d.nbucket, d.bsum, d.I,
+                       d.ninst, d.cstart);
+  }
 }
 
 void launch_sort(const DevBufs &d, hipStream_t s) {
-  hipLaunchKernelGGL(k_scatter, dim3(nblk(d.NA, FIB_BLOCK)), dim3(FIB_BLOCK), 0, s, d.bkey,
-                     d.brank, d.NA, d.cstart, d.perm);
-  hipLaunchKernelGGL(k_fixup, dim3(nblk(d.nbucket, FIB_BLOCK)), dim3(FIB_BLOCK), 0, s, d.cstart,
-                     d.bcnt, d.nbucket, d.perm);
+  {
+    KScope ks_(d, KT_SORT);
+    hipLaunchKernelGGL(k_scatter, dim3(nblk(d.NA, FIB_BLOCK)), dim3(FIB_BLOCK), 0, s, d.bkey,
+                       d.brank, d.NA, d.cstart, d.perm);
+  }
+  {
+    KScope ks_(d, KT_SORT);
+    hipLaunchKernelGGL(k_fixup, dim3(nblk(d.nbucket, FIB_BLOCK)), dim3(FIB_BLOCK), 0, s, d.cstart,
+                       d.bcnt, d.nbucket, d.perm);
+  }
   const int c = d.cur, n = 1 - d.cur;
-  hipLaunchKernelGGL(k_reorder, dim3(nblk(d.NA, FIB_BLOCK)), dim3(FIB_BLOCK), 0, s, d.NA, d.perm,
-                     d.I, d.X[c], d.X[n], d.V[c], d.V[n], d.F[c], d.F[n], d.Xi[c], d.Xi[n],
-                     d.inv);
+  {
+    KScope ks_(d, KT_SORT);
+    hipLaunchKernelGGL(k_reorder, dim3(nblk(d.NA, FIB_BLOCK)), dim3(FIB_BLOCK), 0, s, d.NA, d.perm,
+                       d.I, d.X[c], d.X[n], d.V[c], d.V[n], d.F[c], d.F[n], d.Xi[c], d.Xi[n],
+                       d.inv);
+  }
 }
 
 void launch_nbuild(const DevBufs &d, const FConst &fc, hipStream_t s) {
-  hipLaunchKernelGGL(k_clear_counts, dim3(nblk(d.NA, FIB_BLOCK)), dim3(FIB_BLOCK), 0, s, d.NA,
-                     d.cnt_sw, d.cnt_ion);
-  if (d.total_cells > 0)
+  {
+    KScope ks_(d, KT_LISTAUX);
+    hipLaunchKernelGGL(k_clear_counts, dim3(nblk(d.NA, FIB_BLOCK)), dim3(FIB_BLOCK), 0, s, d.NA,
+                       d.cnt_sw, d.cnt_ion);
+  }
+  if (d.total_cells > 0) {
+    KScope ks_(d, KT_NBUILD);
     hipLaunchKernelGGL(k_nbuild, dim3(d.total_cells), dim3(FIB_WAVE), 0, s, d.X[d.cur], d.I,
                        d.ninst, d.total_cells, d.cstart, d.bcnt, fc.rv2, fc.rv2lj, d.NA, d.Wsw,
                        d.Wion, d.cnt_sw, d.nbr_sw, d.cnt_ion, d.nbr_ion, d.ctl);
-  hipLaunchKernelGGL(k_maxcnt, dim3(d.NA / FIB_BLOCK), dim3(FIB_BLOCK), 0, s, d.cnt_sw,
-                     d.cnt_ion, d.pstride, d.partials);
+  }
+  {
+    KScope ks_(d, KT_LISTAUX);
+    hipLaunchKernelGGL(k_maxcnt, dim3(d.NA / FIB_BLOCK), dim3(FIB_BLOCK), 0, s, d.cnt_sw,
+                       d.cnt_ion, d.pstride, d.partials);
+  }
   const int op[2] = {1, 1};
   const int dst[2] = {DCTL_IDX(maxcnt_sw), DCTL_IDX(maxcnt_ion)};
   finalize(d, 2, FIB_BLOCK, op, dst, s);
 }
 
 void launch_cellid(const DevBufs &d, int inst, int32_t *dout, hipStream_t s) {
-  hipLaunchKernelGGL(k_cellid, dim3(nblk(d.NA, FIB_BLOCK)), dim3(FIB_BLOCK), 0, s, d.X[d.cur],
-                     d.NA, d.I, inst, dout);
+  {
+    KScope ks_(d, KT_IO);
+    hipLaunchKernelGGL(k_cellid, dim3(nblk(d.NA, FIB_BLOCK)), dim3(FIB_BLOCK), 0, s, d.X[d.cur],
+                       d.NA, d.I, inst, dout);
+  }
 }
 
 void launch_rev(const DevBufs &d, hipStream_t s) {
   int sh = 0;
   while ((1 << sh) < d.Wsw) ++sh;
-  hipLaunchKernelGGL(k_rev, dim3((unsigned)(((size_t)d.NA << sh) / FIB_BLOCK)), dim3(FIB_BLOCK), 0,
-                     s, d.NA, sh, d.cnt_sw, d.nbr_sw, d.rev);
+  {
+    KScope ks_(d, KT_REV);
+    hipLaunchKernelGGL(k_rev, dim3((unsigned)(((size_t)d.NA << sh) / FIB_BLOCK)), dim3(FIB_BLOCK), 0,
+                       s, d.NA, sh, d.cnt_sw, d.nbr_sw, d.rev);
+  }
 }
 
 template <bool MIXED, bool EN, bool VIR>
 static void force_ab(const DevBufs &d, const FConst &fc, hipStream_t s) {
   typedef typename Prec<MIXED>::T4 T4;
-  hipLaunchKernelGGL((k_legs<MIXED, EN, VIR>), dim3(d.NA / FIB_BLOCK), dim3(FIB_BLOCK), 0, s,
-                     d.X[d.cur], d.I, fc, d.tab, d.NA, d.Wsw, d.Wion, d.cnt_sw, d.nbr_sw,
-                     d.cnt_ion, d.nbr_ion, (T4 *)d.LA, (T4 *)d.LB, d.cnt_leg, d.amask, d.Fpart,
-                     d.upart, d.pstride, d.partials);
-  hipLaunchKernelGGL((k_tri<MIXED, EN, VIR>), dim3(d.NA / FIB_BLOCK), dim3(FIB_BLOCK), 0, s,
-                     d.X[d.cur], fc, d.NA, d.cnt_leg, (const T4 *)d.LA, (const T4 *)d.LB,
-                     (T4 *)d.G, d.pstride, d.partials + (size_t)FIB_NRED * d.pstride);
+  {
+    KScope ks_(d, KT_LEGS);
+    hipLaunchKernelGGL((k_legs<MIXED, EN, VIR>), dim3(d.NA / FIB_BLOCK), dim3(FIB_BLOCK), 0, s,
+                       d.X[d.cur], d.I, fc, d.tab, d.NA, d.Wsw, d.Wion, d.cnt_sw, d.nbr_sw,
+                       d.cnt_ion, d.nbr_ion, (T4 *)d.LA, (T4 *)d.LB, d.cnt_leg, d.amask, d.Fpart,
+                       d.upart, d.pstride, d.partials);
+  }
+  {
+    KScope ks_(d, KT_TRI);
+    hipLaunchKernelGGL((k_tri<MIXED, EN, VIR>), dim3(d.NA / FIB_BLOCK), dim3(FIB_BLOCK), 0, s,
+                       d.X[d.cur], fc, d.NA, d.cnt_leg, (const T4 *)d.LA, (const T4 *)d.LB,
+                       (T4 *)d.G, d.pstride, d.partials + (size_t)FIB_NRED * d.pstride);
+  }
   if (EN || VIR) {
     int op[10], dst[10];
     for (int k = 0; k < 10; ++k) { op[k] = 0; dst[k] = DCTL_IDX(utot) + k; }
@@ -1392,10 +1465,13 @@ static void force_c(const DevBufs &d, const FConst &fc, const StepArgs &sa, doub
   typedef typename Prec<MIXED>::T4 T4;
   const double fscale = MIXED ? fc.eps[0][0] * fc.isig[0][0] : 1.0;
   const double escale = MIXED ? fc.eps[0][0] : 1.0;
-  hipLaunchKernelGGL((k_force3<MIXED, EN, MODE, SEARCH>), dim3(d.NA / FIB_BLOCK), dim3(FIB_BLOCK),
-                     0, s, d.X[d.cur], d.I, d.NA, d.Wsw, d.nbr_sw, d.rev, (const T4 *)d.G,
-                     d.amask, d.cnt_leg, d.cnt_sw, d.Fpart, d.upart, fscale, escale, Fo,
-                     d.uout, d.V[d.cur], sa, d.pstride, d.partials);
+  {
+    KScope ks_(d, KT_GATHER);
+    hipLaunchKernelGGL((k_force3<MIXED, EN, MODE, SEARCH>), dim3(d.NA / FIB_BLOCK), dim3(FIB_BLOCK),
+                       0, s, d.X[d.cur], d.I, d.NA, d.Wsw, d.nbr_sw, d.rev, (const T4 *)d.G,
+                       d.amask, d.cnt_leg, d.cnt_sw, d.Fpart, d.upart, fscale, escale, Fo,
+                       d.uout, d.V[d.cur], sa, d.pstride, d.partials);
+  }
   if (MODE == 1) finalize_thermo(d, s);
 }
 
@@ -1432,51 +1508,72 @@ void launch_force_b(const DevBufs &d, const FConst &fc, bool mixed, bool energy,
 }
 
 void launch_pos(const DevBufs &d, const StepArgs &sa, hipStream_t s) {
-  hipLaunchKernelGGL(k_pos, dim3(d.NA / FIB_BLOCK), dim3(FIB_BLOCK), 0, s, d.X[d.cur], d.I, d.NA,
-                     d.V[d.cur], d.F[d.cur], d.Xi[d.cur], sa, d.pstride, d.partials, d.ctl);
+  {
+    KScope ks_(d, KT_POS);
+    hipLaunchKernelGGL(k_pos, dim3(d.NA / FIB_BLOCK), dim3(FIB_BLOCK), 0, s, d.X[d.cur], d.I, d.NA,
+                       d.V[d.cur], d.F[d.cur], d.Xi[d.cur], sa, d.pstride, d.partials, d.ctl);
+  }
   const int op[2] = {0, 1};
   const int dst[2] = {DCTL_IDX(eq_sum), DCTL_IDX(kemax)};
   finalize(d, 2, FIB_BLOCK, op, dst, s);
 }
 
 void launch_sums(const DevBufs &d, const StepArgs &sa, hipStream_t s) {
-  hipLaunchKernelGGL(k_sums, dim3(d.NA / FIB_BLOCK), dim3(FIB_BLOCK), 0, s, d.X[d.cur], d.I, d.NA,
-                     d.V[d.cur], sa, d.pstride, d.partials, d.ctl);
+  {
+    KScope ks_(d, KT_REDUCE);
+    hipLaunchKernelGGL(k_sums, dim3(d.NA / FIB_BLOCK), dim3(FIB_BLOCK), 0, s, d.X[d.cur], d.I, d.NA,
+                       d.V[d.cur], sa, d.pstride, d.partials, d.ctl);
+  }
   finalize_thermo(d, s);
 }
 
 void launch_import(const DevBufs &d, int inst, const DInst &in, const double *X,
                    const double *V, const int32_t *atype, const int32_t *xsides,
                    const double *Xi, hipStream_t s) {
-  hipLaunchKernelGGL(k_import, dim3(nblk(in.natm_pad, FIB_BLOCK)), dim3(FIB_BLOCK), 0, s, inst, in,
-                     d.NA, X, V, atype, xsides, Xi, d.X[0], d.X[1], d.V[0], d.V[1], d.F[0],
-                     d.F[1], d.Xi[0], d.Xi[1], d.inv);
+  {
+    KScope ks_(d, KT_IO);
+    hipLaunchKernelGGL(k_import, dim3(nblk(in.natm_pad, FIB_BLOCK)), dim3(FIB_BLOCK), 0, s, inst, in,
+                       d.NA, X, V, atype, xsides, Xi, d.X[0], d.X[1], d.V[0], d.V[1], d.F[0],
+                       d.F[1], d.Xi[0], d.Xi[1], d.inv);
+  }
 }
 
 void launch_export_x(const DevBufs &d, int inst, const DInst &in, double *out, hipStream_t s) {
   (void)inst;
-  hipLaunchKernelGGL(k_export_x, dim3(nblk(in.natm_pad, FIB_BLOCK)), dim3(FIB_BLOCK), 0, s, in,
-                     d.X[d.cur], out);
+  {
+    KScope ks_(d, KT_IO);
+    hipLaunchKernelGGL(k_export_x, dim3(nblk(in.natm_pad, FIB_BLOCK)), dim3(FIB_BLOCK), 0, s, in,
+                       d.X[d.cur], out);
+  }
 }
 
 void launch_export_vec(const DevBufs &d, int inst, const DInst &in, const double *src,
                        double *out, hipStream_t s) {
   (void)inst;
-  hipLaunchKernelGGL(k_export_vec, dim3(nblk(in.natm_pad, FIB_BLOCK)), dim3(FIB_BLOCK), 0, s, in,
-                     d.NA, d.X[d.cur], src, out);
+  {
+    KScope ks_(d, KT_IO);
+    hipLaunchKernelGGL(k_export_vec, dim3(nblk(in.natm_pad, FIB_BLOCK)), dim3(FIB_BLOCK), 0, s, in,
+                       d.NA, d.X[d.cur], src, out);
+  }
 }
 
 void launch_export_scalar(const DevBufs &d, int inst, const DInst &in, const double *src,
                           double *out, hipStream_t s) {
   (void)inst;
-  hipLaunchKernelGGL(k_export_scalar, dim3(nblk(in.natm_pad, FIB_BLOCK)), dim3(FIB_BLOCK), 0, s,
-                     in, d.X[d.cur], src, out);
+  {
+    KScope ks_(d, KT_IO);
+    hipLaunchKernelGGL(k_export_scalar, dim3(nblk(in.natm_pad, FIB_BLOCK)), dim3(FIB_BLOCK), 0, s,
+                       in, d.X[d.cur], src, out);
+  }
 }
 
 void launch_set_atom(const DevBufs &d, int p, const double x[3], const double v[3],
                      hipStream_t s) {
-  hipLaunchKernelGGL(k_set_atom, dim3(1), dim3(FIB_WAVE), 0, s, d.NA, d.X[d.cur], d.V[d.cur], p,
-                     x[0], x[1], x[2], v[0], v[1], v[2]);
+  {
+    KScope ks_(d, KT_IO);
+    hipLaunchKernelGGL(k_set_atom, dim3(1), dim3(FIB_WAVE), 0, s, d.NA, d.X[d.cur], d.V[d.cur], p,
+                       x[0], x[1], x[2], v[0], v[1], v[2]);
+  }
 }
 
 }  // namespace fibk

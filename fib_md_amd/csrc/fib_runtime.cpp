@@ -23,15 +23,16 @@ namespace {
 
 // Phase timers: 0 sort, 1 nbuild, 2 force_a, 3 force_b, 4 integrate.
 struct Timer {
-  static constexpr int NP = 5;
+  static constexpr int NP = FIB_NTIMERS;
   bool on = false;
   hipStream_t s = nullptr;
   std::vector<hipEvent_t> pool;
-  struct Rec { int phase; hipEvent_t a, b; };
+  struct Rec { int key; hipEvent_t a, b; };
   std::vector<Rec> recs;
   size_t used = 0;
-  double acc[NP] = {0, 0, 0, 0, 0};
-  int cnt[NP] = {0, 0, 0, 0, 0};
+  double acc[NP] = {};
+  int cnt[NP] = {};
+  KTimer hook{};
   hipEvent_t get() {
     if (used == pool.size()) {
       hipEvent_t e;
@@ -40,20 +41,21 @@ struct Timer {
     }
     return pool[used++];
   }
-  int begin() {
-    if (!on) return -1;
+  uint32_t mask = 0;  // timers that record (bit per key)
+  int begin(int key) {
+    if (!on || !((mask >> key) & 1u)) return -1;
     hipEvent_t a = get();
     (void)hipEventRecord(a, s);
     recs.push_back({-1, a, nullptr});
     return (int)recs.size() - 1;
   }
-  void end(int id, int phase) {
+  void end(int id, int key) {
     if (!on || id < 0) return;
     hipEvent_t b = get();
     (void)hipEventRecord(b, s);
-    recs[id].phase = phase;
+    recs[id].key = key;
     recs[id].b = b;
-    ++cnt[phase];
+    ++cnt[key];
     if (recs.size() > 4096) flush();
   }
   void flush() {
@@ -61,7 +63,7 @@ struct Timer {
     (void)hipStreamSynchronize(s);
     for (auto &r : recs) {
       float ms = 0;
-      if (r.b && hipEventElapsedTime(&ms, r.a, r.b) == hipSuccess) acc[r.phase] += ms;
+      if (r.key >= 0 && r.b && hipEventElapsedTime(&ms, r.a, r.b) == hipSuccess) acc[r.key] += ms;
     }
     recs.clear();
     used = 0;
@@ -70,7 +72,14 @@ struct Timer {
     flush();
     for (int k = 0; k < NP; ++k) { acc[k] = 0; cnt[k] = 0; }
   }
+  static int begin_cb(void *c, int key) { return static_cast<Timer *>(c)->begin(key); }
+  static void end_cb(void *c, int id, int key) { static_cast<Timer *>(c)->end(id, key); }
 };
+
+static const char *const kTimerNames[KT_N] = {
+    "k_bin", "sort", "k_nbuild", "list_aux", "k_rev", "k_legs",
+    "k_tri", "k_force3", "k_pos", "reduce", "io"};
+static_assert(KT_N <= FIB_NTIMERS, "timer table");
 
 struct RefCtl {
   bool have_run = false;
@@ -345,6 +354,8 @@ extern "C" int32_t fib_create(const fib_params *p, int32_t ninst, const fib_inst
     return FIB_ERR_HIP;
   }
   h->tim.s = h->stream;
+  h->tim.hook = KTimer{&h->tim, &Timer::begin_cb, &Timer::end_cb};
+  h->d.kt = &h->tim.hook;
   int32_t rc_ = FIB_OK;
 #define ALLOC(ptr, n)                                           \
   if (rc_ == FIB_OK && dalloc(&(ptr), (n)) != hipSuccess) {     \
@@ -533,17 +544,13 @@ extern "C" int32_t fib_bin_cells(fib_handle *h, int32_t inst, int32_t *icell) {
 // overflowed (the reference only prints, si.f90:264-272).
 static int32_t build_lists(fib_handle *h) {
   DevBufs &d = h->d;
-  int tid = h->tim.begin();
   fibk::launch_bin(d, h->stream);
   fibk::launch_scan(d, h->stream);
   fibk::launch_sort(d, h->stream);
-  h->tim.end(tid, 0);
   d.cur = 1 - d.cur;
   for (int attempt = 0; attempt < 4; ++attempt) {
     // k_nbuild reports max list lengths into DCtl::maxcnt_*
-    tid = h->tim.begin();
     fibk::launch_nbuild(d, h->fc, h->stream);
-    h->tim.end(tid, 1);
     std::vector<DCtl> c(d.ninst);
     HIPCHK(h, hipMemcpyAsync(c.data(), d.ctl, sizeof(DCtl) * d.ninst, hipMemcpyDeviceToHost,
                              h->stream));
@@ -645,9 +652,7 @@ extern "C" int32_t fib_get_nlist(fib_handle *h, int32_t inst, int32_t *nss, int3
 // list build (MD steps); a single evaluation searches instead.
 static void ensure_rev(fib_handle *h) {
   if (h->rev_valid || !h->have_lists) return;
-  int tid = h->tim.begin();
   fibk::launch_rev(h->d, h->stream);
-  h->tim.end(tid, 1);
   h->rev_valid = true;
 }
 
@@ -658,13 +663,9 @@ static int32_t run_force(fib_handle *h, int flags, int mode, const StepArgs &sa,
   const bool mixed = h->prec == FIB_PREC_MIXED;
   DevBufs &d = h->d;
   double *Fo = mode == 0 ? (dst ? dst : d.Fout) : d.F[d.cur];
-  int tid = h->tim.begin();
   fibk::launch_force_a(d, h->fc, mixed, en, vir, h->stream);
-  h->tim.end(tid, 2);
-  tid = h->tim.begin();
   fibk::launch_force_b(d, h->fc, mixed, en && mode == 0, mode, !h->rev_valid, sa, Fo,
                        h->stream);
-  h->tim.end(tid, 3);
   return FIB_OK;
 }
 
@@ -867,9 +868,7 @@ extern "C" int32_t fib_step(fib_handle *h, const fib_step_ctl *ctl, fib_step_rep
     R.time = R.time + R.ts;
     StepArgs sa = step_args(h, w == 0 ? 0 : 1);
     sa.frz = (lt % R.atlist == 0) ? 1 : 0;
-    int tid = h->tim.begin();
     fibk::launch_pos(h->d, sa, h->stream);
-    h->tim.end(tid, 4);
     if (w == 0) {
       DCtl c;
       int32_t r = read_ctl(h, 0, &c);
@@ -929,6 +928,7 @@ extern "C" int32_t fib_timing_enable(fib_handle *h, int32_t on) {
   if (!h) return FIB_ERR_ARG;
   h->tim.reset();
   h->tim.on = on != 0;
+  h->tim.mask = (uint32_t)on;
   return FIB_OK;
 }
 
@@ -936,13 +936,15 @@ extern "C" int32_t fib_get_timing(fib_handle *h, fib_timing *t) {
   if (!h || !t) return FIB_ERR_ARG;
   h->tim.flush();
   std::memset(t, 0, sizeof(*t));
-  const Timer &T = h->tim;
-  t->sort_ms = T.acc[0]; t->nbuild_ms = T.acc[1]; t->force_a_ms = T.acc[2];
-  t->force_b_ms = T.acc[3]; t->integrate_ms = T.acc[4];
-  t->total_ms = T.acc[0] + T.acc[1] + T.acc[2] + T.acc[3] + T.acc[4];
-  t->n_sort = T.cnt[0]; t->n_nbuild = T.cnt[1]; t->n_force_a = T.cnt[2];
-  t->n_force_b = T.cnt[3]; t->n_integrate = T.cnt[4];
+  for (int k = 0; k < FIB_NTIMERS; ++k) {
+    t->ms[k] = h->tim.acc[k];
+    t->calls[k] = h->tim.cnt[k];
+  }
   return FIB_OK;
+}
+
+extern "C" const char *fib_timer_name(int32_t k) {
+  return (k >= 0 && k < KT_N) ? kTimerNames[k] : nullptr;
 }
 
 extern "C" int32_t fib_synchronize(fib_handle *h) {

@@ -200,13 +200,30 @@ class FibGPU:
         self._chk("fib_temperature", self.L.fib_temperature(self.h, inst, _p(t)))
         return float(t[0])
 
-    def timing(self, enable=None):
+    def timing(self, enable=None, only=None):
+        """timing(enable=True/False[, only=names]) starts/stops the device
+        kernel timers (all, or the named ones); timing() returns
+        {timer name: {ms, calls, avg_ms}} since the start."""
         if enable is not None:
-            self._chk("fib_timing_enable", self.L.fib_timing_enable(self.h, 1 if enable else 0))
+            mask = 0
+            if enable:
+                names = [self.L.fib_timer_name(k) for k in range(L.FIB_NTIMERS)]
+                for k, nm in enumerate(names):
+                    if nm is not None and (only is None or nm.decode() in only):
+                        mask |= 1 << k
+            self._chk("fib_timing_enable", self.L.fib_timing_enable(self.h, mask))
             return None
         t = L.fib_timing()
         self._chk("fib_get_timing", self.L.fib_get_timing(self.h, C.byref(t)))
-        return {k: getattr(t, k) for k, _ in L.fib_timing._fields_ if k != "reserved"}
+        out = {}
+        for k in range(L.FIB_NTIMERS):
+            name = self.L.fib_timer_name(k)
+            if name is None:
+                break
+            n = int(t.calls[k])
+            if n:
+                out[name.decode()] = {"ms": float(t.ms[k]), "calls": n, "avg_ms": float(t.ms[k]) / n}
+        return out
 
     def synchronize(self):
         self._chk("fib_synchronize", self.L.fib_synchronize(self.h))

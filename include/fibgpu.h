@@ -200,14 +200,17 @@ int32_t fib_step(fib_handle *h, const fib_step_ctl *ctl, fib_step_report *rep);
 /* Recompute the reference's temperature (temp.f90:183-205). */
 int32_t fib_temperature(fib_handle *h, int32_t inst, double *temp);
 
-/* Device-side phase timers (HIP events on the handle's stream), accumulated
- * since fib_timing_enable(h, 1); for benchmarks.  sort = binning + counting
- * sort + reorder, nbuild = list build + reverse-slot map, force_a/force_b =
- * the two force kernels, integrate = the position-update kernel. */
+/* Device-side kernel timers (HIP events on the handle's stream around each
+ * launch), accumulated since fib_timing_enable(h, mask); for benchmarks.
+ * Bit k of mask turns on timer k, which covers the kernels fib_timer_name(k)
+ * names; mask 0 turns timing off. */
+#define FIB_NTIMERS 16
 typedef struct fib_timing {
-  double sort_ms, nbuild_ms, force_a_ms, force_b_ms, integrate_ms, total_ms;
-  int32_t n_sort, n_nbuild, n_force_a, n_force_b, n_integrate, reserved;
+  double ms[FIB_NTIMERS];      /* accumulated device time */
+  int32_t calls[FIB_NTIMERS];  /* timed launches */
 } fib_timing;
+/* Name of timer k, or NULL when k is not a timer. */
+const char *fib_timer_name(int32_t k);
 int32_t fib_timing_enable(fib_handle *h, int32_t on);
 int32_t fib_get_timing(fib_handle *h, fib_timing *t);
 int32_t fib_synchronize(fib_handle *h);
