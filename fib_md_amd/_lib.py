@@ -64,7 +64,7 @@ class fib_step_report(C.Structure):
     _fields_ = [("lt", C.c_int64), ("time", C.c_double), ("temp", C.c_double),
                 ("ts", C.c_double), ("steps_done", C.c_int32), ("ended", C.c_int32),
                 ("output_due", C.c_int32), ("regime", C.c_int32),
-                ("nlist_builds", C.c_int32), ("reserved", C.c_int32)]
+                ("nlist_builds", C.c_int32), ("atoms_due", C.c_int32)]
 
 
 FIB_NTIMERS = 16

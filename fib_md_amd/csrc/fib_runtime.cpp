@@ -893,17 +893,19 @@ extern "C" int32_t fib_step(fib_handle *h, const fib_step_ctl *ctl, fib_step_rep
     int32_t r = run_force(h, FIB_EVAL_FORCE, 1, sb);
     if (r != FIB_OK) return r;
     ++steps;
+    // iostuff (out.f90:129-147), called by onestep for whichctrl > 0 only:
+    // on either cadence the temperature is recomputed
     int out = 0;
     if (w > 0) {
-      out = (P.res_out > 0 && lt % P.res_out == 0) || (R.atm_out > 0 && lt % R.atm_out == 0);
+      const int atoms = (R.atm_out > 0 && lt % R.atm_out == 0);
+      out = (P.res_out > 0 && lt % P.res_out == 0) || atoms;
       if (out) {
         DCtl c;
         r = read_ctl(h, 0, &c);
         if (r != FIB_OK) return r;
         R.temp = temp_of(h, c);
+        rep->atoms_due = atoms;
       }
-    } else {
-      out = (R.atm_out > 0 && lt % R.atm_out == 0);
     }
     if (out) {
       rep->output_due = 1;

@@ -45,6 +45,7 @@ class StepReport:
     output_due: bool
     regime: int
     nlist_builds: int
+    atoms_due: bool
 
 
 class FibGPU:
@@ -193,7 +194,7 @@ class FibGPU:
         rep = L.fib_step_report()
         self._chk("fib_step", self.L.fib_step(self.h, C.byref(ctl), C.byref(rep)))
         return StepReport(rep.lt, rep.time, rep.temp, rep.ts, rep.steps_done, bool(rep.ended),
-                          bool(rep.output_due), rep.regime, rep.nlist_builds)
+                          bool(rep.output_due), rep.regime, rep.nlist_builds, bool(rep.atoms_due))
 
     def temperature(self, inst=0):
         t = np.zeros(1)

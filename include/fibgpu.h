@@ -179,10 +179,11 @@ typedef struct fib_step_report {
   double ts;              /* current timestep (s)                          */
   int32_t steps_done;     /* onestep calls in this fib_step                */
   int32_t ended;          /* TI loop condition reached                     */
-  int32_t output_due;     /* last step is on the atm_out/res_out cadence   */
+  int32_t output_due;     /* a step on the atm_out/res_out cadence ended the
+                             call (stop_on_output) or occurred in it      */
   int32_t regime;         /* 1 = fast (Ts_i), 2 = slow (Ts_r)              */
   int32_t nlist_builds;   /* rebuilds performed in this call               */
-  int32_t reserved;
+  int32_t atoms_due;      /* that step is on the atm_out cadence (writeatoms) */
 } fib_step_report;
 
 /* Begin the reference's run on instance 0: lt = lt0, time = time0, initial

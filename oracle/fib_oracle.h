@@ -13,6 +13,11 @@
  * -ffp-contract=off so no FMA is formed (the reference's x86-64 gfortran build
  * has none either).
  *
+ * Parity unpinned: the order of the steps inside orc_onestep / orc_ti_iter
+ * (timeint.f90:36-242) as a sequence.  Each step is pinned on its own by the
+ * tests; their integrated order is a restatement that only the compiled
+ * reference could pin, and that cannot be built without an MPI module.
+ *
  * Array conventions: positions/velocities/forces are Fortran (Natm,3)
  * column-major arrays, i.e. x[0..N), y[0..N), z[0..N).  Atom indices stored in
  * neighbor / triplet lists are 1-based, exactly as in the reference.

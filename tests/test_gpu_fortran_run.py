@@ -1,0 +1,95 @@
+"""End-to-end run of the Fortran host (fortran/mdrun_gpu) on the GPU against
+the oracle: tests/golden/siga_small.in (3x3x2 cells, 2 Ga ions at 500 eV,
+7 deg / 30 deg incidence), fixed seed.  The oracle replays main/tint
+(main.f90:31-74, timeint.f90:36-120) with the same generators; the
+data/mdrun2.xyz frames must agree: same frame count, lt cadence, headers,
+labels and rank column, positions within 2e-4 A and KE within one unit of
+the printed last digit (the two FP64 force sums add in different orders)."""
+import os
+import shutil
+import subprocess
+
+import numpy as np
+import pytest
+
+from oracle.oracle import Oracle, siga_from_values
+from tests.test_fortran_host import EXPECTED, SIGA
+
+pytestmark = pytest.mark.gpu
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+EXE = os.path.join(ROOT, "fortran", "mdrun_gpu")
+SEED = -24680
+MASS = {1: 46.637063e-27, 3: 1.157777417e-25}
+
+
+def oracle_frames():
+    o = Oracle()
+    o.readprms(siga_from_values(EXPECTED), SEED)
+    o.initranions()
+    o.initdata()
+    o.init_nlist(); o.mktables()
+    X, V, _ = o.state()
+    o.si_nlist(X)
+    at, _ = o.types()
+    frames = []
+
+    def frame():
+        X, V, _ = o.state()
+        frames.append((o.getd("time"), X.copy(), V.copy()))
+
+    frames.append((0.0, X.copy(), V.copy()))
+    o.tint_init()
+    o.setd("Tau", 1e-14)
+    o.ti_begin(0)
+    while not o.ti_iter(0, EXPECTED["ttar1"]):
+        pass
+    for impact in range(1, EXPECTED["nlj"] + 1):
+        o.seti("impact", impact); o.seti("atom_is_fast", 1); o.seti("atom_is_slow", 0)
+        o.ionrun(1)
+        o.knock(impact)
+        while not o.ti_iter(1, EXPECTED["ttar1"]):
+            lt, ao = o.geti("lt"), o.geti("atm_out")
+            if ao > 0 and lt % ao == 0:
+                frame()
+        frame()
+    return frames, at
+
+
+def parse(path):
+    lines = open(path).read().split("\n")[:-1]
+    n = int(lines[0])
+    out = []
+    for f in range(len(lines) // (n + 2)):
+        b = f * (n + 2)
+        hdr = lines[b + 1]
+        rows = lines[b + 2:b + 2 + n]
+        out.append((hdr, rows))
+    return n, out
+
+
+def test_fortran_driver_matches_oracle(tmp_path, oracle_lib):
+    assert os.path.exists(EXE), "build fortran/mdrun_gpu first"
+    shutil.copy(SIGA, tmp_path / "siga.in")
+    os.makedirs(tmp_path / "data")
+    print(f"mdrun_gpu --seed {SEED} in {tmp_path}", flush=True)
+    r = subprocess.run([EXE, "--seed", str(SEED)], cwd=tmp_path, capture_output=True, text=True,
+                       timeout=300)
+    assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
+    frames, at = oracle_frames()
+    n, got = parse(tmp_path / "data" / "mdrun2.xyz")
+    assert n == len(at)
+    assert len(got) == len(frames), (len(got), len(frames))
+    for (hdr, rows), (time, X, V) in zip(got, frames):
+        assert hdr.startswith("Time (ps) = ") and hdr[12:24] == f"{time * 1e12:12.5f}"
+        for i in range(n):
+            row = rows[i]
+            x = [float(row[12 * c:12 * c + 12]) for c in range(3)]
+            assert np.allclose(x, X[i] * 1e10, rtol=0, atol=2e-4), (i, row, X[i] * 1e10)
+            assert row[36:44] == f"{'Si' if at[i] == 1 else 'Ga':>8s}"
+            assert row[44:52] == f"{0:8d}"
+            ke = 0.5 * MASS[int(at[i])] * (V[i] ** 2).sum() * 6.252e18
+            got_ke = float(row[52:66])
+            assert abs(got_ke - ke) <= 1e-3 * abs(ke) + 1e-30, (i, row, ke)
+    snap = open(tmp_path / "data" / "3dmol2.xyz").read().split("\n")
+    assert int(snap[0]) == n

@@ -228,3 +228,34 @@ def test_regime_switch_on_max_ke(oracle_lib):
     o.calc_atm()
     assert o.getd("ts") == v["ts_r"] and o.geti("atom_is_fast") == 0
     assert o.geti("ntlist") == v["ntlist_r"] and o.geti("atm_out") == v["atm_out_r"]
+
+
+def test_sputter_layer_rules(oracle_lib):
+    """frzsptrdatms (temp.f90:99-127) with windows in units of T = Lb(3)/10:
+    (outz1, outz2) -> moved to atomz1, (outz3, outz4) -> moved to atomz2, both
+    with V = F = 0; within 0.3 T of atomz1, atomz2 or ionz -> V = F = 0 only;
+    everything else untouched.  Numbers worked by hand for Lb(3) = 100 A."""
+    A = 1e-10
+    z = np.array([20.0, 80.0, 46.0, 64.0, 57.0, 10.0, 40.0, 99.0, 13.1, 34.9]) * A
+    n = len(z)
+    X = np.column_stack([np.arange(n) * 5 * A + 2 * A, np.full(n, 50 * A), z])
+    L = (60 * A, 100 * A, 100 * A)
+    at = np.ones(n, np.int32)
+    V = np.full((n, 3), 123.0)
+    o = gc.oracle_system(X, at, L, n, 0, V=V)
+    v = dict(REFERENCE_SIGA)  # outz (1.3, 3.5, 7.5, 9.8), atomz (4.5, 6.5), ionz 5.5
+    assert tuple(v["outz"]) == (1.3, 3.5, 7.5, 9.8) and tuple(v["atomz"]) == (4.5, 6.5)
+    o.set_run(siga_from_values(v))
+    o.set_force(np.full((n, 3), 7.0))
+    o.frzsptrdatms()
+    Xn, Vn, Fn = o.state()
+    moved = {0: 45.0, 1: 65.0, 8: 45.0, 9: 45.0}
+    zeroed = {0, 1, 2, 3, 4, 8, 9}
+    for k in range(n):
+        exp_z = moved.get(k, z[k] / A)
+        assert abs(Xn[k, 2] / A - exp_z) < 1e-9, (k, Xn[k, 2] / A, exp_z)
+        assert Xn[k, 0] == X[k, 0] and Xn[k, 1] == X[k, 1]
+        if k in zeroed:
+            assert np.all(Vn[k] == 0) and np.all(Fn[k] == 0), k
+        else:
+            assert np.all(Vn[k] == 123.0) and np.all(Fn[k] == 7.0), k
