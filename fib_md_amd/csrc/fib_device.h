@@ -58,6 +58,8 @@ struct FConst {
   int32_t ntable;
   int32_t izero;      // first table index (1-based) whose T and successors are 0
   double mixed_r64;   // mixed precision: Moliere pairs below this use FP64
+  double skin;        // rc - (three-body range): pairs farther apart than rc at
+                      // the build need a relative move above this to form a leg
   double mixed_xleg;  // mixed precision: three-body legs need r/sigma below
                       // this, i.e. E = exp(gamma/(r/sigma - a)) > e^-30
 };
@@ -83,8 +85,9 @@ struct DCtl {
   double maxcnt_sw;     // longest SW / ion neighbour list of the last build
   double maxcnt_ion;
   double eq_sum;        // sum m v^2 of the rescaled V (equilibration)
+  double maxdisp;       // largest displacement since the last list build (m)
   int32_t nonfinite;    // set by k_bin on a NaN/Inf position
-  int32_t pad;
+  int32_t extra_any;    // k_extra found vertex terms outside a centre's list
 };
 
 // Double index of a DCtl field (for the reduction finalize kernel).

@@ -27,6 +27,7 @@ enum KKey {
   KT_LEGS,      // k_legs
   KT_TRI,       // k_tri
   KT_GATHER,    // k_force3
+  KT_EXTRA,     // k_extra
   KT_POS,       // k_pos
   KT_REDUCE,    // k_finalize, k_sums
   KT_IO,        // import / export / set-atom / cell-id kernels
@@ -62,6 +63,11 @@ struct DevBufs {
   uint64_t *amask;           // legs of the last evaluation, by list slot [NA]
   int *cnt_leg;              // legs per atom of the last evaluation [NA]
   double *Fpart, *upart;     // own-term force and energy [3*NA], [NA]
+  double4 *Xb;               // positions at the last list build [NA]
+  float *disp;               // |X - Xb| after the last position update [NA]
+  void *EX;                  // stale-graph terms: force for list slot k, T4 [NA * Wsw]
+  void *EXself;              // stale-graph terms: force on the bridge atom, T4 [NA]
+  uint64_t *exmask;          // slots of EX written in the last evaluation [NA]
   double *Fout, *uout;       // force-only results [3*NA], [NA]
   double *partials;          // [2 * FIB_NRED * pstride], layout [region][slot][block]
   int pstride;               // NA/64 + 1
@@ -79,6 +85,8 @@ void launch_nbuild(const DevBufs &d, const FConst &fc, hipStream_t s);
 void launch_cellid(const DevBufs &d, int inst, int32_t *dout, hipStream_t s);
 // reverse-slot map, after a successful build
 void launch_rev(const DevBufs &d, hipStream_t s);
+// positions of the new build (Xb), displacement bookkeeping reset
+void launch_mark_build(const DevBufs &d, hipStream_t s);
 // passes 1-2 (k_legs, k_tri): own pair terms and legs, centre-term shares
 // into G, energy and virial reductions
 void launch_force_a(const DevBufs &d, const FConst &fc, bool mixed, bool energy, bool virial,

@@ -28,6 +28,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <algorithm>
+
 #include "fib_device.h"
 #include "fib_kernels.h"
 
@@ -158,8 +160,8 @@ __global__ __launch_bounds__(FIN_T) void k_finalize(const DInst *__restrict__ I,
 #pragma unroll
     for (int k = 0; k < NV; ++k) {
       double y = partials[(size_t)k * pstride + blk0 + b];
-      if (rs.nreg > 1) {
-        const double y2 = partials[(size_t)(FIB_NRED + k) * pstride + blk0 + b];
+      for (int r = 1; r < rs.nreg; ++r) {
+        const double y2 = partials[(size_t)(r * FIB_NRED + k) * pstride + blk0 + b];
         y = rs.op[k] ? fmax(y, y2) : y + y2;
       }
       x[k] = rs.op[k] ? fmax(x[k], y) : x[k] + y;
@@ -521,6 +523,12 @@ __global__ __launch_bounds__(FIB_BLOCK) void k_maxcnt(const int *__restrict__ cn
   double v[2] = {(double)cnt_sw[i], (double)cnt_ion[i]};
   const int op[2] = {1, 1};
   block_partials<2, FIB_BLOCK>(v, op, blockIdx.x, pstride, partials);
+}
+
+// Slot of the n-th set bit of m (n is small).
+__device__ __forceinline__ int nth_bit(uint64_t m, int n) {
+  for (int r = 0; r < n; ++r) m &= m - 1;
+  return __ffsll((unsigned long long)m) - 1;
 }
 
 // ---------------------------------------------------------------------------
@@ -967,6 +975,198 @@ __global__ __launch_bounds__(FIB_BLOCK) void k_tri(
 }
 
 // ---------------------------------------------------------------------------
+// Vertex terms of the build-time graph that the centre's own list misses.
+//
+// The reference stores its triplets once per list build (si.f90:191-232):
+// every connected triple of the buffered graph.  Between builds the vertex
+// term centred on c with legs a and b is evaluated whenever both legs are in
+// range *now* (si.f90:491-509), so after atoms have moved more than the list
+// skin a term can exist whose leg c-a is not a listed pair, the triple being
+// connected through the pairs c-b and b-a instead.  k_legs/k_tri only pair
+// legs from c's own list; this pass adds the rest, exactly and without
+// atomics: the term (a, c, b) is found by the bridge atom b, which lists both
+// a and c.  b keeps its own share and writes the forces on a and c into
+// EX[slot][b] by its list slots of a and c; a and c gather them like G.
+// Nothing can be missing unless two atoms moved together by more than the
+// skin, so the pass exits at once while 2 * max displacement <= skin.
+template <bool MIXED, bool EN, bool VIR>
+__global__ __launch_bounds__(FIB_BLOCK) void k_extra(
+    const double4 *__restrict__ X, const DInst *__restrict__ I, const FConst fc, int NA, int Wsw,
+    const int *__restrict__ cnt_sw, const int *__restrict__ nbr_sw,
+    const int *__restrict__ cnt_leg, const uint64_t *__restrict__ amask,
+    const typename Prec<MIXED>::T4 *__restrict__ LA, const typename Prec<MIXED>::T4 *__restrict__ LB,
+    const float *__restrict__ disp, DCtl *__restrict__ ctl, typename Prec<MIXED>::T4 *__restrict__ EX,
+    typename Prec<MIXED>::T4 *__restrict__ EXself, uint64_t *__restrict__ exmask, int pstride,
+    double *__restrict__ partials) {
+  typedef typename Prec<MIXED>::T T;
+  typedef typename Prec<MIXED>::T4 T4;
+  const int lb = xcd_remap(blockIdx.x, gridDim.x);
+  const int b = lb * FIB_BLOCK + threadIdx.x;
+  const double4 xb = X[b];
+  const uint32_t mb = meta_of(xb);
+  const int kin = __builtin_amdgcn_readfirstlane(inst_of(mb));
+  const DInst &in = I[kin];
+  const double isgS = fc.isig[0][0];
+  const double fscale = MIXED ? fc.eps[0][0] * isgS : 1.0;
+  const double escale = MIXED ? fc.eps[0][0] : 1.0;
+  const double lscale = MIXED ? 1.0 / isgS : 1.0;
+  double v[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+  const int op[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+  // block-uniform: one instance per block
+  const bool active = 2.0 * ctl[kin].maxdisp > fc.skin;
+  if (!active) {
+    // nothing to add: clear the instance's flag so the gather skips EX
+    if (threadIdx.x == 0 && lb * FIB_BLOCK == in.atom_off) ctl[kin].extra_any = 0;
+    if (EN || VIR) block_partials<10, FIB_BLOCK>(v, op, lb, pstride, partials);
+    return;
+  }
+  const bool live = !(mb & (META_PAD | META_DEAD));
+  const int tb = type_of(mb);
+  const double lu = MIXED ? 1.0 / fc.eps[0][0] : 1.0;
+  const T lp[4] = {(T)(fc.leps[0][0][0] * lu), (T)(fc.leps[0][0][1] * lu),
+                   (T)(fc.leps[0][1][1] * lu), (T)(fc.leps[1][1][1] * lu)};
+  const float skin = (float)fc.skin;
+  uint64_t em = 0;
+  FAcc<MIXED> fb;
+  T ub = 0, et = 0;
+  T W[9];
+#pragma unroll
+  for (int w = 0; w < 9; ++w) W[w] = 0;
+  if (live && tb < 3) {
+    const int na = cnt_leg[b];
+    const int cs = min(cnt_sw[b], Wsw);
+    const int *rowb = nbr_sw + (size_t)b * Wsw;
+    const uint64_t amb = amask[b];
+    for (int n = 0; n < na; ++n) {
+      Leg<T> Lbc;  // b's leg to c: unit vector from c to b
+      load_leg(LA, LB, (size_t)n * NA + b, Lbc);
+      const int c = ks_index(Lbc.ks);
+      const float dc = disp[c];
+      const double4 xc = X[c];
+      const int tc = type_of(meta_of(xc));
+      const int *rowc = nbr_sw + (size_t)c * Wsw;
+      const int cc = min(cnt_sw[c], Wsw);
+      const int kc = nth_bit(amb, n);  // b's list slot of c
+      for (int k = 0; k < cs; ++k) {
+        const int a = rowb[k];
+        if (a == c || !(disp[a] + dc > skin)) continue;
+        const double4 xa = X[a];
+        const int ta = type_of(meta_of(xa));
+        // leg c -> a: d = x_c - x_a (unit vector from a to the centre c)
+        double d0, d1, d2;
+        min_image_fast(xc, xa, in, d0, d1, d2);
+        const double isg = fc.isig[(tc == 2) ? 1 : 0][(ta == 2) ? 1 : 0];
+        Leg<T> Lca;
+        if (!MIXED) {
+          const double r = sqrt(d0 * d0 + d1 * d1 + d2 * d2);
+          const double rs = r * isg;
+          if (!(rs < fc.cut_sw)) continue;
+          bool listed = false;
+          for (int q = 0; q < cc; ++q) listed |= (rowc[q] == a);
+          if (listed) continue;
+          const double rinv = 1.0 / r, inv_ra = 1.0 / (rs - fc.al);
+          Lca.e0 = d0 * rinv; Lca.e1 = d1 * rinv; Lca.e2 = d2 * rinv;
+          Lca.rinv = rinv;
+          Lca.E = exp(fc.gamma * inv_ra);
+          Lca.g = -fc.gamma * inv_ra * inv_ra * isg;
+        } else {
+          const float r0 = (float)(d0 * isg), r1 = (float)(d1 * isg), r2 = (float)(d2 * isg);
+          const float rr = r0 * r0 + r1 * r1 + r2 * r2;
+          if (!(rr < (float)fc.cut_sw2)) continue;
+          const float rinv = __builtin_amdgcn_rsqf(rr);
+          const float rs = rr * rinv;
+          if (!(rs < (float)fc.mixed_xleg)) continue;
+          bool listed = false;
+          for (int q = 0; q < cc; ++q) listed |= (rowc[q] == a);
+          if (listed) continue;
+          const float rho = (float)(isg / isgS);
+          const float inv_ra = __builtin_amdgcn_rcpf(rs - (float)fc.al);
+          Lca.e0 = r0 * rinv; Lca.e1 = r1 * rinv; Lca.e2 = r2 * rinv;
+          Lca.rinv = rinv * rho;
+          Lca.E = __expf((float)fc.gamma * inv_ra);
+          Lca.g = -(float)fc.gamma * inv_ra * inv_ra * rho;
+        }
+        // leg c -> b is b's leg to c, reversed
+        Leg<T> Lcb = Lbc;
+        Lcb.e0 = -Lbc.e0; Lcb.e1 = -Lbc.e1; Lcb.e2 = -Lbc.e2;
+        const int nge = (tc == 2) + (ta == 2) + (tb == 2);
+        T Fa[3], Fb[3], hh;
+        // vterm gives the force on its first leg: once per end
+        vterm<T>(lp[nge], Lca, Lcb, Fa, hh);
+        vterm<T>(lp[nge], Lcb, Lca, Fb, hh);
+        const T Fc0 = -(Fa[0] + Fb[0]), Fc1 = -(Fa[1] + Fb[1]), Fc2 = -(Fa[2] + Fb[2]);
+        fb.add(Fb[0], Fb[1], Fb[2]);
+        const T third = hh * T(1.0 / 3.0);
+        // forces on a (slot k) and c (slot kc) for them to gather
+        const uint64_t bits[2] = {1ull << k, 1ull << kc};
+        const int slots[2] = {k, kc};
+        const T fx[2] = {Fa[0], Fc0}, fy[2] = {Fa[1], Fc1}, fz[2] = {Fa[2], Fc2};
+#pragma unroll
+        for (int e = 0; e < 2; ++e) {
+          const size_t at = (size_t)slots[e] * NA + b;
+          T4 g;
+          if (em & bits[e]) {
+            g = EX[at];
+          } else {
+            g.x = g.y = g.z = g.w = T(0);
+          }
+          g.x += fx[e]; g.y += fy[e]; g.z += fz[e]; g.w += third;
+          EX[at] = g;
+          em |= bits[e];
+        }
+        ub += third;
+        et += hh;
+        if (VIR) {
+          // relative to the centre: r_a = x_a - x_c = -e_ca / rinv_ca, same for b
+          const T ra = -T(1) / Lca.rinv, rb = -T(1) / Lcb.rinv;
+          const T a0 = ra * Lca.e0, a1 = ra * Lca.e1, a2 = ra * Lca.e2;
+          const T b0 = rb * Lcb.e0, b1 = rb * Lcb.e1, b2 = rb * Lcb.e2;
+          W[0] += a0 * Fa[0] + b0 * Fb[0]; W[1] += a0 * Fa[1] + b0 * Fb[1];
+          W[2] += a0 * Fa[2] + b0 * Fb[2]; W[3] += a1 * Fa[0] + b1 * Fb[0];
+          W[4] += a1 * Fa[1] + b1 * Fb[1]; W[5] += a1 * Fa[2] + b1 * Fb[2];
+          W[6] += a2 * Fa[0] + b2 * Fb[0]; W[7] += a2 * Fa[1] + b2 * Fb[1];
+          W[8] += a2 * Fa[2] + b2 * Fb[2];
+        }
+      }
+    }
+  }
+  exmask[b] = em;
+  {
+    double f0, f1, f2;
+    fb.get(1.0, f0, f1, f2);
+    T4 g;
+    g.x = (T)f0; g.y = (T)f1; g.z = (T)f2; g.w = ub;
+    EXself[b] = g;
+  }
+  if (em) {
+    // rare: only after atoms outran the list skin (global atomic, vector memory)
+    __hip_atomic_fetch_or(&ctl[kin].extra_any, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
+  if (EN || VIR) {
+    v[0] = (double)et * escale;
+    const double ws = MIXED ? fscale * lscale : 1.0;
+#pragma unroll
+    for (int w = 0; w < 9; ++w) v[1 + w] = (double)W[w] * ws;
+    block_partials<10, FIB_BLOCK>(v, op, lb, pstride, partials);
+  }
+}
+
+// After a list build: positions of the build, displacement bookkeeping reset.
+__global__ void k_mark_build(int NA, int ninst, const double4 *__restrict__ X,
+                             double4 *__restrict__ Xb, float *__restrict__ disp,
+                             DCtl *__restrict__ ctl) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < NA) {
+    Xb[i] = X[i];
+    disp[i] = 0.0f;
+  }
+  if (i < ninst) {
+    ctl[i].maxdisp = 0.0;
+    ctl[i].extra_any = 0;
+  }
+}
+
+// ---------------------------------------------------------------------------
 // force pass 3: own terms + the G entries neighbours wrote for this atom;
 // MODE 0 = store forces, MODE 1 = reference velocity update
 // (timeint.f90:231-236) + thermostat sums.  SEARCH: scan the neighbour's list
@@ -979,7 +1179,9 @@ __global__ __launch_bounds__(FIB_BLOCK) void k_force3(
     const int *__restrict__ cnt_leg, const int *__restrict__ cnt_sw,
     const double *__restrict__ Fpart, const double *__restrict__ upart, double fscale,
     double escale, double *__restrict__ Fout, double *__restrict__ uout, double *__restrict__ V,
-    StepArgs sa, int pstride, double *__restrict__ partials) {
+    StepArgs sa, int pstride, double *__restrict__ partials, const DCtl *__restrict__ ctl,
+    const typename Prec<MIXED>::T4 *__restrict__ EX,
+    const typename Prec<MIXED>::T4 *__restrict__ EXself, const uint64_t *__restrict__ exmask) {
   typedef typename Prec<MIXED>::T4 T4;
   const int lb = xcd_remap(blockIdx.x, gridDim.x);
   const int i = lb * FIB_BLOCK + threadIdx.x;
@@ -1036,6 +1238,32 @@ __global__ __launch_bounds__(FIB_BLOCK) void k_force3(
       g0 += (double)g.x; g1 += (double)g.y; g2 += (double)g.z;
       if (EN) gu += (double)g.w;
     }
+    // vertex terms outside the centre's list (k_extra), when there are any
+    if (ctl[kin].extra_any) {
+      const T4 gs = EXself[i];
+      g0 += (double)gs.x; g1 += (double)gs.y; g2 += (double)gs.z;
+      if (EN) gu += (double)gs.w;
+      const int cs = min(cnt_sw[i], Wsw);
+      for (int k = 0; k < cs; ++k) {
+        const int j = nbr_sw[row + k];
+        const uint64_t mj = exmask[j];
+        if (!mj) continue;
+        int r = 255;
+        if (SEARCH) {
+          const int cj = min(cnt_sw[j], Wsw);
+          const int *rj = nbr_sw + (size_t)j * Wsw;
+          for (int q = 0; q < cj; ++q)
+            if (r == 255 && rj[q] == i) r = q;
+        } else {
+          r = rev[row + k];
+        }
+        if (r < Wsw && ((mj >> r) & 1ull)) {
+          const T4 g = EX[(size_t)r * NA + j];
+          g0 += (double)g.x; g1 += (double)g.y; g2 += (double)g.z;
+          if (EN) gu += (double)g.w;
+        }
+      }
+    }
     f0 += g0 * fscale; f1 += g1 * fscale; f2 += g2 * fscale;
     if (EN) u += gu * escale;
   }
@@ -1088,7 +1316,9 @@ __global__ __launch_bounds__(FIB_BLOCK) void k_pos(double4 *__restrict__ X,
                                                    double *__restrict__ V, double *__restrict__ F,
                                                    const double *__restrict__ Xi, StepArgs sa,
                                                    int pstride, double *__restrict__ partials,
-                                                   DCtl *__restrict__ ctl) {
+                                                   DCtl *__restrict__ ctl,
+                                                   const double4 *__restrict__ Xb,
+                                                   float *__restrict__ disp) {
 #pragma clang fp contract(off)
   const int nb = gridDim.x;
   const int lb = xcd_remap(blockIdx.x, nb);
@@ -1100,7 +1330,7 @@ __global__ __launch_bounds__(FIB_BLOCK) void k_pos(double4 *__restrict__ X,
   const int kin = __builtin_amdgcn_readfirstlane(inst_of(mi));
   const DInst &in = I[kin];
   const bool live = !(mi & (META_PAD | META_DEAD));
-  double red[2] = {0.0, -1.0};
+  double red[3] = {0.0, -1.0, 0.0};
   if (live) {
     const int t = type_of(mi);
     const double m = sa.mass[t - 1];
@@ -1160,13 +1390,21 @@ __global__ __launch_bounds__(FIB_BLOCK) void k_pos(double4 *__restrict__ X,
     red[1] = 0.5 * m * (v[0] * v[0] + v[1] * v[1] + v[2] * v[2]) / sa.ec;
     x.x = xx[0]; x.y = xx[1]; x.z = xx[2];
     X[i] = x;
+    {
+      // displacement since the list build (minimum image across the rebox)
+      double e0, e1, e2;
+      min_image_fast(x, Xb[i], in, e0, e1, e2);
+      const double dd = sqrt(e0 * e0 + e1 * e1 + e2 * e2);
+      disp[i] = (float)dd;
+      red[2] = dd;
+    }
     for (int c = 0; c < 3; ++c) {
       V[(size_t)c * NA + i] = v[c];
       F[(size_t)c * NA + i] = f[c];
     }
   }
-  const int op[2] = {0, 1};
-  block_partials<2, FIB_BLOCK>(red, op, lb, pstride, partials);
+  const int op[3] = {0, 1, 1};
+  block_partials<3, FIB_BLOCK>(red, op, lb, pstride, partials);
 }
 
 // Thermostat / temperature sums of the current state (same as k_force_b's
@@ -1322,6 +1560,13 @@ static void finalize(const DevBufs &d, int nv, int bs, const int *op, const int 
                            d.pstride, rs, d.ctl);
       }
       break;
+    case 3:
+      {
+        KScope ks_(d, KT_REDUCE);
+        hipLaunchKernelGGL(k_finalize<3>, dim3(d.ninst), dim3(FIN_T), 0, s, d.I, d.partials,
+                           d.pstride, rs, d.ctl);
+      }
+      break;
     case 5:
       {
         KScope ks_(d, KT_REDUCE);
@@ -1452,10 +1697,18 @@ static void force_ab(const DevBufs &d, const FConst &fc, hipStream_t s) {
                        d.X[d.cur], fc, d.NA, d.cnt_leg, (const T4 *)d.LA, (const T4 *)d.LB,
                        (T4 *)d.G, d.pstride, d.partials + (size_t)FIB_NRED * d.pstride);
   }
+  {
+    KScope ks_(d, KT_EXTRA);
+    hipLaunchKernelGGL((k_extra<MIXED, EN, VIR>), dim3(d.NA / FIB_BLOCK), dim3(FIB_BLOCK), 0, s,
+                       d.X[d.cur], d.I, fc, d.NA, d.Wsw, d.cnt_sw, d.nbr_sw, d.cnt_leg, d.amask,
+                       (const T4 *)d.LA, (const T4 *)d.LB, d.disp, d.ctl, (T4 *)d.EX,
+                       (T4 *)d.EXself, d.exmask, d.pstride,
+                       d.partials + (size_t)2 * FIB_NRED * d.pstride);
+  }
   if (EN || VIR) {
     int op[10], dst[10];
     for (int k = 0; k < 10; ++k) { op[k] = 0; dst[k] = DCTL_IDX(utot) + k; }
-    finalize(d, 10, FIB_BLOCK, op, dst, s, 2);
+    finalize(d, 10, FIB_BLOCK, op, dst, s, 3);
   }
 }
 
@@ -1470,7 +1723,8 @@ static void force_c(const DevBufs &d, const FConst &fc, const StepArgs &sa, doub
     hipLaunchKernelGGL((k_force3<MIXED, EN, MODE, SEARCH>), dim3(d.NA / FIB_BLOCK), dim3(FIB_BLOCK),
                        0, s, d.X[d.cur], d.I, d.NA, d.Wsw, d.nbr_sw, d.rev, (const T4 *)d.G,
                        d.amask, d.cnt_leg, d.cnt_sw, d.Fpart, d.upart, fscale, escale, Fo,
-                       d.uout, d.V[d.cur], sa, d.pstride, d.partials);
+                       d.uout, d.V[d.cur], sa, d.pstride, d.partials, d.ctl, (const T4 *)d.EX,
+                       (const T4 *)d.EXself, d.exmask);
   }
   if (MODE == 1) finalize_thermo(d, s);
 }
@@ -1511,11 +1765,18 @@ void launch_pos(const DevBufs &d, const StepArgs &sa, hipStream_t s) {
   {
     KScope ks_(d, KT_POS);
     hipLaunchKernelGGL(k_pos, dim3(d.NA / FIB_BLOCK), dim3(FIB_BLOCK), 0, s, d.X[d.cur], d.I, d.NA,
-                       d.V[d.cur], d.F[d.cur], d.Xi[d.cur], sa, d.pstride, d.partials, d.ctl);
+                       d.V[d.cur], d.F[d.cur], d.Xi[d.cur], sa, d.pstride, d.partials, d.ctl,
+                       d.Xb, d.disp);
   }
-  const int op[2] = {0, 1};
-  const int dst[2] = {DCTL_IDX(eq_sum), DCTL_IDX(kemax)};
-  finalize(d, 2, FIB_BLOCK, op, dst, s);
+  const int op[3] = {0, 1, 1};
+  const int dst[3] = {DCTL_IDX(eq_sum), DCTL_IDX(kemax), DCTL_IDX(maxdisp)};
+  finalize(d, 3, FIB_BLOCK, op, dst, s);
+}
+
+void launch_mark_build(const DevBufs &d, hipStream_t s) {
+  KScope ks_(d, KT_LISTAUX);
+  hipLaunchKernelGGL(k_mark_build, dim3(nblk(std::max(d.NA, d.ninst), FIB_BLOCK)), dim3(FIB_BLOCK), 0,
+                     s, d.NA, d.ninst, d.X[d.cur], d.Xb, d.disp, d.ctl);
 }
 
 void launch_sums(const DevBufs &d, const StepArgs &sa, hipStream_t s) {

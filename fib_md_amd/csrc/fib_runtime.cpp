@@ -78,7 +78,7 @@ struct Timer {
 
 static const char *const kTimerNames[KT_N] = {
     "k_bin", "sort", "k_nbuild", "list_aux", "k_rev", "k_legs",
-    "k_tri", "k_force3", "k_pos", "reduce", "io"};
+    "k_tri", "k_force3", "k_extra", "k_pos", "reduce", "io"};
 static_assert(KT_N <= FIB_NTIMERS, "timer table");
 
 struct RefCtl {
@@ -228,6 +228,7 @@ static void derive_constants(fib_handle *h) {
   c.izero = izero;
   c.cut_sw2 = c.cut_sw * c.cut_sw * (1.0 + 1e-6);
   c.mixed_xleg = std::min(c.cut_sw, c.al - c.gamma / 30.0);
+  c.skin = rc - c.cut_sw * std::max(p.sigma_si, p.sigma_ge);
   {
     const double ug = std::max(c.umax * (double)(izero - 1) / (double)(nt - 1), c.cut_sw);
     c.r2gate = (1.001 * ug) * (1.001 * ug);
@@ -249,7 +250,7 @@ static int32_t free_bufs(fib_handle *h) {
   DevBufs &d = h->d;
   void *ptrs[] = {d.I, d.ctl, d.X[0], d.X[1], d.V[0], d.V[1], d.F[0], d.F[1], d.Xi[0], d.Xi[1],
                   d.inv, d.bkey, d.brank, d.bcnt, d.cstart, d.perm, d.bsum, d.cnt_sw, d.nbr_sw,
-                  d.cnt_ion, d.nbr_ion, d.rev, d.LA, d.LB, d.G, d.amask, d.cnt_leg, d.Fpart, d.upart, d.Fout, d.uout,
+                  d.cnt_ion, d.nbr_ion, d.rev, d.LA, d.LB, d.G, d.amask, d.cnt_leg, d.Xb, d.disp, d.EX, d.EXself, d.exmask, d.Fpart, d.upart, d.Fout, d.uout,
                   d.partials, d.tab, d.xfer};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
@@ -266,11 +267,11 @@ static int32_t alloc_lists(fib_handle *h, int Wsw, int Wion) {
   DevBufs &d = h->d;
   if (d.nbr_sw) (void)hipFree(d.nbr_sw);
   if (d.nbr_ion) (void)hipFree(d.nbr_ion);
-  void *old[] = {d.rev, d.LA, d.LB, d.G};
+  void *old[] = {d.rev, d.LA, d.LB, d.G, d.EX};
   for (void *q : old)
     if (q) (void)hipFree(q);
   d.nbr_sw = nullptr; d.nbr_ion = nullptr; d.rev = nullptr;
-  d.LA = nullptr; d.LB = nullptr; d.G = nullptr;
+  d.LA = nullptr; d.LB = nullptr; d.G = nullptr; d.EX = nullptr;
   d.Wsw = Wsw;
   d.Wion = Wion;
   const size_t NA = (size_t)d.NA;
@@ -282,6 +283,7 @@ static int32_t alloc_lists(fib_handle *h, int Wsw, int Wion) {
   HIPCHK(h, hipMalloc(&d.LA, gsz * (size_t)Wsw * NA));
   HIPCHK(h, hipMalloc(&d.LB, gsz * (size_t)Wsw * NA));
   HIPCHK(h, hipMalloc(&d.G, gsz * (size_t)Wsw * NA));
+  HIPCHK(h, hipMalloc(&d.EX, gsz * (size_t)Wsw * NA));
   return FIB_OK;
 }
 
@@ -381,12 +383,21 @@ extern "C" int32_t fib_create(const fib_params *p, int32_t ninst, const fib_inst
   ALLOC(d.cnt_ion, NA);
   ALLOC(d.amask, NA);
   ALLOC(d.cnt_leg, NA);
+  ALLOC(d.Xb, NA);
+  ALLOC(d.disp, NA);
+  ALLOC(d.exmask, NA);
+  if (rc_ == FIB_OK &&
+      hipMalloc(&d.EXself, (h->prec == FIB_PREC_MIXED ? sizeof(float4) : sizeof(double4)) * NA) !=
+          hipSuccess) {
+    rc_ = FIB_ERR_HIP;
+    h->err = "hipMalloc failed for d.EXself";
+  }
   ALLOC(d.Fpart, 3 * NA);
   ALLOC(d.upart, NA);
   ALLOC(d.Fout, 3 * NA);
   ALLOC(d.uout, NA);
   d.pstride = d.NA / FIB_WAVE + 1;
-  ALLOC(d.partials, (size_t)d.pstride * FIB_NRED * 2);
+  ALLOC(d.partials, (size_t)d.pstride * FIB_NRED * 3);
   ALLOC(d.tab, h->tab.size());
   ALLOC(d.xfer, 3 * NA);
 #undef ALLOC
@@ -403,6 +414,8 @@ extern "C" int32_t fib_create(const fib_params *p, int32_t ninst, const fib_inst
   if (e == hipSuccess) e = hipMemset(d.cnt_ion, 0, sizeof(int) * NA);
   if (e == hipSuccess) e = hipMemset(d.amask, 0, sizeof(uint64_t) * NA);
   if (e == hipSuccess) e = hipMemset(d.cnt_leg, 0, sizeof(int) * NA);
+  if (e == hipSuccess) e = hipMemset(d.exmask, 0, sizeof(uint64_t) * NA);
+  if (e == hipSuccess) e = hipMemset(d.disp, 0, sizeof(float) * NA);
   if (e == hipSuccess)
     e = hipMemcpy(d.tab, h->tab.data(), sizeof(TabEntry) * h->tab.size(), hipMemcpyHostToDevice);
   if (e != hipSuccess) {
@@ -569,6 +582,7 @@ static int32_t build_lists(fib_handle *h) {
     }
     if (nonfin) return fail(h, FIB_ERR_NONFINITE, "non-finite position detected while binning");
     if (msw <= d.Wsw && mion <= d.Wion) {
+      fibk::launch_mark_build(d, h->stream);
       h->rev_valid = false;
       h->have_lists = true;
       h->have_force = false;
