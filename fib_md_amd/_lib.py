@@ -70,6 +70,18 @@ class fib_step_report(C.Structure):
 FIB_NTIMERS = 16
 
 
+class fib_md_params(C.Structure):
+    _fields_ = [("dt_max", C.c_double), ("dx_max", C.c_double), ("remove_height", C.c_double),
+                ("shell_width", C.c_double), ("ttar", C.c_double), ("tau", C.c_double),
+                ("displaced_r", C.c_double), ("use_graph", C.c_int32), ("reserved", C.c_int32)]
+
+
+class fib_md_report(C.Structure):
+    _fields_ = [("time", C.c_double), ("dt", C.c_double), ("ion_z", C.c_double),
+                ("ke", C.c_double), ("steps", C.c_int64), ("removed", C.c_int32),
+                ("ions_removed", C.c_int32), ("displaced", C.c_int32), ("builds", C.c_int32)]
+
+
 class fib_timing(C.Structure):
     _fields_ = [("ms", C.c_double * FIB_NTIMERS), ("calls", C.c_int32 * FIB_NTIMERS)]
 
@@ -100,6 +112,10 @@ SIGNATURES = {
     "fib_timing_enable": ([vp, i32], i32),
     "fib_get_timing": ([vp, P(fib_timing)], i32),
     "fib_timer_name": ([i32], C.c_char_p),
+    "fib_md_begin": ([vp, P(fib_md_params), vp, vp], i32),
+    "fib_md_fire": ([vp, i32, i32, vp, vp], i32),
+    "fib_md_run": ([vp, i32], i32),
+    "fib_md_get_report": ([vp, i32, P(fib_md_report)], i32),
     "fib_synchronize": ([vp], i32),
 }
 

@@ -90,3 +90,70 @@ def force_microbench(n: int = 64, seed: int = 1002):
     X = amorphous_sphere(X, L, 20e-10, rho, 2.0e-10, seed + 1)
     X = np.mod(X, L)
     return X, np.ones(len(X), np.int32), (L, L, L)
+
+
+# Ga launch speed per sqrt(eV) (knock.f90:28): v = 1663.6334 sqrt(E/eV) m/s
+GA_SPEED = 1663.6334
+# height of the unfired ion above the surface, inside the removal height so it
+# stays in the simulation (unfired ions carry no forces)
+ION_PARK = 8.0e-10
+
+
+def si_slab(n: int, seed: int, nz: int | None = None, vacuum: float = 40e-10, T: float = 300.0,
+            a: float = A_SI, nions: int = 1):
+    """Si(100) slab of n x n x nz cells for the benchmark configurations:
+    periodic in x/y, free top (box z = slab + vacuum), the bottom two atomic
+    layers (z = 0 and a/4) pinned, Maxwell-Boltzmann velocities at T (pinned
+    atoms at rest), `nions` Ga ions parked above the surface.
+    Returns dict(X, V, atype, xsides, L, nsg, nlj, surface_z, bottom_z)."""
+    nz = n if nz is None else nz
+    S = diamond(n, n, nz, a)
+    L = (n * a, n * a, nz * a + vacuum)
+    pinned = S[:, 2] < 0.3 * a
+    V = maxwell_boltzmann(np.full(len(S), MASS_SI), T, seed)
+    V[pinned] = 0.0
+    surface = float(S[:, 2].max())
+    rng = np.random.default_rng(seed + 7919)
+    ions = np.column_stack([rng.random(nions) * L[0], rng.random(nions) * L[1],
+                            np.full(nions, surface + ION_PARK)])
+    X = np.vstack([S, ions])
+    V = np.vstack([V, np.zeros((nions, 3))])
+    atype = np.concatenate([np.ones(len(S), np.int32), np.full(nions, 3, np.int32)])
+    xsides = np.zeros(len(X), np.int32)
+    xsides[np.nonzero(pinned)[0]] = np.nonzero(pinned)[0] + 1
+    return dict(X=X, V=V, atype=atype, xsides=xsides, L=L, nsg=len(S), nlj=nions,
+                surface_z=surface, bottom_z=0.0)
+
+
+def ion_launch(L, surface_z: float, energy_ev: float, polar_deg: float, azimuth_deg: float,
+               rng, height: float = 5e-10):
+    """Ion start 5 A above the surface at a uniform point of the surface cell,
+    speed for `energy_ev`, direction (polar from -z, azimuth from +x)."""
+    x = np.array([rng.random() * L[0], rng.random() * L[1], surface_z + height])
+    th, ph = math.radians(polar_deg), math.radians(azimuth_deg)
+    s = GA_SPEED * math.sqrt(energy_ev)
+    v = np.array([s * math.sin(th) * math.cos(ph), s * math.sin(th) * math.sin(ph),
+                  -s * math.cos(th)])
+    return x, v
+
+
+def ensemble(n_inst: int = 64, seed: int = 2000, nmin: float = 1e4, nmax: float = 2.5e5,
+             emin: float = 1e3, emax: float = 1e4, polar_max: float = 60.0):
+    """The batched-ensemble configuration: instance k (seed + k) is a cubic
+    Si(100) slab of n = round((N/8)^(1/3)) cells with N log-uniform in
+    [nmin, nmax], hit by one Ga ion of energy log-uniform in [emin, emax] eV,
+    polar angle uniform in [0, polar_max] deg, azimuth uniform.
+    Returns a list of dicts (si_slab fields plus ion x, v, energy, angles)."""
+    out = []
+    for k in range(n_inst):
+        rng = np.random.default_rng(seed + k)
+        N = math.exp(rng.uniform(math.log(nmin), math.log(nmax)))
+        n = max(3, int(round((N / 8.0) ** (1.0 / 3.0))))
+        E = math.exp(rng.uniform(math.log(emin), math.log(emax)))
+        th = rng.uniform(0.0, polar_max)
+        ph = rng.uniform(0.0, 360.0)
+        slab = si_slab(n, seed + k)
+        x, v = ion_launch(slab["L"], slab["surface_z"], E, th, ph, rng)
+        slab.update(ion_x=x, ion_v=v, energy_ev=E, polar=th, azimuth=ph, ncell=n)
+        out.append(slab)
+    return out

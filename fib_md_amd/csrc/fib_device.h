@@ -36,6 +36,7 @@ struct DInst {
   int32_t cell_real_off;       // prefix of N0*N1*N2 over earlier instances
   int32_t block_off;           // first thread block of this instance
   int32_t nblocks;             // natm_pad / FIB_BLOCK
+  double surf_z, bot_z;        // MD-step mode: slab top and bottom heights (m)
 };
 
 // Force-field constants common to all instances (kernel argument by value).
@@ -86,8 +87,23 @@ struct DCtl {
   double maxcnt_ion;
   double eq_sum;        // sum m v^2 of the rescaled V (equilibration)
   double maxdisp;       // largest displacement since the last list build (m)
+  double builds;        // list builds performed (count)
+  // MD-step mode (per-instance adaptive dt)
+  double cfg_dt;        // timestep of the next step (s)
+  double cfg_time;      // simulated time (s)
+  double cfg_lambda;    // Berendsen factor of the thermostat shells, next step
+  double cfg_steps;     // steps taken
+  double n_removed;     // SW atoms removed above the surface (sputtered)
+  double n_ga_removed;  // ions removed (backscattered)
+  double n_displaced;   // SW atoms farther than the threshold from their site
+  double ion_z;         // height of the instance's last ion (m)
+  double ke_shell;      // sum m v^2 over thermostat-shell atoms
+  double n_shell;       // their count
+  double e_pot;         // potential energy at the last energy evaluation (J)
   int32_t nonfinite;    // set by k_bin on a NaN/Inf position
   int32_t extra_any;    // k_extra found vertex terms outside a centre's list
+  int32_t overflow;     // a neighbour list exceeded its width (sticky)
+  int32_t pad;
 };
 
 // Double index of a DCtl field (for the reduction finalize kernel).

@@ -15,6 +15,12 @@ struct StepArgs {
   double outz[4], atomz[2], ionz;
   int32_t mode;              // 0 equilibrate, 1 impact (side shells), 2 free
   int32_t frz;               // apply frzsptrdatms in this step
+  // MD-step mode
+  double dt_max, dx_max;     // dt = min(dt_max, dx_max / |v|max) per instance
+  double remove_dz;          // atoms higher than surf_z + remove_dz are removed
+  double shell_w;            // Berendsen shells: x/y sides and the bottom (0 = off)
+  double disp_r;             // displaced-atom threshold
+  double skin;               // list rebuild when 2 * max displacement exceeds it
 };
 
 // Per-kernel timing keys (index into fib_timing.ms).
@@ -55,6 +61,7 @@ struct DevBufs {
   double4 *X[2];
   double *V[2], *F[2], *Xi[2];
   int *inv;                  // [NA] atom_off + orig -> sorted position
+  int *ibase;                // [ninst] scan offset of each instance's first bucket
   int *bkey, *brank, *bcnt, *cstart, *perm, *bsum;
   int *cnt_sw, *nbr_sw, *cnt_ion, *nbr_ion;
   uint8_t *rev;              // [NA * Wsw]: slot of i in its k-th neighbour's list
@@ -74,12 +81,14 @@ struct DevBufs {
   TabEntry *tab;             // [ntable + 3]
   double *xfer;              // staging [3*NA] for host transfers
   const KTimer *kt;          // timing hook (may be null)
+  const int *gate;           // non-null: [1 + ninst] build gates, see fib_kernels.hip
 };
 
 namespace fibk {
 void launch_bin(const DevBufs &d, hipStream_t s);
 void launch_scan(const DevBufs &d, hipStream_t s);
-void launch_sort(const DevBufs &d, hipStream_t s);  // scatter + fixup + reorder (flips cur)
+// scatter + fixup + reorder: flips cur, or (inplace) copies back and keeps cur
+void launch_sort(const DevBufs &d, hipStream_t s, bool inplace = false);
 // list build + longest-list reduction (DCtl::maxcnt_*)
 void launch_nbuild(const DevBufs &d, const FConst &fc, hipStream_t s);
 void launch_cellid(const DevBufs &d, int inst, int32_t *dout, hipStream_t s);
@@ -97,6 +106,14 @@ void launch_force_a(const DevBufs &d, const FConst &fc, bool mixed, bool energy,
 void launch_force_b(const DevBufs &d, const FConst &fc, bool mixed, bool energy, int mode,
                     bool search, const StepArgs &sa, double *Fout, hipStream_t s);
 void launch_pos(const DevBufs &d, const StepArgs &sa, hipStream_t s);
+// MD-step mode: head of the step (kick, drift, pins, removal) + rebuild gate
+void launch_md_pos(const DevBufs &d, const StepArgs &sa, int *gate, hipStream_t s);
+// MD-step mode: time, next dt, shell factor (after the MODE 2 force pass)
+void launch_md_ctl(const DevBufs &d, const StepArgs &sa, hipStream_t s);
+// MD-step mode: dt from the current |v|max (reset = also time, steps, factor)
+void launch_md_init(const DevBufs &d, const StepArgs &sa, bool reset, hipStream_t s);
+// MD-step mode: removed / displaced counts and ion height per instance
+void launch_md_report(const DevBufs &d, const StepArgs &sa, hipStream_t s);
 void launch_sums(const DevBufs &d, const StepArgs &sa, hipStream_t s);
 void launch_import(const DevBufs &d, int inst, const DInst &in, const double *X,
                    const double *V, const int32_t *atype, const int32_t *xsides,

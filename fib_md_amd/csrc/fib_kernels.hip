@@ -35,6 +35,19 @@
 
 namespace fibk {
 
+// Kernels of the list build take `gate`: null, or gate[0] = "some instance
+// rebuilds" and gate[1 + k] = "instance k rebuilds".  With gate[0] zero they
+// return at once (block-uniform), so a build can be enqueued -- or captured
+// in a graph -- and skipped on the device; atoms, cells and lists of an
+// instance whose own flag is zero are left untouched.
+#define GATE_OR_RETURN(gate) \
+  do {                       \
+    if ((gate) && !(gate)[0]) return; \
+  } while (0)
+__device__ __forceinline__ bool gated_out(const int *gate, int inst) {
+  return gate && !gate[1 + inst];
+}
+
 // ---------------------------------------------------------------------------
 // helpers
 __device__ __forceinline__ uint64_t wbits(const double4 &x) {
@@ -148,7 +161,9 @@ template <int NV>
 __global__ __launch_bounds__(FIN_T) void k_finalize(const DInst *__restrict__ I,
                                                     const double *__restrict__ partials,
                                                     int pstride, RedSpec rs,
-                                                    DCtl *__restrict__ ctl) {
+                                                    DCtl *__restrict__ ctl,
+                                                    const int *__restrict__ gate) {
+  GATE_OR_RETURN(gate);
   __shared__ double red[FIN_T / FIB_WAVE][NV];
   const int inst = blockIdx.x;
   const int blk0 = I[inst].atom_off / rs.bs, nblk = I[inst].natm_pad / rs.bs;
@@ -188,12 +203,13 @@ __global__ __launch_bounds__(FIN_T) void k_finalize(const DInst *__restrict__ I,
 // binning
 __global__ void k_bin(const double4 *__restrict__ X, int NA, const DInst *__restrict__ I,
                       int *__restrict__ bkey, int *__restrict__ brank, int *__restrict__ bcnt,
-                      DCtl *__restrict__ ctl) {
+                      DCtl *__restrict__ ctl, const int *__restrict__ gate) {
+  GATE_OR_RETURN(gate);
   int p = blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= NA) return;
   double4 x = X[p];
   uint32_t m = meta_of(x);
-  if (m & META_PAD) { bkey[p] = -1; return; }
+  if ((m & META_PAD) || gated_out(gate, inst_of(m))) { bkey[p] = -1; return; }
   const DInst &in = I[inst_of(m)];
   int local;
   if (m & META_DEAD) {
@@ -219,7 +235,8 @@ __global__ void k_bin(const double4 *__restrict__ X, int NA, const DInst *__rest
 }
 
 // Zero fill (used instead of a memset node so the step can be captured).
-__global__ void k_zero_i32(int *__restrict__ p, int n) {
+__global__ void k_zero_i32(int *__restrict__ p, int n, const int *__restrict__ gate) {
+  GATE_OR_RETURN(gate);
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k < n) p[k] = 0;
 }
@@ -265,7 +282,9 @@ __device__ __forceinline__ int block_scan_excl(int v, int &total) {
   return off + x - v;
 }
 
-__global__ void k_scan_reduce(const int *__restrict__ in, int n, int *__restrict__ bsum) {
+__global__ void k_scan_reduce(const int *__restrict__ in, int n, int *__restrict__ bsum,
+                              const int *__restrict__ gate) {
+  GATE_OR_RETURN(gate);
   int base = blockIdx.x * SCAN_CH;
   int s = 0;
   for (int e = 0; e < SCAN_E; ++e) {
@@ -277,7 +296,8 @@ __global__ void k_scan_reduce(const int *__restrict__ in, int n, int *__restrict
   if (threadIdx.x == 0) bsum[blockIdx.x] = tot;
 }
 
-__global__ void k_scan_top(int *__restrict__ bsum, int nb) {
+__global__ void k_scan_top(int *__restrict__ bsum, int nb, const int *__restrict__ gate) {
+  GATE_OR_RETURN(gate);
   int carry = 0;
   for (int base = 0; base < nb; base += SCAN_T) {
     int idx = base + threadIdx.x;
@@ -290,9 +310,12 @@ __global__ void k_scan_top(int *__restrict__ bsum, int nb) {
   }
 }
 
-// out[b] = exclusive prefix + pad_before of b's instance
+// out[b] = exclusive prefix over all buckets (made instance-local by
+// k_scan_base / k_scan_fix)
 __global__ void k_scan_down(const int *__restrict__ in, int n, const int *__restrict__ bsum,
-                            const DInst *__restrict__ I, int ninst, int *__restrict__ out) {
+                            const DInst *__restrict__ I, int ninst, int *__restrict__ out,
+                            const int *__restrict__ gate) {
+  GATE_OR_RETURN(gate);
   int base = blockIdx.x * SCAN_CH + threadIdx.x * SCAN_E;
   int vals[SCAN_E];
   int s = 0;
@@ -305,16 +328,32 @@ __global__ void k_scan_down(const int *__restrict__ in, int n, const int *__rest
   int ex = block_scan_excl(s, tot) + bsum[blockIdx.x];
   for (int e = 0; e < SCAN_E; ++e) {
     int idx = base + e;
-    if (idx < n) {
-      int k = find_inst_by_bucket(I, ninst, idx);
-      out[idx] = ex + I[k].pad_before;
-    }
+    if (idx < n) out[idx] = ex;
     ex += vals[e];
   }
 }
 
+// Instance-local start positions: bucket b of instance k starts at
+// atom_off(k) + (atoms of k in buckets before b), whatever other instances
+// hold, so an instance can be sorted while the others stay in place.
+__global__ void k_scan_base(const int *__restrict__ out, const DInst *__restrict__ I, int ninst,
+                            int *__restrict__ ibase, const int *__restrict__ gate) {
+  GATE_OR_RETURN(gate);
+  for (int k = threadIdx.x; k < ninst; k += blockDim.x)
+    ibase[k] = out[I[k].bucket_off] - I[k].atom_off;
+}
+__global__ void k_scan_fix(int *__restrict__ out, int n, const DInst *__restrict__ I, int ninst,
+                           const int *__restrict__ ibase, const int *__restrict__ gate) {
+  GATE_OR_RETURN(gate);
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= n) return;
+  out[b] -= ibase[find_inst_by_bucket(I, ninst, b)];
+}
+
 __global__ void k_scatter(const int *__restrict__ bkey, const int *__restrict__ brank, int NA,
-                          const int *__restrict__ cstart, int *__restrict__ perm) {
+                          const int *__restrict__ cstart, int *__restrict__ perm,
+                          const int *__restrict__ gate) {
+  GATE_OR_RETURN(gate);
   int p = blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= NA) return;
   int b = bkey[p];
@@ -324,7 +363,8 @@ __global__ void k_scatter(const int *__restrict__ bkey, const int *__restrict__ 
 
 // Deterministic order inside a bucket: ascending previous position.
 __global__ void k_fixup(const int *__restrict__ cstart, const int *__restrict__ bcnt, int nb,
-                        int *__restrict__ perm) {
+                        int *__restrict__ perm, const int *__restrict__ gate) {
+  GATE_OR_RETURN(gate);
   int b = blockIdx.x * blockDim.x + threadIdx.x;
   if (b >= nb) return;
   int n = bcnt[b];
@@ -342,11 +382,12 @@ __global__ void k_reorder(int NA, const int *__restrict__ perm, const DInst *__r
                           const double *__restrict__ V0, double *__restrict__ V1,
                           const double *__restrict__ F0, double *__restrict__ F1,
                           const double *__restrict__ Xi0, double *__restrict__ Xi1,
-                          int *__restrict__ inv) {
+                          int *__restrict__ inv, const int *__restrict__ gate) {
+  GATE_OR_RETURN(gate);
   int p = blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= NA) return;
   uint32_t mp = meta_of(X0[p]);
-  if (mp & META_PAD) return;
+  if ((mp & META_PAD) || gated_out(gate, inst_of(mp))) return;
   int s = perm[p];
   double4 x = X0[s];
   X1[p] = x;
@@ -360,6 +401,26 @@ __global__ void k_reorder(int NA, const int *__restrict__ perm, const DInst *__r
   inv[in.atom_off + orig_of(x)] = p;
 }
 
+// In-place variant of the sort: copy the reordered arrays back, so the
+// arrays stay where a captured step expects them.
+__global__ void k_copyback(int NA, const double4 *__restrict__ X1, double4 *__restrict__ X0,
+                           const double *__restrict__ V1, double *__restrict__ V0,
+                           const double *__restrict__ F1, double *__restrict__ F0,
+                           const double *__restrict__ Xi1, double *__restrict__ Xi0,
+                           const int *__restrict__ gate) {
+  GATE_OR_RETURN(gate);
+  int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= NA) return;
+  if (gated_out(gate, inst_of(meta_of(X0[p])))) return;
+  X0[p] = X1[p];
+  for (int c = 0; c < 3; ++c) {
+    const size_t a = (size_t)c * NA + p;
+    V0[a] = V1[a];
+    F0[a] = F1[a];
+    Xi0[a] = Xi1[a];
+  }
+}
+
 // ---------------------------------------------------------------------------
 // neighbour build: one 64-lane block per real cell
 #define NB_CHUNK 128
@@ -368,7 +429,9 @@ __global__ __launch_bounds__(FIB_WAVE) void k_nbuild(
     const double4 *__restrict__ X, const DInst *__restrict__ I, int ninst, int total_cells,
     const int *__restrict__ cstart, const int *__restrict__ bcnt, double rv2, double rv2lj,
     int NA, int Wsw, int Wion, int *__restrict__ cnt_sw, int *__restrict__ nbr_sw,
-    int *__restrict__ cnt_ion, int *__restrict__ nbr_ion, DCtl *__restrict__ ctl) {
+    int *__restrict__ cnt_ion, int *__restrict__ nbr_ion, DCtl *__restrict__ ctl,
+    const int *__restrict__ gate) {
+  GATE_OR_RETURN(gate);
   __shared__ double4 s_x[NB_CHUNK];
   __shared__ int s_j[NB_CHUNK];
   __shared__ int s_rs[27], s_off[28];
@@ -377,6 +440,7 @@ __global__ __launch_bounds__(FIB_WAVE) void k_nbuild(
   const int gcell = xcd_remap(blockIdx.x, gridDim.x);
   if (gcell >= total_cells) return;
   const int k = find_inst_by_cell(I, ninst, gcell);
+  if (gated_out(gate, k)) return;
   const DInst &in = I[k];
   const int N0 = in.nc[0], N1 = in.nc[1];
   const int loc = gcell - in.cell_real_off;
@@ -475,7 +539,12 @@ __global__ __launch_bounds__(FIB_WAVE) void k_nbuild(
 
 // Atoms outside every scanned cell (dropped, dead, or phantom i-atoms) get
 // empty lists; k_nbuild only writes the atoms of real cells.
-__global__ void k_clear_counts(int NA, int *__restrict__ cnt_sw, int *__restrict__ cnt_ion) {
+__global__ void k_clear_counts(int NA, int *__restrict__ cnt_sw, int *__restrict__ cnt_ion,
+                               const double4 *__restrict__ X, const int *__restrict__ gate) {
+  GATE_OR_RETURN(gate);
+  if (gate && blockIdx.x * blockDim.x + threadIdx.x < NA &&
+      gated_out(gate, inst_of(meta_of(X[blockIdx.x * blockDim.x + threadIdx.x]))))
+    return;
   int p = blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= NA) return;
   cnt_sw[p] = 0;
@@ -487,11 +556,15 @@ __global__ void k_clear_counts(int NA, int *__restrict__ cnt_sw, int *__restrict
 // list is one contiguous row read as int4.
 __global__ __launch_bounds__(FIB_BLOCK) void k_rev(int NA, int wshift, const int *__restrict__ cnt_sw,
                                                    const int *__restrict__ nbr_sw,
-                                                   uint8_t *__restrict__ rev) {
+                                                   uint8_t *__restrict__ rev,
+                                                   const double4 *__restrict__ X,
+                                                   const int *__restrict__ gate) {
+  GATE_OR_RETURN(gate);
   const int Wsw = 1 << wshift;
   const size_t t = (size_t)blockIdx.x * FIB_BLOCK + threadIdx.x;
   const int i = (int)(t >> wshift), k = (int)(t & (size_t)(Wsw - 1));
   if (i >= NA) return;
+  if (gate && gated_out(gate, inst_of(meta_of(X[i])))) return;
   const int c = min(cnt_sw[i], Wsw);
   if (k >= c) return;
   const int j = nbr_sw[t];
@@ -518,7 +591,9 @@ __global__ __launch_bounds__(FIB_BLOCK) void k_rev(int NA, int wshift, const int
 // Longest SW / ion lists of the build (overflow check).
 __global__ __launch_bounds__(FIB_BLOCK) void k_maxcnt(const int *__restrict__ cnt_sw,
                                                       const int *__restrict__ cnt_ion,
-                                                      int pstride, double *__restrict__ partials) {
+                                                      int pstride, double *__restrict__ partials,
+                                                      const int *__restrict__ gate) {
+  GATE_OR_RETURN(gate);
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   double v[2] = {(double)cnt_sw[i], (double)cnt_ion[i]};
   const int op[2] = {1, 1};
@@ -1152,18 +1227,178 @@ __global__ __launch_bounds__(FIB_BLOCK) void k_extra(
 }
 
 // After a list build: positions of the build, displacement bookkeeping reset.
-__global__ void k_mark_build(int NA, int ninst, const double4 *__restrict__ X,
+__global__ void k_mark_build(int NA, int ninst, int Wsw, int Wion, const double4 *__restrict__ X,
                              double4 *__restrict__ Xb, float *__restrict__ disp,
-                             DCtl *__restrict__ ctl) {
+                             DCtl *__restrict__ ctl, const int *__restrict__ gate) {
+  GATE_OR_RETURN(gate);
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < NA) {
+  if (i < NA && !gated_out(gate, inst_of(meta_of(X[i])))) {
     Xb[i] = X[i];
     disp[i] = 0.0f;
   }
-  if (i < ninst) {
+  if (i < ninst && !gated_out(gate, i)) {
     ctl[i].maxdisp = 0.0;
     ctl[i].extra_any = 0;
+    // a list longer than its width was cut: sticky, checked by the host
+    if (ctl[i].maxcnt_sw > Wsw || ctl[i].maxcnt_ion > Wion) ctl[i].overflow = 1;
+    ctl[i].builds += 1.0;
   }
+}
+
+// End of a gated build: clear the gate (its own kernel, so that every block
+// of the build kernels has read it).
+__global__ void k_clear_gate(int *__restrict__ gate, int ninst) {
+  for (int k = threadIdx.x; k <= ninst; k += blockDim.x) gate[k] = 0;
+}
+
+// ---------------------------------------------------------------------------
+// MD-step mode: velocity Verlet with a per-instance adaptive timestep
+// dt = min(dt_max, dx_max / |v|max), all step control on the device, so one
+// step is a fixed sequence of launches that can be captured in a graph:
+//   k_md_pos     Berendsen shells, half kick, drift, pinned atoms, removal
+//                above the surface, displacement since the list build;
+//                k_md_decide sets the rebuild gate (2 * max displacement > skin)
+//   gated build  bin / sort in place / k_nbuild / k_rev (skipped on the
+//                device when the gate is 0)
+//   forces       k_legs, k_tri, k_extra, k_force3 MODE 2 (second half kick,
+//                |v|max, shell sums)
+//   k_md_ctl     time += dt, next dt, next shell factor
+__device__ __forceinline__ bool in_shell(double x, double y, double z, const DInst &in, double w) {
+  return x < w || x > in.lb[0] - w || y < w || y > in.lb[1] - w || z < in.bot_z + w;
+}
+
+__global__ __launch_bounds__(FIB_BLOCK) void k_md_pos(double4 *__restrict__ X,
+                                                      const DInst *__restrict__ I, int NA,
+                                                      double *__restrict__ V,
+                                                      const double *__restrict__ F,
+                                                      const double *__restrict__ Xi, StepArgs sa,
+                                                      int pstride, double *__restrict__ partials,
+                                                      const DCtl *__restrict__ ctl,
+                                                      const double4 *__restrict__ Xb,
+                                                      float *__restrict__ disp) {
+  const int lb = xcd_remap(blockIdx.x, gridDim.x);
+  const int i = lb * FIB_BLOCK + threadIdx.x;
+  double4 x = X[i];
+  uint32_t mi = meta_of(x);
+  const int kin = __builtin_amdgcn_readfirstlane(inst_of(mi));
+  const DInst &in = I[kin];
+  const bool live = !(mi & (META_PAD | META_DEAD));
+  const double dt = ctl[kin].cfg_dt, lam = ctl[kin].cfg_lambda;
+  double red[1] = {0.0};
+  if (live) {
+    const int t = type_of(mi);
+    const double m = sa.mass[t - 1];
+    double v[3], xx[3] = {x.x, x.y, x.z};
+    for (int c = 0; c < 3; ++c) v[c] = V[(size_t)c * NA + i];
+    if (sa.shell_w > 0.0 && t < 3 && !(mi & META_FROZEN) &&
+        in_shell(xx[0], xx[1], xx[2], in, sa.shell_w))
+      for (int c = 0; c < 3; ++c) v[c] *= lam;
+    for (int c = 0; c < 3; ++c) {
+      v[c] += 0.5 * F[(size_t)c * NA + i] * dt / m;
+      xx[c] += v[c] * dt;
+    }
+    if (mi & META_FROZEN) {
+      for (int c = 0; c < 3; ++c) {
+        xx[c] = Xi[(size_t)c * NA + i];
+        v[c] = 0.0;
+      }
+    }
+    for (int c = 0; c < 3; ++c) xx[c] = fmod(xx[c] + 2. * in.lb[c], in.lb[c]);
+    if (xx[2] - in.surf_z > sa.remove_dz && xx[2] < in.lb[2] - sa.remove_dz) {
+      // sputtered (or backscattered): out of the simulation from now on
+      mi |= META_DEAD;
+      for (int c = 0; c < 3; ++c) v[c] = 0.0;
+      x.w = __longlong_as_double((long long)((wbits(x) & ~0xffffffffull) | mi));
+    }
+    x.x = xx[0]; x.y = xx[1]; x.z = xx[2];
+    X[i] = x;
+    for (int c = 0; c < 3; ++c) V[(size_t)c * NA + i] = v[c];
+    double e0, e1, e2;
+    min_image_fast(x, Xb[i], in, e0, e1, e2);
+    const double dd = sqrt(e0 * e0 + e1 * e1 + e2 * e2);
+    disp[i] = (float)dd;
+    red[0] = dd;
+  }
+  const int op[1] = {1};
+  block_partials<1, FIB_BLOCK>(red, op, lb, pstride, partials);
+}
+
+// Rebuild gates of the step: each instance whose 2 * max displacement
+// exceeds the skin, and whether there is any.
+__global__ void k_md_decide(int ninst, const DCtl *__restrict__ ctl, double skin,
+                            int *__restrict__ gate) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  int any = 0;
+  for (int k = 0; k < ninst; ++k) {
+    const int g = (2.0 * ctl[k].maxdisp > skin) ? 1 : 0;
+    gate[1 + k] = g;
+    any |= g;
+  }
+  gate[0] = any;
+}
+
+// After the step's second half kick: advance time, choose the next dt and
+// the next Berendsen factor of the shells.
+__global__ void k_md_ctl(int ninst, DCtl *__restrict__ ctl, StepArgs sa) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= ninst) return;
+  DCtl &c = ctl[k];
+  c.cfg_time += c.cfg_dt;
+  c.cfg_steps += 1.0;
+  const double dt = c.vmax > 0.0 ? fmin(sa.dt_max, sa.dx_max / c.vmax) : sa.dt_max;
+  c.cfg_dt = dt;
+  double lam = 1.0;
+  if (sa.shell_w > 0.0 && c.n_shell > 0.0 && c.ke_shell > 0.0) {
+    const double ts = c.ke_shell / (3.0 * c.n_shell * sa.kB);
+    lam = sqrt(1.0 + dt / sa.tau * (sa.ttar / ts - 1.0));
+  }
+  c.cfg_lambda = lam;
+}
+
+// dt from the current |v|max (DCtl::vmax, from k_sums); reset also zeroes the
+// clock and the shell factor.
+__global__ void k_md_init(int ninst, DCtl *__restrict__ ctl, StepArgs sa, int reset) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= ninst) return;
+  DCtl &c = ctl[k];
+  c.cfg_dt = c.vmax > 0.0 ? fmin(sa.dt_max, sa.dx_max / c.vmax) : sa.dt_max;
+  if (reset) {
+    c.cfg_time = 0.0;
+    c.cfg_steps = 0.0;
+    c.cfg_lambda = 1.0;
+  }
+}
+
+// Cascade outputs: removed SW atoms (sputter yield), removed ions, displaced
+// SW atoms (farther than disp_r from their initial site Xi), height of the
+// instance's most recently fired ion.
+__global__ __launch_bounds__(FIB_BLOCK) void k_md_report(const double4 *__restrict__ X,
+                                                         const DInst *__restrict__ I, int NA,
+                                                         const double *__restrict__ Xi,
+                                                         StepArgs sa, int pstride,
+                                                         double *__restrict__ partials) {
+  const int lb = xcd_remap(blockIdx.x, gridDim.x);
+  const int i = lb * FIB_BLOCK + threadIdx.x;
+  const double4 x = X[i];
+  const uint32_t mi = meta_of(x);
+  const int kin = __builtin_amdgcn_readfirstlane(inst_of(mi));
+  const DInst &in = I[kin];
+  double v[4] = {0.0, 0.0, 0.0, -1.0e300};
+  if (!(mi & META_PAD)) {
+    const int t = type_of(mi);
+    if (mi & META_DEAD) {
+      if (t < 3) v[0] = 1.0; else v[1] = 1.0;
+    } else if (t < 3) {
+      double e0, e1, e2;
+      double4 xi;
+      xi.x = Xi[i]; xi.y = Xi[(size_t)NA + i]; xi.z = Xi[2 * (size_t)NA + i]; xi.w = 0.0;
+      min_image_fast(x, xi, in, e0, e1, e2);
+      if (e0 * e0 + e1 * e1 + e2 * e2 > sa.disp_r * sa.disp_r) v[2] = 1.0;
+    }
+    if (t == 3 && orig_of(x) == in.nsg + in.impact - 1) v[3] = (mi & META_DEAD) ? 1.0e300 : x.z;
+  }
+  const int op[4] = {0, 0, 0, 1};
+  block_partials<4, FIB_BLOCK>(v, op, lb, pstride, partials);
 }
 
 // ---------------------------------------------------------------------------
@@ -1272,6 +1507,35 @@ __global__ __launch_bounds__(FIB_BLOCK) void k_force3(
       Fout[i] = f0; Fout[(size_t)NA + i] = f1; Fout[2 * (size_t)NA + i] = f2;
       if (EN) uout[i] = u;
     }
+    return;
+  }
+  if (MODE == 2) {
+    // MD-step mode: second half kick with the instance's dt; F = F'
+    // reductions: |v|max, kinetic energy, shell sum m v^2 and count
+    double v[4] = {0, 0, 0, 0};
+    if (live) {
+      const int t = type_of(mi);
+      const double m = sa.mass[t - 1];
+      const double dt = ctl[kin].cfg_dt;
+      const bool pinned = (mi & META_FROZEN) != 0;
+      const double fn[3] = {f0, f1, f2};
+      double vv[3];
+      for (int c = 0; c < 3; ++c) {
+        const size_t a = (size_t)c * NA + i;
+        vv[c] = pinned ? 0.0 : V[a] + 0.5 * fn[c] * dt / m;
+        V[a] = vv[c];
+        Fout[a] = fn[c];
+      }
+      const double v2 = vv[0] * vv[0] + vv[1] * vv[1] + vv[2] * vv[2];
+      v[0] = sqrt(v2);
+      v[1] = 0.5 * m * v2;
+      if (sa.shell_w > 0.0 && t < 3 && !pinned && in_shell(xi.x, xi.y, xi.z, in, sa.shell_w)) {
+        v[2] = m * v2;
+        v[3] = 1.0;
+      }
+    }
+    const int op[4] = {1, 0, 0, 0};
+    block_partials<4, FIB_BLOCK>(v, op, lb, pstride, partials);
     return;
   }
   // MODE 1: V += 0.5*(F+Fp)*Ts/m ; F = Fp ; thermostat / temperature sums
@@ -1543,7 +1807,7 @@ struct KScope {
 };
 
 static void finalize(const DevBufs &d, int nv, int bs, const int *op, const int *dst,
-                     hipStream_t s, int nreg = 1) {
+                     hipStream_t s, int nreg = 1, const int *gate = nullptr) {
   RedSpec rs;
   rs.nv = nv;
   rs.bs = bs;
@@ -1553,32 +1817,46 @@ static void finalize(const DevBufs &d, int nv, int bs, const int *op, const int 
     rs.dst[k] = k < nv ? dst[k] : -1;
   }
   switch (nv) {
+    case 1:
+      {
+        KScope ks_(d, KT_REDUCE);
+        hipLaunchKernelGGL(k_finalize<1>, dim3(d.ninst), dim3(FIN_T), 0, s, d.I, d.partials,
+                           d.pstride, rs, d.ctl, gate);
+      }
+      break;
     case 2:
       {
         KScope ks_(d, KT_REDUCE);
         hipLaunchKernelGGL(k_finalize<2>, dim3(d.ninst), dim3(FIN_T), 0, s, d.I, d.partials,
-                           d.pstride, rs, d.ctl);
+                           d.pstride, rs, d.ctl, gate);
       }
       break;
     case 3:
       {
         KScope ks_(d, KT_REDUCE);
         hipLaunchKernelGGL(k_finalize<3>, dim3(d.ninst), dim3(FIN_T), 0, s, d.I, d.partials,
-                           d.pstride, rs, d.ctl);
+                           d.pstride, rs, d.ctl, gate);
+      }
+      break;
+    case 4:
+      {
+        KScope ks_(d, KT_REDUCE);
+        hipLaunchKernelGGL(k_finalize<4>, dim3(d.ninst), dim3(FIN_T), 0, s, d.I, d.partials,
+                           d.pstride, rs, d.ctl, gate);
       }
       break;
     case 5:
       {
         KScope ks_(d, KT_REDUCE);
         hipLaunchKernelGGL(k_finalize<5>, dim3(d.ninst), dim3(FIN_T), 0, s, d.I, d.partials,
-                           d.pstride, rs, d.ctl);
+                           d.pstride, rs, d.ctl, gate);
       }
       break;
     default:
       {
         KScope ks_(d, KT_REDUCE);
         hipLaunchKernelGGL(k_finalize<10>, dim3(d.ninst), dim3(FIN_T), 0, s, d.I, d.partials,
-                           d.pstride, rs, d.ctl);
+                           d.pstride, rs, d.ctl, gate);
       }
       break;
   }
@@ -1595,12 +1873,12 @@ void launch_bin(const DevBufs &d, hipStream_t s) {
   {
     KScope ks_(d, KT_BIN);
     hipLaunchKernelGGL(k_zero_i32, dim3(nblk(d.nbucket, FIB_BLOCK)), dim3(FIB_BLOCK), 0, s, d.bcnt,
-                       d.nbucket);
+                       d.nbucket, d.gate);
   }
   {
     KScope ks_(d, KT_BIN);
     hipLaunchKernelGGL(k_bin, dim3(nblk(d.NA, FIB_BLOCK)), dim3(FIB_BLOCK), 0, s, d.X[d.cur], d.NA,
-                       d.I, d.bkey, d.brank, d.bcnt, d.ctl);
+                       d.I, d.bkey, d.brank, d.bcnt, d.ctl, d.gate);
   }
 }
 
@@ -1608,36 +1886,52 @@ void launch_scan(const DevBufs &d, hipStream_t s) {
   const int nb = nblk(d.nbucket, SCAN_CH);
   {
     KScope ks_(d, KT_SORT);
-    hipLaunchKernelGGL(k_scan_reduce, dim3(nb), dim3(SCAN_T), 0, s, d.bcnt, d.nbucket, d.bsum);
+    hipLaunchKernelGGL(k_scan_reduce, dim3(nb), dim3(SCAN_T), 0, s, d.bcnt, d.nbucket, d.bsum,
+                       d.gate);
   }
   {
     KScope ks_(d, KT_SORT);
-    hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(SCAN_T), 0, s, d.bsum, nb);
+    hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(SCAN_T), 0, s, d.bsum, nb, d.gate);
   }
   {
     KScope ks_(d, KT_SORT);
     hipLaunchKernelGGL(k_scan_down, dim3(nb), dim3(SCAN_T), 0, s, d.bcnt, d.nbucket, d.bsum, d.I,
-                       d.ninst, d.cstart);
+                       d.ninst, d.cstart, d.gate);
+  }
+  {
+    KScope ks_(d, KT_SORT);
+    hipLaunchKernelGGL(k_scan_base, dim3(1), dim3(SCAN_T), 0, s, d.cstart, d.I, d.ninst, d.ibase,
+                       d.gate);
+  }
+  {
+    KScope ks_(d, KT_SORT);
+    hipLaunchKernelGGL(k_scan_fix, dim3(nblk(d.nbucket, FIB_BLOCK)), dim3(FIB_BLOCK), 0, s,
+                       d.cstart, d.nbucket, d.I, d.ninst, d.ibase, d.gate);
   }
 }
 
-void launch_sort(const DevBufs &d, hipStream_t s) {
+void launch_sort(const DevBufs &d, hipStream_t s, bool inplace) {
   {
     KScope ks_(d, KT_SORT);
     hipLaunchKernelGGL(k_scatter, dim3(nblk(d.NA, FIB_BLOCK)), dim3(FIB_BLOCK), 0, s, d.bkey,
-                       d.brank, d.NA, d.cstart, d.perm);
+                       d.brank, d.NA, d.cstart, d.perm, d.gate);
   }
   {
     KScope ks_(d, KT_SORT);
     hipLaunchKernelGGL(k_fixup, dim3(nblk(d.nbucket, FIB_BLOCK)), dim3(FIB_BLOCK), 0, s, d.cstart,
-                       d.bcnt, d.nbucket, d.perm);
+                       d.bcnt, d.nbucket, d.perm, d.gate);
   }
   const int c = d.cur, n = 1 - d.cur;
   {
     KScope ks_(d, KT_SORT);
     hipLaunchKernelGGL(k_reorder, dim3(nblk(d.NA, FIB_BLOCK)), dim3(FIB_BLOCK), 0, s, d.NA, d.perm,
                        d.I, d.X[c], d.X[n], d.V[c], d.V[n], d.F[c], d.F[n], d.Xi[c], d.Xi[n],
-                       d.inv);
+                       d.inv, d.gate);
+  }
+  if (inplace) {
+    KScope ks_(d, KT_SORT);
+    hipLaunchKernelGGL(k_copyback, dim3(nblk(d.NA, FIB_BLOCK)), dim3(FIB_BLOCK), 0, s, d.NA, d.X[n],
+                       d.X[c], d.V[n], d.V[c], d.F[n], d.F[c], d.Xi[n], d.Xi[c], d.gate);
   }
 }
 
@@ -1645,22 +1939,22 @@ void launch_nbuild(const DevBufs &d, const FConst &fc, hipStream_t s) {
   {
     KScope ks_(d, KT_LISTAUX);
     hipLaunchKernelGGL(k_clear_counts, dim3(nblk(d.NA, FIB_BLOCK)), dim3(FIB_BLOCK), 0, s, d.NA,
-                       d.cnt_sw, d.cnt_ion);
+                       d.cnt_sw, d.cnt_ion, d.X[d.cur], d.gate);
   }
   if (d.total_cells > 0) {
     KScope ks_(d, KT_NBUILD);
     hipLaunchKernelGGL(k_nbuild, dim3(d.total_cells), dim3(FIB_WAVE), 0, s, d.X[d.cur], d.I,
                        d.ninst, d.total_cells, d.cstart, d.bcnt, fc.rv2, fc.rv2lj, d.NA, d.Wsw,
-                       d.Wion, d.cnt_sw, d.nbr_sw, d.cnt_ion, d.nbr_ion, d.ctl);
+                       d.Wion, d.cnt_sw, d.nbr_sw, d.cnt_ion, d.nbr_ion, d.ctl, d.gate);
   }
   {
     KScope ks_(d, KT_LISTAUX);
     hipLaunchKernelGGL(k_maxcnt, dim3(d.NA / FIB_BLOCK), dim3(FIB_BLOCK), 0, s, d.cnt_sw,
-                       d.cnt_ion, d.pstride, d.partials);
+                       d.cnt_ion, d.pstride, d.partials, d.gate);
   }
   const int op[2] = {1, 1};
   const int dst[2] = {DCTL_IDX(maxcnt_sw), DCTL_IDX(maxcnt_ion)};
-  finalize(d, 2, FIB_BLOCK, op, dst, s);
+  finalize(d, 2, FIB_BLOCK, op, dst, s, 1, d.gate);
 }
 
 void launch_cellid(const DevBufs &d, int inst, int32_t *dout, hipStream_t s) {
@@ -1677,7 +1971,7 @@ void launch_rev(const DevBufs &d, hipStream_t s) {
   {
     KScope ks_(d, KT_REV);
     hipLaunchKernelGGL(k_rev, dim3((unsigned)(((size_t)d.NA << sh) / FIB_BLOCK)), dim3(FIB_BLOCK), 0,
-                       s, d.NA, sh, d.cnt_sw, d.nbr_sw, d.rev);
+                       s, d.NA, sh, d.cnt_sw, d.nbr_sw, d.rev, d.X[d.cur], d.gate);
   }
 }
 
@@ -1727,6 +2021,11 @@ static void force_c(const DevBufs &d, const FConst &fc, const StepArgs &sa, doub
                        (const T4 *)d.EXself, d.exmask);
   }
   if (MODE == 1) finalize_thermo(d, s);
+  if (MODE == 2) {
+    const int op[4] = {1, 0, 0, 0};
+    const int dst[4] = {DCTL_IDX(vmax), DCTL_IDX(ke_sum), DCTL_IDX(ke_shell), DCTL_IDX(n_shell)};
+    finalize(d, 4, FIB_BLOCK, op, dst, s);
+  }
 }
 
 void launch_force_a(const DevBufs &d, const FConst &fc, bool mixed, bool energy, bool virial,
@@ -1746,6 +2045,8 @@ static void force_c_mode(const DevBufs &d, const FConst &fc, bool energy, int mo
   if (mode == 1) {
     if (search) force_c<MIXED, false, 1, true>(d, fc, sa, Fout, s);
     else force_c<MIXED, false, 1, false>(d, fc, sa, Fout, s);
+  } else if (mode == 2) {
+    force_c<MIXED, false, 2, false>(d, fc, sa, Fout, s);
   } else if (energy) {
     if (search) force_c<MIXED, true, 0, true>(d, fc, sa, Fout, s);
     else force_c<MIXED, true, 0, false>(d, fc, sa, Fout, s);
@@ -1773,10 +2074,47 @@ void launch_pos(const DevBufs &d, const StepArgs &sa, hipStream_t s) {
   finalize(d, 3, FIB_BLOCK, op, dst, s);
 }
 
+void launch_md_pos(const DevBufs &d, const StepArgs &sa, int *gate, hipStream_t s) {
+  {
+    KScope ks_(d, KT_POS);
+    hipLaunchKernelGGL(k_md_pos, dim3(d.NA / FIB_BLOCK), dim3(FIB_BLOCK), 0, s, d.X[d.cur], d.I,
+                       d.NA, d.V[d.cur], d.F[d.cur], d.Xi[d.cur], sa, d.pstride, d.partials,
+                       d.ctl, d.Xb, d.disp);
+  }
+  const int op[1] = {1};
+  const int dst[1] = {DCTL_IDX(maxdisp)};
+  finalize(d, 1, FIB_BLOCK, op, dst, s);
+  hipLaunchKernelGGL(k_md_decide, dim3(1), dim3(FIB_WAVE), 0, s, d.ninst, d.ctl, sa.skin, gate);
+}
+
+void launch_md_ctl(const DevBufs &d, const StepArgs &sa, hipStream_t s) {
+  hipLaunchKernelGGL(k_md_ctl, dim3(nblk(d.ninst, FIB_BLOCK)), dim3(FIB_BLOCK), 0, s, d.ninst,
+                     d.ctl, sa);
+}
+
+void launch_md_init(const DevBufs &d, const StepArgs &sa, bool reset, hipStream_t s) {
+  hipLaunchKernelGGL(k_md_init, dim3(nblk(d.ninst, FIB_BLOCK)), dim3(FIB_BLOCK), 0, s, d.ninst,
+                     d.ctl, sa, reset ? 1 : 0);
+}
+
+void launch_md_report(const DevBufs &d, const StepArgs &sa, hipStream_t s) {
+  {
+    KScope ks_(d, KT_REDUCE);
+    hipLaunchKernelGGL(k_md_report, dim3(d.NA / FIB_BLOCK), dim3(FIB_BLOCK), 0, s, d.X[d.cur],
+                       d.I, d.NA, d.Xi[d.cur], sa, d.pstride, d.partials);
+  }
+  const int op[4] = {0, 0, 0, 1};
+  const int dst[4] = {DCTL_IDX(n_removed), DCTL_IDX(n_ga_removed), DCTL_IDX(n_displaced),
+                      DCTL_IDX(ion_z)};
+  finalize(d, 4, FIB_BLOCK, op, dst, s);
+}
+
 void launch_mark_build(const DevBufs &d, hipStream_t s) {
   KScope ks_(d, KT_LISTAUX);
   hipLaunchKernelGGL(k_mark_build, dim3(nblk(std::max(d.NA, d.ninst), FIB_BLOCK)), dim3(FIB_BLOCK), 0,
-                     s, d.NA, d.ninst, d.X[d.cur], d.Xb, d.disp, d.ctl);
+                     s, d.NA, d.ninst, d.Wsw, d.Wion, d.X[d.cur], d.Xb, d.disp, d.ctl, d.gate);
+  if (d.gate)
+    hipLaunchKernelGGL(k_clear_gate, dim3(1), dim3(FIB_BLOCK), 0, s, (int *)d.gate, d.ninst);
 }
 
 void launch_sums(const DevBufs &d, const StepArgs &sa, hipStream_t s) {

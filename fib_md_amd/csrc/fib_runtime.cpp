@@ -108,6 +108,14 @@ struct fib_handle {
   DevBufs d{};
   bool have_state = false, have_lists = false, have_force = false;
   bool rev_valid = false;  // rev matches the current lists
+  // MD-step mode
+  bool md_on = false;
+  fib_md_params md{};
+  StepArgs md_sa{};
+  int *md_gate = nullptr;       // device [1 + ninst]: rebuild gates of the step
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t graph_exec = nullptr;
+  bool graph_valid = false;
   int last_flags = 0;
   std::string err;
   RefCtl ref;
@@ -248,7 +256,7 @@ static void derive_constants(fib_handle *h) {
 
 static int32_t free_bufs(fib_handle *h) {
   DevBufs &d = h->d;
-  void *ptrs[] = {d.I, d.ctl, d.X[0], d.X[1], d.V[0], d.V[1], d.F[0], d.F[1], d.Xi[0], d.Xi[1],
+  void *ptrs[] = {d.I, d.ctl, d.ibase, d.X[0], d.X[1], d.V[0], d.V[1], d.F[0], d.F[1], d.Xi[0], d.Xi[1],
                   d.inv, d.bkey, d.brank, d.bcnt, d.cstart, d.perm, d.bsum, d.cnt_sw, d.nbr_sw,
                   d.cnt_ion, d.nbr_ion, d.rev, d.LA, d.LB, d.G, d.amask, d.cnt_leg, d.Xb, d.disp, d.EX, d.EXself, d.exmask, d.Fpart, d.upart, d.Fout, d.uout,
                   d.partials, d.tab, d.xfer};
@@ -373,6 +381,7 @@ extern "C" int32_t fib_create(const fib_params *p, int32_t ninst, const fib_inst
     ALLOC(d.Xi[c], 3 * NA);
   }
   ALLOC(d.inv, NA);
+  ALLOC(d.ibase, (size_t)ninst);
   ALLOC(d.bkey, NA);
   ALLOC(d.brank, NA);
   ALLOC(d.bcnt, (size_t)d.nbucket);
@@ -445,6 +454,9 @@ extern "C" int32_t fib_destroy(fib_handle *h) {
   if (!h) return FIB_ERR_ARG;
   (void)hipStreamSynchronize(h->stream);
   for (auto e : h->tim.pool) (void)hipEventDestroy(e);
+  if (h->graph_exec) (void)hipGraphExecDestroy(h->graph_exec);
+  if (h->graph) (void)hipGraphDestroy(h->graph);
+  if (h->md_gate) (void)hipFree(h->md_gate);
   free_bufs(h);
   (void)hipStreamDestroy(h->stream);
   delete h;
@@ -502,6 +514,7 @@ extern "C" int32_t fib_set_state(fib_handle *h, int32_t inst, const double *X, c
   (void)hipFree(dt);
   if (ds) (void)hipFree(ds);
   h->have_state = true;
+  h->md_on = false;  // MD-step mode restarts from fib_md_begin
   h->have_lists = false;
   h->have_force = false;
   return FIB_OK;
@@ -584,6 +597,7 @@ static int32_t build_lists(fib_handle *h) {
     if (msw <= d.Wsw && mion <= d.Wion) {
       fibk::launch_mark_build(d, h->stream);
       h->rev_valid = false;
+      h->graph_valid = false;  // the sort moved the arrays a graph refers to
       h->have_lists = true;
       h->have_force = false;
       ++h->n_builds;
@@ -937,6 +951,153 @@ extern "C" int32_t fib_step(fib_handle *h, const fib_step_ctl *ctl, fib_step_rep
   rep->regime = R.regime;
   rep->nlist_builds = h->n_builds - builds0;
   h->have_force = false;
+  return FIB_OK;
+}
+
+// ---------------------------------------------------------------------------
+// MD-step mode
+
+static void enqueue_md_step(fib_handle *h) {
+  DevBufs &d = h->d;
+  const StepArgs &sa = h->md_sa;
+  fibk::launch_md_pos(d, sa, h->md_gate, h->stream);
+  // list rebuild, skipped on the device unless the gate is set; the sort is
+  // in place so every array stays where the step (or its graph) expects it
+  d.gate = h->md_gate;
+  fibk::launch_bin(d, h->stream);
+  fibk::launch_scan(d, h->stream);
+  fibk::launch_sort(d, h->stream, true);
+  fibk::launch_nbuild(d, h->fc, h->stream);
+  fibk::launch_rev(d, h->stream);
+  fibk::launch_mark_build(d, h->stream);
+  d.gate = nullptr;
+  const bool mixed = h->prec == FIB_PREC_MIXED;
+  fibk::launch_force_a(d, h->fc, mixed, false, false, h->stream);
+  fibk::launch_force_b(d, h->fc, mixed, false, 2, false, sa, d.F[d.cur], h->stream);
+  fibk::launch_md_ctl(d, sa, h->stream);
+}
+
+// forces of the current state into F, rev, and dt from the current |v|max
+static int32_t md_refresh(fib_handle *h, bool reset) {
+  DevBufs &d = h->d;
+  ensure_rev(h);
+  int32_t r = run_force(h, FIB_EVAL_FORCE, 0, h->md_sa, d.F[d.cur]);
+  if (r != FIB_OK) return r;
+  fibk::launch_sums(d, h->md_sa, h->stream);
+  fibk::launch_md_init(d, h->md_sa, reset, h->stream);
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  return FIB_OK;
+}
+
+extern "C" int32_t fib_md_begin(fib_handle *h, const fib_md_params *p, const double *surface_z,
+                                const double *bottom_z) {
+  if (!h || !p || !surface_z || !bottom_z) return FIB_ERR_ARG;
+  if (!(p->dt_max > 0) || !(p->dx_max > 0) || !(p->remove_height > 0) || p->shell_width < 0 ||
+      (p->shell_width > 0 && !(p->tau > 0 && p->ttar > 0)) || p->displaced_r < 0)
+    return fail(h, FIB_ERR_ARG, "fib_md_begin: bad parameters");
+  if (!h->have_state) return fail(h, FIB_ERR_STATE, "fib_md_begin: set the state first");
+  if (!h->md_gate) {
+    HIPCHK(h, hipMalloc(&h->md_gate, sizeof(int) * (h->inst.size() + 1)));
+    HIPCHK(h, hipMemset(h->md_gate, 0, sizeof(int) * (h->inst.size() + 1)));
+  }
+  for (size_t k = 0; k < h->inst.size(); ++k) {
+    h->inst[k].surf_z = surface_z[k];
+    h->inst[k].bot_z = bottom_z[k];
+  }
+  HIPCHK(h, hipMemcpy(h->d.I, h->inst.data(), sizeof(DInst) * h->inst.size(),
+                      hipMemcpyHostToDevice));
+  h->md = *p;
+  StepArgs &sa = h->md_sa;
+  sa = h->base;
+  sa.dt_max = p->dt_max;
+  sa.dx_max = p->dx_max;
+  sa.remove_dz = p->remove_height;
+  sa.shell_w = p->shell_width;
+  sa.ttar = p->ttar;
+  sa.tau = p->tau > 0 ? p->tau : 1.0;
+  sa.disp_r = p->displaced_r;
+  sa.skin = h->fc.skin;
+  if (!h->have_lists) {
+    int32_t r = build_lists(h);
+    if (r != FIB_OK) return r;
+  }
+  h->md_on = true;
+  h->graph_valid = false;
+  return md_refresh(h, true);
+}
+
+extern "C" int32_t fib_md_fire(fib_handle *h, int32_t inst, int32_t ion, const double x[3],
+                               const double v[3]) {
+  if (!h || !x || !v) return FIB_ERR_ARG;
+  if (!h->md_on) return fail(h, FIB_ERR_STATE, "fib_md_fire: call fib_md_begin first");
+  if (bad_inst(h, inst)) return fail(h, FIB_ERR_ARG, "fib_md_fire: bad instance");
+  DInst &in = h->inst[inst];
+  if (ion < 1 || ion > in.nlj) return fail(h, FIB_ERR_ARG, "fib_md_fire: ion out of range");
+  int p = -1;
+  HIPCHK(h, hipMemcpy(&p, h->d.inv + in.atom_off + in.nsg + ion - 1, sizeof(int),
+                      hipMemcpyDeviceToHost));
+  fibk::launch_set_atom(h->d, p, x, v, h->stream);
+  in.impact = ion;
+  HIPCHK(h, hipMemcpyAsync(&h->d.I[inst], &in, sizeof(DInst), hipMemcpyHostToDevice, h->stream));
+  // the ion jumped: fresh lists, forces and dt
+  int32_t r = build_lists(h);
+  if (r != FIB_OK) return r;
+  return md_refresh(h, false);
+}
+
+extern "C" int32_t fib_md_run(fib_handle *h, int32_t nsteps) {
+  if (!h || nsteps < 0) return FIB_ERR_ARG;
+  if (!h->md_on) return fail(h, FIB_ERR_STATE, "fib_md_run: call fib_md_begin first");
+  if (nsteps == 0) return FIB_OK;
+  ensure_rev(h);
+  if (h->md.use_graph) {
+    if (!h->graph_valid) {
+      if (h->graph_exec) { (void)hipGraphExecDestroy(h->graph_exec); h->graph_exec = nullptr; }
+      if (h->graph) { (void)hipGraphDestroy(h->graph); h->graph = nullptr; }
+      const KTimer *kt = h->d.kt;
+      h->d.kt = nullptr;  // no timing events inside the graph
+      HIPCHK(h, hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
+      enqueue_md_step(h);
+      hipError_t e = hipStreamEndCapture(h->stream, &h->graph);
+      h->d.kt = kt;
+      HIPCHK(h, e);
+      HIPCHK(h, hipGraphInstantiate(&h->graph_exec, h->graph, nullptr, nullptr, 0));
+      h->graph_valid = true;
+    }
+    for (int k = 0; k < nsteps; ++k) HIPCHK(h, hipGraphLaunch(h->graph_exec, h->stream));
+  } else {
+    for (int k = 0; k < nsteps; ++k) enqueue_md_step(h);
+  }
+  HIPCHK(h, hipGetLastError());
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  h->have_force = false;
+  std::vector<DCtl> c(h->inst.size());
+  HIPCHK(h, hipMemcpy(c.data(), h->d.ctl, sizeof(DCtl) * c.size(), hipMemcpyDeviceToHost));
+  for (auto &x : c) {
+    if (x.overflow) return fail(h, FIB_ERR_OVERFLOW, "fib_md_run: a neighbour list outgrew its width");
+    if (x.nonfinite) return fail(h, FIB_ERR_NONFINITE, "fib_md_run: non-finite position");
+  }
+  return FIB_OK;
+}
+
+extern "C" int32_t fib_md_get_report(fib_handle *h, int32_t inst, fib_md_report *r) {
+  if (!h || !r) return FIB_ERR_ARG;
+  if (!h->md_on) return fail(h, FIB_ERR_STATE, "fib_md_get_report: call fib_md_begin first");
+  if (bad_inst(h, inst)) return fail(h, FIB_ERR_ARG, "fib_md_get_report: bad instance");
+  fibk::launch_md_report(h->d, h->md_sa, h->stream);
+  DCtl c;
+  HIPCHK(h, hipMemcpyAsync(&c, &h->d.ctl[inst], sizeof(DCtl), hipMemcpyDeviceToHost, h->stream));
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  std::memset(r, 0, sizeof(*r));
+  r->time = c.cfg_time;
+  r->dt = c.cfg_dt;
+  r->ion_z = c.ion_z;
+  r->ke = c.ke_sum;
+  r->steps = (int64_t)c.cfg_steps;
+  r->removed = (int32_t)c.n_removed;
+  r->ions_removed = (int32_t)c.n_ga_removed;
+  r->displaced = (int32_t)c.n_displaced;
+  r->builds = (int32_t)c.builds;
   return FIB_OK;
 }
 

@@ -226,6 +226,29 @@ class FibGPU:
                 out[name.decode()] = {"ms": float(t.ms[k]), "calls": n, "avg_ms": float(t.ms[k]) / n}
         return out
 
+    # ---- MD-step mode (batched impacts, adaptive dt) ------------------------
+    def md_begin(self, surface_z, bottom_z, dt_max=1e-15, dx_max=0.05e-10, remove_height=10e-10,
+                 shell_width=0.0, ttar=300.0, tau=1e-13, displaced_r=1.5e-10, use_graph=True):
+        p = L.fib_md_params(dt_max, dx_max, remove_height, shell_width, ttar, tau, displaced_r,
+                            1 if use_graph else 0, 0)
+        sz = np.ascontiguousarray(surface_z, dtype=np.float64)
+        bz = np.ascontiguousarray(bottom_z, dtype=np.float64)
+        assert sz.shape == (len(self.inst),) and bz.shape == (len(self.inst),)
+        self._chk("fib_md_begin", self.L.fib_md_begin(self.h, C.byref(p), _p(sz), _p(bz)))
+
+    def md_fire(self, inst, ion, x, v):
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        v = np.ascontiguousarray(v, dtype=np.float64)
+        self._chk("fib_md_fire", self.L.fib_md_fire(self.h, int(inst), int(ion), _p(x), _p(v)))
+
+    def md_run(self, nsteps):
+        self._chk("fib_md_run", self.L.fib_md_run(self.h, int(nsteps)))
+
+    def md_report(self, inst=0):
+        r = L.fib_md_report()
+        self._chk("fib_md_get_report", self.L.fib_md_get_report(self.h, int(inst), C.byref(r)))
+        return {k: getattr(r, k) for k, _ in L.fib_md_report._fields_}
+
     def synchronize(self):
         self._chk("fib_synchronize", self.L.fib_synchronize(self.h))
 

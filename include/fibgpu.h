@@ -201,6 +201,51 @@ int32_t fib_step(fib_handle *h, const fib_step_ctl *ctl, fib_step_report *rep);
 /* Recompute the reference's temperature (temp.f90:183-205). */
 int32_t fib_temperature(fib_handle *h, int32_t inst, double *temp);
 
+/* ---- MD-step mode: impacts as the benchmark configurations define them ---
+ * Velocity Verlet with a per-instance timestep dt = min(dt_max, dx_max /
+ * |v|max), lists rebuilt when an atom has moved half the list skin, atoms
+ * pinned through xsides (e.g. the bottom layers), atoms leaving the slab top
+ * by more than remove_height taken out (sputtered), optional Berendsen
+ * shells on the x/y sides and the bottom.  Every instance of the handle
+ * steps together (a batched ensemble) with its own dt; a step runs entirely
+ * on the device and can be replayed as a captured HIP graph.  This mode has
+ * no reference counterpart: the reference's own loop is fib_step. */
+typedef struct fib_md_params {
+  double dt_max;          /* s, timestep cap                                    */
+  double dx_max;          /* m, largest displacement per step                   */
+  double remove_height;   /* m above surface_z beyond which atoms are removed   */
+  double shell_width;     /* m, Berendsen shells (0: off)                       */
+  double ttar, tau;       /* shell target temperature (K), time constant (s)    */
+  double displaced_r;     /* m, displacement that counts an atom as displaced   */
+  int32_t use_graph;      /* replay the step as a captured HIP graph            */
+  int32_t reserved;
+} fib_md_params;
+
+/* Enter MD-step mode on all instances: surface_z / bottom_z (per instance,
+ * m) are the slab's top and bottom; forces of the current state, first dt. */
+int32_t fib_md_begin(fib_handle *h, const fib_md_params *p, const double *surface_z,
+                     const double *bottom_z);
+/* Fire ion `ion` (1-based among the instance's nlj ions) from x with
+ * velocity v: the ion counts as fired from now on, lists and forces are
+ * refreshed and the instance's dt follows the ion's speed. */
+int32_t fib_md_fire(fib_handle *h, int32_t inst, int32_t ion, const double x[3],
+                    const double v[3]);
+/* Advance every instance by nsteps steps of its own dt. */
+int32_t fib_md_run(fib_handle *h, int32_t nsteps);
+
+typedef struct fib_md_report {
+  double time;            /* s, simulated time of the instance                  */
+  double dt;              /* s, timestep of its next step                       */
+  double ion_z;           /* m, height of its last fired ion (1e300: removed)   */
+  double ke;              /* J, kinetic energy after the last step              */
+  int64_t steps;          /* steps taken since fib_md_begin                     */
+  int32_t removed;        /* SW atoms removed above the surface (sputter yield) */
+  int32_t ions_removed;   /* ions removed (backscattered)                       */
+  int32_t displaced;      /* SW atoms farther than displaced_r from their site  */
+  int32_t builds;         /* list builds so far                                 */
+} fib_md_report;
+int32_t fib_md_get_report(fib_handle *h, int32_t inst, fib_md_report *r);
+
 /* Device-side kernel timers (HIP events on the handle's stream around each
  * launch), accumulated since fib_timing_enable(h, mask); for benchmarks.
  * Bit k of mask turns on timer k, which covers the kernels fib_timer_name(k)

@@ -1,0 +1,160 @@
+"""MD-step mode (the benchmark configurations' integrator): per-instance
+adaptive timestep dt = min(dt_max, dx_max / |v|max), half-skin list rebuilds
+gated on the device, pinned atoms, removal above the surface, batched
+instances, HIP-graph replay.
+
+Pins (no reference counterpart exists for this mode):
+- free flight of isolated atoms: x_n = x_0 + n dt v with the closed-form dt,
+  simulated time n dt;
+- removal of an atom crossing surface + 10 A at the step the closed form
+  gives; pinned atoms stay at their sites at rest;
+- energy conservation (NVE, FP64) of a thermal slab with a live ion;
+- batching: every instance of a batch follows bit for bit the trajectory it
+  has alone (FP64 and mixed);
+- a captured step replayed as a graph equals the eager step, bit for bit.
+"""
+import math
+
+import numpy as np
+import pytest
+
+from fib_md_amd import configs
+
+pytestmark = pytest.mark.gpu
+
+A = 1e-10
+
+
+def _gpu(instances, precision="fp64"):
+    from fib_md_amd import FibGPU
+    return FibGPU([(len(s["X"]), s["nsg"], s["nlj"], s["L"]) for s in instances],
+                  precision=precision)
+
+
+def _load(g, instances):
+    for k, s in enumerate(instances):
+        g.set_state(s["X"], V=s["V"], atype=s["atype"], xsides=s["xsides"], Xi=s["X"], inst=k)
+
+
+def test_md_free_flight_closed_form():
+    rng = np.random.default_rng(3)
+    n = 27
+    g3 = np.stack(np.meshgrid(*[np.arange(3)] * 3, indexing="ij"), -1).reshape(-1, 3)
+    X = g3 * 12 * A + 6 * A
+    V = rng.normal(0, 4000.0, (n, 3))
+    s = dict(X=X, V=V, atype=np.ones(n, np.int32), xsides=np.zeros(n, np.int32),
+             L=(60 * A, 60 * A, 60 * A), nsg=n, nlj=0)
+    g = _gpu([s])
+    _load(g, [s])
+    g.md_begin([55 * A], [0.0], use_graph=False)
+    vmax = np.sqrt((V ** 2).sum(1)).max()
+    dt = min(1e-15, 0.05e-10 / vmax)
+    nsteps = 25
+    g.md_run(nsteps)
+    Xg, Vg, Fg = g.state()
+    r = g.md_report()
+    assert r["steps"] == nsteps and abs(r["time"] - nsteps * dt) <= 1e-12 * nsteps * dt
+    assert r["dt"] == dt
+    np.testing.assert_array_equal(Vg, V)
+    assert np.all(Fg == 0)
+    Xe = np.mod(X + nsteps * dt * V, 60 * A)
+    assert np.abs(Xg - Xe).max() <= 1e-22
+    assert r["removed"] == 0 and r["displaced"] == int((np.sqrt(((Xe - X) ** 2).sum(1)) > 1.5 * A).sum())
+
+
+def test_md_removal_and_pinned_atoms():
+    X = np.array([[10 * A, 10 * A, 25 * A],    # flies up, crosses surface + 10 A
+                  [30 * A, 30 * A, 5 * A],     # pinned
+                  [50 * A, 50 * A, 15 * A]])   # free, slow
+    V = np.array([[0, 0, 3000.0], [800.0, 0, 0], [0, 100.0, 0]])
+    s = dict(X=X, V=V, atype=np.ones(3, np.int32), xsides=np.array([0, 2, 0], np.int32),
+             L=(60 * A, 60 * A, 80 * A), nsg=3, nlj=0)
+    g = _gpu([s])
+    _load(g, [s])
+    surface = 20 * A
+    g.md_begin([surface], [0.0], use_graph=False)
+    dt = min(1e-15, 0.05e-10 / 3000.0)
+    # z0 + k dt v > surface + 10 A first at k = ceil((30 - 25) A / (dt v))
+    k_cross = math.floor((30 * A - 25 * A) / (dt * 3000.0)) + 1
+    g.md_run(k_cross - 1)
+    assert g.md_report()["removed"] == 0
+    g.md_run(1)
+    assert g.md_report()["removed"] == 1
+    Xg, Vg, _ = g.state()
+    assert np.all(Vg[0] == 0)
+    # pinned to its site, then reboxed like every atom: fmod(x + 2 Lb, Lb)
+    Lv = np.array(s["L"])
+    np.testing.assert_array_equal(Xg[1], np.fmod(X[1] + 2 * Lv, Lv))
+    assert np.all(Vg[1] == 0)
+
+
+def _slab_batch():
+    out = []
+    for k, (n, nz, E, th, ph) in enumerate([(3, 3, 1000.0, 20.0, 45.0), (4, 3, 2000.0, 7.0, 0.0),
+                                            (3, 4, 1500.0, 45.0, 200.0)]):
+        s = configs.si_slab(n, 500 + k, nz=nz)
+        x, v = configs.ion_launch(s["L"], s["surface_z"], E, th, ph, np.random.default_rng(k))
+        s.update(ion_x=x, ion_v=v)
+        out.append(s)
+    return out
+
+
+def _run(instances, precision, steps, use_graph):
+    g = _gpu(instances, precision)
+    _load(g, instances)
+    g.md_begin([s["surface_z"] for s in instances], [s["bottom_z"] for s in instances],
+               use_graph=use_graph)
+    for k, s in enumerate(instances):
+        g.md_fire(k, 1, s["ion_x"], s["ion_v"])
+    g.md_run(steps)
+    out = [(*g.state(inst=k), g.md_report(k)) for k in range(len(instances))]
+    g.close()
+    return out
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_md_batch_equals_single(precision):
+    batch = _slab_batch()
+    together = _run(batch, precision, 60, use_graph=True)
+    for k, s in enumerate(batch):
+        alone = _run([s], precision, 60, use_graph=True)[0]
+        Xb, Vb, _, rb = together[k]
+        Xa, Va, _, ra = alone
+        np.testing.assert_array_equal(Xb, Xa)
+        np.testing.assert_array_equal(Vb, Va)
+        assert rb["time"] == ra["time"]
+        # each fib_md_fire rebuilds every instance; the gated rebuilds during
+        # the run are the instance's own
+        assert rb["builds"] - ra["builds"] == len(batch) - 1
+    # the ions went in and the lists were rebuilt on the way
+    assert all(r[3]["builds"] >= 2 for r in together)
+
+
+def test_md_graph_equals_eager():
+    s = _slab_batch()[1]
+    a = _run([s], "fp64", 40, use_graph=True)[0]
+    b = _run([s], "fp64", 40, use_graph=False)[0]
+    np.testing.assert_array_equal(a[0], b[0])
+    np.testing.assert_array_equal(a[1], b[1])
+
+
+def test_md_energy_conservation_fp64():
+    s = configs.si_slab(4, 77, nz=4)
+    s["xsides"][:] = 0  # nothing pinned: isolated NVE system
+    x, v = configs.ion_launch(s["L"], s["surface_z"], 300.0, 10.0, 30.0, np.random.default_rng(1))
+    g = _gpu([s])
+    _load(g, [s])
+    g.md_begin([s["surface_z"]], [0.0], dt_max=0.5e-15, use_graph=True)
+    g.md_fire(0, 1, x, v)
+
+    def etot():
+        g.compute(energy=True)
+        return g.forces(0, energy=True)["utot"] + g.md_report(0)["ke"]
+
+    e0 = etot()
+    ke0 = g.md_report(0)["ke"]
+    g.md_run(400)
+    e1 = etot()
+    r = g.md_report(0)
+    assert r["removed"] == 0 and r["ions_removed"] == 0
+    assert abs(e1 - e0) <= 2e-3 * ke0, (e0, e1, ke0)
