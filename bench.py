@@ -154,6 +154,86 @@ def run_gpu(X, at, L, precision, steps, warmup):
     return out
 
 
+STEPS_PER_IMPACT = 2000  # the single-impact configuration's step count
+
+
+def _md_handle(instances, precision, use_graph=True):
+    from fib_md_amd import FibGPU
+    g = FibGPU([(len(s["X"]), s["nsg"], s["nlj"], s["L"]) for s in instances], precision=precision)
+    for k, s in enumerate(instances):
+        g.set_state(s["X"], V=s["V"], atype=s["atype"], xsides=s["xsides"], Xi=s["X"], inst=k)
+    g.md_begin([s["surface_z"] for s in instances], [s["bottom_z"] for s in instances],
+               use_graph=use_graph)
+    for k, s in enumerate(instances):
+        g.md_fire(k, 1, s["ion_x"], s["ion_v"])
+    return g
+
+
+def md_rate(instances, precision, steps, warmup=5, use_graph=True):
+    """Timed MD steps of a batch: atom-steps/s over all instances."""
+    import torch
+    g = _md_handle(instances, precision, use_graph)
+    g.md_run(warmup)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    g.md_run(steps)
+    torch.cuda.synchronize()
+    t = time.perf_counter() - t0
+    n = sum(len(s["X"]) for s in instances)
+    reps = [g.md_report(k) for k in range(len(instances))]
+    g.close()
+    return dict(atom_steps_per_s=n * steps / t, ms_per_step=t * 1e3 / steps, atoms=n,
+                builds=sum(r["builds"] for r in reps), removed=sum(r["removed"] for r in reps))
+
+
+def md_bench(args):
+    """MD-step throughput: the single 500 eV impact on an 8^3-cell slab, and
+    the batched ensemble (64 slabs of 10k-250k atoms, 1-10 keV, 0-60 deg),
+    mixed-batched against the plain FP64 path (one instance at a time)."""
+    from fib_md_amd import configs
+    out = {}
+    # single impact: 8x8x8 cells, 500 eV normal incidence, 300 K, seed 1001
+    s = configs.si_slab(8, 1001)
+    x, v = configs.ion_launch(s["L"], s["surface_z"], 500.0, 0.0, 0.0, np.random.default_rng(1001))
+    s.update(ion_x=x, ion_v=v)
+    for prec in ("fp64", "mixed"):
+        r = md_rate([s], prec, args.md_steps)
+        r["impacts_per_hour"] = 3600.0 / (r["ms_per_step"] * 1e-3 * STEPS_PER_IMPACT)
+        out[f"single_impact_4097_{prec}"] = r
+        log(f"  single impact {prec}: {r['ms_per_step']:.3f} ms/step, "
+            f"{r['atom_steps_per_s']:.4g} atom-steps/s")
+    ens = configs.ensemble(args.ensemble)
+    nat = sum(len(e["X"]) for e in ens)
+    log(f"  ensemble: {len(ens)} instances, {nat} atoms")
+    mb = md_rate(ens, "mixed", args.md_steps)
+    fb = md_rate(ens, "fp64", args.md_steps)
+    # plain FP64 path: each instance on its own, one after the other, with
+    # the step replayed as a graph and with ordinary launches
+    half = max(1, args.md_steps // 2)
+    seq = {}
+    for name, graph in (("fp64_one_at_a_time_graph", True), ("fp64_one_at_a_time_launches", False)):
+        tot_t, tot_as = 0.0, 0.0
+        for e in ens:
+            r = md_rate([e], "fp64", half, warmup=2, use_graph=graph)
+            tot_t += r["ms_per_step"] * 1e-3 * half
+            tot_as += r["atoms"] * half
+        seq[name] = dict(atom_steps_per_s=tot_as / tot_t, atoms=nat,
+                         impacts_per_hour=3600.0 * len(ens) / (tot_t / half * STEPS_PER_IMPACT))
+    for r in (mb, fb):
+        r["impacts_per_hour"] = 3600.0 * len(ens) / (r["ms_per_step"] * 1e-3 * STEPS_PER_IMPACT)
+    out["ensemble"] = {"instances": len(ens), "atoms": nat, "steps_timed": args.md_steps,
+                       "mixed_batched": mb, "fp64_batched": fb, **seq,
+                       "mixed_batched_over_fp64_one_at_a_time_graph":
+                           mb["atom_steps_per_s"] / seq["fp64_one_at_a_time_graph"]["atom_steps_per_s"],
+                       "mixed_batched_over_fp64_one_at_a_time_launches":
+                           mb["atom_steps_per_s"] / seq["fp64_one_at_a_time_launches"]["atom_steps_per_s"]}
+    log(f"  ensemble mixed batched {mb['atom_steps_per_s']:.4g}, fp64 batched "
+        f"{fb['atom_steps_per_s']:.4g}, fp64 one at a time (graph) "
+        f"{seq['fp64_one_at_a_time_graph']['atom_steps_per_s']:.4g}, (launches) "
+        f"{seq['fp64_one_at_a_time_launches']['atom_steps_per_s']:.4g} atom-steps/s")
+    return out
+
+
 def rel_err(Fg, Fo, floor_frac=1e-3):
     d = np.sqrt(((Fg - Fo) ** 2).sum(1))
     m = np.sqrt((Fo ** 2).sum(1))
@@ -195,7 +275,10 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--ncell", type=int, default=64)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--profile", choices=["fp64", "mixed"], default=None)
+    ap.add_argument("--no-md", action="store_true")
+    ap.add_argument("--md-steps", type=int, default=100)
+    ap.add_argument("--ensemble", type=int, default=64)
+    ap.add_argument("--profile", choices=["fp64", "mixed", "md-fp64", "md-mixed"], default=None)
     ap.add_argument("--summarize-stats", nargs=3, metavar=("DIR", "OUT", "CMD"))
     ap.add_argument("--summarize-pmc", nargs=4, metavar=("FETCH_DIR", "WRITE_DIR", "OUT", "CMD"))
     args = ap.parse_args()
@@ -217,6 +300,13 @@ def main():
     X, at, L = make_workload(args.ncell)
     n = len(X)
     log(f"workload: {n} atoms, box {L[0]*1e10:.2f} A, generated in {time.perf_counter()-t0:.1f} s")
+    if args.profile in ("md-fp64", "md-mixed"):
+        # the batched ensemble, a few MD steps (for rocprofv3)
+        from fib_md_amd import configs
+        ens = configs.ensemble(args.ensemble)
+        r = md_rate(ens, args.profile[3:], 20, warmup=2)
+        log(json.dumps(r))
+        return
     if args.profile:
         # a plain copy of known size calibrates the FETCH/WRITE counters
         a = torch.empty(256 * 1024 * 1024 // 4, dtype=torch.float32, device="cuda").fill_(1.0)
@@ -249,6 +339,10 @@ def main():
         for name, k in pmc.get("kernels", {}).items():
             if name.split("<")[0] == dom and pmc.get("atoms") == n:
                 traffic = k["traffic_bytes"]
+    md = None
+    if not args.no_md:
+        log("MD steps ...")
+        md = md_bench(args)
     cpu = None
     if not args.no_cpu_baseline:
         log("cpu baseline ...")
@@ -279,6 +373,7 @@ def main():
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "bytes_per_atom": bpa, "avg_launch_ms": dur_ms},
+        "md_steps": md,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)
