@@ -29,7 +29,6 @@ enum KKey {
   KT_SORT,      // scan, scatter, fixup, reorder
   KT_NBUILD,    // k_nbuild
   KT_LISTAUX,   // k_clear_counts, k_maxcnt
-  KT_REV,       // k_rev
   KT_LEGS,      // k_legs
   KT_TRI,       // k_tri
   KT_GATHER,    // k_force3
@@ -64,7 +63,8 @@ struct DevBufs {
   int *ibase;                // [ninst] scan offset of each instance's first bucket
   int *bkey, *brank, *bcnt, *cstart, *perm, *bsum;
   int *cnt_sw, *nbr_sw, *cnt_ion, *nbr_ion;
-  uint8_t *rev;              // [NA * Wsw]: slot of i in its k-th neighbour's list
+  uint16_t *rk_sw;           // [NA * Wsw]: (stencil index << 8) | rank of each row entry
+  uint8_t *off_sw;           // [NA * 32]: row start of each stencil cell, [27] = has a row
   void *LA, *LB;             // leg scratch rows, double4 or float4 [NA * Wsw] each
   void *G;                   // centre-term force on each leg, T4 [NA * Wsw]
   uint64_t *amask;           // legs of the last evaluation, by list slot [NA]
@@ -92,19 +92,17 @@ void launch_sort(const DevBufs &d, hipStream_t s, bool inplace = false);
 // list build + longest-list reduction (DCtl::maxcnt_*)
 void launch_nbuild(const DevBufs &d, const FConst &fc, hipStream_t s);
 void launch_cellid(const DevBufs &d, int inst, int32_t *dout, hipStream_t s);
-// reverse-slot map, after a successful build
-void launch_rev(const DevBufs &d, hipStream_t s);
 // positions of the new build (Xb), displacement bookkeeping reset
 void launch_mark_build(const DevBufs &d, hipStream_t s);
 // passes 1-2 (k_legs, k_tri): own pair terms and legs, centre-term shares
 // into G, energy and virial reductions
 void launch_force_a(const DevBufs &d, const FConst &fc, bool mixed, bool energy, bool virial,
                     hipStream_t s);
-// pass 3 (k_force3): gather of G (through rev, or by search when rev is
-// stale); mode 0: store forces into Fout; mode 1: velocity update with Fout
-// the persistent force array F (timeint.f90:231-236)
+// pass 3 (k_force3): gather of G; mode 0: store forces into Fout; mode 1:
+// velocity update with Fout the persistent force array F (timeint.f90:231-236);
+// mode 2: MD-step mode second half kick
 void launch_force_b(const DevBufs &d, const FConst &fc, bool mixed, bool energy, int mode,
-                    bool search, const StepArgs &sa, double *Fout, hipStream_t s);
+                    const StepArgs &sa, double *Fout, hipStream_t s);
 void launch_pos(const DevBufs &d, const StepArgs &sa, hipStream_t s);
 // MD-step mode: head of the step (kick, drift, pins, removal) + rebuild gate
 void launch_md_pos(const DevBufs &d, const StepArgs &sa, int *gate, hipStream_t s);

@@ -10,7 +10,6 @@
 //   k_nbuild    one wavefront per cell: the 27-cell halo is staged in LDS,
 //               the reference pair predicate (si.f90:156-170) is evaluated
 //               lane-parallel and survivors are compacted with ballots
-//   k_rev       slot of i inside each neighbour's list (for the force gather)
 // Per force evaluation (replaces si_force, si.f90:384-552):
 //   k_legs      one thread per atom: SW pair and Moliere forces in registers,
 //               the atom's three-body legs to the leg arrays
@@ -424,17 +423,31 @@ __global__ void k_copyback(int NA, const double4 *__restrict__ X1, double4 *__re
 // ---------------------------------------------------------------------------
 // neighbour build: one 64-lane block per real cell
 #define NB_CHUNK 128
+#define NB_ACC 384  // candidates whose acceptance masks are kept for the ranks
+#define OFF_W 32    // bytes per atom of the stencil-offset table (27 used)
 
+// Stencil bookkeeping for the force gather.  The list row of an atom is the
+// concatenation, in the fixed order s = 0..26 of the 27-cell stencil, of the
+// accepted atoms of each stencil cell in cell order.  For every atom the
+// build stores off[s] = row position where stencil cell s starts (255 for an
+// atom without a row), and for every row entry (i -> j) the stencil index s
+// of j's cell and the rank of i among the atoms of i's cell that j accepts.
+// Stencil offsets are symmetric (cell 26 - s seen from j) and so is the pair
+// predicate, so the position of i in j's row is off_j[26 - s] + rank -- no
+// search of j's row is needed.  The rank is counted from the acceptance
+// masks of the cell's earlier atoms: j accepts a exactly when a accepted j.
 __global__ __launch_bounds__(FIB_WAVE) void k_nbuild(
     const double4 *__restrict__ X, const DInst *__restrict__ I, int ninst, int total_cells,
     const int *__restrict__ cstart, const int *__restrict__ bcnt, double rv2, double rv2lj,
     int NA, int Wsw, int Wion, int *__restrict__ cnt_sw, int *__restrict__ nbr_sw,
-    int *__restrict__ cnt_ion, int *__restrict__ nbr_ion, DCtl *__restrict__ ctl,
-    const int *__restrict__ gate) {
+    int *__restrict__ cnt_ion, int *__restrict__ nbr_ion, uint16_t *__restrict__ rk_sw,
+    uint8_t *__restrict__ off_sw, DCtl *__restrict__ ctl, const int *__restrict__ gate) {
   GATE_OR_RETURN(gate);
   __shared__ double4 s_x[NB_CHUNK];
   __shared__ int s_j[NB_CHUNK];
+  __shared__ uint8_t s_st[NB_CHUNK];
   __shared__ int s_rs[27], s_off[28];
+  __shared__ uint32_t s_acc[NB_ACC];
   const int lane = threadIdx.x;
   // each XCD walks a contiguous run of cells, so halos stay in its L2
   const int gcell = xcd_remap(blockIdx.x, gridDim.x);
@@ -479,6 +492,11 @@ __global__ __launch_bounds__(FIB_WAVE) void k_nbuild(
   if (lane < 27) { s_rs[lane] = rs; s_off[lane + 1] = sc; }
   if (lane == 0) s_off[0] = 0;
   const int total = __shfl(sc, 26);
+  // lane s < 27 owns stencil cell s: candidates [my_lo, my_hi)
+  const int my_lo = sc - rc;
+  const bool masks = total <= NB_ACC && ni <= 32;
+  if (masks)
+    for (int g = lane; g < total; g += FIB_WAVE) s_acc[g] = 0;
   __syncthreads();
 
   const double4 *Xc = X;
@@ -487,29 +505,38 @@ __global__ __launch_bounds__(FIB_WAVE) void k_nbuild(
     const double4 xi = Xc[i];
     const int ti = type_of(meta_of(xi));
     int nsw = 0, nion = 0;
+    int before = 0;  // lane s: accepted candidates before stencil cell s
     for (int base = 0; base < total; base += NB_CHUNK) {
       const int n = min(NB_CHUNK, total - base);
       // (re)stage the chunk; a single chunk is loaded once per cell
       if (ii == 0 || total > NB_CHUNK) {
         __syncthreads();
         for (int q = lane; q < n; q += FIB_WAVE) {
-          int g = base + q;
-          int r = 0;
-          while (s_off[r + 1] <= g) ++r;
-          int j = s_rs[r] + (g - s_off[r]);
+          const int g = base + q;
+          // stencil range holding g: binary search over the 28 offsets
+          int lo = 0, hi = 26;
+          while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (s_off[mid] <= g) lo = mid; else hi = mid - 1;
+          }
+          const int j = s_rs[lo] + (g - s_off[lo]);
           s_j[q] = j;
+          s_st[q] = (uint8_t)lo;
           s_x[q] = Xc[j];
         }
         __syncthreads();
       }
       for (int q0 = 0; q0 < n; q0 += FIB_WAVE) {
         const int q = q0 + lane;
+        const int g = base + q;
         bool sw = false, io = false;
-        int j = -1;
+        int j = -1, st = 0;
+        double4 xj;
         if (q < n) {
           j = s_j[q];
           if (j != i) {
-            const double4 xj = s_x[q];
+            xj = s_x[q];
+            st = s_st[q];
             const int tj = type_of(meta_of(xj));
             const double r2 = ref_r2(xi, xj, in);
             if (max(ti, tj) < 3) sw = r2 < rv2;
@@ -519,17 +546,42 @@ __global__ __launch_bounds__(FIB_WAVE) void k_nbuild(
         uint64_t msw = __ballot(sw), mio = __ballot(io);
         uint64_t below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
         if (sw) {
-          int pos = nsw + __popcll(msw & below);
-          if (pos < Wsw) nbr_sw[(size_t)i * Wsw + pos] = j;
+          // rank of i among the atoms of its cell that j lists (in cell order)
+          int rank = 0;
+          if (masks) {
+            const uint32_t acc = s_acc[g];
+            rank = __popc(acc & ((ii == 0) ? 0u : (~0u >> (32 - ii))));
+            s_acc[g] = acc | (1u << ii);
+          } else {
+            for (int a = 0; a < ii; ++a) {
+              const int ia = i0 + a;
+              if (ia == j) continue;
+              const double4 xa = Xc[ia];
+              if (type_of(meta_of(xa)) < 3 && ref_r2(xj, xa, in) < rv2) ++rank;
+            }
+          }
+          const int pos = nsw + __popcll(msw & below);
+          if (pos < Wsw) {
+            nbr_sw[(size_t)i * Wsw + pos] = j;
+            rk_sw[(size_t)i * Wsw + pos] = (uint16_t)((st << 8) | (rank & 0xff));
+          }
         }
         if (io) {
           int pos = nion + __popcll(mio & below);
           if (pos < Wion) nbr_ion[(size_t)i * Wion + pos] = j;
         }
+        // stencil cell s starts at candidate my_lo: count the accepted ones
+        // of this batch of 64 candidates that lie before it
+        {
+          const int cut = my_lo - (base + q0);
+          if (cut >= 64) before += __popcll(msw);
+          else if (cut > 0) before += __popcll(msw & (~0ull >> (64 - cut)));
+        }
         nsw += __popcll(msw);
         nion += __popcll(mio);
       }
     }
+    if (lane < 27) off_sw[(size_t)i * OFF_W + lane] = (uint8_t)min(before, 254);
     if (lane == 0) {
       cnt_sw[i] = nsw;
       cnt_ion[i] = nion;
@@ -540,7 +592,8 @@ __global__ __launch_bounds__(FIB_WAVE) void k_nbuild(
 // Atoms outside every scanned cell (dropped, dead, or phantom i-atoms) get
 // empty lists; k_nbuild only writes the atoms of real cells.
 __global__ void k_clear_counts(int NA, int *__restrict__ cnt_sw, int *__restrict__ cnt_ion,
-                               const double4 *__restrict__ X, const int *__restrict__ gate) {
+                               uint8_t *__restrict__ off_sw, const double4 *__restrict__ X,
+                               const int *__restrict__ gate) {
   GATE_OR_RETURN(gate);
   if (gate && blockIdx.x * blockDim.x + threadIdx.x < NA &&
       gated_out(gate, inst_of(meta_of(X[blockIdx.x * blockDim.x + threadIdx.x]))))
@@ -549,43 +602,10 @@ __global__ void k_clear_counts(int NA, int *__restrict__ cnt_sw, int *__restrict
   if (p >= NA) return;
   cnt_sw[p] = 0;
   cnt_ion[p] = 0;
-}
-
-// Reverse slots: rev[i][k] = slot of i inside the list of its k-th
-// neighbour j.  One thread per (atom, slot); lists are atom-major, so j's
-// list is one contiguous row read as int4.
-__global__ __launch_bounds__(FIB_BLOCK) void k_rev(int NA, int wshift, const int *__restrict__ cnt_sw,
-                                                   const int *__restrict__ nbr_sw,
-                                                   uint8_t *__restrict__ rev,
-                                                   const double4 *__restrict__ X,
-                                                   const int *__restrict__ gate) {
-  GATE_OR_RETURN(gate);
-  const int Wsw = 1 << wshift;
-  const size_t t = (size_t)blockIdx.x * FIB_BLOCK + threadIdx.x;
-  const int i = (int)(t >> wshift), k = (int)(t & (size_t)(Wsw - 1));
-  if (i >= NA) return;
-  if (gate && gated_out(gate, inst_of(meta_of(X[i])))) return;
-  const int c = min(cnt_sw[i], Wsw);
-  if (k >= c) return;
-  const int j = nbr_sw[t];
-  const int cj = min(cnt_sw[j], Wsw);
-  const int4 *rj = reinterpret_cast<const int4 *>(nbr_sw + (size_t)j * Wsw);
-  int r = 255;
-  // the first 32 slots in one batch of loads, the rest (Wsw = 64) after
-  for (int q0 = 0; q0 < cj && r == 255; q0 += 32) {
-    int4 e[8];
-#pragma unroll
-    for (int u = 0; u < 8; ++u) e[u] = rj[(q0 >> 2) + u];
-#pragma unroll
-    for (int u = 7; u >= 0; --u) {
-      const int q = q0 + 4 * u;
-      if (q + 3 < cj && e[u].w == i) r = q + 3;
-      if (q + 2 < cj && e[u].z == i) r = q + 2;
-      if (q + 1 < cj && e[u].y == i) r = q + 1;
-      if (q < cj && e[u].x == i) r = q;
-    }
-  }
-  rev[t] = (uint8_t)r;
+  // no row until k_nbuild writes one: offsets past any row (see k_force3)
+  uint4 *o = reinterpret_cast<uint4 *>(off_sw + (size_t)p * OFF_W);
+  o[0] = make_uint4(~0u, ~0u, ~0u, ~0u);
+  o[1] = make_uint4(~0u, ~0u, ~0u, ~0u);
 }
 
 // Longest SW / ion lists of the build (overflow check).
@@ -627,8 +647,7 @@ __device__ __forceinline__ int nth_bit(uint64_t m, int n) {
 // own G row (the reaction on the centre) + the G entries its neighbours wrote
 // for it.  Neighbour c's entry for i is the leg of c in the list slot r that
 // holds i; its compact index is the popcount of c's leg mask below r.  r
-// comes from rev (computed once per list build) or, for a single evaluation
-// after a build, from scanning c's list row.  This
+// follows from the stencil offsets the list build stores (see k_nbuild).  This
 // turns the reference's scatter F(i) += fi, F(j) += fj, F(k) += fk
 // (si.f90:544-546) into a gather.  In MD mode the same pass applies the
 // velocity update of onestep (timeint.f90:231-236).
@@ -1258,7 +1277,7 @@ __global__ void k_clear_gate(int *__restrict__ gate, int ninst) {
 //   k_md_pos     Berendsen shells, half kick, drift, pinned atoms, removal
 //                above the surface, displacement since the list build;
 //                k_md_decide sets the rebuild gate (2 * max displacement > skin)
-//   gated build  bin / sort in place / k_nbuild / k_rev (skipped on the
+//   gated build  bin / sort in place / k_nbuild (skipped on the
 //                device when the gate is 0)
 //   forces       k_legs, k_tri, k_extra, k_force3 MODE 2 (second half kick,
 //                |v|max, shell sums)
@@ -1404,12 +1423,12 @@ __global__ __launch_bounds__(FIB_BLOCK) void k_md_report(const double4 *__restri
 // ---------------------------------------------------------------------------
 // force pass 3: own terms + the G entries neighbours wrote for this atom;
 // MODE 0 = store forces, MODE 1 = reference velocity update
-// (timeint.f90:231-236) + thermostat sums.  SEARCH: scan the neighbour's list
-// row for the atom instead of reading rev.
-template <bool MIXED, bool EN, int MODE, bool SEARCH>
+// (timeint.f90:231-236) + thermostat sums; MODE 2 = MD-step mode.
+template <bool MIXED, bool EN, int MODE>
 __global__ __launch_bounds__(FIB_BLOCK) void k_force3(
     const double4 *__restrict__ X, const DInst *__restrict__ I, int NA, int Wsw,
-    const int *__restrict__ nbr_sw, const uint8_t *__restrict__ rev,
+    const int *__restrict__ nbr_sw, const uint16_t *__restrict__ rk_sw,
+    const uint8_t *__restrict__ off_sw,
     const typename Prec<MIXED>::T4 *__restrict__ G, const uint64_t *__restrict__ amask,
     const int *__restrict__ cnt_leg, const int *__restrict__ cnt_sw,
     const double *__restrict__ Fpart, const double *__restrict__ upart, double fscale,
@@ -1439,39 +1458,42 @@ __global__ __launch_bounds__(FIB_BLOCK) void k_force3(
       g0 -= (double)g.x; g1 -= (double)g.y; g2 -= (double)g.z;
       if (EN) gu += 0.5 * (double)g.w;
     }
+    // legs four at a time, so the gathers of different legs overlap
     while (m) {
-      const int k = __ffsll((unsigned long long)m) - 1;
-      m &= m - 1;
-      const int j = nbr_sw[row + k];
-      int n = -1;
-      if (SEARCH) {
-        // slot of i in j's list row (contiguous, atom-major), as k_rev does
-        const int cj = min(cnt_sw[j], Wsw);
-        const int4 *rj = reinterpret_cast<const int4 *>(nbr_sw + (size_t)j * Wsw);
-        int r = 255;
-        for (int q0 = 0; q0 < cj && r == 255; q0 += 32) {
-          int4 e[8];
+      int kk[4], jj[4], sr[4];
+      int cnt = 0;
 #pragma unroll
-          for (int u = 0; u < 8; ++u) e[u] = rj[(q0 >> 2) + u];
-#pragma unroll
-          for (int u = 7; u >= 0; --u) {
-            const int q = q0 + 4 * u;
-            if (q + 3 < cj && e[u].w == i) r = q + 3;
-            if (q + 2 < cj && e[u].z == i) r = q + 2;
-            if (q + 1 < cj && e[u].y == i) r = q + 1;
-            if (q < cj && e[u].x == i) r = q;
-          }
-        }
-        if (r < Wsw) n = __popcll(amask[j] & ((1ull << r) - 1ull));
-      } else {
-        const int r = rev[row + k];
-        // r == 255: j never lists i (a phantom-layer atom, see k_nbuild)
-        if (r < Wsw) n = __popcll(amask[j] & ((1ull << r) - 1ull));
+      for (int u = 0; u < 4; ++u) {
+        kk[u] = m ? __ffsll((unsigned long long)m) - 1 : -1;
+        if (m) { m &= m - 1; ++cnt; }
       }
-      if (n < 0) continue;
-      const T4 g = G[(size_t)n * NA + j];
-      g0 += (double)g.x; g1 += (double)g.y; g2 += (double)g.z;
-      if (EN) gu += (double)g.w;
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        jj[u] = kk[u] >= 0 ? nbr_sw[row + kk[u]] : i;
+        sr[u] = kk[u] >= 0 ? rk_sw[row + kk[u]] : 0;
+      }
+      uint64_t am4[4];
+      int r4[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        // slot of i in j's row from the stencil offsets (see k_nbuild); an
+        // atom without a row (phantom layer) has offsets 255: r >= Wsw
+        am4[u] = amask[jj[u]];
+        r4[u] = off_sw[(size_t)jj[u] * OFF_W + 26 - (sr[u] >> 8)] + (sr[u] & 0xff);
+      }
+      T4 g4[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const bool ok = u < cnt && r4[u] < Wsw;
+        const int n = ok ? __popcll(am4[u] & ((1ull << r4[u]) - 1ull)) : 0;
+        g4[u] = G[(size_t)n * NA + (ok ? jj[u] : i)];
+        if (!ok) { g4[u].x = g4[u].y = g4[u].z = g4[u].w = 0; }
+      }
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        g0 += (double)g4[u].x; g1 += (double)g4[u].y; g2 += (double)g4[u].z;
+        if (EN) gu += (double)g4[u].w;
+      }
     }
     // vertex terms outside the centre's list (k_extra), when there are any
     if (ctl[kin].extra_any) {
@@ -1483,15 +1505,8 @@ __global__ __launch_bounds__(FIB_BLOCK) void k_force3(
         const int j = nbr_sw[row + k];
         const uint64_t mj = exmask[j];
         if (!mj) continue;
-        int r = 255;
-        if (SEARCH) {
-          const int cj = min(cnt_sw[j], Wsw);
-          const int *rj = nbr_sw + (size_t)j * Wsw;
-          for (int q = 0; q < cj; ++q)
-            if (r == 255 && rj[q] == i) r = q;
-        } else {
-          r = rev[row + k];
-        }
+        const int sr = rk_sw[row + k];
+        const int r = off_sw[(size_t)j * OFF_W + 26 - (sr >> 8)] + (sr & 0xff);
         if (r < Wsw && ((mj >> r) & 1ull)) {
           const T4 g = EX[(size_t)r * NA + j];
           g0 += (double)g.x; g1 += (double)g.y; g2 += (double)g.z;
@@ -1939,13 +1954,14 @@ void launch_nbuild(const DevBufs &d, const FConst &fc, hipStream_t s) {
   {
     KScope ks_(d, KT_LISTAUX);
     hipLaunchKernelGGL(k_clear_counts, dim3(nblk(d.NA, FIB_BLOCK)), dim3(FIB_BLOCK), 0, s, d.NA,
-                       d.cnt_sw, d.cnt_ion, d.X[d.cur], d.gate);
+                       d.cnt_sw, d.cnt_ion, d.off_sw, d.X[d.cur], d.gate);
   }
   if (d.total_cells > 0) {
     KScope ks_(d, KT_NBUILD);
     hipLaunchKernelGGL(k_nbuild, dim3(d.total_cells), dim3(FIB_WAVE), 0, s, d.X[d.cur], d.I,
                        d.ninst, d.total_cells, d.cstart, d.bcnt, fc.rv2, fc.rv2lj, d.NA, d.Wsw,
-                       d.Wion, d.cnt_sw, d.nbr_sw, d.cnt_ion, d.nbr_ion, d.ctl, d.gate);
+                       d.Wion, d.cnt_sw, d.nbr_sw, d.cnt_ion, d.nbr_ion, d.rk_sw, d.off_sw,
+                       d.ctl, d.gate);
   }
   {
     KScope ks_(d, KT_LISTAUX);
@@ -1962,16 +1978,6 @@ void launch_cellid(const DevBufs &d, int inst, int32_t *dout, hipStream_t s) {
     KScope ks_(d, KT_IO);
     hipLaunchKernelGGL(k_cellid, dim3(nblk(d.NA, FIB_BLOCK)), dim3(FIB_BLOCK), 0, s, d.X[d.cur],
                        d.NA, d.I, inst, dout);
-  }
-}
-
-void launch_rev(const DevBufs &d, hipStream_t s) {
-  int sh = 0;
-  while ((1 << sh) < d.Wsw) ++sh;
-  {
-    KScope ks_(d, KT_REV);
-    hipLaunchKernelGGL(k_rev, dim3((unsigned)(((size_t)d.NA << sh) / FIB_BLOCK)), dim3(FIB_BLOCK), 0,
-                       s, d.NA, sh, d.cnt_sw, d.nbr_sw, d.rev, d.X[d.cur], d.gate);
   }
 }
 
@@ -2006,7 +2012,7 @@ static void force_ab(const DevBufs &d, const FConst &fc, hipStream_t s) {
   }
 }
 
-template <bool MIXED, bool EN, int MODE, bool SEARCH>
+template <bool MIXED, bool EN, int MODE>
 static void force_c(const DevBufs &d, const FConst &fc, const StepArgs &sa, double *Fo,
                     hipStream_t s) {
   typedef typename Prec<MIXED>::T4 T4;
@@ -2014,8 +2020,9 @@ static void force_c(const DevBufs &d, const FConst &fc, const StepArgs &sa, doub
   const double escale = MIXED ? fc.eps[0][0] : 1.0;
   {
     KScope ks_(d, KT_GATHER);
-    hipLaunchKernelGGL((k_force3<MIXED, EN, MODE, SEARCH>), dim3(d.NA / FIB_BLOCK), dim3(FIB_BLOCK),
-                       0, s, d.X[d.cur], d.I, d.NA, d.Wsw, d.nbr_sw, d.rev, (const T4 *)d.G,
+    hipLaunchKernelGGL((k_force3<MIXED, EN, MODE>), dim3(d.NA / FIB_BLOCK), dim3(FIB_BLOCK),
+                       0, s, d.X[d.cur], d.I, d.NA, d.Wsw, d.nbr_sw, d.rk_sw, d.off_sw,
+                       (const T4 *)d.G,
                        d.amask, d.cnt_leg, d.cnt_sw, d.Fpart, d.upart, fscale, escale, Fo,
                        d.uout, d.V[d.cur], sa, d.pstride, d.partials, d.ctl, (const T4 *)d.EX,
                        (const T4 *)d.EXself, d.exmask);
@@ -2040,26 +2047,18 @@ void launch_force_a(const DevBufs &d, const FConst &fc, bool mixed, bool energy,
 }
 
 template <bool MIXED>
-static void force_c_mode(const DevBufs &d, const FConst &fc, bool energy, int mode, bool search,
+static void force_c_mode(const DevBufs &d, const FConst &fc, bool energy, int mode,
                          const StepArgs &sa, double *Fout, hipStream_t s) {
-  if (mode == 1) {
-    if (search) force_c<MIXED, false, 1, true>(d, fc, sa, Fout, s);
-    else force_c<MIXED, false, 1, false>(d, fc, sa, Fout, s);
-  } else if (mode == 2) {
-    force_c<MIXED, false, 2, false>(d, fc, sa, Fout, s);
-  } else if (energy) {
-    if (search) force_c<MIXED, true, 0, true>(d, fc, sa, Fout, s);
-    else force_c<MIXED, true, 0, false>(d, fc, sa, Fout, s);
-  } else {
-    if (search) force_c<MIXED, false, 0, true>(d, fc, sa, Fout, s);
-    else force_c<MIXED, false, 0, false>(d, fc, sa, Fout, s);
-  }
+  if (mode == 1) force_c<MIXED, false, 1>(d, fc, sa, Fout, s);
+  else if (mode == 2) force_c<MIXED, false, 2>(d, fc, sa, Fout, s);
+  else if (energy) force_c<MIXED, true, 0>(d, fc, sa, Fout, s);
+  else force_c<MIXED, false, 0>(d, fc, sa, Fout, s);
 }
 
 void launch_force_b(const DevBufs &d, const FConst &fc, bool mixed, bool energy, int mode,
-                    bool search, const StepArgs &sa, double *Fout, hipStream_t s) {
-  if (mixed) force_c_mode<true>(d, fc, energy, mode, search, sa, Fout, s);
-  else force_c_mode<false>(d, fc, energy, mode, search, sa, Fout, s);
+                    const StepArgs &sa, double *Fout, hipStream_t s) {
+  if (mixed) force_c_mode<true>(d, fc, energy, mode, sa, Fout, s);
+  else force_c_mode<false>(d, fc, energy, mode, sa, Fout, s);
 }
 
 void launch_pos(const DevBufs &d, const StepArgs &sa, hipStream_t s) {

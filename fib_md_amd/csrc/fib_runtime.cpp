@@ -77,7 +77,7 @@ struct Timer {
 };
 
 static const char *const kTimerNames[KT_N] = {
-    "k_bin", "sort", "k_nbuild", "list_aux", "k_rev", "k_legs",
+    "k_bin", "sort", "k_nbuild", "list_aux", "k_legs",
     "k_tri", "k_force3", "k_extra", "k_pos", "reduce", "io"};
 static_assert(KT_N <= FIB_NTIMERS, "timer table");
 
@@ -107,7 +107,6 @@ struct fib_handle {
   std::vector<DInst> inst;
   DevBufs d{};
   bool have_state = false, have_lists = false, have_force = false;
-  bool rev_valid = false;  // rev matches the current lists
   // MD-step mode
   bool md_on = false;
   fib_md_params md{};
@@ -258,7 +257,7 @@ static int32_t free_bufs(fib_handle *h) {
   DevBufs &d = h->d;
   void *ptrs[] = {d.I, d.ctl, d.ibase, d.X[0], d.X[1], d.V[0], d.V[1], d.F[0], d.F[1], d.Xi[0], d.Xi[1],
                   d.inv, d.bkey, d.brank, d.bcnt, d.cstart, d.perm, d.bsum, d.cnt_sw, d.nbr_sw,
-                  d.cnt_ion, d.nbr_ion, d.rev, d.LA, d.LB, d.G, d.amask, d.cnt_leg, d.Xb, d.disp, d.EX, d.EXself, d.exmask, d.Fpart, d.upart, d.Fout, d.uout,
+                  d.cnt_ion, d.nbr_ion, d.rk_sw, d.off_sw, d.LA, d.LB, d.G, d.amask, d.cnt_leg, d.Xb, d.disp, d.EX, d.EXself, d.exmask, d.Fpart, d.upart, d.Fout, d.uout,
                   d.partials, d.tab, d.xfer};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
@@ -275,17 +274,17 @@ static int32_t alloc_lists(fib_handle *h, int Wsw, int Wion) {
   DevBufs &d = h->d;
   if (d.nbr_sw) (void)hipFree(d.nbr_sw);
   if (d.nbr_ion) (void)hipFree(d.nbr_ion);
-  void *old[] = {d.rev, d.LA, d.LB, d.G, d.EX};
+  void *old[] = {d.rk_sw, d.LA, d.LB, d.G, d.EX};
   for (void *q : old)
     if (q) (void)hipFree(q);
-  d.nbr_sw = nullptr; d.nbr_ion = nullptr; d.rev = nullptr;
+  d.nbr_sw = nullptr; d.nbr_ion = nullptr; d.rk_sw = nullptr;
   d.LA = nullptr; d.LB = nullptr; d.G = nullptr; d.EX = nullptr;
   d.Wsw = Wsw;
   d.Wion = Wion;
   const size_t NA = (size_t)d.NA;
   HIPCHK(h, dalloc(&d.nbr_sw, (size_t)Wsw * NA));
   HIPCHK(h, dalloc(&d.nbr_ion, (size_t)Wion * NA));
-  HIPCHK(h, dalloc(&d.rev, (size_t)Wsw * NA));
+  HIPCHK(h, dalloc(&d.rk_sw, (size_t)Wsw * NA));
   // a leg is a listed pair, so Wsw slots always hold every leg of an atom
   const size_t gsz = h->prec == FIB_PREC_MIXED ? sizeof(float4) : sizeof(double4);
   HIPCHK(h, hipMalloc(&d.LA, gsz * (size_t)Wsw * NA));
@@ -392,6 +391,7 @@ extern "C" int32_t fib_create(const fib_params *p, int32_t ninst, const fib_inst
   ALLOC(d.cnt_ion, NA);
   ALLOC(d.amask, NA);
   ALLOC(d.cnt_leg, NA);
+  ALLOC(d.off_sw, (size_t)32 * NA);
   ALLOC(d.Xb, NA);
   ALLOC(d.disp, NA);
   ALLOC(d.exmask, NA);
@@ -424,6 +424,7 @@ extern "C" int32_t fib_create(const fib_params *p, int32_t ninst, const fib_inst
   if (e == hipSuccess) e = hipMemset(d.amask, 0, sizeof(uint64_t) * NA);
   if (e == hipSuccess) e = hipMemset(d.cnt_leg, 0, sizeof(int) * NA);
   if (e == hipSuccess) e = hipMemset(d.exmask, 0, sizeof(uint64_t) * NA);
+  if (e == hipSuccess) e = hipMemset(d.off_sw, 0, (size_t)32 * NA);
   if (e == hipSuccess) e = hipMemset(d.disp, 0, sizeof(float) * NA);
   if (e == hipSuccess)
     e = hipMemcpy(d.tab, h->tab.data(), sizeof(TabEntry) * h->tab.size(), hipMemcpyHostToDevice);
@@ -596,7 +597,6 @@ static int32_t build_lists(fib_handle *h) {
     if (nonfin) return fail(h, FIB_ERR_NONFINITE, "non-finite position detected while binning");
     if (msw <= d.Wsw && mion <= d.Wion) {
       fibk::launch_mark_build(d, h->stream);
-      h->rev_valid = false;
       h->graph_valid = false;  // the sort moved the arrays a graph refers to
       h->have_lists = true;
       h->have_force = false;
@@ -676,14 +676,6 @@ extern "C" int32_t fib_get_nlist(fib_handle *h, int32_t inst, int32_t *nss, int3
   return FIB_OK;
 }
 
-// The reverse-slot map pays off when several force evaluations share one
-// list build (MD steps); a single evaluation searches instead.
-static void ensure_rev(fib_handle *h) {
-  if (h->rev_valid || !h->have_lists) return;
-  fibk::launch_rev(h->d, h->stream);
-  h->rev_valid = true;
-}
-
 // mode 0: forces into dst (default d.Fout); mode 1: MD velocity update.
 static int32_t run_force(fib_handle *h, int flags, int mode, const StepArgs &sa,
                          double *dst = nullptr) {
@@ -692,8 +684,7 @@ static int32_t run_force(fib_handle *h, int flags, int mode, const StepArgs &sa,
   DevBufs &d = h->d;
   double *Fo = mode == 0 ? (dst ? dst : d.Fout) : d.F[d.cur];
   fibk::launch_force_a(d, h->fc, mixed, en, vir, h->stream);
-  fibk::launch_force_b(d, h->fc, mixed, en && mode == 0, mode, !h->rev_valid, sa, Fo,
-                       h->stream);
+  fibk::launch_force_b(d, h->fc, mixed, en && mode == 0, mode, sa, Fo, h->stream);
   return FIB_OK;
 }
 
@@ -879,7 +870,6 @@ extern "C" int32_t fib_step(fib_handle *h, const fib_step_ctl *ctl, fib_step_rep
   const fib_run_params &P = R.rp;
   const int builds0 = h->n_builds;
   int steps = 0;
-  ensure_rev(h);
   for (;;) {
     if (ctl->max_steps > 0 && steps >= ctl->max_steps) break;
     if (w == 0) {
@@ -906,7 +896,6 @@ extern "C" int32_t fib_step(fib_handle *h, const fib_step_ctl *ctl, fib_step_rep
     if (lt % R.ntlist == 0) {
       int32_t r = build_lists(h);
       if (r != FIB_OK) return r;
-      ensure_rev(h);
     }
     if (lt % R.dslist == 0 && w == 1) {
       DCtl c;
@@ -968,19 +957,17 @@ static void enqueue_md_step(fib_handle *h) {
   fibk::launch_scan(d, h->stream);
   fibk::launch_sort(d, h->stream, true);
   fibk::launch_nbuild(d, h->fc, h->stream);
-  fibk::launch_rev(d, h->stream);
   fibk::launch_mark_build(d, h->stream);
   d.gate = nullptr;
   const bool mixed = h->prec == FIB_PREC_MIXED;
   fibk::launch_force_a(d, h->fc, mixed, false, false, h->stream);
-  fibk::launch_force_b(d, h->fc, mixed, false, 2, false, sa, d.F[d.cur], h->stream);
+  fibk::launch_force_b(d, h->fc, mixed, false, 2, sa, d.F[d.cur], h->stream);
   fibk::launch_md_ctl(d, sa, h->stream);
 }
 
-// forces of the current state into F, rev, and dt from the current |v|max
+// forces of the current state into F, and dt from the current |v|max
 static int32_t md_refresh(fib_handle *h, bool reset) {
   DevBufs &d = h->d;
-  ensure_rev(h);
   int32_t r = run_force(h, FIB_EVAL_FORCE, 0, h->md_sa, d.F[d.cur]);
   if (r != FIB_OK) return r;
   fibk::launch_sums(d, h->md_sa, h->stream);
@@ -1049,7 +1036,6 @@ extern "C" int32_t fib_md_run(fib_handle *h, int32_t nsteps) {
   if (!h || nsteps < 0) return FIB_ERR_ARG;
   if (!h->md_on) return fail(h, FIB_ERR_STATE, "fib_md_run: call fib_md_begin first");
   if (nsteps == 0) return FIB_OK;
-  ensure_rev(h);
   if (h->md.use_graph) {
     if (!h->graph_valid) {
       if (h->graph_exec) { (void)hipGraphExecDestroy(h->graph_exec); h->graph_exec = nullptr; }
