@@ -304,7 +304,7 @@ def main():
         # the batched ensemble, a few MD steps (for rocprofv3)
         from fib_md_amd import configs
         ens = configs.ensemble(args.ensemble)
-        r = md_rate(ens, args.profile[3:], 20, warmup=2)
+        r = md_rate(ens, args.profile[3:], 60, warmup=2)
         log(json.dumps(r))
         return
     if args.profile:

@@ -500,12 +500,16 @@ __global__ __launch_bounds__(FIB_WAVE) void k_nbuild(
   __syncthreads();
 
   const double4 *Xc = X;
+  // the cell's own atoms are stencil range 13: read them from the staged
+  // chunk when it holds them (always, unless the halo is very dense)
+  const int own0 = s_off[13];
+  const bool own_staged = own0 + ni <= NB_CHUNK && total <= NB_CHUNK;
   for (int ii = 0; ii < ni; ++ii) {
     const int i = i0 + ii;
-    const double4 xi = Xc[i];
-    const int ti = type_of(meta_of(xi));
     int nsw = 0, nion = 0;
     int before = 0;  // lane s: accepted candidates before stencil cell s
+    double4 xi;
+    int ti = 0;
     for (int base = 0; base < total; base += NB_CHUNK) {
       const int n = min(NB_CHUNK, total - base);
       // (re)stage the chunk; a single chunk is loaded once per cell
@@ -525,6 +529,10 @@ __global__ __launch_bounds__(FIB_WAVE) void k_nbuild(
           s_x[q] = Xc[j];
         }
         __syncthreads();
+      }
+      if (base == 0) {
+        xi = own_staged ? s_x[own0 + ii] : Xc[i];
+        ti = type_of(meta_of(xi));
       }
       for (int q0 = 0; q0 < n; q0 += FIB_WAVE) {
         const int q = q0 + lane;
@@ -882,19 +890,36 @@ __global__ __launch_bounds__(FIB_BLOCK) void k_legs(
   if (sw_atom) {
     const int cs = min(cnt_sw[i], Wsw);
     const int4 *r4 = reinterpret_cast<const int4 *>(nbr_sw + row);
-    for (int k0 = 0; k0 < cs; k0 += 4) {
+    // the first 24 slots in one batch of index loads (all in flight), then
+    // the positions four at a time
+    int4 rq[6];
+#pragma unroll
+    for (int u = 0; u < 6; ++u) rq[u] = (4 * u < cs) ? r4[u] : make_int4(i, i, i, i);
+#pragma unroll
+    for (int u = 0; u < 6; ++u) {
+      const int k0 = 4 * u;
+      if (k0 >= cs) break;
+      const int jq[4] = {rq[u].x, (k0 + 1 < cs) ? rq[u].y : i, (k0 + 2 < cs) ? rq[u].z : i,
+                         (k0 + 3 < cs) ? rq[u].w : i};
+      double4 xq[4];
+#pragma unroll
+      for (int v = 0; v < 4; ++v) xq[v] = X[jq[v]];
+#pragma unroll
+      for (int v = 0; v < 4; ++v)
+        if (k0 + v < cs) pair(k0 + v, jq[v], xq[v]);
+    }
+    for (int k0 = 24; k0 < cs; k0 += 4) {
       const int4 q = r4[k0 >> 2];
       const int jq[4] = {q.x, (k0 + 1 < cs) ? q.y : i, (k0 + 2 < cs) ? q.z : i,
                          (k0 + 3 < cs) ? q.w : i};
       double4 xq[4];
 #pragma unroll
-      for (int u = 0; u < 4; ++u) xq[u] = X[jq[u]];
+      for (int v = 0; v < 4; ++v) xq[v] = X[jq[v]];
 #pragma unroll
-      for (int u = 0; u < 4; ++u)
-        if (k0 + u < cs) pair(k0 + u, jq[u], xq[u]);
+      for (int v = 0; v < 4; ++v)
+        if (k0 + v < cs) pair(k0 + v, jq[v], xq[v]);
     }
   }
-
   cnt_leg[i] = na;
   amask[i] = am;
 
