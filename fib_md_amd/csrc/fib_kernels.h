@@ -67,6 +67,8 @@ struct DevBufs {
   uint8_t *off_sw;           // [NA * 32]: row start of each stencil cell, [27] = has a row
   void *LA, *LB;             // leg scratch rows, double4 or float4 [NA * Wsw] each
   void *G;                   // centre-term force on each leg, T4 [NA * Wsw]
+  uint32_t *RA;              // per leg [NA * Wsw]: index into G of the partner's force
+                             // on the atom (~0: none); needs NA * Wsw < 2^32
   uint64_t *amask;           // legs of the last evaluation, by list slot [NA]
   int *cnt_leg;              // legs per atom of the last evaluation [NA]
   double *Fpart, *upart;     // own-term force and energy [3*NA], [NA]
@@ -81,7 +83,8 @@ struct DevBufs {
   TabEntry *tab;             // [ntable + 3]
   double *xfer;              // staging [3*NA] for host transfers
   const KTimer *kt;          // timing hook (may be null)
-  const int *gate;           // non-null: [1 + ninst] build gates, see fib_kernels.hip
+  const int *gate;           // non-null: [2 + 2*ninst] build gates and the cell prefix of
+                             // the gated instances, see k_md_decide
 };
 
 namespace fibk {

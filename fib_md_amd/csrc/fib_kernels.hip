@@ -424,6 +424,7 @@ __global__ void k_copyback(int NA, const double4 *__restrict__ X1, double4 *__re
 // neighbour build: one 64-lane block per real cell
 #define NB_CHUNK 128
 #define NB_ACC 384  // candidates whose acceptance masks are kept for the ranks
+#define NB_CELLS 8  // cells per k_nbuild workgroup
 #define OFF_W 32    // bytes per atom of the stencil-offset table (27 used)
 
 // Stencil bookkeeping for the force gather.  The list row of an atom is the
@@ -436,25 +437,33 @@ __global__ void k_copyback(int NA, const double4 *__restrict__ X1, double4 *__re
 // predicate, so the position of i in j's row is off_j[26 - s] + rank -- no
 // search of j's row is needed.  The rank is counted from the acceptance
 // masks of the cell's earlier atoms: j accepts a exactly when a accepted j.
-__global__ __launch_bounds__(FIB_WAVE) void k_nbuild(
-    const double4 *__restrict__ X, const DInst *__restrict__ I, int ninst, int total_cells,
-    const int *__restrict__ cstart, const int *__restrict__ bcnt, double rv2, double rv2lj,
-    int NA, int Wsw, int Wion, int *__restrict__ cnt_sw, int *__restrict__ nbr_sw,
-    int *__restrict__ cnt_ion, int *__restrict__ nbr_ion, uint16_t *__restrict__ rk_sw,
-    uint8_t *__restrict__ off_sw, DCtl *__restrict__ ctl, const int *__restrict__ gate) {
-  GATE_OR_RETURN(gate);
-  __shared__ double4 s_x[NB_CHUNK];
-  __shared__ int s_j[NB_CHUNK];
-  __shared__ uint8_t s_st[NB_CHUNK];
-  __shared__ int s_rs[27], s_off[28];
-  __shared__ uint32_t s_acc[NB_ACC];
+//
+// One cell c of the build (one wave); see k_nbuild for the numbering of c.
+__device__ __forceinline__ void nbuild_cell(
+    const double4 *__restrict__ X, const DInst *__restrict__ I, int ninst, int c,
+    const int *__restrict__ cplan, const int *__restrict__ cstart, const int *__restrict__ bcnt,
+    double rv2, double rv2lj, int Wsw, int Wion, int *__restrict__ cnt_sw,
+    int *__restrict__ nbr_sw, int *__restrict__ cnt_ion, int *__restrict__ nbr_ion,
+    uint16_t *__restrict__ rk_sw, uint8_t *__restrict__ off_sw, double4 *s_x, int *s_j,
+    uint8_t *s_st, int *s_rs, int *s_off, uint32_t *s_acc) {
   const int lane = threadIdx.x;
-  // each XCD walks a contiguous run of cells, so halos stay in its L2
-  const int gcell = xcd_remap(blockIdx.x, gridDim.x);
-  if (gcell >= total_cells) return;
-  const int k = find_inst_by_cell(I, ninst, gcell);
-  if (gated_out(gate, k)) return;
+  int k, gcell;
+  if (cplan) {
+    // last instance whose prefix is <= c (gated-out instances have width 0)
+    int lo = 0, hi = ninst - 1;
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (cplan[mid] <= c) lo = mid; else hi = mid - 1;
+    }
+    k = lo;
+    gcell = I[k].cell_real_off + (c - cplan[k]);
+  } else {
+    gcell = c;
+    k = find_inst_by_cell(I, ninst, gcell);
+  }
   const DInst &in = I[k];
+  // the previous cell of this wave is done with the shared arrays
+  __syncthreads();
   const int N0 = in.nc[0], N1 = in.nc[1];
   const int loc = gcell - in.cell_real_off;
   const int cx = loc % N0, cy = (loc / N0) % N1, cz = loc / (N0 * N1);
@@ -597,6 +606,36 @@ __global__ __launch_bounds__(FIB_WAVE) void k_nbuild(
   }
 }
 
+// Each workgroup builds NB_CELLS consecutive cells.  The cells to build are
+// all real cells, or under a gate the cells of the gated instances, numbered
+// by the prefix that k_md_decide writes after the gates; the grid covers all
+// real cells and the workgroups past the gated ones leave at once (gated-out
+// instances cost a launch of empty workgroups, no more).  The busy
+// workgroups are remapped so that each XCD builds a contiguous run of cells,
+// whose halos then stay in its L2.
+__global__ __launch_bounds__(FIB_WAVE) void k_nbuild(
+    const double4 *__restrict__ X, const DInst *__restrict__ I, int ninst, int total_cells,
+    const int *__restrict__ cstart, const int *__restrict__ bcnt, double rv2, double rv2lj,
+    int Wsw, int Wion, int *__restrict__ cnt_sw, int *__restrict__ nbr_sw,
+    int *__restrict__ cnt_ion, int *__restrict__ nbr_ion, uint16_t *__restrict__ rk_sw,
+    uint8_t *__restrict__ off_sw, const int *__restrict__ gate) {
+  GATE_OR_RETURN(gate);
+  __shared__ double4 s_x[NB_CHUNK];
+  __shared__ int s_j[NB_CHUNK];
+  __shared__ uint8_t s_st[NB_CHUNK];
+  __shared__ int s_rs[27], s_off[28];
+  __shared__ uint32_t s_acc[NB_ACC];
+  const int *cplan = gate ? gate + 1 + ninst : nullptr;
+  const int ncell = gate ? cplan[ninst] : total_cells;
+  const int nbusy = (ncell + NB_CELLS - 1) / NB_CELLS;
+  if ((int)blockIdx.x >= nbusy) return;
+  const int c0 = xcd_remap(blockIdx.x, nbusy) * NB_CELLS;
+  const int c1 = min(ncell, c0 + NB_CELLS);
+  for (int c = c0; c < c1; ++c)
+    nbuild_cell(X, I, ninst, c, cplan, cstart, bcnt, rv2, rv2lj, Wsw, Wion, cnt_sw, nbr_sw,
+                cnt_ion, nbr_ion, rk_sw, off_sw, s_x, s_j, s_st, s_rs, s_off, s_acc);
+}
+
 // Atoms outside every scanned cell (dropped, dead, or phantom i-atoms) get
 // empty lists; k_nbuild only writes the atoms of real cells.
 __global__ void k_clear_counts(int NA, int *__restrict__ cnt_sw, int *__restrict__ cnt_ion,
@@ -679,6 +718,7 @@ struct Leg {
   T E;           // exp(gamma/(r/sigma - a))
   T g;           // d ln E / dr (same length unit as rinv)
   uint32_t ks;   // neighbour index | (species == Ge) << 31
+  uint32_t slot; // list slot of the neighbour in the centre's row
 };
 
 __device__ __forceinline__ double ks_pack(double, uint32_t ks) {
@@ -699,7 +739,7 @@ __device__ __forceinline__ void store_leg(T4 *__restrict__ LA, T4 *__restrict__ 
                                           const Leg<T> &L) {
   T4 a, b;
   a.x = L.e0; a.y = L.e1; a.z = L.e2; a.w = L.E;
-  b.x = L.g; b.y = L.rinv; b.z = ks_pack(T(0), L.ks); b.w = T(0);
+  b.x = L.g; b.y = L.rinv; b.z = ks_pack(T(0), L.ks); b.w = ks_pack(T(0), L.slot);
   LA[at] = a;
   LB[at] = b;
 }
@@ -707,7 +747,7 @@ template <typename T, typename T4>
 __device__ __forceinline__ void load_leg(const T4 *LA, const T4 *LB, size_t at, Leg<T> &L) {
   const T4 a = LA[at], b = LB[at];
   L.e0 = a.x; L.e1 = a.y; L.e2 = a.z; L.E = a.w;
-  L.g = b.x; L.rinv = b.y; L.ks = ks_unpack(b.z);
+  L.g = b.x; L.rinv = b.y; L.ks = ks_unpack(b.z); L.slot = ks_unpack(b.w);
 }
 
 // Force on leg a of one vertex term (centre c, legs a and b), and h.
@@ -804,13 +844,17 @@ __global__ __launch_bounds__(FIB_BLOCK) void k_legs(
   uint64_t am = 0;
   // one buffered SW pair in list slot k: pair force, pair energy, leg
   auto pair = [&](int k, int jn, const double4 &xj) {
-    const int tj = type_of(meta_of(xj));
+    const uint32_t mj = meta_of(xj);
+    // an atom removed since the build is out of the simulation (MD-step mode)
+    if (mj & META_DEAD) return;
+    const int tj = type_of(mj);
     const bool jA = tj != 2;
     const double isg = jA ? isgA : isgB;
     double d0, d1, d2;
     min_image_fast(xi, xj, in, d0, d1, d2);
     Leg<T> L;
     L.ks = (uint32_t)jn | (jA ? 0u : 0x80000000u);
+    L.slot = (uint32_t)k;
     if (!MIXED) {
       const double rr = d0 * d0 + d1 * d1 + d2 * d2;
       if (!(rr * (isg * isg) < fc.r2gate)) return;
@@ -1001,10 +1045,12 @@ __global__ __launch_bounds__(FIB_BLOCK) void k_legs(
 #define TRI_MAXLEG 2048  // 64 atoms x 32 legs; more legs are taken in batches
 template <bool MIXED, bool EN, bool VIR>
 __global__ __launch_bounds__(FIB_BLOCK) void k_tri(
-    const double4 *__restrict__ X, const FConst fc, int NA, const int *__restrict__ cnt_leg,
-    const typename Prec<MIXED>::T4 *__restrict__ LA,
-    const typename Prec<MIXED>::T4 *__restrict__ LB, typename Prec<MIXED>::T4 *__restrict__ G,
-    int pstride, double *__restrict__ partials) {
+    const double4 *__restrict__ X, const FConst fc, int NA, int Wsw,
+    const int *__restrict__ cnt_leg, const typename Prec<MIXED>::T4 *__restrict__ LA,
+    const typename Prec<MIXED>::T4 *__restrict__ LB, const uint16_t *__restrict__ rk_sw,
+    const uint8_t *__restrict__ off_sw, const uint64_t *__restrict__ amask,
+    typename Prec<MIXED>::T4 *__restrict__ G, uint32_t *__restrict__ RA, int pstride,
+    double *__restrict__ partials) {
   typedef typename Prec<MIXED>::T T;
   typedef typename Prec<MIXED>::T4 T4;
   __shared__ uint8_t owner[FIB_BLOCK / FIB_WAVE][TRI_MAXLEG];
@@ -1055,6 +1101,14 @@ __global__ __launch_bounds__(FIB_BLOCK) void k_tri(
       const int a = g - soff[wid][o];
       Leg<T> La;
       load_leg(LA, LB, (size_t)a * NA + i, La);
+      // Where the partner j keeps the force of its own terms on i: G[n * NA
+      // + j], n = j's leg index of i.  The slot r of i in j's row follows
+      // from the stencil offsets (see k_nbuild), n from j's leg mask (legs
+      // are symmetric); 255 offsets (r >= Wsw) mark a j without a row.
+      const int j = ks_index(La.ks);
+      const uint64_t amj = amask[j];
+      const int rk = rk_sw[(size_t)i * Wsw + La.slot];
+      const int r = off_sw[(size_t)j * OFF_W + 26 - (rk >> 8)] + (rk & 0xff);
       const int gi = (type_of(meta_of(X[i])) == 2) ? 1 : 0;
       const int ga = gi + ks_ge(La.ks);
       T Ga0 = 0, Ga1 = 0, Ga2 = 0, Gu = 0;
@@ -1070,6 +1124,9 @@ __global__ __launch_bounds__(FIB_BLOCK) void k_tri(
       T4 gv;
       gv.x = Ga0; gv.y = Ga1; gv.z = Ga2; gv.w = Gu * T(1.0 / 3.0);
       G[(size_t)a * NA + i] = gv;
+      RA[(size_t)a * NA + i] =
+          r < Wsw ? (uint32_t)__popcll(amj & ((1ull << r) - 1ull)) * (uint32_t)NA + (uint32_t)j
+                  : ~0u;
       uc += Gu;
       if (VIR) {
         // r_a = x_a - x_i = -e_a / rinv_a
@@ -1369,16 +1426,34 @@ __global__ __launch_bounds__(FIB_BLOCK) void k_md_pos(double4 *__restrict__ X,
 
 // Rebuild gates of the step: each instance whose 2 * max displacement
 // exceeds the skin, and whether there is any.
-__global__ void k_md_decide(int ninst, const DCtl *__restrict__ ctl, double skin,
-                            int *__restrict__ gate) {
-  if (threadIdx.x != 0 || blockIdx.x != 0) return;
-  int any = 0;
-  for (int k = 0; k < ninst; ++k) {
-    const int g = (2.0 * ctl[k].maxdisp > skin) ? 1 : 0;
-    gate[1 + k] = g;
-    any |= g;
+__global__ void k_md_decide(int ninst, const DInst *__restrict__ I, const DCtl *__restrict__ ctl,
+                            double skin, int *__restrict__ gate) {
+  // one wave; behind the gates, the prefix over instances of the real cells
+  // of the gated ones (the cell numbering of a gated k_nbuild)
+  const int lane = threadIdx.x;
+  int *cplan = gate + 1 + ninst;
+  int any = 0, carry = 0;
+  for (int k0 = 0; k0 < ninst; k0 += FIB_WAVE) {
+    const int k = k0 + lane;
+    int g = 0, w = 0;
+    if (k < ninst) {
+      g = (2.0 * ctl[k].maxdisp > skin) ? 1 : 0;
+      gate[1 + k] = g;
+      if (g) w = I[k].nc[0] * I[k].nc[1] * I[k].nc[2];
+    }
+    int sc = w;
+    for (int o = 1; o < FIB_WAVE; o <<= 1) {
+      const int y = __shfl_up(sc, o);
+      if (lane >= o) sc += y;
+    }
+    if (k < ninst) cplan[k] = carry + sc - w;
+    carry += __shfl(sc, FIB_WAVE - 1);
+    any |= __ballot(g) != 0ull;
   }
-  gate[0] = any;
+  if (lane == 0) {
+    cplan[ninst] = carry;
+    gate[0] = any;
+  }
 }
 
 // After the step's second half kick: advance time, choose the next dt and
@@ -1453,8 +1528,8 @@ template <bool MIXED, bool EN, int MODE>
 __global__ __launch_bounds__(FIB_BLOCK) void k_force3(
     const double4 *__restrict__ X, const DInst *__restrict__ I, int NA, int Wsw,
     const int *__restrict__ nbr_sw, const uint16_t *__restrict__ rk_sw,
-    const uint8_t *__restrict__ off_sw,
-    const typename Prec<MIXED>::T4 *__restrict__ G, const uint64_t *__restrict__ amask,
+    const uint8_t *__restrict__ off_sw, const typename Prec<MIXED>::T4 *__restrict__ G,
+    const uint32_t *__restrict__ RA,
     const int *__restrict__ cnt_leg, const int *__restrict__ cnt_sw,
     const double *__restrict__ Fpart, const double *__restrict__ upart, double fscale,
     double escale, double *__restrict__ Fout, double *__restrict__ uout, double *__restrict__ V,
@@ -1473,7 +1548,6 @@ __global__ __launch_bounds__(FIB_BLOCK) void k_force3(
   if (live) {
     f0 = Fpart[i]; f1 = Fpart[(size_t)NA + i]; f2 = Fpart[2 * (size_t)NA + i];
     if (EN) u = upart[i];
-    uint64_t m = amask[i];
     const size_t row = (size_t)i * Wsw;
     double g0 = 0, g1 = 0, g2 = 0, gu = 0;
     // reaction on the centre: minus the forces its own terms put on its legs
@@ -1483,42 +1557,27 @@ __global__ __launch_bounds__(FIB_BLOCK) void k_force3(
       g0 -= (double)g.x; g1 -= (double)g.y; g2 -= (double)g.z;
       if (EN) gu += 0.5 * (double)g.w;
     }
-    // legs four at a time, so the gathers of different legs overlap
-    while (m) {
-      int kk[4], jj[4], sr[4];
-      int cnt = 0;
-#pragma unroll
-      for (int u = 0; u < 4; ++u) {
-        kk[u] = m ? __ffsll((unsigned long long)m) - 1 : -1;
-        if (m) { m &= m - 1; ++cnt; }
-      }
-#pragma unroll
-      for (int u = 0; u < 4; ++u) {
-        jj[u] = kk[u] >= 0 ? nbr_sw[row + kk[u]] : i;
-        sr[u] = kk[u] >= 0 ? rk_sw[row + kk[u]] : 0;
-      }
-      uint64_t am4[4];
-      int r4[4];
-#pragma unroll
-      for (int u = 0; u < 4; ++u) {
-        // slot of i in j's row from the stencil offsets (see k_nbuild); an
-        // atom without a row (phantom layer) has offsets 255: r >= Wsw
-        am4[u] = amask[jj[u]];
-        r4[u] = off_sw[(size_t)jj[u] * OFF_W + 26 - (sr[u] >> 8)] + (sr[u] & 0xff);
-      }
+    // forces the partners' terms put on this atom, at the places k_tri
+    // noted for its legs: one lookup per leg, four in flight
+    for (int n0 = 0; n0 < na; n0 += 4) {
+      uint32_t ra[4];
       T4 g4[4];
 #pragma unroll
-      for (int u = 0; u < 4; ++u) {
-        const bool ok = u < cnt && r4[u] < Wsw;
-        const int n = ok ? __popcll(am4[u] & ((1ull << r4[u]) - 1ull)) : 0;
-        g4[u] = G[(size_t)n * NA + (ok ? jj[u] : i)];
-        if (!ok) { g4[u].x = g4[u].y = g4[u].z = g4[u].w = 0; }
-      }
+      for (int u = 0; u < 4; ++u) ra[u] = (n0 + u < na) ? RA[(size_t)(n0 + u) * NA + i] : ~0u;
 #pragma unroll
       for (int u = 0; u < 4; ++u) {
-        g0 += (double)g4[u].x; g1 += (double)g4[u].y; g2 += (double)g4[u].z;
-        if (EN) gu += (double)g4[u].w;
+        if (ra[u] != ~0u) {
+          g4[u] = G[ra[u]];
+        } else {
+          g4[u].x = g4[u].y = g4[u].z = g4[u].w = 0;
+        }
       }
+#pragma unroll
+      for (int u = 0; u < 4; ++u)
+        if (n0 + u < na) {
+          g0 += (double)g4[u].x; g1 += (double)g4[u].y; g2 += (double)g4[u].z;
+          if (EN) gu += (double)g4[u].w;
+        }
     }
     // vertex terms outside the centre's list (k_extra), when there are any
     if (ctl[kin].extra_any) {
@@ -1983,10 +2042,9 @@ void launch_nbuild(const DevBufs &d, const FConst &fc, hipStream_t s) {
   }
   if (d.total_cells > 0) {
     KScope ks_(d, KT_NBUILD);
-    hipLaunchKernelGGL(k_nbuild, dim3(d.total_cells), dim3(FIB_WAVE), 0, s, d.X[d.cur], d.I,
-                       d.ninst, d.total_cells, d.cstart, d.bcnt, fc.rv2, fc.rv2lj, d.NA, d.Wsw,
-                       d.Wion, d.cnt_sw, d.nbr_sw, d.cnt_ion, d.nbr_ion, d.rk_sw, d.off_sw,
-                       d.ctl, d.gate);
+    hipLaunchKernelGGL(k_nbuild, dim3(nblk(d.total_cells, NB_CELLS)), dim3(FIB_WAVE), 0, s, d.X[d.cur], d.I, d.ninst,
+                       d.total_cells, d.cstart, d.bcnt, fc.rv2, fc.rv2lj, d.Wsw, d.Wion,
+                       d.cnt_sw, d.nbr_sw, d.cnt_ion, d.nbr_ion, d.rk_sw, d.off_sw, d.gate);
   }
   {
     KScope ks_(d, KT_LISTAUX);
@@ -2019,8 +2077,9 @@ static void force_ab(const DevBufs &d, const FConst &fc, hipStream_t s) {
   {
     KScope ks_(d, KT_TRI);
     hipLaunchKernelGGL((k_tri<MIXED, EN, VIR>), dim3(d.NA / FIB_BLOCK), dim3(FIB_BLOCK), 0, s,
-                       d.X[d.cur], fc, d.NA, d.cnt_leg, (const T4 *)d.LA, (const T4 *)d.LB,
-                       (T4 *)d.G, d.pstride, d.partials + (size_t)FIB_NRED * d.pstride);
+                       d.X[d.cur], fc, d.NA, d.Wsw, d.cnt_leg, (const T4 *)d.LA,
+                       (const T4 *)d.LB, d.rk_sw, d.off_sw, d.amask, (T4 *)d.G, d.RA, d.pstride,
+                       d.partials + (size_t)FIB_NRED * d.pstride);
   }
   {
     KScope ks_(d, KT_EXTRA);
@@ -2047,8 +2106,8 @@ static void force_c(const DevBufs &d, const FConst &fc, const StepArgs &sa, doub
     KScope ks_(d, KT_GATHER);
     hipLaunchKernelGGL((k_force3<MIXED, EN, MODE>), dim3(d.NA / FIB_BLOCK), dim3(FIB_BLOCK),
                        0, s, d.X[d.cur], d.I, d.NA, d.Wsw, d.nbr_sw, d.rk_sw, d.off_sw,
-                       (const T4 *)d.G,
-                       d.amask, d.cnt_leg, d.cnt_sw, d.Fpart, d.upart, fscale, escale, Fo,
+                       (const T4 *)d.G, d.RA, d.cnt_leg, d.cnt_sw, d.Fpart,
+                       d.upart, fscale, escale, Fo,
                        d.uout, d.V[d.cur], sa, d.pstride, d.partials, d.ctl, (const T4 *)d.EX,
                        (const T4 *)d.EXself, d.exmask);
   }
@@ -2108,7 +2167,8 @@ void launch_md_pos(const DevBufs &d, const StepArgs &sa, int *gate, hipStream_t 
   const int op[1] = {1};
   const int dst[1] = {DCTL_IDX(maxdisp)};
   finalize(d, 1, FIB_BLOCK, op, dst, s);
-  hipLaunchKernelGGL(k_md_decide, dim3(1), dim3(FIB_WAVE), 0, s, d.ninst, d.ctl, sa.skin, gate);
+  hipLaunchKernelGGL(k_md_decide, dim3(1), dim3(FIB_WAVE), 0, s, d.ninst, d.I, d.ctl, sa.skin,
+                     gate);
 }
 
 void launch_md_ctl(const DevBufs &d, const StepArgs &sa, hipStream_t s) {

@@ -111,7 +111,8 @@ struct fib_handle {
   bool md_on = false;
   fib_md_params md{};
   StepArgs md_sa{};
-  int *md_gate = nullptr;       // device [1 + ninst]: rebuild gates of the step
+  int *md_gate = nullptr;       // device [2 + 2*ninst]: rebuild gates of the step and
+                                // the cell prefix of the gated instances
   hipGraph_t graph = nullptr;
   hipGraphExec_t graph_exec = nullptr;
   bool graph_valid = false;
@@ -257,7 +258,7 @@ static int32_t free_bufs(fib_handle *h) {
   DevBufs &d = h->d;
   void *ptrs[] = {d.I, d.ctl, d.ibase, d.X[0], d.X[1], d.V[0], d.V[1], d.F[0], d.F[1], d.Xi[0], d.Xi[1],
                   d.inv, d.bkey, d.brank, d.bcnt, d.cstart, d.perm, d.bsum, d.cnt_sw, d.nbr_sw,
-                  d.cnt_ion, d.nbr_ion, d.rk_sw, d.off_sw, d.LA, d.LB, d.G, d.amask, d.cnt_leg, d.Xb, d.disp, d.EX, d.EXself, d.exmask, d.Fpart, d.upart, d.Fout, d.uout,
+                  d.cnt_ion, d.nbr_ion, d.rk_sw, d.off_sw, d.LA, d.LB, d.G, d.RA, d.amask, d.cnt_leg, d.Xb, d.disp, d.EX, d.EXself, d.exmask, d.Fpart, d.upart, d.Fout, d.uout,
                   d.partials, d.tab, d.xfer};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
@@ -274,14 +275,18 @@ static int32_t alloc_lists(fib_handle *h, int Wsw, int Wion) {
   DevBufs &d = h->d;
   if (d.nbr_sw) (void)hipFree(d.nbr_sw);
   if (d.nbr_ion) (void)hipFree(d.nbr_ion);
-  void *old[] = {d.rk_sw, d.LA, d.LB, d.G, d.EX};
+  void *old[] = {d.rk_sw, d.LA, d.LB, d.G, d.RA, d.EX};
   for (void *q : old)
     if (q) (void)hipFree(q);
   d.nbr_sw = nullptr; d.nbr_ion = nullptr; d.rk_sw = nullptr;
-  d.LA = nullptr; d.LB = nullptr; d.G = nullptr; d.EX = nullptr;
+  d.LA = nullptr; d.LB = nullptr; d.G = nullptr; d.RA = nullptr;
+  d.EX = nullptr;
   d.Wsw = Wsw;
   d.Wion = Wion;
   const size_t NA = (size_t)d.NA;
+  // leg lookups (RA) are 32-bit indices into the slot-major force rows
+  if ((uint64_t)Wsw * NA >= (1ull << 32))
+    return fail(h, FIB_ERR_ARG, "too many atoms for one handle: atoms x list width >= 2^32");
   HIPCHK(h, dalloc(&d.nbr_sw, (size_t)Wsw * NA));
   HIPCHK(h, dalloc(&d.nbr_ion, (size_t)Wion * NA));
   HIPCHK(h, dalloc(&d.rk_sw, (size_t)Wsw * NA));
@@ -290,6 +295,7 @@ static int32_t alloc_lists(fib_handle *h, int Wsw, int Wion) {
   HIPCHK(h, hipMalloc(&d.LA, gsz * (size_t)Wsw * NA));
   HIPCHK(h, hipMalloc(&d.LB, gsz * (size_t)Wsw * NA));
   HIPCHK(h, hipMalloc(&d.G, gsz * (size_t)Wsw * NA));
+  HIPCHK(h, dalloc(&d.RA, (size_t)Wsw * NA));
   HIPCHK(h, hipMalloc(&d.EX, gsz * (size_t)Wsw * NA));
   return FIB_OK;
 }
@@ -984,8 +990,8 @@ extern "C" int32_t fib_md_begin(fib_handle *h, const fib_md_params *p, const dou
     return fail(h, FIB_ERR_ARG, "fib_md_begin: bad parameters");
   if (!h->have_state) return fail(h, FIB_ERR_STATE, "fib_md_begin: set the state first");
   if (!h->md_gate) {
-    HIPCHK(h, hipMalloc(&h->md_gate, sizeof(int) * (h->inst.size() + 1)));
-    HIPCHK(h, hipMemset(h->md_gate, 0, sizeof(int) * (h->inst.size() + 1)));
+    HIPCHK(h, hipMalloc(&h->md_gate, sizeof(int) * (2 * h->inst.size() + 2)));
+    HIPCHK(h, hipMemset(h->md_gate, 0, sizeof(int) * (2 * h->inst.size() + 2)));
   }
   for (size_t k = 0; k < h->inst.size(); ++k) {
     h->inst[k].surf_z = surface_z[k];
