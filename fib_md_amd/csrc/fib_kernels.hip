@@ -85,6 +85,17 @@ __device__ __forceinline__ double ref_r2(const double4 &a, const double4 &b, con
   double x2 = d2 - round(d2 * in.ilb[2]) * in.lb[2];
   return x0 * x0 + x1 * x1 + x2 * x2;
 }
+// The raw differences a - b of ref_r2, and its sum of squares when every
+// NINT shift is zero (|d| < L/4 makes |d/L| < 1/2): then d - 0*L = d and the
+// result equals ref_r2 bit for bit.
+__device__ __forceinline__ void ref_diff(const double4 &a, const double4 &b, double &d0,
+                                         double &d1, double &d2) {
+  d0 = a.x - b.x; d1 = a.y - b.y; d2 = a.z - b.z;
+}
+__device__ __forceinline__ double ref_sq(double d0, double d1, double d2) {
+#pragma clang fp contract(off)
+  return d0 * d0 + d1 * d1 + d2 * d2;
+}
 
 // Minimum-image difference a - b (force kernels: contraction allowed).
 __device__ __forceinline__ void min_image(const double4 &a, const double4 &b, const DInst &in,
@@ -513,6 +524,7 @@ __device__ __forceinline__ void nbuild_cell(
   // chunk when it holds them (always, unless the halo is very dense)
   const int own0 = s_off[13];
   const bool own_staged = own0 + ni <= NB_CHUNK && total <= NB_CHUNK;
+  const double qb0 = 0.25 * in.lb[0], qb1 = 0.25 * in.lb[1], qb2 = 0.25 * in.lb[2];
   for (int ii = 0; ii < ni; ++ii) {
     const int i = i0 + ii;
     int nsw = 0, nion = 0;
@@ -546,19 +558,26 @@ __device__ __forceinline__ void nbuild_cell(
       for (int q0 = 0; q0 < n; q0 += FIB_WAVE) {
         const int q = q0 + lane;
         const int g = base + q;
-        bool sw = false, io = false;
+        bool sw = false, io = false, far = false;
         int j = -1, st = 0;
         double4 xj;
-        if (q < n) {
-          j = s_j[q];
-          if (j != i) {
-            xj = s_x[q];
-            st = s_st[q];
-            const int tj = type_of(meta_of(xj));
-            const double r2 = ref_r2(xi, xj, in);
-            if (max(ti, tj) < 3) sw = r2 < rv2;
-            else io = r2 < rv2lj;
-          }
+        double d0 = 0, d1 = 0, d2 = 0;
+        const bool cand = q < n && (j = s_j[q]) != i;
+        if (cand) {
+          xj = s_x[q];
+          st = s_st[q];
+          ref_diff(xi, xj, d0, d1, d2);
+          far = !(fabs(d0) < qb0 && fabs(d1) < qb1 && fabs(d2) < qb2);
+        }
+        // With every difference of the batch under a quarter box, the
+        // reference's NINT shifts are all zero: r^2 is the plain sum
+        // (bit-identical); otherwise the full minimum image
+        const bool near_all = __ballot(far) == 0ull;
+        if (cand) {
+          const int tj = type_of(meta_of(xj));
+          const double r2 = near_all ? ref_sq(d0, d1, d2) : ref_r2(xi, xj, in);
+          if (max(ti, tj) < 3) sw = r2 < rv2;
+          else io = r2 < rv2lj;
         }
         uint64_t msw = __ballot(sw), mio = __ballot(io);
         uint64_t below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
