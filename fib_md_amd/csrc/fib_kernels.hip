@@ -1060,8 +1060,14 @@ __global__ __launch_bounds__(FIB_BLOCK) void k_legs(
 
 
 // Lane-per-leg three-body pass.  Block = 4 wavefronts, each owning 64
-// consecutive atoms.
-#define TRI_MAXLEG 2048  // 64 atoms x 32 legs; more legs are taken in batches
+// consecutive atoms.  A wavefront stages the legs of a run of its atoms in
+// LDS (one coalesced read per leg plane), so the vertex terms, which read
+// every leg of the centre once per leg, read LDS instead of re-fetching the
+// leg rows.  Runs end on atom boundaries and hold at most TRI_STAGE legs.
+template <bool MIXED>
+struct TriStage {
+  static constexpr int N = MIXED ? 256 : 128;  // legs per run: 8 KB of each table
+};
 template <bool MIXED, bool EN, bool VIR>
 __global__ __launch_bounds__(FIB_BLOCK) void k_tri(
     const double4 *__restrict__ X, const FConst fc, int NA, int Wsw,
@@ -1072,8 +1078,16 @@ __global__ __launch_bounds__(FIB_BLOCK) void k_tri(
     double *__restrict__ partials) {
   typedef typename Prec<MIXED>::T T;
   typedef typename Prec<MIXED>::T4 T4;
-  __shared__ uint8_t owner[FIB_BLOCK / FIB_WAVE][TRI_MAXLEG];
-  __shared__ int soff[FIB_BLOCK / FIB_WAVE][FIB_WAVE + 1];
+  constexpr int NS = TriStage<MIXED>::N;
+  constexpr int NW = FIB_BLOCK / FIB_WAVE;
+  // sA: e0, e1, e2 and E, negated when the leg's atom is Ge (E > 0);
+  // sB: the LB record (g, rinv, neighbour bits, list slot bits)
+  __shared__ T4 sA[NW][NS];
+  __shared__ T4 sB[NW][NS];
+  __shared__ uint8_t owner[NW][NS];
+  __shared__ int soff[NW][FIB_WAVE + 1];
+  __shared__ uint8_t sge[NW][FIB_WAVE];
+  __shared__ uint32_t sR[NW][NS];
   const int lb = xcd_remap(blockIdx.x, gridDim.x);
   const int lane = threadIdx.x % FIB_WAVE, wid = threadIdx.x / FIB_WAVE;
   const int i0 = lb * FIB_BLOCK + wid * FIB_WAVE;
@@ -1086,40 +1100,61 @@ __global__ __launch_bounds__(FIB_BLOCK) void k_tri(
   const T lp[4] = {(T)(fc.leps[0][0][0] * lu), (T)(fc.leps[0][0][1] * lu),
                    (T)(fc.leps[0][1][1] * lu), (T)(fc.leps[1][1][1] * lu)};
 
-  // legs of the wave's atoms: exclusive prefix sum, owner table in LDS
+  // legs of the wave's atoms: exclusive prefix sum in LDS
   const int nl = cnt_leg[i0 + lane];
   int inc = nl;
   for (int o = 1; o < FIB_WAVE; o <<= 1) {
     const int y = __shfl_up(inc, o);
     if (lane >= o) inc += y;
   }
-  const int total = __shfl(inc, FIB_WAVE - 1);
   soff[wid][lane] = inc - nl;
-  if (lane == FIB_WAVE - 1) soff[wid][FIB_WAVE] = total;
+  if (lane == FIB_WAVE - 1) soff[wid][FIB_WAVE] = inc;
+  sge[wid][lane] = (type_of(meta_of(X[i0 + lane])) == 2) ? 1 : 0;
 
   T uc = 0;
   T W[9];
 #pragma unroll
   for (int w = 0; w < 9; ++w) W[w] = 0;
 
-  // batches of up to TRI_MAXLEG legs (one batch unless the atoms are dense)
-  for (int g0 = 0; g0 < total; g0 += TRI_MAXLEG) {
-    const int g1 = min(total, g0 + TRI_MAXLEG);
+  for (int o0 = 0; o0 < FIB_WAVE;) {
+    // run of atoms [o0, o1) whose legs [s0, s1) fit the stage (an atom has
+    // at most Wsw <= 64 legs, so every run takes at least one atom)
+    const int s0 = __shfl(inc - nl, o0);
+    const uint64_t fit = __ballot(lane >= o0 && inc - s0 <= NS);
+    const int o1 = o0 + __popcll(fit);
+    const int s1 = __shfl(inc, o1 - 1);
+    const bool mine = lane >= o0 && lane < o1;
     __builtin_amdgcn_wave_barrier();
-    {
-      const int s0 = max(inc - nl, g0), s1 = min(inc, g1);
-      for (int g = s0; g < s1; ++g) owner[wid][g - g0] = (uint8_t)lane;
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    // stage: plane by plane, each lane its atom's legs (coalesced rows)
+    for (int a = 0; __ballot(mine && a < nl) != 0ull; ++a) {
+      if (mine && a < nl) {
+        const size_t at = (size_t)a * NA + i0 + lane;
+        T4 x = LA[at];
+        const T4 y = LB[at];
+        if (ks_ge(ks_unpack(y.z))) x.w = -x.w;
+        const int q = inc - nl + a - s0;
+        sA[wid][q] = x;
+        sB[wid][q] = y;
+        owner[wid][q] = (uint8_t)lane;
+      }
     }
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-    for (int g = g0 + lane; g < g1; g += FIB_WAVE) {
-      const int o = owner[wid][g - g0];
+    for (int g = s0 + lane; g < s1; g += FIB_WAVE) {
+      const int q = g - s0;
+      const int o = owner[wid][q];
       const int i = i0 + o;
+      const int c0 = soff[wid][o] - s0;  // the centre's first leg in the stage
       const int na = soff[wid][o + 1] - soff[wid][o];
       const int a = g - soff[wid][o];
       Leg<T> La;
-      load_leg(LA, LB, (size_t)a * NA + i, La);
+      {
+        const T4 x = sA[wid][q], y = sB[wid][q];
+        La.e0 = x.x; La.e1 = x.y; La.e2 = x.z; La.E = fabs(x.w);
+        La.g = y.x; La.rinv = y.y; La.ks = ks_unpack(y.z); La.slot = ks_unpack(y.w);
+      }
       // Where the partner j keeps the force of its own terms on i: G[n * NA
       // + j], n = j's leg index of i.  The slot r of i in j's row follows
       // from the stencil offsets (see k_nbuild), n from j's leg mask (legs
@@ -1128,22 +1163,24 @@ __global__ __launch_bounds__(FIB_BLOCK) void k_tri(
       const uint64_t amj = amask[j];
       const int rk = rk_sw[(size_t)i * Wsw + La.slot];
       const int r = off_sw[(size_t)j * OFF_W + 26 - (rk >> 8)] + (rk & 0xff);
-      const int gi = (type_of(meta_of(X[i])) == 2) ? 1 : 0;
-      const int ga = gi + ks_ge(La.ks);
+      const int ga = sge[wid][o] + ks_ge(La.ks);
       T Ga0 = 0, Ga1 = 0, Ga2 = 0, Gu = 0;
       for (int b = 0; b < na; ++b) {
         if (b == a) continue;
+        const T4 xb = sA[wid][c0 + b];
         Leg<T> Lb;
-        load_leg(LA, LB, (size_t)b * NA + i, Lb);
+        Lb.e0 = xb.x; Lb.e1 = xb.y; Lb.e2 = xb.z; Lb.E = fabs(xb.w);
         T Fa[3], hh;
-        vterm<T>(lp[ga + ks_ge(Lb.ks)], La, Lb, Fa, hh);
+        vterm<T>(lp[ga + (xb.w < T(0) ? 1 : 0)], La, Lb, Fa, hh);
         Ga0 += Fa[0]; Ga1 += Fa[1]; Ga2 += Fa[2];
         Gu += hh;
       }
       T4 gv;
       gv.x = Ga0; gv.y = Ga1; gv.z = Ga2; gv.w = Gu * T(1.0 / 3.0);
-      G[(size_t)a * NA + i] = gv;
-      RA[(size_t)a * NA + i] =
+      // results go back through the stage (this lane alone reads sB[q]) and
+      // leave plane by plane below, so the stores coalesce
+      sB[wid][q] = gv;
+      sR[wid][q] =
           r < Wsw ? (uint32_t)__popcll(amj & ((1ull << r) - 1ull)) * (uint32_t)NA + (uint32_t)j
                   : ~0u;
       uc += Gu;
@@ -1156,6 +1193,18 @@ __global__ __launch_bounds__(FIB_BLOCK) void k_tri(
         W[6] += d2 * Ga0; W[7] += d2 * Ga1; W[8] += d2 * Ga2;
       }
     }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    for (int a = 0; __ballot(mine && a < nl) != 0ull; ++a) {
+      if (mine && a < nl) {
+        const size_t at = (size_t)a * NA + i0 + lane;
+        const int q = inc - nl + a - s0;
+        G[at] = sB[wid][q];
+        RA[at] = sR[wid][q];
+      }
+    }
+    o0 = o1;
   }
   if (EN || VIR) {
     double v[10];
