@@ -186,6 +186,46 @@ def md_rate(instances, precision, steps, warmup=5, use_graph=True):
                 builds=sum(r["builds"] for r in reps), removed=sum(r["removed"] for r in reps))
 
 
+def cascade_stats(instances, steps):
+    """Per-instance cascade outputs after `steps` MD steps of the batch, FP64
+    and mixed: sputter yield (Si removed above the surface), Ga projected
+    range (surface - ion z, A; ions removed are left out), displaced Si
+    (> 1.5 A from the initial site).  Ensemble means, standard errors and the
+    Welch z of mixed against FP64 (cascades are chaotic: the statistics,
+    not single runs, must agree)."""
+    per = {}
+    for prec in ("fp64", "mixed"):
+        g = _md_handle(instances, prec)
+        g.md_run(steps)
+        reps = [g.md_report(k) for k in range(len(instances))]
+        g.close()
+        per[prec] = {
+            "yield": np.array([r["removed"] for r in reps], float),
+            "range_A": np.array([(s["surface_z"] - r["ion_z"]) * 1e10 if r["ion_z"] < 1e299
+                                 else np.nan for s, r in zip(instances, reps)]),
+            "displaced": np.array([r["displaced"] for r in reps], float),
+            "ions_removed": int(sum(r["ions_removed"] for r in reps)),
+        }
+    out = {"steps": steps, "instances": len(instances),
+           "ions_removed": {p: per[p]["ions_removed"] for p in per}}
+    for key in ("yield", "range_A", "displaced"):
+        st = {}
+        for p in ("fp64", "mixed"):
+            a = per[p][key][np.isfinite(per[p][key])]
+            st[p] = float(a.mean())
+            st[p + "_sem"] = float(a.std(ddof=1) / np.sqrt(len(a)))
+        se = float(np.hypot(st["fp64_sem"], st["mixed_sem"]))
+        st["z_mixed_vs_fp64"] = (st["mixed"] - st["fp64"]) / se if se > 0 else 0.0
+        # the same impacts in both precisions: per-instance differences too
+        d = per["mixed"][key] - per["fp64"][key]
+        d = d[np.isfinite(d)]
+        sd = float(d.std(ddof=1) / np.sqrt(len(d))) if len(d) > 1 else 0.0
+        st["paired_mean_diff"] = float(d.mean()) if len(d) else 0.0
+        st["paired_z"] = st["paired_mean_diff"] / sd if sd > 0 else 0.0
+        out[key] = st
+    return out
+
+
 def md_bench(args):
     """MD-step throughput: the single 500 eV impact on an 8^3-cell slab, and
     the batched ensemble (64 slabs of 10k-250k atoms, 1-10 keV, 0-60 deg),
@@ -231,6 +271,13 @@ def md_bench(args):
         f"{fb['atom_steps_per_s']:.4g}, fp64 one at a time (graph) "
         f"{seq['fp64_one_at_a_time_graph']['atom_steps_per_s']:.4g}, (launches) "
         f"{seq['fp64_one_at_a_time_launches']['atom_steps_per_s']:.4g} atom-steps/s")
+    if args.stats_steps > 0:
+        cs = cascade_stats(ens, args.stats_steps)
+        out["ensemble"]["cascade_stats"] = cs
+        for key in ("yield", "range_A", "displaced"):
+            st = cs[key]
+            log(f"  {key:9s} fp64 {st['fp64']:.3f} +- {st['fp64_sem']:.3f}, mixed "
+                f"{st['mixed']:.3f} +- {st['mixed_sem']:.3f}, z {st['z_mixed_vs_fp64']:.2f}")
     return out
 
 
@@ -278,6 +325,8 @@ def main():
     ap.add_argument("--no-md", action="store_true")
     ap.add_argument("--md-steps", type=int, default=100)
     ap.add_argument("--ensemble", type=int, default=64)
+    ap.add_argument("--stats-steps", type=int, default=2000,
+                    help="MD steps of the cascade-statistics run (0: skip)")
     ap.add_argument("--profile", choices=["fp64", "mixed", "md-fp64", "md-mixed"], default=None)
     ap.add_argument("--summarize-stats", nargs=3, metavar=("DIR", "OUT", "CMD"))
     ap.add_argument("--summarize-pmc", nargs=4, metavar=("FETCH_DIR", "WRITE_DIR", "OUT", "CMD"))
