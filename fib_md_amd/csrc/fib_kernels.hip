@@ -13,13 +13,17 @@
 // Per force evaluation (replaces si_force, si.f90:384-552):
 //   k_legs      one thread per atom: SW pair and Moliere forces in registers,
 //               the atom's three-body legs to the leg arrays
-//   k_tri       one lane per leg: the vertex terms centred on the leg's atom,
-//               force on the leg's neighbour summed into G -- no atomics
-//   k_force3    one thread per atom: own terms + gather of the G entries its
-//               neighbours wrote for it; in MD mode also the velocity update
+//   k_tri       one lane per leg, legs staged in LDS: the vertex terms
+//               centred on the leg's atom, force on the leg's neighbour into
+//               G, and where the neighbour keeps its force on the atom (RA)
+//   k_extra     vertex terms of the stale build-time triplet graph
+//   k_force3    one thread per atom: own terms + the G entries its neighbours
+//               wrote for it; the velocity update of either integrator
 // Per step (replaces the inline Verlet of onestep, timeint.f90:165-236):
 //   k_pos       thermostat scaling, position update, side freeze, rebox and
 //               the sputter-layer rules in one pass
+//   k_md_pos    the benchmark configurations' integrator (MD-step mode):
+//               shells, half kick, drift, pins, removal, displacements
 //
 // Reductions (energy, virial, thermostat sums, max KE) use per-block
 // partials; the last block of an instance sums them in block order, so every
@@ -709,14 +713,15 @@ __device__ __forceinline__ int nth_bit(uint64_t m, int n) {
 // atoms in turn (a prefix sum over the atoms' leg counts in LDS maps lanes to
 // legs), so atoms with many legs do not stall the others.
 //
-// k_force2 (one thread per atom): F_i = own pair/Moliere force - sum of its
+// k_force3 (one thread per atom): F_i = own pair/Moliere force - sum of its
 // own G row (the reaction on the centre) + the G entries its neighbours wrote
 // for it.  Neighbour c's entry for i is the leg of c in the list slot r that
 // holds i; its compact index is the popcount of c's leg mask below r.  r
-// follows from the stencil offsets the list build stores (see k_nbuild).  This
-// turns the reference's scatter F(i) += fi, F(j) += fj, F(k) += fk
-// (si.f90:544-546) into a gather.  In MD mode the same pass applies the
-// velocity update of onestep (timeint.f90:231-236).
+// follows from the stencil offsets the list build stores (see k_nbuild); k_tri
+// works the index out per leg and leaves it in RA, so k_force3 makes one
+// lookup per leg.  This turns the reference's scatter F(i) += fi, F(j) += fj,
+// F(k) += fk (si.f90:544-546) into a gather.  MODE 1 applies the velocity
+// update of onestep (timeint.f90:231-236), MODE 2 the MD-step second half kick.
 template <bool MIXED>
 struct Prec;
 template <>

@@ -60,6 +60,18 @@ module fibgpu
      integer(c_int32_t) :: calls(FIB_NTIMERS)
   end type fib_timing
 
+  ! MD-step mode (the benchmark configurations' integrator)
+  type, bind(C) :: fib_md_params
+     real(c_double) :: dt_max, dx_max, remove_height, shell_width, ttar, tau, displaced_r
+     integer(c_int32_t) :: use_graph, reserved
+  end type fib_md_params
+
+  type, bind(C) :: fib_md_report
+     real(c_double) :: time, dt, ion_z, ke
+     integer(c_int64_t) :: steps
+     integer(c_int32_t) :: removed, ions_removed, displaced, builds
+  end type fib_md_report
+
   interface
      subroutine fib_default_params(p) bind(C, name="fib_default_params")
        import :: fib_params
@@ -211,6 +223,34 @@ module fibgpu
        import :: c_int32_t, c_ptr
        type(c_ptr), value :: h
      end function fib_synchronize
+
+     integer(c_int32_t) function fib_md_begin(h, p, surface_z, bottom_z) &
+          bind(C, name="fib_md_begin")
+       import :: c_int32_t, c_double, c_ptr, fib_md_params
+       type(c_ptr), value :: h
+       type(fib_md_params), intent(in) :: p
+       real(c_double), intent(in) :: surface_z(*), bottom_z(*)
+     end function fib_md_begin
+
+     integer(c_int32_t) function fib_md_fire(h, inst, ion, x, v) bind(C, name="fib_md_fire")
+       import :: c_int32_t, c_double, c_ptr
+       type(c_ptr), value :: h
+       integer(c_int32_t), value :: inst, ion
+       real(c_double), intent(in) :: x(3), v(3)
+     end function fib_md_fire
+
+     integer(c_int32_t) function fib_md_run(h, nsteps) bind(C, name="fib_md_run")
+       import :: c_int32_t, c_ptr
+       type(c_ptr), value :: h
+       integer(c_int32_t), value :: nsteps
+     end function fib_md_run
+
+     integer(c_int32_t) function fib_md_get_report(h, inst, r) bind(C, name="fib_md_get_report")
+       import :: c_int32_t, c_ptr, fib_md_report
+       type(c_ptr), value :: h
+       integer(c_int32_t), value :: inst
+       type(fib_md_report), intent(out) :: r
+     end function fib_md_get_report
   end interface
 
 contains
