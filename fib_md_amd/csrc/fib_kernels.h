@@ -47,7 +47,8 @@ struct KTimer {
 };
 
 // All device buffers of a handle (sorted order, see fib_device.h).  Neighbour
-// lists, reverse slots, leg rows and G are atom-major: row i = [i * W, i * W + W).
+// lists and their stencil bookkeeping are atom-major (row i = [i * W, i * W + W));
+// G and RA are slot-major by compact leg index ([n * NA + i]).
 struct DevBufs {
   int NA;                    // total padded atoms
   int ninst;
@@ -65,7 +66,6 @@ struct DevBufs {
   int *cnt_sw, *nbr_sw, *cnt_ion, *nbr_ion;
   uint16_t *rk_sw;           // [NA * Wsw]: (stencil index << 8) | rank of each row entry
   uint8_t *off_sw;           // [NA * 32]: row start of each stencil cell, [27] = has a row
-  void *LA, *LB;             // leg scratch rows, double4 or float4 [NA * Wsw] each
   void *G;                   // centre-term force on each leg, T4 [NA * Wsw]
   uint32_t *RA;              // per leg [NA * Wsw]: index into G of the partner's force
                              // on the atom (~0: none); needs NA * Wsw < 2^32

@@ -70,6 +70,14 @@ __device__ __forceinline__ int xcd_remap(int w, int nb) {
   return (x < r) ? x * (q + 1) + loc : r * (q + 1) + (x - r) * q + loc;
 }
 
+// LDS hand-off between the lanes of one wavefront (no workgroup barrier:
+// the wavefronts of a workgroup may work independently).
+__device__ __forceinline__ void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 // Reference cell index of one coordinate (nlist.f90:29), no contraction.
 __device__ __forceinline__ int ref_cell(double x, double ilb, double lb, double h) {
 #pragma clang fp contract(off)
@@ -439,7 +447,8 @@ __global__ void k_copyback(int NA, const double4 *__restrict__ X1, double4 *__re
 // neighbour build: one 64-lane block per real cell
 #define NB_CHUNK 128
 #define NB_ACC 384  // candidates whose acceptance masks are kept for the ranks
-#define NB_CELLS 8  // cells per k_nbuild workgroup
+#define NB_CELLS 8  // cells per k_nbuild wavefront
+#define NB_WAVES 4  // independent wavefronts per k_nbuild workgroup
 #define OFF_W 32    // bytes per atom of the stencil-offset table (27 used)
 
 // Stencil bookkeeping for the force gather.  The list row of an atom is the
@@ -461,7 +470,7 @@ __device__ __forceinline__ void nbuild_cell(
     int *__restrict__ nbr_sw, int *__restrict__ cnt_ion, int *__restrict__ nbr_ion,
     uint16_t *__restrict__ rk_sw, uint8_t *__restrict__ off_sw, double4 *s_x, int *s_j,
     uint8_t *s_st, int *s_rs, int *s_off, uint32_t *s_acc) {
-  const int lane = threadIdx.x;
+  const int lane = threadIdx.x % FIB_WAVE;
   int k, gcell;
   if (cplan) {
     // last instance whose prefix is <= c (gated-out instances have width 0)
@@ -478,7 +487,7 @@ __device__ __forceinline__ void nbuild_cell(
   }
   const DInst &in = I[k];
   // the previous cell of this wave is done with the shared arrays
-  __syncthreads();
+  wave_lds_sync();
   const int N0 = in.nc[0], N1 = in.nc[1];
   const int loc = gcell - in.cell_real_off;
   const int cx = loc % N0, cy = (loc / N0) % N1, cz = loc / (N0 * N1);
@@ -521,7 +530,7 @@ __device__ __forceinline__ void nbuild_cell(
   const bool masks = total <= NB_ACC && ni <= 32;
   if (masks)
     for (int g = lane; g < total; g += FIB_WAVE) s_acc[g] = 0;
-  __syncthreads();
+  wave_lds_sync();
 
   const double4 *Xc = X;
   // the cell's own atoms are stencil range 13: read them from the staged
@@ -539,7 +548,7 @@ __device__ __forceinline__ void nbuild_cell(
       const int n = min(NB_CHUNK, total - base);
       // (re)stage the chunk; a single chunk is loaded once per cell
       if (ii == 0 || total > NB_CHUNK) {
-        __syncthreads();
+        wave_lds_sync();
         for (int q = lane; q < n; q += FIB_WAVE) {
           const int g = base + q;
           // stencil range holding g: binary search over the 28 offsets
@@ -553,7 +562,7 @@ __device__ __forceinline__ void nbuild_cell(
           s_st[q] = (uint8_t)lo;
           s_x[q] = Xc[j];
         }
-        __syncthreads();
+        wave_lds_sync();
       }
       if (base == 0) {
         xi = own_staged ? s_x[own0 + ii] : Xc[i];
@@ -629,34 +638,40 @@ __device__ __forceinline__ void nbuild_cell(
   }
 }
 
-// Each workgroup builds NB_CELLS consecutive cells.  The cells to build are
-// all real cells, or under a gate the cells of the gated instances, numbered
-// by the prefix that k_md_decide writes after the gates; the grid covers all
-// real cells and the workgroups past the gated ones leave at once (gated-out
-// instances cost a launch of empty workgroups, no more).  The busy
-// workgroups are remapped so that each XCD builds a contiguous run of cells,
-// whose halos then stay in its L2.
-__global__ __launch_bounds__(FIB_WAVE) void k_nbuild(
+// Each wavefront builds NB_CELLS consecutive cells; a workgroup holds
+// NB_WAVES independent wavefronts (their own LDS, wave-level sync only), so
+// that the workgroup-per-CU limit does not cap the resident waves.  The cells
+// to build are all real cells, or under a gate the cells of the gated
+// instances, numbered by the prefix that k_md_decide writes after the gates;
+// the grid covers all real cells and the workgroups past the gated ones leave
+// at once (gated-out instances cost a launch of empty workgroups, no more).
+// The busy workgroups are remapped so that each XCD builds a contiguous run
+// of cells, whose halos then stay in its L2.
+__global__ __launch_bounds__(NB_WAVES * FIB_WAVE) __attribute__((amdgpu_waves_per_eu(6)))
+void k_nbuild(
     const double4 *__restrict__ X, const DInst *__restrict__ I, int ninst, int total_cells,
     const int *__restrict__ cstart, const int *__restrict__ bcnt, double rv2, double rv2lj,
     int Wsw, int Wion, int *__restrict__ cnt_sw, int *__restrict__ nbr_sw,
     int *__restrict__ cnt_ion, int *__restrict__ nbr_ion, uint16_t *__restrict__ rk_sw,
     uint8_t *__restrict__ off_sw, const int *__restrict__ gate) {
   GATE_OR_RETURN(gate);
-  __shared__ double4 s_x[NB_CHUNK];
-  __shared__ int s_j[NB_CHUNK];
-  __shared__ uint8_t s_st[NB_CHUNK];
-  __shared__ int s_rs[27], s_off[28];
-  __shared__ uint32_t s_acc[NB_ACC];
+  __shared__ double4 s_x[NB_WAVES][NB_CHUNK];
+  __shared__ int s_j[NB_WAVES][NB_CHUNK];
+  __shared__ uint8_t s_st[NB_WAVES][NB_CHUNK];
+  __shared__ int s_rs[NB_WAVES][27], s_off[NB_WAVES][28];
+  __shared__ uint32_t s_acc[NB_WAVES][NB_ACC];
+  const int w = __builtin_amdgcn_readfirstlane(threadIdx.x / FIB_WAVE);  // wave-uniform
   const int *cplan = gate ? gate + 1 + ninst : nullptr;
   const int ncell = gate ? cplan[ninst] : total_cells;
-  const int nbusy = (ncell + NB_CELLS - 1) / NB_CELLS;
+  constexpr int CPB = NB_WAVES * NB_CELLS;  // cells per workgroup
+  const int nbusy = (ncell + CPB - 1) / CPB;
   if ((int)blockIdx.x >= nbusy) return;
-  const int c0 = xcd_remap(blockIdx.x, nbusy) * NB_CELLS;
+  const int c0 = xcd_remap(blockIdx.x, nbusy) * CPB + w * NB_CELLS;
   const int c1 = min(ncell, c0 + NB_CELLS);
   for (int c = c0; c < c1; ++c)
     nbuild_cell(X, I, ninst, c, cplan, cstart, bcnt, rv2, rv2lj, Wsw, Wion, cnt_sw, nbr_sw,
-                cnt_ion, nbr_ion, rk_sw, off_sw, s_x, s_j, s_st, s_rs, s_off, s_acc);
+                cnt_ion, nbr_ion, rk_sw, off_sw, s_x[w], s_j[w], s_st[w], s_rs[w], s_off[w],
+                s_acc[w]);
 }
 
 // Atoms outside every scanned cell (dropped, dead, or phantom i-atoms) get
@@ -756,24 +771,6 @@ __device__ __forceinline__ uint32_t ks_unpack(float v) { return __float_as_uint(
 __device__ __forceinline__ int ks_index(uint32_t ks) { return (int)(ks & 0x7fffffffu); }
 __device__ __forceinline__ int ks_ge(uint32_t ks) { return (int)(ks >> 31); }
 
-// Leg scratch: two T4 per leg, slot-major by compact leg index n
-// ([n * NA + i]), so a wave's threads touching their n-th legs coalesce.
-template <typename T, typename T4>
-__device__ __forceinline__ void store_leg(T4 *__restrict__ LA, T4 *__restrict__ LB, size_t at,
-                                          const Leg<T> &L) {
-  T4 a, b;
-  a.x = L.e0; a.y = L.e1; a.z = L.e2; a.w = L.E;
-  b.x = L.g; b.y = L.rinv; b.z = ks_pack(T(0), L.ks); b.w = ks_pack(T(0), L.slot);
-  LA[at] = a;
-  LB[at] = b;
-}
-template <typename T, typename T4>
-__device__ __forceinline__ void load_leg(const T4 *LA, const T4 *LB, size_t at, Leg<T> &L) {
-  const T4 a = LA[at], b = LB[at];
-  L.e0 = a.x; L.e1 = a.y; L.e2 = a.z; L.E = a.w;
-  L.g = b.x; L.rinv = b.y; L.ks = ks_unpack(b.z); L.slot = ks_unpack(b.w);
-}
-
 // Force on leg a of one vertex term (centre c, legs a and b), and h.
 // h = leps E(ra) E(rb) (cos+1/3)^2 with d cos/dx_a = -(e_b - cos e_a)/r_a and
 // d r_a/dx_a = -e_a, so F_a = -dh/dx_a = h g_a e_a + 2 leps Ea Eb D (e_b - cos e_a)/r_a.
@@ -799,6 +796,44 @@ __device__ __forceinline__ void min_image_fast(const double4 &a, const double4 &
   x0 = d0 - rint(d0 * in.ilb[0]) * in.lb[0];
   x1 = d1 - rint(d1 * in.ilb[1]) * in.lb[1];
   x2 = d2 - rint(d2 * in.ilb[2]) * in.lb[2];
+}
+
+// The leg from centre c to neighbour n (unit vector from n to c, 1/r, E, g),
+// with k_legs' arithmetic: FP64 as the reference (si.f90:491-509), or the
+// FP32 terms of the mixed mode.  Legs are not stored between passes: k_legs
+// decides which pairs are legs (its leg mask), and the passes that need the
+// values recompute them here from the positions.
+template <bool MIXED>
+__device__ __forceinline__ void make_leg(const double4 &xc, const double4 &xn, const DInst &in,
+                                         const FConst &fc, int tc, int tn, uint32_t jn,
+                                         uint32_t slot, Leg<typename Prec<MIXED>::T> &L) {
+  double d0, d1, d2;
+  min_image_fast(xc, xn, in, d0, d1, d2);
+  const double isg = fc.isig[tc == 2 ? 1 : 0][tn == 2 ? 1 : 0];
+  L.ks = jn | (tn == 2 ? 0x80000000u : 0u);
+  L.slot = slot;
+  if (!MIXED) {
+    const double rr = d0 * d0 + d1 * d1 + d2 * d2;
+    const double r = sqrt(rr);
+    const double rinv = 1.0 / r;
+    const double rs = r * isg;
+    const double inv_ra = 1.0 / (rs - fc.al);
+    L.e0 = d0 * rinv; L.e1 = d1 * rinv; L.e2 = d2 * rinv;
+    L.rinv = rinv;
+    L.E = exp(fc.gamma * inv_ra);
+    L.g = -fc.gamma * inv_ra * inv_ra * isg;
+  } else {
+    const float r0 = (float)(d0 * isg), r1 = (float)(d1 * isg), r2 = (float)(d2 * isg);
+    const float rr = r0 * r0 + r1 * r1 + r2 * r2;
+    const float rinv = __builtin_amdgcn_rsqf(rr);
+    const float rs = rr * rinv;
+    const float inv_ra = __builtin_amdgcn_rcpf(rs - (float)fc.al);
+    const float rho = (float)(isg / fc.isig[0][0]);
+    L.e0 = r0 * rinv; L.e1 = r1 * rinv; L.e2 = r2 * rinv;
+    L.rinv = rinv * rho;
+    L.E = __expf((float)fc.gamma * inv_ra);
+    L.g = -(float)fc.gamma * inv_ra * inv_ra * rho;
+  }
 }
 
 // Register accumulator of one force vector: FP64, or FP32 with a TwoSum
@@ -829,8 +864,7 @@ __global__ __launch_bounds__(FIB_BLOCK) void k_legs(
     const double4 *__restrict__ X, const DInst *__restrict__ I, const FConst fc,
     const TabEntry *__restrict__ tab, int NA, int Wsw, int Wion, const int *__restrict__ cnt_sw,
     const int *__restrict__ nbr_sw, const int *__restrict__ cnt_ion,
-    const int *__restrict__ nbr_ion, typename Prec<MIXED>::T4 *LA, typename Prec<MIXED>::T4 *LB,
-    int *__restrict__ cnt_leg, uint64_t *__restrict__ amask,
+    const int *__restrict__ nbr_ion, int *__restrict__ cnt_leg, uint64_t *__restrict__ amask,
     double *__restrict__ Fpart, double *__restrict__ upart, int pstride,
     double *__restrict__ partials) {
   typedef typename Prec<MIXED>::T T;
@@ -876,9 +910,6 @@ __global__ __launch_bounds__(FIB_BLOCK) void k_legs(
     const double isg = jA ? isgA : isgB;
     double d0, d1, d2;
     min_image_fast(xi, xj, in, d0, d1, d2);
-    Leg<T> L;
-    L.ks = (uint32_t)jn | (jA ? 0u : 0x80000000u);
-    L.slot = (uint32_t)k;
     if (!MIXED) {
       const double rr = d0 * d0 + d1 * d1 + d2 * d2;
       if (!(rr * (isg * isg) < fc.r2gate)) return;
@@ -898,10 +929,6 @@ __global__ __launch_bounds__(FIB_BLOCK) void k_legs(
       }
       if (rs < fc.cut_sw) {
         const double inv_ra = 1.0 / (rs - fc.al);
-        L.e0 = d0 * rinv; L.e1 = d1 * rinv; L.e2 = d2 * rinv;
-        L.rinv = rinv;
-        L.E = exp(fc.gamma * inv_ra);
-        L.g = -fc.gamma * inv_ra * inv_ra * isg;
         if (EN) {
           const double rs2 = rs * rs;
           const double ul =
@@ -909,7 +936,6 @@ __global__ __launch_bounds__(FIB_BLOCK) void k_legs(
           ui += ul;
           etot += ul;
         }
-        store_leg(LA, LB, (size_t)na * NA + i, L);
         am |= 1ull << k;
         ++na;
       }
@@ -944,11 +970,6 @@ __global__ __launch_bounds__(FIB_BLOCK) void k_legs(
       // legs with E = exp(gamma/(r-a)) < e^-30 are left out of the
       // three-body sum: each of their terms is below 1e-9 eps/sigma
       if (rs < (float)fc.mixed_xleg) {
-        L.e0 = r0 * rinv; L.e1 = r1 * rinv; L.e2 = r2 * rinv;
-        L.rinv = rinv * rho;
-        L.E = __expf((float)fc.gamma * inv_ra);
-        L.g = -(float)fc.gamma * inv_ra * inv_ra * rho;
-        store_leg(LA, LB, (size_t)na * NA + i, L);
         am |= 1ull << k;
         ++na;
       }
@@ -1065,28 +1086,28 @@ __global__ __launch_bounds__(FIB_BLOCK) void k_legs(
 
 
 // Lane-per-leg three-body pass.  Block = 4 wavefronts, each owning 64
-// consecutive atoms.  A wavefront stages the legs of a run of its atoms in
-// LDS (one coalesced read per leg plane), so the vertex terms, which read
-// every leg of the centre once per leg, read LDS instead of re-fetching the
-// leg rows.  Runs end on atom boundaries and hold at most TRI_STAGE legs.
+// consecutive atoms.  A wavefront computes the legs of a run of its atoms
+// (the pairs of their leg masks, from the positions) into LDS, so the vertex
+// terms, which read every leg of the centre once per leg, read LDS; no leg
+// record goes through HBM.  Runs end on atom boundaries and hold at most
+// TriStage::N legs.
 template <bool MIXED>
 struct TriStage {
-  static constexpr int N = MIXED ? 256 : 128;  // legs per run: 8 KB of each table
+  static constexpr int N = MIXED ? 256 : 128;  // legs per run: 4 KB of each table per wave
 };
 template <bool MIXED, bool EN, bool VIR>
 __global__ __launch_bounds__(FIB_BLOCK) void k_tri(
-    const double4 *__restrict__ X, const FConst fc, int NA, int Wsw,
-    const int *__restrict__ cnt_leg, const typename Prec<MIXED>::T4 *__restrict__ LA,
-    const typename Prec<MIXED>::T4 *__restrict__ LB, const uint16_t *__restrict__ rk_sw,
-    const uint8_t *__restrict__ off_sw, const uint64_t *__restrict__ amask,
-    typename Prec<MIXED>::T4 *__restrict__ G, uint32_t *__restrict__ RA, int pstride,
-    double *__restrict__ partials) {
+    const double4 *__restrict__ X, const DInst *__restrict__ I, const FConst fc, int NA,
+    int Wsw, const int *__restrict__ cnt_leg, const int *__restrict__ nbr_sw,
+    const uint16_t *__restrict__ rk_sw, const uint8_t *__restrict__ off_sw,
+    const uint64_t *__restrict__ amask, typename Prec<MIXED>::T4 *__restrict__ G,
+    uint32_t *__restrict__ RA, int pstride, double *__restrict__ partials) {
   typedef typename Prec<MIXED>::T T;
   typedef typename Prec<MIXED>::T4 T4;
   constexpr int NS = TriStage<MIXED>::N;
   constexpr int NW = FIB_BLOCK / FIB_WAVE;
   // sA: e0, e1, e2 and E, negated when the leg's atom is Ge (E > 0);
-  // sB: the LB record (g, rinv, neighbour bits, list slot bits)
+  // sB: g, 1/r, neighbour bits, list slot bits
   __shared__ T4 sA[NW][NS];
   __shared__ T4 sB[NW][NS];
   __shared__ uint8_t owner[NW][NS];
@@ -1114,7 +1135,14 @@ __global__ __launch_bounds__(FIB_BLOCK) void k_tri(
   }
   soff[wid][lane] = inc - nl;
   if (lane == FIB_WAVE - 1) soff[wid][FIB_WAVE] = inc;
-  sge[wid][lane] = (type_of(meta_of(X[i0 + lane])) == 2) ? 1 : 0;
+  // this lane's atom as a centre; every block lies inside one instance
+  const int ic = i0 + lane;
+  const double4 xc = X[ic];
+  const int tci = type_of(meta_of(xc));
+  const uint64_t amc = amask[ic];
+  const size_t rowc = (size_t)ic * Wsw;
+  const DInst &in = I[__builtin_amdgcn_readfirstlane(inst_of(meta_of(xc)))];
+  sge[wid][lane] = (tci == 2) ? 1 : 0;
 
   T uc = 0;
   T W[9];
@@ -1131,17 +1159,37 @@ __global__ __launch_bounds__(FIB_BLOCK) void k_tri(
     const bool mine = lane >= o0 && lane < o1;
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-    // stage: plane by plane, each lane its atom's legs (coalesced rows)
-    for (int a = 0; __ballot(mine && a < nl) != 0ull; ++a) {
-      if (mine && a < nl) {
-        const size_t at = (size_t)a * NA + i0 + lane;
-        T4 x = LA[at];
-        const T4 y = LB[at];
-        if (ks_ge(ks_unpack(y.z))) x.w = -x.w;
-        const int q = inc - nl + a - s0;
-        sA[wid][q] = x;
-        sB[wid][q] = y;
-        owner[wid][q] = (uint8_t)lane;
+    // stage: each lane computes its atom's legs (the pairs of its leg mask,
+    // in list order) from the positions, four neighbour loads in flight
+    {
+      uint64_t m = amc;
+      for (int a0 = 0; __ballot(mine && a0 < nl) != 0ull; a0 += 4) {
+        int kq[4], jq[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          const bool ok = mine && a0 + u < nl;
+          kq[u] = ok ? __ffsll((unsigned long long)m) - 1 : 0;
+          if (ok) m &= m - 1;
+          jq[u] = ok ? nbr_sw[rowc + kq[u]] : ic;
+        }
+        double4 xq[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) xq[u] = X[jq[u]];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          if (mine && a0 + u < nl) {
+            Leg<T> L;
+            make_leg<MIXED>(xc, xq[u], in, fc, tci, type_of(meta_of(xq[u])), (uint32_t)jq[u],
+                            (uint32_t)kq[u], L);
+            T4 x, y;
+            x.x = L.e0; x.y = L.e1; x.z = L.e2; x.w = ks_ge(L.ks) ? -L.E : L.E;
+            y.x = L.g; y.y = L.rinv; y.z = ks_pack(T(0), L.ks); y.w = ks_pack(T(0), L.slot);
+            const int q = inc - nl + a0 + u - s0;
+            sA[wid][q] = x;
+            sB[wid][q] = y;
+            owner[wid][q] = (uint8_t)lane;
+          }
+        }
       }
     }
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -1243,7 +1291,6 @@ __global__ __launch_bounds__(FIB_BLOCK) void k_extra(
     const double4 *__restrict__ X, const DInst *__restrict__ I, const FConst fc, int NA, int Wsw,
     const int *__restrict__ cnt_sw, const int *__restrict__ nbr_sw,
     const int *__restrict__ cnt_leg, const uint64_t *__restrict__ amask,
-    const typename Prec<MIXED>::T4 *__restrict__ LA, const typename Prec<MIXED>::T4 *__restrict__ LB,
     const float *__restrict__ disp, DCtl *__restrict__ ctl, typename Prec<MIXED>::T4 *__restrict__ EX,
     typename Prec<MIXED>::T4 *__restrict__ EXself, uint64_t *__restrict__ exmask, int pstride,
     double *__restrict__ partials) {
@@ -1287,15 +1334,15 @@ __global__ __launch_bounds__(FIB_BLOCK) void k_extra(
     const int *rowb = nbr_sw + (size_t)b * Wsw;
     const uint64_t amb = amask[b];
     for (int n = 0; n < na; ++n) {
-      Leg<T> Lbc;  // b's leg to c: unit vector from c to b
-      load_leg(LA, LB, (size_t)n * NA + b, Lbc);
-      const int c = ks_index(Lbc.ks);
+      const int kc = nth_bit(amb, n);  // b's list slot of c
+      const int c = rowb[kc];
       const float dc = disp[c];
       const double4 xc = X[c];
       const int tc = type_of(meta_of(xc));
+      Leg<T> Lbc;  // b's leg to c: unit vector from c to b
+      make_leg<MIXED>(xb, xc, in, fc, tb, tc, (uint32_t)c, (uint32_t)kc, Lbc);
       const int *rowc = nbr_sw + (size_t)c * Wsw;
       const int cc = min(cnt_sw[c], Wsw);
-      const int kc = nth_bit(amb, n);  // b's list slot of c
       for (int k = 0; k < cs; ++k) {
         const int a = rowb[k];
         if (a == c || !(disp[a] + dc > skin)) continue;
@@ -2115,7 +2162,8 @@ void launch_nbuild(const DevBufs &d, const FConst &fc, hipStream_t s) {
   }
   if (d.total_cells > 0) {
     KScope ks_(d, KT_NBUILD);
-    hipLaunchKernelGGL(k_nbuild, dim3(nblk(d.total_cells, NB_CELLS)), dim3(FIB_WAVE), 0, s, d.X[d.cur], d.I, d.ninst,
+    hipLaunchKernelGGL(k_nbuild, dim3(nblk(d.total_cells, NB_CELLS * NB_WAVES)),
+                       dim3(NB_WAVES * FIB_WAVE), 0, s, d.X[d.cur], d.I, d.ninst,
                        d.total_cells, d.cstart, d.bcnt, fc.rv2, fc.rv2lj, d.Wsw, d.Wion,
                        d.cnt_sw, d.nbr_sw, d.cnt_ion, d.nbr_ion, d.rk_sw, d.off_sw, d.gate);
   }
@@ -2144,21 +2192,21 @@ static void force_ab(const DevBufs &d, const FConst &fc, hipStream_t s) {
     KScope ks_(d, KT_LEGS);
     hipLaunchKernelGGL((k_legs<MIXED, EN, VIR>), dim3(d.NA / FIB_BLOCK), dim3(FIB_BLOCK), 0, s,
                        d.X[d.cur], d.I, fc, d.tab, d.NA, d.Wsw, d.Wion, d.cnt_sw, d.nbr_sw,
-                       d.cnt_ion, d.nbr_ion, (T4 *)d.LA, (T4 *)d.LB, d.cnt_leg, d.amask, d.Fpart,
-                       d.upart, d.pstride, d.partials);
+                       d.cnt_ion, d.nbr_ion, d.cnt_leg, d.amask, d.Fpart, d.upart, d.pstride,
+                       d.partials);
   }
   {
     KScope ks_(d, KT_TRI);
     hipLaunchKernelGGL((k_tri<MIXED, EN, VIR>), dim3(d.NA / FIB_BLOCK), dim3(FIB_BLOCK), 0, s,
-                       d.X[d.cur], fc, d.NA, d.Wsw, d.cnt_leg, (const T4 *)d.LA,
-                       (const T4 *)d.LB, d.rk_sw, d.off_sw, d.amask, (T4 *)d.G, d.RA, d.pstride,
+                       d.X[d.cur], d.I, fc, d.NA, d.Wsw, d.cnt_leg, d.nbr_sw, d.rk_sw,
+                       d.off_sw, d.amask, (T4 *)d.G, d.RA, d.pstride,
                        d.partials + (size_t)FIB_NRED * d.pstride);
   }
   {
     KScope ks_(d, KT_EXTRA);
     hipLaunchKernelGGL((k_extra<MIXED, EN, VIR>), dim3(d.NA / FIB_BLOCK), dim3(FIB_BLOCK), 0, s,
                        d.X[d.cur], d.I, fc, d.NA, d.Wsw, d.cnt_sw, d.nbr_sw, d.cnt_leg, d.amask,
-                       (const T4 *)d.LA, (const T4 *)d.LB, d.disp, d.ctl, (T4 *)d.EX,
+                       d.disp, d.ctl, (T4 *)d.EX,
                        (T4 *)d.EXself, d.exmask, d.pstride,
                        d.partials + (size_t)2 * FIB_NRED * d.pstride);
   }

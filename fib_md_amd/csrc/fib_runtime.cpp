@@ -258,7 +258,7 @@ static int32_t free_bufs(fib_handle *h) {
   DevBufs &d = h->d;
   void *ptrs[] = {d.I, d.ctl, d.ibase, d.X[0], d.X[1], d.V[0], d.V[1], d.F[0], d.F[1], d.Xi[0], d.Xi[1],
                   d.inv, d.bkey, d.brank, d.bcnt, d.cstart, d.perm, d.bsum, d.cnt_sw, d.nbr_sw,
-                  d.cnt_ion, d.nbr_ion, d.rk_sw, d.off_sw, d.LA, d.LB, d.G, d.RA, d.amask, d.cnt_leg, d.Xb, d.disp, d.EX, d.EXself, d.exmask, d.Fpart, d.upart, d.Fout, d.uout,
+                  d.cnt_ion, d.nbr_ion, d.rk_sw, d.off_sw, d.G, d.RA, d.amask, d.cnt_leg, d.Xb, d.disp, d.EX, d.EXself, d.exmask, d.Fpart, d.upart, d.Fout, d.uout,
                   d.partials, d.tab, d.xfer};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
@@ -275,11 +275,11 @@ static int32_t alloc_lists(fib_handle *h, int Wsw, int Wion) {
   DevBufs &d = h->d;
   if (d.nbr_sw) (void)hipFree(d.nbr_sw);
   if (d.nbr_ion) (void)hipFree(d.nbr_ion);
-  void *old[] = {d.rk_sw, d.LA, d.LB, d.G, d.RA, d.EX};
+  void *old[] = {d.rk_sw, d.G, d.RA, d.EX};
   for (void *q : old)
     if (q) (void)hipFree(q);
   d.nbr_sw = nullptr; d.nbr_ion = nullptr; d.rk_sw = nullptr;
-  d.LA = nullptr; d.LB = nullptr; d.G = nullptr; d.RA = nullptr;
+  d.G = nullptr; d.RA = nullptr;
   d.EX = nullptr;
   d.Wsw = Wsw;
   d.Wion = Wion;
@@ -292,8 +292,6 @@ static int32_t alloc_lists(fib_handle *h, int Wsw, int Wion) {
   HIPCHK(h, dalloc(&d.rk_sw, (size_t)Wsw * NA));
   // a leg is a listed pair, so Wsw slots always hold every leg of an atom
   const size_t gsz = h->prec == FIB_PREC_MIXED ? sizeof(float4) : sizeof(double4);
-  HIPCHK(h, hipMalloc(&d.LA, gsz * (size_t)Wsw * NA));
-  HIPCHK(h, hipMalloc(&d.LB, gsz * (size_t)Wsw * NA));
   HIPCHK(h, hipMalloc(&d.G, gsz * (size_t)Wsw * NA));
   HIPCHK(h, dalloc(&d.RA, (size_t)Wsw * NA));
   HIPCHK(h, hipMalloc(&d.EX, gsz * (size_t)Wsw * NA));
