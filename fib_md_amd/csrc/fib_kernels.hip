@@ -798,6 +798,13 @@ __device__ __forceinline__ void min_image_fast(const double4 &a, const double4 &
   x2 = d2 - rint(d2 * in.ilb[2]) * in.lb[2];
 }
 
+// v[n] of a 4-entry register table by selects (a dynamic index would put
+// the table in scratch memory).
+template <typename T>
+__device__ __forceinline__ T pick4(const T (&v)[4], int n) {
+  return n == 0 ? v[0] : n == 1 ? v[1] : n == 2 ? v[2] : v[3];
+}
+
 // The leg from centre c to neighbour n (unit vector from n to c, 1/r, E, g),
 // with k_legs' arithmetic: FP64 as the reference (si.f90:491-509), or the
 // FP32 terms of the mixed mode.  Legs are not stored between passes: k_legs
@@ -1091,6 +1098,7 @@ __global__ __launch_bounds__(FIB_BLOCK) void k_legs(
 // terms, which read every leg of the centre once per leg, read LDS; no leg
 // record goes through HBM.  Runs end on atom boundaries and hold at most
 // TriStage::N legs.
+#define TRI_BATCH 4  // legs per lane whose neighbour loads are in flight together
 template <bool MIXED>
 struct TriStage {
   static constexpr int N = MIXED ? 256 : 128;  // legs per run: 4 KB of each table per wave
@@ -1160,23 +1168,35 @@ __global__ __launch_bounds__(FIB_BLOCK) void k_tri(
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     // stage: each lane computes its atom's legs (the pairs of its leg mask,
-    // in list order) from the positions, four neighbour loads in flight
+    // in list order) from the positions, TRI_BATCH neighbours in flight
     {
       uint64_t m = amc;
-      for (int a0 = 0; __ballot(mine && a0 < nl) != 0ull; a0 += 4) {
-        int kq[4], jq[4];
+      for (int a0 = 0; __ballot(mine && a0 < nl) != 0ull; a0 += TRI_BATCH) {
+        int kq[TRI_BATCH], jq[TRI_BATCH], rq[TRI_BATCH];
 #pragma unroll
-        for (int u = 0; u < 4; ++u) {
+        for (int u = 0; u < TRI_BATCH; ++u) {
           const bool ok = mine && a0 + u < nl;
           kq[u] = ok ? __ffsll((unsigned long long)m) - 1 : 0;
           if (ok) m &= m - 1;
           jq[u] = ok ? nbr_sw[rowc + kq[u]] : ic;
+          rq[u] = ok ? rk_sw[rowc + kq[u]] : 0;
         }
-        double4 xq[4];
+        // Where the partner j keeps the force of its own terms on this atom:
+        // G[n * NA + j], n = j's leg index of the atom.  The slot r of the
+        // atom in j's row follows from the stencil offsets (see k_nbuild), n
+        // from j's leg mask (legs are symmetric); 255 offsets (r >= Wsw)
+        // mark a j without a row.
+        double4 xq[TRI_BATCH];
+        uint64_t aq[TRI_BATCH];
+        int oq[TRI_BATCH];
 #pragma unroll
-        for (int u = 0; u < 4; ++u) xq[u] = X[jq[u]];
+        for (int u = 0; u < TRI_BATCH; ++u) {
+          xq[u] = X[jq[u]];
+          aq[u] = amask[jq[u]];
+          oq[u] = off_sw[(size_t)jq[u] * OFF_W + 26 - (rq[u] >> 8)];
+        }
 #pragma unroll
-        for (int u = 0; u < 4; ++u) {
+        for (int u = 0; u < TRI_BATCH; ++u) {
           if (mine && a0 + u < nl) {
             Leg<T> L;
             make_leg<MIXED>(xc, xq[u], in, fc, tci, type_of(meta_of(xq[u])), (uint32_t)jq[u],
@@ -1188,6 +1208,10 @@ __global__ __launch_bounds__(FIB_BLOCK) void k_tri(
             sA[wid][q] = x;
             sB[wid][q] = y;
             owner[wid][q] = (uint8_t)lane;
+            const int r = oq[u] + (rq[u] & 0xff);
+            sR[wid][q] = r < Wsw ? (uint32_t)__popcll(aq[u] & ((1ull << r) - 1ull)) *
+                                           (uint32_t)NA + (uint32_t)jq[u]
+                                 : ~0u;
           }
         }
       }
@@ -1198,7 +1222,6 @@ __global__ __launch_bounds__(FIB_BLOCK) void k_tri(
     for (int g = s0 + lane; g < s1; g += FIB_WAVE) {
       const int q = g - s0;
       const int o = owner[wid][q];
-      const int i = i0 + o;
       const int c0 = soff[wid][o] - s0;  // the centre's first leg in the stage
       const int na = soff[wid][o + 1] - soff[wid][o];
       const int a = g - soff[wid][o];
@@ -1208,14 +1231,6 @@ __global__ __launch_bounds__(FIB_BLOCK) void k_tri(
         La.e0 = x.x; La.e1 = x.y; La.e2 = x.z; La.E = fabs(x.w);
         La.g = y.x; La.rinv = y.y; La.ks = ks_unpack(y.z); La.slot = ks_unpack(y.w);
       }
-      // Where the partner j keeps the force of its own terms on i: G[n * NA
-      // + j], n = j's leg index of i.  The slot r of i in j's row follows
-      // from the stencil offsets (see k_nbuild), n from j's leg mask (legs
-      // are symmetric); 255 offsets (r >= Wsw) mark a j without a row.
-      const int j = ks_index(La.ks);
-      const uint64_t amj = amask[j];
-      const int rk = rk_sw[(size_t)i * Wsw + La.slot];
-      const int r = off_sw[(size_t)j * OFF_W + 26 - (rk >> 8)] + (rk & 0xff);
       const int ga = sge[wid][o] + ks_ge(La.ks);
       T Ga0 = 0, Ga1 = 0, Ga2 = 0, Gu = 0;
       for (int b = 0; b < na; ++b) {
@@ -1224,18 +1239,15 @@ __global__ __launch_bounds__(FIB_BLOCK) void k_tri(
         Leg<T> Lb;
         Lb.e0 = xb.x; Lb.e1 = xb.y; Lb.e2 = xb.z; Lb.E = fabs(xb.w);
         T Fa[3], hh;
-        vterm<T>(lp[ga + (xb.w < T(0) ? 1 : 0)], La, Lb, Fa, hh);
+        vterm<T>(pick4(lp, ga + (xb.w < T(0) ? 1 : 0)), La, Lb, Fa, hh);
         Ga0 += Fa[0]; Ga1 += Fa[1]; Ga2 += Fa[2];
         Gu += hh;
       }
       T4 gv;
       gv.x = Ga0; gv.y = Ga1; gv.z = Ga2; gv.w = Gu * T(1.0 / 3.0);
       // results go back through the stage (this lane alone reads sB[q]) and
-      // leave plane by plane below, so the stores coalesce
+      // leave plane by plane below, with sR, so the stores coalesce
       sB[wid][q] = gv;
-      sR[wid][q] =
-          r < Wsw ? (uint32_t)__popcll(amj & ((1ull << r) - 1ull)) * (uint32_t)NA + (uint32_t)j
-                  : ~0u;
       uc += Gu;
       if (VIR) {
         // r_a = x_a - x_i = -e_a / rinv_a
@@ -1388,8 +1400,8 @@ __global__ __launch_bounds__(FIB_BLOCK) void k_extra(
         const int nge = (tc == 2) + (ta == 2) + (tb == 2);
         T Fa[3], Fb[3], hh;
         // vterm gives the force on its first leg: once per end
-        vterm<T>(lp[nge], Lca, Lcb, Fa, hh);
-        vterm<T>(lp[nge], Lcb, Lca, Fb, hh);
+        vterm<T>(pick4(lp, nge), Lca, Lcb, Fa, hh);
+        vterm<T>(pick4(lp, nge), Lcb, Lca, Fb, hh);
         const T Fc0 = -(Fa[0] + Fb[0]), Fc1 = -(Fa[1] + Fb[1]), Fc2 = -(Fa[2] + Fb[2]);
         fb.add(Fb[0], Fb[1], Fb[2]);
         const T third = hh * T(1.0 / 3.0);
