@@ -5,6 +5,9 @@
 
 #include "fib_device.h"
 
+// Workgroup size of the per-instance reductions (k_finalize).
+#define FIB_FIN_T 1024
+
 // Per-step arguments of the integrate kernels (onestep, timeint.f90:158-242).
 struct StepArgs {
   double mass[3];            // kg by type 1..3
@@ -79,6 +82,9 @@ struct DevBufs {
   uint64_t *exmask;          // slots of EX written in the last evaluation [NA]
   double *Fout, *uout;       // force-only results [3*NA], [NA]
   double *partials;          // [2 * FIB_NRED * pstride], layout [region][slot][block]
+  int max_pad;               // largest padded atom count of an instance
+  double *fin_tmp;           // chunk results of big instances' reductions
+                             // [ninst * (max_pad / 64 / FIB_FIN_T + 1) * FIB_NRED]
   int pstride;               // NA/64 + 1
   TabEntry *tab;             // [ntable + 3]
   double *xfer;              // staging [3*NA] for host transfers

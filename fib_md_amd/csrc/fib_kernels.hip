@@ -175,25 +175,44 @@ __device__ void block_partials(double (&v)[NV], const int (&op)[NV], int lblock,
   }
 }
 
-#define FIN_T 1024
-// One block per instance: combine the instance's partials in a fixed order
-// (thread t sums blocks t, t+FIN_T, ... then a fixed tree), all NV slots at
-// once so the loads of every slot are in flight together.
+#define FIN_T FIB_FIN_T
+// Per instance, combine the partials of its blocks in a fixed order.  One
+// workgroup per instance when the instance has at most FIN_T blocks (thread t
+// takes block t, then a fixed tree); otherwise workgroup (inst, c) reduces
+// blocks [c*FIN_T, (c+1)*FIN_T) into tmp and k_finalize2 combines the
+// chunks in order -- a big instance then spreads over many CUs.  All NV
+// slots at once, so the loads of every slot are in flight together.
+template <int NV>
+__device__ __forceinline__ void fin_block_tree(double (&x)[NV], const RedSpec &rs,
+                                               double (*red)[NV]) {
+  const int lane = threadIdx.x % FIB_WAVE, wid = threadIdx.x / FIB_WAVE;
+#pragma unroll
+  for (int k = 0; k < NV; ++k) {
+    for (int o = FIB_WAVE / 2; o > 0; o >>= 1) {
+      const double y = __shfl_xor(x[k], o);
+      x[k] = rs.op[k] ? fmax(x[k], y) : x[k] + y;
+    }
+    if (lane == 0) red[wid][k] = x[k];
+  }
+  __syncthreads();
+}
 template <int NV>
 __global__ __launch_bounds__(FIN_T) void k_finalize(const DInst *__restrict__ I,
                                                     const double *__restrict__ partials,
                                                     int pstride, RedSpec rs,
                                                     DCtl *__restrict__ ctl,
-                                                    const int *__restrict__ gate) {
+                                                    const int *__restrict__ gate,
+                                                    double *__restrict__ tmp) {
   GATE_OR_RETURN(gate);
   __shared__ double red[FIN_T / FIB_WAVE][NV];
-  const int inst = blockIdx.x;
+  const int inst = blockIdx.x, ch = blockIdx.y, nch = gridDim.y;
   const int blk0 = I[inst].atom_off / rs.bs, nblk = I[inst].natm_pad / rs.bs;
-  const int lane = threadIdx.x % FIB_WAVE, wid = threadIdx.x / FIB_WAVE;
   double x[NV];
 #pragma unroll
   for (int k = 0; k < NV; ++k) x[k] = rs.op[k] ? -1.0e300 : 0.0;
-  for (int b = threadIdx.x; b < nblk; b += FIN_T) {
+  const int b_lo = nch == 1 ? 0 : ch * FIN_T;
+  const int b_hi = nch == 1 ? nblk : min(nblk, b_lo + FIN_T);
+  for (int b = b_lo + (int)threadIdx.x; b < b_hi; b += FIN_T) {
 #pragma unroll
     for (int k = 0; k < NV; ++k) {
       double y = partials[(size_t)k * pstride + blk0 + b];
@@ -204,21 +223,30 @@ __global__ __launch_bounds__(FIN_T) void k_finalize(const DInst *__restrict__ I,
       x[k] = rs.op[k] ? fmax(x[k], y) : x[k] + y;
     }
   }
-#pragma unroll
-  for (int k = 0; k < NV; ++k) {
-    for (int o = FIB_WAVE / 2; o > 0; o >>= 1) {
-      const double y = __shfl_xor(x[k], o);
-      x[k] = rs.op[k] ? fmax(x[k], y) : x[k] + y;
-    }
-    if (lane == 0) red[wid][k] = x[k];
-  }
-  __syncthreads();
+  fin_block_tree<NV>(x, rs, red);
   if (threadIdx.x < NV) {
     const int k = threadIdx.x;
     double t = red[0][k];
     for (int w = 1; w < FIN_T / FIB_WAVE; ++w) t = rs.op[k] ? fmax(t, red[w][k]) : t + red[w][k];
-    if (rs.dst[k] >= 0) reinterpret_cast<double *>(&ctl[inst])[rs.dst[k]] = t;
+    if (nch == 1) {
+      if (rs.dst[k] >= 0) reinterpret_cast<double *>(&ctl[inst])[rs.dst[k]] = t;
+    } else {
+      tmp[((size_t)inst * nch + ch) * NV + k] = t;
+    }
   }
+}
+template <int NV>
+__global__ void k_finalize2(const DInst *__restrict__ I, int nch, RedSpec rs,
+                            DCtl *__restrict__ ctl, const int *__restrict__ gate,
+                            const double *__restrict__ tmp) {
+  GATE_OR_RETURN(gate);
+  const int inst = blockIdx.x, k = threadIdx.x;
+  if (k >= NV) return;
+  const int nci = (I[inst].natm_pad / rs.bs + FIN_T - 1) / FIN_T;
+  const double *t0 = tmp + (size_t)inst * nch * NV + k;
+  double t = rs.op[k] ? -1.0e300 : 0.0;
+  for (int c = 0; c < nci; ++c) t = rs.op[k] ? fmax(t, t0[(size_t)c * NV]) : t + t0[(size_t)c * NV];
+  if (rs.dst[k] >= 0) reinterpret_cast<double *>(&ctl[inst])[rs.dst[k]] = t;
 }
 
 // ---------------------------------------------------------------------------
@@ -2037,6 +2065,16 @@ struct KScope {
   }
 };
 
+template <int NV>
+static void fin_launch(const DevBufs &d, const RedSpec &rs, int nch, const int *gate,
+                       hipStream_t s) {
+  hipLaunchKernelGGL(k_finalize<NV>, dim3(d.ninst, nch), dim3(FIN_T), 0, s, d.I, d.partials,
+                     d.pstride, rs, d.ctl, gate, d.fin_tmp);
+  if (nch > 1)
+    hipLaunchKernelGGL(k_finalize2<NV>, dim3(d.ninst), dim3(FIB_WAVE), 0, s, d.I, nch, rs, d.ctl,
+                       gate, d.fin_tmp);
+}
+
 static void finalize(const DevBufs &d, int nv, int bs, const int *op, const int *dst,
                      hipStream_t s, int nreg = 1, const int *gate = nullptr) {
   RedSpec rs;
@@ -2047,49 +2085,16 @@ static void finalize(const DevBufs &d, int nv, int bs, const int *op, const int 
     rs.op[k] = k < nv ? op[k] : 0;
     rs.dst[k] = k < nv ? dst[k] : -1;
   }
+  // chunks per instance: one workgroup up to FIN_T blocks of the producer
+  const int nch = (d.max_pad / bs + FIN_T - 1) / FIN_T;
+  KScope ks_(d, KT_REDUCE);
   switch (nv) {
-    case 1:
-      {
-        KScope ks_(d, KT_REDUCE);
-        hipLaunchKernelGGL(k_finalize<1>, dim3(d.ninst), dim3(FIN_T), 0, s, d.I, d.partials,
-                           d.pstride, rs, d.ctl, gate);
-      }
-      break;
-    case 2:
-      {
-        KScope ks_(d, KT_REDUCE);
-        hipLaunchKernelGGL(k_finalize<2>, dim3(d.ninst), dim3(FIN_T), 0, s, d.I, d.partials,
-                           d.pstride, rs, d.ctl, gate);
-      }
-      break;
-    case 3:
-      {
-        KScope ks_(d, KT_REDUCE);
-        hipLaunchKernelGGL(k_finalize<3>, dim3(d.ninst), dim3(FIN_T), 0, s, d.I, d.partials,
-                           d.pstride, rs, d.ctl, gate);
-      }
-      break;
-    case 4:
-      {
-        KScope ks_(d, KT_REDUCE);
-        hipLaunchKernelGGL(k_finalize<4>, dim3(d.ninst), dim3(FIN_T), 0, s, d.I, d.partials,
-                           d.pstride, rs, d.ctl, gate);
-      }
-      break;
-    case 5:
-      {
-        KScope ks_(d, KT_REDUCE);
-        hipLaunchKernelGGL(k_finalize<5>, dim3(d.ninst), dim3(FIN_T), 0, s, d.I, d.partials,
-                           d.pstride, rs, d.ctl, gate);
-      }
-      break;
-    default:
-      {
-        KScope ks_(d, KT_REDUCE);
-        hipLaunchKernelGGL(k_finalize<10>, dim3(d.ninst), dim3(FIN_T), 0, s, d.I, d.partials,
-                           d.pstride, rs, d.ctl, gate);
-      }
-      break;
+    case 1: fin_launch<1>(d, rs, nch, gate, s); break;
+    case 2: fin_launch<2>(d, rs, nch, gate, s); break;
+    case 3: fin_launch<3>(d, rs, nch, gate, s); break;
+    case 4: fin_launch<4>(d, rs, nch, gate, s); break;
+    case 5: fin_launch<5>(d, rs, nch, gate, s); break;
+    default: fin_launch<10>(d, rs, nch, gate, s); break;
   }
 }
 

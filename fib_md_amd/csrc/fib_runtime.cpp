@@ -259,7 +259,7 @@ static int32_t free_bufs(fib_handle *h) {
   void *ptrs[] = {d.I, d.ctl, d.ibase, d.X[0], d.X[1], d.V[0], d.V[1], d.F[0], d.F[1], d.Xi[0], d.Xi[1],
                   d.inv, d.bkey, d.brank, d.bcnt, d.cstart, d.perm, d.bsum, d.cnt_sw, d.nbr_sw,
                   d.cnt_ion, d.nbr_ion, d.rk_sw, d.off_sw, d.G, d.RA, d.amask, d.cnt_leg, d.Xb, d.disp, d.EX, d.EXself, d.exmask, d.Fpart, d.upart, d.Fout, d.uout,
-                  d.partials, d.tab, d.xfer};
+                  d.partials, d.fin_tmp, d.tab, d.xfer};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
   std::memset(&d, 0, sizeof(d));
@@ -411,6 +411,9 @@ extern "C" int32_t fib_create(const fib_params *p, int32_t ninst, const fib_inst
   ALLOC(d.uout, NA);
   d.pstride = d.NA / FIB_WAVE + 1;
   ALLOC(d.partials, (size_t)d.pstride * FIB_NRED * 3);
+  d.max_pad = 0;
+  for (const DInst &in : h->inst) d.max_pad = std::max(d.max_pad, in.natm_pad);
+  ALLOC(d.fin_tmp, (size_t)ninst * (d.max_pad / FIB_WAVE / FIB_FIN_T + 1) * FIB_NRED);
   ALLOC(d.tab, h->tab.size());
   ALLOC(d.xfer, 3 * NA);
 #undef ALLOC
