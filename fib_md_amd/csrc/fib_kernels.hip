@@ -597,29 +597,24 @@ __device__ __forceinline__ void nbuild_cell(
         ti = type_of(meta_of(xi));
       }
       for (int q0 = 0; q0 < n; q0 += FIB_WAVE) {
-        const int q = q0 + lane;
+        const int q = q0 + lane;  // < NB_CHUNK: the stage arrays are read unguarded
         const int g = base + q;
-        bool sw = false, io = false, far = false;
-        int j = -1, st = 0;
-        double4 xj;
-        double d0 = 0, d1 = 0, d2 = 0;
-        const bool cand = q < n && (j = s_j[q]) != i;
-        if (cand) {
-          xj = s_x[q];
-          st = s_st[q];
-          ref_diff(xi, xj, d0, d1, d2);
-          far = !(fabs(d0) < qb0 && fabs(d1) < qb1 && fabs(d2) < qb2);
-        }
+        const int j = s_j[q];
+        const bool cand = q < n && j != i;
+        const double4 xj = s_x[q];
+        const int st = s_st[q];
+        double d0, d1, d2;
+        ref_diff(xi, xj, d0, d1, d2);
+        const bool far = cand && !(fabs(d0) < qb0 && fabs(d1) < qb1 && fabs(d2) < qb2);
         // With every difference of the batch under a quarter box, the
         // reference's NINT shifts are all zero: r^2 is the plain sum
         // (bit-identical); otherwise the full minimum image
-        const bool near_all = __ballot(far) == 0ull;
-        if (cand) {
-          const int tj = type_of(meta_of(xj));
-          const double r2 = near_all ? ref_sq(d0, d1, d2) : ref_r2(xi, xj, in);
-          if (max(ti, tj) < 3) sw = r2 < rv2;
-          else io = r2 < rv2lj;
-        }
+        double r2;
+        if (__ballot(far) == 0ull) r2 = ref_sq(d0, d1, d2);
+        else r2 = ref_r2(xi, xj, in);
+        const bool swp = max(ti, type_of(meta_of(xj))) < 3;
+        const bool sw = cand && swp && r2 < rv2;
+        const bool io = cand && !swp && r2 < rv2lj;
         uint64_t msw = __ballot(sw), mio = __ballot(io);
         uint64_t below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
         if (sw) {
@@ -650,9 +645,9 @@ __device__ __forceinline__ void nbuild_cell(
         // stencil cell s starts at candidate my_lo: count the accepted ones
         // of this batch of 64 candidates that lie before it
         {
-          const int cut = my_lo - (base + q0);
-          if (cut >= 64) before += __popcll(msw);
-          else if (cut > 0) before += __popcll(msw & (~0ull >> (64 - cut)));
+          const int cut = min(max(my_lo - (base + q0), 0), 64);
+          const uint64_t lm = cut >= 64 ? ~0ull : ((1ull << cut) - 1ull);
+          before += __popcll(msw & lm);
         }
         nsw += __popcll(msw);
         nion += __popcll(mio);
