@@ -157,22 +157,15 @@ def run_gpu(X, at, L, precision, steps, warmup):
 STEPS_PER_IMPACT = 2000  # the single-impact configuration's step count
 
 
-def _md_handle(instances, precision, use_graph=True):
-    from fib_md_amd import FibGPU
-    g = FibGPU([(len(s["X"]), s["nsg"], s["nlj"], s["L"]) for s in instances], precision=precision)
-    for k, s in enumerate(instances):
-        g.set_state(s["X"], V=s["V"], atype=s["atype"], xsides=s["xsides"], Xi=s["X"], inst=k)
-    g.md_begin([s["surface_z"] for s in instances], [s["bottom_z"] for s in instances],
-               use_graph=use_graph)
-    for k, s in enumerate(instances):
-        g.md_fire(k, 1, s["ion_x"], s["ion_v"])
-    return g
+def _md_handle(instances, precision, use_graph=True, params=None):
+    from fib_md_amd.autotune import md_handle
+    return md_handle(instances, precision, params=params, use_graph=use_graph)
 
 
-def md_rate(instances, precision, steps, warmup=5, use_graph=True):
+def md_rate(instances, precision, steps, warmup=5, use_graph=True, params=None):
     """Timed MD steps of a batch: atom-steps/s over all instances."""
     import torch
-    g = _md_handle(instances, precision, use_graph)
+    g = _md_handle(instances, precision, use_graph, params)
     g.md_run(warmup)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -247,6 +240,16 @@ def md_bench(args):
     log(f"  ensemble: {len(ens)} instances, {nat} atoms")
     mb = md_rate(ens, "mixed", args.md_steps)
     fb = md_rate(ens, "fp64", args.md_steps)
+    # the list radius factor (cell size and skin) chosen by the autotuner
+    from fib_md_amd import autotune
+    from fib_md_amd.gpu import default_params
+    lf, table = autotune.tune_listfac(ens, "mixed")
+    pt = default_params()
+    pt.listfac = lf
+    mt = md_rate(ens, "mixed", args.md_steps, params=pt)
+    mt.update(listfac=lf, tuning_ms_per_step={f"{a:.2f}": b for a, b, _ in table})
+    log(f"  ensemble mixed batched, autotuned listfac {lf:.2f}: {mt['atom_steps_per_s']:.4g} "
+        f"atom-steps/s (default 1.10: {mb['atom_steps_per_s']:.4g})")
     # plain FP64 path: each instance on its own, one after the other, with
     # the step replayed as a graph and with ordinary launches
     half = max(1, args.md_steps // 2)
@@ -259,10 +262,13 @@ def md_bench(args):
             tot_as += r["atoms"] * half
         seq[name] = dict(atom_steps_per_s=tot_as / tot_t, atoms=nat,
                          impacts_per_hour=3600.0 * len(ens) / (tot_t / half * STEPS_PER_IMPACT))
-    for r in (mb, fb):
+    for r in (mb, fb, mt):
         r["impacts_per_hour"] = 3600.0 * len(ens) / (r["ms_per_step"] * 1e-3 * STEPS_PER_IMPACT)
     out["ensemble"] = {"instances": len(ens), "atoms": nat, "steps_timed": args.md_steps,
-                       "mixed_batched": mb, "fp64_batched": fb, **seq,
+                       "mixed_batched": mb, "mixed_batched_autotuned": mt, "fp64_batched": fb,
+                       **seq,
+                       "mixed_batched_autotuned_over_fp64_one_at_a_time_graph":
+                           mt["atom_steps_per_s"] / seq["fp64_one_at_a_time_graph"]["atom_steps_per_s"],
                        "mixed_batched_over_fp64_one_at_a_time_graph":
                            mb["atom_steps_per_s"] / seq["fp64_one_at_a_time_graph"]["atom_steps_per_s"],
                        "mixed_batched_over_fp64_one_at_a_time_launches":
