@@ -176,7 +176,45 @@ def md_rate(instances, precision, steps, warmup=5, use_graph=True, params=None):
     reps = [g.md_report(k) for k in range(len(instances))]
     g.close()
     return dict(atom_steps_per_s=n * steps / t, ms_per_step=t * 1e3 / steps, atoms=n,
-                builds=sum(r["builds"] for r in reps), removed=sum(r["removed"] for r in reps))
+                builds=sum(r["builds"] for r in reps), removed=sum(r["removed"] for r in reps),
+                dt_min_s=min(r["dt"] for r in reps), sim_time_s=max(r["time"] for r in reps))
+
+
+def dose_run(n_rep, ncell, impacts, steps_per_impact, precision):
+    """Cumulative dose: `n_rep` replicas of ncell^3 cells, `impacts` sequential
+    2 keV normal Ga impacts each (all replicas fire together), the Berendsen
+    side and bottom 3 A shells at 300 K throughout, sputtered atoms removed,
+    Ga retained.  Sputter yield per replica after each impact, and the rate."""
+    import torch
+    from fib_md_amd import configs
+    reps, shots = configs.dose_replicas(n_rep, ncell, impacts)
+    from fib_md_amd import FibGPU
+    g = FibGPU([(len(s["X"]), s["nsg"], s["nlj"], s["L"]) for s in reps], precision=precision)
+    for k, s in enumerate(reps):
+        g.set_state(s["X"], V=s["V"], atype=s["atype"], xsides=s["xsides"], Xi=s["X"], inst=k)
+    g.md_begin([s["surface_z"] for s in reps], [s["bottom_z"] for s in reps],
+               shell_width=3e-10, ttar=300.0, tau=1e-13)
+    yields, ke = [], []
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for m in range(impacts):
+        for k in range(n_rep):
+            g.md_fire(k, m + 1, *shots[k][m])
+        g.md_run(steps_per_impact)
+        rr = [g.md_report(k) for k in range(n_rep)]
+        yields.append([r["removed"] for r in rr])
+        ke.append(float(np.mean([r["ke"] for r in rr])))
+    torch.cuda.synchronize()
+    t = time.perf_counter() - t0
+    g.close()
+    n = sum(len(s["X"]) for s in reps)
+    y = np.array(yields, float)
+    return dict(replicas=n_rep, atoms=n, impacts=impacts, steps_per_impact=steps_per_impact,
+                atom_steps_per_s=n * impacts * steps_per_impact / t,
+                ms_per_step=t * 1e3 / (impacts * steps_per_impact),
+                cumulative_yield_mean=y.mean(1).tolist(),
+                cumulative_yield_sem=(y.std(1, ddof=1) / np.sqrt(n_rep)).tolist(),
+                mean_ke_J=ke)
 
 
 def cascade_stats(instances, steps):
@@ -284,6 +322,24 @@ def md_bench(args):
             st = cs[key]
             log(f"  {key:9s} fp64 {st['fp64']:.3f} +- {st['fp64_sem']:.3f}, mixed "
                 f"{st['mixed']:.3f} +- {st['mixed_sem']:.3f}, z {st['z_mixed_vs_fp64']:.2f}")
+    # the single large impact: 60 x 60 x 80 cells (2,304,000 atoms), 30 keV at
+    # 7 deg; impacts/hour at 5000 steps per impact
+    if args.big_steps > 0:
+        s3 = configs.big_impact()
+        for prec in ("mixed", "fp64"):
+            r = md_rate([s3], prec, args.big_steps)
+            r["impacts_per_hour"] = 3600.0 / (r["ms_per_step"] * 1e-3 * 5000)
+            out[f"big_impact_2304001_{prec}"] = r
+            log(f"  30 keV impact, 2.3 M atoms, {prec}: {r['ms_per_step']:.3f} ms/step, "
+                f"{r['atom_steps_per_s']:.4g} atom-steps/s, dt down to {r['dt_min_s'] * 1e15:.4f} fs")
+    # cumulative dose, shortened: 16 replicas of 40^3 cells, a few impacts
+    if args.dose_impacts > 0:
+        out["dose"] = {}
+        for prec in ("mixed", "fp64"):
+            r = dose_run(16, 40, args.dose_impacts, args.dose_steps, prec)
+            out["dose"][prec] = r
+            log(f"  dose {prec}: {r['atoms']} atoms, {r['atom_steps_per_s']:.4g} atom-steps/s, "
+                f"cumulative yield {['%.2f' % y for y in r['cumulative_yield_mean']]}")
     return out
 
 
@@ -333,6 +389,12 @@ def main():
     ap.add_argument("--ensemble", type=int, default=64)
     ap.add_argument("--stats-steps", type=int, default=2000,
                     help="MD steps of the cascade-statistics run (0: skip)")
+    ap.add_argument("--big-steps", type=int, default=40,
+                    help="timed MD steps of the 2.3 M-atom 30 keV impact (0: skip)")
+    ap.add_argument("--dose-impacts", type=int, default=2,
+                    help="sequential impacts per replica of the shortened dose run (0: skip)")
+    ap.add_argument("--dose-steps", type=int, default=1500,
+                    help="MD steps per impact of the dose run")
     ap.add_argument("--profile", choices=["fp64", "mixed", "md-fp64", "md-mixed"], default=None)
     ap.add_argument("--summarize-stats", nargs=3, metavar=("DIR", "OUT", "CMD"))
     ap.add_argument("--summarize-pmc", nargs=4, metavar=("FETCH_DIR", "WRITE_DIR", "OUT", "CMD"))

@@ -157,3 +157,33 @@ def ensemble(n_inst: int = 64, seed: int = 2000, nmin: float = 1e4, nmax: float 
         slab.update(ion_x=x, ion_v=v, energy_ev=E, polar=th, azimuth=ph, ncell=n)
         out.append(slab)
     return out
+
+
+def big_impact(seed: int = 1003, n: int = 60, nz: int = 80, energy_ev: float = 30e3,
+               polar_deg: float = 7.0):
+    """The single large impact: an n x n x nz-cell Si(100) slab at 300 K
+    (2,304,000 atoms for 60 x 60 x 80) hit by one Ga ion of `energy_ev` at
+    `polar_deg` off normal, azimuth drawn from the seed."""
+    slab = si_slab(n, seed, nz=nz)
+    rng = np.random.default_rng(seed + 1)
+    az = rng.uniform(0.0, 360.0)
+    x, v = ion_launch(slab["L"], slab["surface_z"], energy_ev, polar_deg, az, rng)
+    slab.update(ion_x=x, ion_v=v, energy_ev=energy_ev, polar=polar_deg, azimuth=az)
+    return slab
+
+
+def dose_replicas(n_rep: int = 16, n: int = 40, impacts: int = 200, seed: int = 3000,
+                  energy_ev: float = 2e3):
+    """The cumulative-dose configuration: `n_rep` Si(100) slabs of n^3 cells
+    (512,000 atoms for n = 40), each with `impacts` Ga ions parked above the
+    surface, fired one after another at normal incidence from seeded random
+    (x, y) points 5 A above the surface.  Returns the slabs and, per replica,
+    the (x, v) of every impact in firing order."""
+    reps, shots = [], []
+    for r in range(n_rep):
+        slab = si_slab(n, seed + r, nions=impacts)
+        rng = np.random.default_rng(seed + r + 104729)
+        shots.append([ion_launch(slab["L"], slab["surface_z"], energy_ev, 0.0, 0.0, rng)
+                      for _ in range(impacts)])
+        reps.append(slab)
+    return reps, shots
