@@ -60,9 +60,10 @@ module fibhost
   end type state_t
 
   public :: read_siga, ran1, pureepi, initranions, initdata, knock_draw, writeatoms, &
-       open_outputs, close_outputs, host_temperature, proc_of, pi_ref
+       open_outputs, close_outputs, host_temperature, proc_of, pi_ref, write_restart, &
+       read_restart
 
-  integer, parameter :: atm_unit = 41, snap_unit = 42
+  integer, parameter :: atm_unit = 41, snap_unit = 42, res_unit = 12
 
 contains
 
@@ -416,5 +417,116 @@ contains
     close(snap_unit)
     write(*, "(I10,F10.4,F10.2)") lt, time * 1e12_dp, temp
   end subroutine writeatoms
+
+  ! --------------------------------------------------------------------------
+  ! Restart records (writerestart, out.f90:34-59): D/restart.in_old and
+  ! D/restart.in<impact:6>_<lt:9>, unformatted sequential, records
+  ! (lt, restart_time), (X, V, mass, atype), (impact) -- as the reference
+  ! writes them, restart_time being the clock value the run started from.
+  ! Beside each goes D/restart.rng<same suffix> with what the reference does
+  ! not keep: the ran1 state and the impact spots, so that a restart carries
+  ! on the random sequence where the run left it.
+  subroutine write_restart(s, st, lt, restart_time, impact)
+    type(siga_t), intent(in) :: s
+    type(state_t), intent(in) :: st
+    integer, intent(in) :: lt, impact
+    real(dp), intent(in) :: restart_time
+    character(len=64) :: fn(2), fr(2)
+    integer :: k
+    fn(1) = 'D/restart.in_old'
+    fr(1) = 'D/restart.rng_old'
+    write(fn(2), "('D/restart.in',I6.6,'_',I9.9)") impact, lt
+    write(fr(2), "('D/restart.rng',I6.6,'_',I9.9)") impact, lt
+    do k = 1, 2
+       open(res_unit, file=trim(fn(k)), form='unformatted', status='replace')
+       write(res_unit) lt, restart_time
+       write(res_unit) st%X, st%V, st%mass, st%atype
+       write(res_unit) impact
+       close(res_unit)
+       open(res_unit, file=trim(fr(k)), form='unformatted', status='replace')
+       write(res_unit) ranseed, sh_y, sh_v
+       write(res_unit) s%nrand, st%xrand
+       close(res_unit)
+    end do
+  end subroutine write_restart
+
+  ! Initial state of a restarted run, whichic = 0 (ic, data.f90:71-93): Xi
+  ! and species from si.dat, then X, V, masses, species, lt, the clock and
+  ! the number of ions already fired from the restart record
+  ! (D/restart.in_old for startimpact 0, else the first D/restart.in
+  ! <startimpact:6>* file, found through F/restart_list_000.dat as the
+  ! reference does); ions keep their stored positions as sites, unfired ions
+  ! rest; impact spots from fwhm.dat.  With the record's companion
+  ! D/restart.rng* present, the ran1 state and the impact spots come from it.
+  subroutine read_restart(s, st, nt0, restart_time, ions, ios)
+    type(siga_t), intent(in) :: s
+    type(state_t), intent(inout) :: st
+    integer, intent(out) :: nt0, ions, ios
+    real(dp), intent(out) :: restart_time
+    character(len=28) :: fn
+    character(len=64) :: fr, cmd
+    integer :: i, u, t, nr
+    logical :: have
+    allocate(st%X(st%natm,3), st%Xi(st%natm,3), st%V(st%natm,3), st%mass(st%natm), &
+         st%atype(st%natm), st%xrand(s%nrand,2))
+    open(newunit=u, file='si.dat', status='old', action='read', iostat=ios)
+    if (ios /= 0) return
+    do i = 1, st%nsg
+       read(u, *, iostat=ios) st%Xi(i,:), t
+       if (ios /= 0) then
+          close(u)
+          return
+       end if
+       st%atype(i) = t
+    end do
+    close(u)
+    if (s%startimpact == 0) then
+       fn = 'D/restart.in_old'
+    else
+       write(cmd, "('ls D/restart.in',I6.6,'* > F/restart_list_000.dat')") s%startimpact
+       call execute_command_line(trim(cmd), exitstat=ios)
+       if (ios /= 0) return
+       open(newunit=u, file='F/restart_list_000.dat', status='old', action='read', iostat=ios)
+       if (ios /= 0) return
+       read(u, "(A28)", iostat=ios) fn
+       close(u)
+       if (ios /= 0) return
+    end if
+    open(res_unit, file=trim(fn), form='unformatted', status='old', action='read', iostat=ios)
+    if (ios /= 0) return
+    read(res_unit, iostat=ios) nt0, restart_time
+    if (ios == 0) read(res_unit, iostat=ios) st%X, st%V, st%mass, st%atype
+    if (ios == 0) read(res_unit, iostat=ios) ions
+    close(res_unit)
+    if (ios /= 0) return
+    st%Xi(st%nsg+1:st%natm,:) = st%X(st%nsg+1:st%natm,:)
+    st%V(st%nsg+ions+1:st%natm,:) = 0
+    open(newunit=u, file='fwhm.dat', status='old', action='read', iostat=ios)
+    if (ios /= 0) return
+    do i = 1, s%nrand
+       read(u, *, iostat=ios) st%xrand(i,:)
+       if (ios /= 0) then
+          close(u)
+          return
+       end if
+    end do
+    close(u)
+    ! the companion file: same name with 'rng' for 'in'
+    fr = 'D/restart.rng' // trim(fn(13:))
+    inquire(file=trim(fr), exist=have)
+    if (have) then
+       open(res_unit, file=trim(fr), form='unformatted', status='old', action='read', iostat=ios)
+       if (ios /= 0) return
+       read(res_unit, iostat=ios) ranseed, sh_y, sh_v
+       if (ios == 0) read(res_unit, iostat=ios) nr
+       if (ios == 0 .and. nr == s%nrand) then
+          backspace(res_unit)
+          read(res_unit, iostat=ios) nr, st%xrand
+       end if
+       close(res_unit)
+       if (ios /= 0) return
+    end if
+    ios = 0
+  end subroutine read_restart
 
 end module fibhost

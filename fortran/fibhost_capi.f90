@@ -126,6 +126,28 @@ contains
     call close_outputs(atm_out)
   end subroutine fh_close_outputs
 
+  subroutine fh_write_restart(lt, restart_time, impact) bind(C, name="fh_write_restart")
+    integer(c_int), value :: lt, impact
+    real(c_double), value :: restart_time
+    call write_restart(s, st, lt, restart_time, impact)
+  end subroutine fh_write_restart
+
+  ! whichic-0 initial state from the restart record (the state of fh_setup
+  ! is replaced); returns the iostat
+  integer(c_int) function fh_read_restart(nt0, restart_time, ions) &
+       bind(C, name="fh_read_restart")
+    integer(c_int), intent(out) :: nt0, ions
+    real(c_double), intent(out) :: restart_time
+    integer :: ios, n0, ni
+    real(dp) :: rt
+    if (allocated(st%X)) deallocate(st%X, st%Xi, st%V, st%mass, st%xrand, st%atype)
+    call read_restart(s, st, n0, rt, ni, ios)
+    nt0 = n0
+    ions = ni
+    restart_time = rt
+    fh_read_restart = ios
+  end function fh_read_restart
+
   subroutine fh_writeatoms(lt, time, temp) bind(C, name="fh_writeatoms")
     integer(c_int64_t), value :: lt
     real(c_double), value :: time, temp

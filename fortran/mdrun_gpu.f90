@@ -26,7 +26,8 @@ program mdrun_gpu
   type(fib_step_ctl) :: ctl
   type(fib_step_report) :: rep
   type(c_ptr) :: h
-  integer :: ios, k, nargs, prec, impact, vals(8)
+  integer :: ios, k, nargs, prec, impact, vals(8), nt0, ions
+  real(dp) :: restart_time
   integer(c_int32_t) :: rc
   integer(c_int64_t) :: lt
   real(dp) :: time, temp, xion(3), vion(3)
@@ -63,8 +64,9 @@ program mdrun_gpu
      write(0, *) 'mdrun_gpu: cannot read siga.in'
      stop 1
   end if
-  if (s%amorcrys /= 0 .or. s%whichic /= 4) then
-     write(0, *) 'mdrun_gpu: only from-scratch crystalline runs (amorcrys 0, whichic 4)'
+  if (s%amorcrys /= 0 .or. (s%whichic /= 4 .and. s%whichic /= 0)) then
+     write(0, *) 'mdrun_gpu: crystalline runs only (amorcrys 0), from scratch (whichic 4) ', &
+          'or from a restart record (whichic 0)'
      stop 1
   end if
   if (.not. have_seed) then
@@ -76,9 +78,17 @@ program mdrun_gpu
   ! ---- initran, initdata (main.f90:40-48) -----------------------------------
   call initranions(s)
   print *, 'initran done'
-  call initdata(s, st, ios)
+  ! initdata / ic (data.f90:48-93): from scratch, or from a restart record
+  nt0 = 0
+  ions = 0
+  restart_time = 0
+  if (s%whichic == 4) then
+     call initdata(s, st, ios)
+  else
+     call read_restart(s, st, nt0, restart_time, ions, ios)
+  end if
   if (ios /= 0) then
-     write(0, *) 'mdrun_gpu: cannot read si.dat / fwhm.dat back'
+     write(0, *) 'mdrun_gpu: cannot read si.dat / fwhm.dat / the restart record'
      stop 1
   end if
   print *, 'initdata done'
@@ -125,28 +135,30 @@ program mdrun_gpu
   write(*, *) '--------------------------------'
 
   ! ---- tint (timeint.f90:36-120) --------------------------------------------
-  lt = 0
-  time = 0
+  lt = nt0
+  time = restart_time
   temp = host_temperature(st)
-  call writeatoms(s, st, lt, time, temp)
+  if (lt == 0) call writeatoms(s, st, lt, time, temp)
   call check(fib_tint_init(h, lt, time), 'fib_tint_init')
 
-  print *, 'Initializing target temperature. . .'
-  call check(fib_ti_begin(h, FIB_TI_EQUILIBRATE, 0), 'fib_ti_begin')
-  ctl%whichctrl = FIB_TI_EQUILIBRATE
   ctl%max_steps = 0
-  ctl%stop_on_output = 0
   ctl%reserved = 0
-  do
-     call check(fib_step(h, ctl, rep), 'fib_step')
-     if (rep%ended /= 0) exit
-  end do
-  temp = rep%temp
-  lt = rep%lt
-  time = rep%time
-  print *, 'Equalized to temperature ', temp
+  if (nt0 == 0) then
+     print *, 'Initializing target temperature. . .'
+     call check(fib_ti_begin(h, FIB_TI_EQUILIBRATE, 0), 'fib_ti_begin')
+     ctl%whichctrl = FIB_TI_EQUILIBRATE
+     ctl%stop_on_output = 0
+     do
+        call check(fib_step(h, ctl, rep), 'fib_step')
+        if (rep%ended /= 0) exit
+     end do
+     temp = rep%temp
+     lt = rep%lt
+     time = rep%time
+     print *, 'Equalized to temperature ', temp
+  end if
 
-  do impact = 1, st%nlj
+  do impact = ions + 1, st%nlj
      call check(fib_ti_begin(h, FIB_TI_IMPACT, impact), 'fib_ti_begin')
      call knock_draw(s, st, xion, vion)
      print *, 'KNOCK    --=>*<=--    ion ID: ', st%nsg + impact
@@ -158,6 +170,13 @@ program mdrun_gpu
         lt = rep%lt
         time = rep%time
         temp = rep%temp
+        ! iostuff (out.f90:129-150): restart record, then the frame
+        if (rep%output_due /= 0 .and. s%res_out > 0) then
+           if (mod(lt, int(s%res_out, c_int64_t)) == 0) then
+              call fetch()
+              call write_restart(s, st, int(lt), restart_time, impact)
+           end if
+        end if
         if (rep%atoms_due /= 0 .and. s%atm_out_i > 0) then
            call fetch()
            call writeatoms(s, st, lt, time, temp)

@@ -8,7 +8,10 @@ entry points in libfibhost.so:
   are pinned in test_oracle_init_integrate.py);
 - data/mdrun2.xyz and data/3dmol2.xyz: the reference's format strings
   (out.f90:74-96), checked field by field against independently formatted
-  expectations.
+  expectations;
+- restart records (out.f90:34-59): the unformatted sequential layout of the
+  reference's three records, parsed byte by byte, read back as a whichic-0
+  start (data.f90:71-93), and the ran1 sequence carried on across it.
 """
 import ctypes as C
 import os
@@ -49,6 +52,9 @@ def fh():
     L.fh_writeatoms.argtypes = [C.c_int64, C.c_double, C.c_double]
     L.fh_open_outputs.argtypes = [C.c_int]
     L.fh_close_outputs.argtypes = [C.c_int]
+    L.fh_write_restart.argtypes = [C.c_int, C.c_double, C.c_int]
+    L.fh_read_restart.argtypes = [C.POINTER(C.c_int), C.POINTER(C.c_double),
+                                  C.POINTER(C.c_int)]
     return L
 
 
@@ -168,3 +174,66 @@ def test_writers_layout(fh, tmp_path, oracle_lib):
     for i in (0, n - 1):
         lab = "Si" if at[i] == 1 else "Ga"
         assert snap[2 + i] == f"{lab:>8s}" + "".join(f"{x * 1e10:10.4f}" for x in X[i])
+
+
+def _records(path):
+    """The records of an unformatted sequential file: 4-byte length markers
+    before and after each record (the layout of gfortran and flang)."""
+    b = open(path, "rb").read()
+    out, p = [], 0
+    while p < len(b):
+        n = int(np.frombuffer(b[p:p + 4], np.int32)[0])
+        out.append(b[p + 4:p + 4 + n])
+        assert int(np.frombuffer(b[p + 4 + n:p + 8 + n], np.int32)[0]) == n
+        p += 8 + n
+    return out
+
+
+def test_restart_records(fh, tmp_path):
+    n, nsg, lb, X, V, at, xs, Xi, xr = _setup(fh, tmp_path)
+    rng = np.random.default_rng(3)
+    X2 = np.asfortranarray(X + rng.normal(0, 1e-11, X.shape))
+    V2 = np.asfortranarray(rng.normal(0, 500.0, V.shape))
+    fh.fh_set_xv(X2.ctypes.data_as(C.c_void_p), V2.ctypes.data_as(C.c_void_p))
+    for _ in range(7):
+        fh.fh_ran1()
+    cwd = os.getcwd()
+    os.chdir(tmp_path)
+    try:
+        os.makedirs("D", exist_ok=True)
+        fh.fh_write_restart(1234, 2.5e-13, 1)
+        after = [fh.fh_ran1() for _ in range(5)]
+        for name in ("D/restart.in_old", "D/restart.in000001_000001234"):
+            r = _records(name)
+            assert len(r) == 3
+            assert np.frombuffer(r[0][:4], np.int32)[0] == 1234
+            assert np.frombuffer(r[0][4:], np.float64)[0] == 2.5e-13
+            assert len(r[1]) == n * (7 * 8 + 4)
+            body = np.frombuffer(r[1][:n * 56], np.float64)
+            np.testing.assert_array_equal(body[:3 * n].reshape(3, n).T, X2)
+            np.testing.assert_array_equal(body[3 * n:6 * n].reshape(3, n).T, V2)
+            m = body[6 * n:]
+            assert np.all(m[:nsg] == 46.637063e-27) and np.all(m[nsg:] == 1.157777417e-25)
+            np.testing.assert_array_equal(np.frombuffer(r[1][n * 56:], np.int32), at)
+            assert np.frombuffer(r[2], np.int32)[0] == 1
+        # a whichic-0 start from the record: state back, clock and counters
+        # as recorded, unfired ions at rest, ion sites at their positions, and
+        # the random sequence carried on
+        nt0, rt, ions = C.c_int(), C.c_double(), C.c_int()
+        assert fh.fh_read_restart(C.byref(nt0), C.byref(rt), C.byref(ions)) == 0
+    finally:
+        os.chdir(cwd)
+    assert (nt0.value, rt.value, ions.value) == (1234, 2.5e-13, 1)
+    X3 = np.zeros((n, 3), order="F"); V3 = np.zeros((n, 3), order="F")
+    Xi3 = np.zeros((n, 3), order="F")
+    at3 = np.zeros(n, np.int32); xs3 = np.zeros(n, np.int32)
+    xr3 = np.zeros((EXPECTED["nrand"], 2), order="F")
+    fh.fh_get_state(*(a.ctypes.data_as(C.c_void_p) for a in (X3, V3, at3, xs3, Xi3, xr3)))
+    np.testing.assert_array_equal(X3, X2)
+    np.testing.assert_array_equal(V3[:nsg + 1], V2[:nsg + 1])
+    assert np.all(V3[nsg + 1:] == 0)
+    np.testing.assert_array_equal(Xi3[nsg:], X2[nsg:])
+    np.testing.assert_array_equal(Xi3[:nsg], Xi[:nsg])
+    np.testing.assert_array_equal(at3, at)
+    np.testing.assert_array_equal(xr3, xr)
+    assert [fh.fh_ran1() for _ in range(5)] == after

@@ -93,3 +93,43 @@ def test_fortran_driver_matches_oracle(tmp_path, oracle_lib):
             assert abs(got_ke - ke) <= 1e-3 * abs(ke) + 1e-30, (i, row, ke)
     snap = open(tmp_path / "data" / "3dmol2.xyz").read().split("\n")
     assert int(snap[0]) == n
+
+
+def test_fortran_driver_restart(tmp_path):
+    """Restart records on the res_out cadence (out.f90:129-150) and a
+    whichic-0 start from one (data.f90:71-93, timeint.f90:51-111): the
+    restarted run skips equilibration, resumes lt and the impact count from
+    the record, and fires only the ions not fired yet."""
+    assert os.path.exists(EXE), "build fortran/mdrun_gpu first"
+    text = open(SIGA).read()
+    assert "-5000    \\ res_out" in text
+    text = text.replace("-5000    \\ res_out", "100    \\ res_out")
+    (tmp_path / "siga.in").write_text(text)
+    for d in ("data", "D", "F"):
+        os.makedirs(tmp_path / d)
+    print(f"mdrun_gpu --seed {SEED} (res_out 100) in {tmp_path}", flush=True)
+    r = subprocess.run([EXE, "--seed", str(SEED)], cwd=tmp_path, capture_output=True, text=True,
+                       timeout=300)
+    assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
+    names = sorted(os.listdir(tmp_path / "D"))
+    assert "restart.in_old" in names and "restart.rng_old" in names
+    first = [f for f in names if f.startswith("restart.in000001_")]
+    assert first, names
+    lt_rec = int(first[0].split("_")[1])
+    assert lt_rec % 100 == 0
+    text = text.replace("4    \\ whichic", "0    \\ whichic").replace(
+        "0    \\ startimpact", "1    \\ startimpact")
+    (tmp_path / "siga.in").write_text(text)
+    print("mdrun_gpu restarted from impact 1", flush=True)
+    r2 = subprocess.run([EXE, "--seed", str(SEED)], cwd=tmp_path, capture_output=True, text=True,
+                        timeout=300)
+    assert r2.returncode == 0, r2.stdout[-2000:] + r2.stderr[-2000:]
+    out = r2.stdout
+    nsg = 8 * 3 * 3 * 2
+    assert "Initializing target temperature" not in out
+    knocks = [ln for ln in out.splitlines() if "KNOCK" in ln]
+    assert len(knocks) == 1 and knocks[0].split()[-1] == str(nsg + 2), knocks
+    # progress lines of the restarted run continue the step counter
+    lts = [int(ln.split()[0]) for ln in out.splitlines()
+           if len(ln.split()) == 3 and ln.split()[0].isdigit()]
+    assert lts and min(lts) > lt_rec, (lts[:3], lt_rec)
