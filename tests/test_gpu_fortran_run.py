@@ -103,11 +103,11 @@ def test_fortran_driver_restart(tmp_path):
     assert os.path.exists(EXE), "build fortran/mdrun_gpu first"
     text = open(SIGA).read()
     assert "-5000    \\ res_out" in text
-    text = text.replace("-5000    \\ res_out", "100    \\ res_out")
+    text = text.replace("-5000    \\ res_out", "25    \\ res_out")
     (tmp_path / "siga.in").write_text(text)
     for d in ("data", "D", "F"):
         os.makedirs(tmp_path / d)
-    print(f"mdrun_gpu --seed {SEED} (res_out 100) in {tmp_path}", flush=True)
+    print(f"mdrun_gpu --seed {SEED} (res_out 25) in {tmp_path}", flush=True)
     r = subprocess.run([EXE, "--seed", str(SEED)], cwd=tmp_path, capture_output=True, text=True,
                        timeout=300)
     assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
@@ -116,7 +116,7 @@ def test_fortran_driver_restart(tmp_path):
     first = [f for f in names if f.startswith("restart.in000001_")]
     assert first, names
     lt_rec = int(first[0].split("_")[1])
-    assert lt_rec % 100 == 0
+    assert lt_rec % 25 == 0
     text = text.replace("4    \\ whichic", "0    \\ whichic").replace(
         "0    \\ startimpact", "1    \\ startimpact")
     (tmp_path / "siga.in").write_text(text)
