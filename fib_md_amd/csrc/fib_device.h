@@ -87,6 +87,10 @@ struct DCtl {
   double maxcnt_ion;
   double eq_sum;        // sum m v^2 of the rescaled V (equilibration)
   double maxdisp;       // largest displacement since the last list build (m)
+  double maxdisp2;      // second largest (MD-step mode; equal to maxdisp otherwise):
+                        // a pair unlisted at the build can be within the cutoff
+                        // only when maxdisp + maxdisp2 exceeds the skin
+  double nmaxdisp;      // atoms at maxdisp (MD-step mode)
   double builds;        // list builds performed (count)
   // MD-step mode (per-instance adaptive dt)
   double cfg_dt;        // timestep of the next step (s)

@@ -1344,7 +1344,7 @@ __global__ __launch_bounds__(FIB_BLOCK) void k_extra(
   double v[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
   const int op[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
   // block-uniform: one instance per block
-  const bool active = 2.0 * ctl[kin].maxdisp > fc.skin;
+  const bool active = ctl[kin].maxdisp + ctl[kin].maxdisp2 > fc.skin;
   if (!active) {
     // nothing to add: clear the instance's flag so the gather skips EX
     if (threadIdx.x == 0 && lb * FIB_BLOCK == in.atom_off) ctl[kin].extra_any = 0;
@@ -1494,6 +1494,8 @@ __global__ void k_mark_build(int NA, int ninst, int Wsw, int Wion, const double4
   }
   if (i < ninst && !gated_out(gate, i)) {
     ctl[i].maxdisp = 0.0;
+    ctl[i].maxdisp2 = 0.0;
+    ctl[i].nmaxdisp = 0.0;
     ctl[i].extra_any = 0;
     // a list longer than its width was cut: sticky, checked by the host
     if (ctl[i].maxcnt_sw > Wsw || ctl[i].maxcnt_ion > Wion) ctl[i].overflow = 1;
@@ -1513,7 +1515,9 @@ __global__ void k_clear_gate(int *__restrict__ gate, int ninst) {
 // step is a fixed sequence of launches that can be captured in a graph:
 //   k_md_pos     Berendsen shells, half kick, drift, pinned atoms, removal
 //                above the surface, displacement since the list build;
-//                k_md_decide sets the rebuild gate (2 * max displacement > skin)
+//                k_disp2      the second largest displacement
+//                k_md_decide sets the rebuild gate (two largest displacements
+//                add up to more than the skin)
 //   gated build  bin / sort in place / k_nbuild (skipped on the
 //                device when the gate is 0)
 //   forces       k_legs, k_tri, k_extra, k_force3 MODE 2 (second half kick,
@@ -1571,17 +1575,39 @@ __global__ __launch_bounds__(FIB_BLOCK) void k_md_pos(double4 *__restrict__ X,
     for (int c = 0; c < 3; ++c) V[(size_t)c * NA + i] = v[c];
     double e0, e1, e2;
     min_image_fast(x, Xb[i], in, e0, e1, e2);
-    const double dd = sqrt(e0 * e0 + e1 * e1 + e2 * e2);
-    disp[i] = (float)dd;
-    red[0] = dd;
+    const float dd = (float)sqrt(e0 * e0 + e1 * e1 + e2 * e2);
+    // the reduction takes the stored value, so that k_disp2 can find the
+    // atoms at the maximum by comparison
+    disp[i] = dd;
+    red[0] = (double)dd;
+  } else {
+    disp[i] = 0.0f;
   }
   const int op[1] = {1};
   block_partials<1, FIB_BLOCK>(red, op, lb, pstride, partials);
 }
 
-// Rebuild gates of the step: each instance whose 2 * max displacement
-// exceeds the skin, and whether there is any.
-__global__ void k_md_decide(int ninst, const DInst *__restrict__ I, const DCtl *__restrict__ ctl,
+// Second largest displacement since the build per instance (MD-step mode):
+// the largest below the maximum, and how many atoms sit at the maximum.
+__global__ __launch_bounds__(FIB_BLOCK) void k_disp2(const double4 *__restrict__ X, int NA,
+                                                     const float *__restrict__ disp,
+                                                     const DCtl *__restrict__ ctl, int pstride,
+                                                     double *__restrict__ partials) {
+  const int lb = xcd_remap(blockIdx.x, gridDim.x);
+  const int i = lb * FIB_BLOCK + threadIdx.x;
+  // every block lies inside one instance (its first atom names it)
+  const int kin = __builtin_amdgcn_readfirstlane(inst_of(meta_of(X[lb * FIB_BLOCK])));
+  const double m1 = ctl[kin].maxdisp;
+  const double d = (double)disp[i];
+  double v[2] = {d < m1 ? d : 0.0, d == m1 ? 1.0 : 0.0};
+  const int op[2] = {1, 0};
+  block_partials<2, FIB_BLOCK>(v, op, lb, pstride, partials);
+}
+
+// Rebuild gates of the step: each instance whose two largest displacements
+// since the build add up to more than the skin (only then can a pair the
+// build left out have come within the cutoff), and whether there is any.
+__global__ void k_md_decide(int ninst, const DInst *__restrict__ I, DCtl *__restrict__ ctl,
                             double skin, int *__restrict__ gate) {
   // one wave; behind the gates, the prefix over instances of the real cells
   // of the gated ones (the cell numbering of a gated k_nbuild)
@@ -1592,7 +1618,10 @@ __global__ void k_md_decide(int ninst, const DInst *__restrict__ I, const DCtl *
     const int k = k0 + lane;
     int g = 0, w = 0;
     if (k < ninst) {
-      g = (2.0 * ctl[k].maxdisp > skin) ? 1 : 0;
+      const DCtl &c = ctl[k];
+      const double m2 = c.nmaxdisp >= 2.0 ? c.maxdisp : c.maxdisp2;
+      g = (c.maxdisp + m2 > skin) ? 1 : 0;
+      ctl[k].maxdisp2 = m2;  // the effective second maximum, for k_extra
       gate[1 + k] = g;
       if (g) w = I[k].nc[0] * I[k].nc[1] * I[k].nc[2];
     }
@@ -1848,7 +1877,9 @@ __global__ __launch_bounds__(FIB_BLOCK) void k_pos(double4 *__restrict__ X,
   const int kin = __builtin_amdgcn_readfirstlane(inst_of(mi));
   const DInst &in = I[kin];
   const bool live = !(mi & (META_PAD | META_DEAD));
-  double red[3] = {0.0, -1.0, 0.0};
+  // slots: thermostat sum, max KE, max displacement, and the displacement
+  // again as the second maximum (k_extra tests maxdisp + maxdisp2)
+  double red[4] = {0.0, -1.0, 0.0, 0.0};
   if (live) {
     const int t = type_of(mi);
     const double m = sa.mass[t - 1];
@@ -1915,14 +1946,15 @@ __global__ __launch_bounds__(FIB_BLOCK) void k_pos(double4 *__restrict__ X,
       const double dd = sqrt(e0 * e0 + e1 * e1 + e2 * e2);
       disp[i] = (float)dd;
       red[2] = dd;
+      red[3] = dd;
     }
     for (int c = 0; c < 3; ++c) {
       V[(size_t)c * NA + i] = v[c];
       F[(size_t)c * NA + i] = f[c];
     }
   }
-  const int op[3] = {0, 1, 1};
-  block_partials<3, FIB_BLOCK>(red, op, lb, pstride, partials);
+  const int op[4] = {0, 1, 1, 1};
+  block_partials<4, FIB_BLOCK>(red, op, lb, pstride, partials);
 }
 
 // Thermostat / temperature sums of the current state (same as k_force_b's
@@ -2285,9 +2317,9 @@ void launch_pos(const DevBufs &d, const StepArgs &sa, hipStream_t s) {
                        d.V[d.cur], d.F[d.cur], d.Xi[d.cur], sa, d.pstride, d.partials, d.ctl,
                        d.Xb, d.disp);
   }
-  const int op[3] = {0, 1, 1};
-  const int dst[3] = {DCTL_IDX(eq_sum), DCTL_IDX(kemax), DCTL_IDX(maxdisp)};
-  finalize(d, 3, FIB_BLOCK, op, dst, s);
+  const int op[4] = {0, 1, 1, 1};
+  const int dst[4] = {DCTL_IDX(eq_sum), DCTL_IDX(kemax), DCTL_IDX(maxdisp), DCTL_IDX(maxdisp2)};
+  finalize(d, 4, FIB_BLOCK, op, dst, s);
 }
 
 void launch_md_pos(const DevBufs &d, const StepArgs &sa, int *gate, hipStream_t s) {
@@ -2300,6 +2332,14 @@ void launch_md_pos(const DevBufs &d, const StepArgs &sa, int *gate, hipStream_t 
   const int op[1] = {1};
   const int dst[1] = {DCTL_IDX(maxdisp)};
   finalize(d, 1, FIB_BLOCK, op, dst, s);
+  {
+    KScope ks_(d, KT_POS);
+    hipLaunchKernelGGL(k_disp2, dim3(d.NA / FIB_BLOCK), dim3(FIB_BLOCK), 0, s, d.X[d.cur], d.NA,
+                       d.disp, d.ctl, d.pstride, d.partials);
+  }
+  const int op2[2] = {1, 0};
+  const int dst2[2] = {DCTL_IDX(maxdisp2), DCTL_IDX(nmaxdisp)};
+  finalize(d, 2, FIB_BLOCK, op2, dst2, s);
   hipLaunchKernelGGL(k_md_decide, dim3(1), dim3(FIB_WAVE), 0, s, d.ninst, d.I, d.ctl, sa.skin,
                      gate);
 }
