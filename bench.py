@@ -39,6 +39,7 @@ PMC_FILE = os.path.join(ROOT, "profiles", "pmc_traffic.json")
 
 
 COPY_KIB = 256 * 1024  # the plain copy of --profile mode: 256 MiB read, 256 MiB written
+EV = 1.602176634e-19  # J
 
 
 def short_name(n: str) -> str:
@@ -167,6 +168,7 @@ def md_rate(instances, precision, steps, warmup=5, use_graph=True, params=None):
     import torch
     g = _md_handle(instances, precision, use_graph, params)
     g.md_run(warmup)
+    b0 = [g.md_report(k)["builds"] for k in range(len(instances))]
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     g.md_run(steps)
@@ -175,8 +177,12 @@ def md_rate(instances, precision, steps, warmup=5, use_graph=True, params=None):
     n = sum(len(s["X"]) for s in instances)
     reps = [g.md_report(k) for k in range(len(instances))]
     g.close()
+    # list builds per instance during the timed steps (each instance counts
+    # its own gated rebuilds)
+    nb = [r["builds"] - b for r, b in zip(reps, b0)]
     return dict(atom_steps_per_s=n * steps / t, ms_per_step=t * 1e3 / steps, atoms=n,
-                builds=sum(r["builds"] for r in reps), removed=sum(r["removed"] for r in reps),
+                builds_per_instance_timed=float(np.mean(nb)),
+                removed=sum(r["removed"] for r in reps),
                 dt_min_s=min(r["dt"] for r in reps), sim_time_s=max(r["time"] for r in reps))
 
 
@@ -184,7 +190,12 @@ def dose_run(n_rep, ncell, impacts, steps_per_impact, precision):
     """Cumulative dose: `n_rep` replicas of ncell^3 cells, `impacts` sequential
     2 keV normal Ga impacts each (all replicas fire together), the Berendsen
     side and bottom 3 A shells at 300 K throughout, sputtered atoms removed,
-    Ga retained.  Sputter yield per replica after each impact, and the rate."""
+    Ga retained.  Sputter yield per replica after each impact, the FP64
+    energy ledger after each impact (kinetic + potential + energy the shells
+    took + energy removed atoms carried - energy the ions brought, minus its
+    value before the first impact: the integration drift), and the rate.
+    The timed region holds the firing and the steps; reports and the
+    ledger's energy evaluations lie outside it."""
     import torch
     from fib_md_amd import configs
     reps, shots = configs.dose_replicas(n_rep, ncell, impacts)
@@ -194,18 +205,23 @@ def dose_run(n_rep, ncell, impacts, steps_per_impact, precision):
         g.set_state(s["X"], V=s["V"], atype=s["atype"], xsides=s["xsides"], Xi=s["X"], inst=k)
     g.md_begin([s["surface_z"] for s in reps], [s["bottom_z"] for s in reps],
                shell_width=3e-10, ttar=300.0, tau=1e-13)
-    yields, ke = [], []
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
+    L0 = g.md_ledger()
+    yields, ke, drift, fired, bath = [], [], [], [], []
+    t = 0.0
     for m in range(impacts):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
         for k in range(n_rep):
             g.md_fire(k, m + 1, *shots[k][m])
         g.md_run(steps_per_impact)
+        torch.cuda.synchronize()
+        t += time.perf_counter() - t0
         rr = [g.md_report(k) for k in range(n_rep)]
         yields.append([r["removed"] for r in rr])
         ke.append(float(np.mean([r["ke"] for r in rr])))
-    torch.cuda.synchronize()
-    t = time.perf_counter() - t0
+        drift.append((g.md_ledger() - L0) / EV)
+        fired.append(float(np.mean([r["e_fired"] for r in rr])) / EV)
+        bath.append(float(np.mean([r["e_bath"] for r in rr])) / EV)
     g.close()
     n = sum(len(s["X"]) for s in reps)
     y = np.array(yields, float)
@@ -214,7 +230,10 @@ def dose_run(n_rep, ncell, impacts, steps_per_impact, precision):
                 ms_per_step=t * 1e3 / (impacts * steps_per_impact),
                 cumulative_yield_mean=y.mean(1).tolist(),
                 cumulative_yield_sem=(y.std(1, ddof=1) / np.sqrt(n_rep)).tolist(),
-                mean_ke_J=ke)
+                mean_ke_J=ke, ledger_drift_eV_mean=[float(d.mean()) for d in drift],
+                ledger_drift_eV_maxabs=[float(np.abs(d).max()) for d in drift],
+                ledger_drift_eV_per_replica=[d.tolist() for d in drift],
+                fired_eV_mean=fired, shells_took_eV_mean=bath)
 
 
 def cascade_stats(instances, steps):
@@ -268,7 +287,8 @@ def md_bench(args):
     x, v = configs.ion_launch(s["L"], s["surface_z"], 500.0, 0.0, 0.0, np.random.default_rng(1001))
     s.update(ion_x=x, ion_v=v)
     for prec in ("fp64", "mixed"):
-        r = md_rate([s], prec, args.md_steps)
+        # the whole impact: all 2000 steps of the configuration are timed
+        r = md_rate([s], prec, STEPS_PER_IMPACT)
         r["impacts_per_hour"] = 3600.0 / (r["ms_per_step"] * 1e-3 * STEPS_PER_IMPACT)
         out[f"single_impact_4097_{prec}"] = r
         log(f"  single impact {prec}: {r['ms_per_step']:.3f} ms/step, "
@@ -339,7 +359,14 @@ def md_bench(args):
             r = dose_run(16, 40, args.dose_impacts, args.dose_steps, prec)
             out["dose"][prec] = r
             log(f"  dose {prec}: {r['atoms']} atoms, {r['atom_steps_per_s']:.4g} atom-steps/s, "
-                f"cumulative yield {['%.2f' % y for y in r['cumulative_yield_mean']]}")
+                f"cumulative yield {['%.2f' % y for y in r['cumulative_yield_mean']]}, ledger drift "
+                f"{['%.3g' % d for d in r['ledger_drift_eV_mean']]} eV of "
+                f"{['%.0f' % e for e in r['fired_eV_mean']]} eV fired")
+        # energy drift of the mixed run against the FP64 run, replica by replica
+        dm = np.array(out["dose"]["mixed"]["ledger_drift_eV_per_replica"])
+        df = np.array(out["dose"]["fp64"]["ledger_drift_eV_per_replica"])
+        out["dose"]["ledger_drift_mixed_minus_fp64_eV"] = {
+            "mean": (dm - df).mean(1).tolist(), "maxabs": np.abs(dm - df).max(1).tolist()}
     return out
 
 
@@ -389,8 +416,9 @@ def main():
     ap.add_argument("--ensemble", type=int, default=64)
     ap.add_argument("--stats-steps", type=int, default=2000,
                     help="MD steps of the cascade-statistics run (0: skip)")
-    ap.add_argument("--big-steps", type=int, default=40,
-                    help="timed MD steps of the 2.3 M-atom 30 keV impact (0: skip)")
+    ap.add_argument("--big-steps", type=int, default=5000,
+                    help="timed MD steps of the 2.3 M-atom 30 keV impact (0: skip; "
+                         "the configuration's impact is 5000 steps)")
     ap.add_argument("--dose-impacts", type=int, default=2,
                     help="sequential impacts per replica of the shortened dose run (0: skip)")
     ap.add_argument("--dose-steps", type=int, default=1500,

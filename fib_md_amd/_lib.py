@@ -79,7 +79,8 @@ class fib_md_params(C.Structure):
 class fib_md_report(C.Structure):
     _fields_ = [("time", C.c_double), ("dt", C.c_double), ("ion_z", C.c_double),
                 ("ke", C.c_double), ("steps", C.c_int64), ("removed", C.c_int32),
-                ("ions_removed", C.c_int32), ("displaced", C.c_int32), ("builds", C.c_int32)]
+                ("ions_removed", C.c_int32), ("displaced", C.c_int32), ("builds", C.c_int32),
+                ("e_bath", C.c_double), ("e_removed", C.c_double), ("e_fired", C.c_double)]
 
 
 class fib_timing(C.Structure):
@@ -116,6 +117,7 @@ SIGNATURES = {
     "fib_md_fire": ([vp, i32, i32, vp, vp], i32),
     "fib_md_run": ([vp, i32], i32),
     "fib_md_get_report": ([vp, i32, P(fib_md_report)], i32),
+    "fib_md_energy": ([vp, vp], i32),
     "fib_synchronize": ([vp], i32),
 }
 

@@ -2,9 +2,9 @@
 
 The list radius factor `listfac` (rc = listfac * a * sigma) sets both the
 cell size of the list build and the skin rc - 0.999 a sigma that decides
-how often an instance rebuilds (2 x max displacement > skin).  A larger
-factor means fewer rebuilds but more candidates per build and longer lists
-for the pair pass.  The best value depends on the workload (instance sizes,
+how often an instance rebuilds (when the two largest displacements since
+the build add up to more than the skin).  A larger factor means fewer
+rebuilds but more candidates per build and longer lists for the pair pass.  The best value depends on the workload (instance sizes,
 ion energies, precision), so it is measured: each candidate runs a short
 stretch of the actual batch on the GPU and the fastest steps win.  Lists
 hold every pair within rc and rc lies beyond the SW cutoff, so the mixed

@@ -103,7 +103,13 @@ struct DCtl {
   double ion_z;         // height of the instance's last ion (m)
   double ke_shell;      // sum m v^2 over thermostat-shell atoms
   double n_shell;       // their count
-  double e_pot;         // potential energy at the last energy evaluation (J)
+  // energy ledger of MD-step mode (J, cumulative since fib_md_begin): the
+  // total energy plus what left minus what came in stays constant
+  double e_bath;        // taken out by the Berendsen shells, sum 1/2 m v^2 (1 - lambda^2)
+  double e_out;         // carried out by removed atoms, their 1/2 m v^2 at removal
+  double e_in;          // brought in by fired ions, 1/2 m v^2 at firing
+  double e_bath_step;   // the bath and removal terms of the last step (k_md_ctl
+  double e_out_step;    // adds them in)
   int32_t nonfinite;    // set by k_bin on a NaN/Inf position
   int32_t extra_any;    // k_extra found vertex terms outside a centre's list
   int32_t overflow;     // a neighbour list exceeded its width (sticky)

@@ -23,7 +23,7 @@ struct StepArgs {
   double remove_dz;          // atoms higher than surf_z + remove_dz are removed
   double shell_w;            // Berendsen shells: x/y sides and the bottom (0 = off)
   double disp_r;             // displaced-atom threshold
-  double skin;               // list rebuild when 2 * max displacement exceeds it
+  double skin;               // rebuild when the two largest displacements exceed it
 };
 
 // Per-kernel timing keys (index into fib_timing.ms).
@@ -130,6 +130,7 @@ void launch_export_vec(const DevBufs &d, int inst, const DInst &in, const double
                        double *out, hipStream_t s);
 void launch_export_scalar(const DevBufs &d, int inst, const DInst &in, const double *src,
                           double *out, hipStream_t s);
+// sa supplies the masses for the ledger's e_in
 void launch_set_atom(const DevBufs &d, int p, const double x[3], const double v[3],
-                     hipStream_t s);
+                     const StepArgs &sa, hipStream_t s);
 }  // namespace fibk

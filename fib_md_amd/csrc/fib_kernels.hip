@@ -1320,7 +1320,7 @@ __global__ __launch_bounds__(FIB_BLOCK) void k_tri(
 // a and c.  b keeps its own share and writes the forces on a and c into
 // EX[slot][b] by its list slots of a and c; a and c gather them like G.
 // Nothing can be missing unless two atoms moved together by more than the
-// skin, so the pass exits at once while 2 * max displacement <= skin.
+// skin, so the pass exits at once while maxdisp + maxdisp2 <= skin.
 template <bool MIXED, bool EN, bool VIR>
 __global__ __launch_bounds__(FIB_BLOCK) void k_extra(
     const double4 *__restrict__ X, const DInst *__restrict__ I, const FConst fc, int NA, int Wsw,
@@ -1544,15 +1544,18 @@ __global__ __launch_bounds__(FIB_BLOCK) void k_md_pos(double4 *__restrict__ X,
   const DInst &in = I[kin];
   const bool live = !(mi & (META_PAD | META_DEAD));
   const double dt = ctl[kin].cfg_dt, lam = ctl[kin].cfg_lambda;
-  double red[1] = {0.0};
+  // max displacement, energy taken by the shells, energy of removed atoms
+  double red[3] = {0.0, 0.0, 0.0};
   if (live) {
     const int t = type_of(mi);
     const double m = sa.mass[t - 1];
     double v[3], xx[3] = {x.x, x.y, x.z};
     for (int c = 0; c < 3; ++c) v[c] = V[(size_t)c * NA + i];
     if (sa.shell_w > 0.0 && t < 3 && !(mi & META_FROZEN) &&
-        in_shell(xx[0], xx[1], xx[2], in, sa.shell_w))
+        in_shell(xx[0], xx[1], xx[2], in, sa.shell_w)) {
+      red[1] = 0.5 * m * (v[0] * v[0] + v[1] * v[1] + v[2] * v[2]) * (1.0 - lam * lam);
       for (int c = 0; c < 3; ++c) v[c] *= lam;
+    }
     for (int c = 0; c < 3; ++c) {
       v[c] += 0.5 * F[(size_t)c * NA + i] * dt / m;
       xx[c] += v[c] * dt;
@@ -1565,8 +1568,10 @@ __global__ __launch_bounds__(FIB_BLOCK) void k_md_pos(double4 *__restrict__ X,
     }
     for (int c = 0; c < 3; ++c) xx[c] = fmod(xx[c] + 2. * in.lb[c], in.lb[c]);
     if (xx[2] - in.surf_z > sa.remove_dz && xx[2] < in.lb[2] - sa.remove_dz) {
-      // sputtered (or backscattered): out of the simulation from now on
+      // sputtered (or backscattered): out of the simulation from now on,
+      // with its kinetic energy (beyond every cutoff, it has no potential)
       mi |= META_DEAD;
+      red[2] = 0.5 * m * (v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
       for (int c = 0; c < 3; ++c) v[c] = 0.0;
       x.w = __longlong_as_double((long long)((wbits(x) & ~0xffffffffull) | mi));
     }
@@ -1583,8 +1588,8 @@ __global__ __launch_bounds__(FIB_BLOCK) void k_md_pos(double4 *__restrict__ X,
   } else {
     disp[i] = 0.0f;
   }
-  const int op[1] = {1};
-  block_partials<1, FIB_BLOCK>(red, op, lb, pstride, partials);
+  const int op[3] = {1, 0, 0};
+  block_partials<3, FIB_BLOCK>(red, op, lb, pstride, partials);
 }
 
 // Second largest displacement since the build per instance (MD-step mode):
@@ -1648,6 +1653,8 @@ __global__ void k_md_ctl(int ninst, DCtl *__restrict__ ctl, StepArgs sa) {
   DCtl &c = ctl[k];
   c.cfg_time += c.cfg_dt;
   c.cfg_steps += 1.0;
+  c.e_bath += c.e_bath_step;
+  c.e_out += c.e_out_step;
   const double dt = c.vmax > 0.0 ? fmin(sa.dt_max, sa.dx_max / c.vmax) : sa.dt_max;
   c.cfg_dt = dt;
   double lam = 1.0;
@@ -1669,6 +1676,7 @@ __global__ void k_md_init(int ninst, DCtl *__restrict__ ctl, StepArgs sa, int re
     c.cfg_time = 0.0;
     c.cfg_steps = 0.0;
     c.cfg_lambda = 1.0;
+    c.e_bath = c.e_out = c.e_in = 0.0;
   }
 }
 
@@ -2067,10 +2075,17 @@ __global__ void k_export_scalar(DInst in, const double4 *__restrict__ X,
   out[orig_of(x)] = src[p];
 }
 
+// Places atom p (a fired ion) and books its change of kinetic energy as
+// energy brought into its instance.
 __global__ void k_set_atom(int NA, double4 *__restrict__ X, double *__restrict__ V, int p,
-                           double x0, double x1, double x2, double v0, double v1, double v2) {
+                           double x0, double x1, double x2, double v0, double v1, double v2,
+                           StepArgs sa, DCtl *__restrict__ ctl) {
   if (threadIdx.x != 0 || blockIdx.x != 0) return;
   double4 x = X[p];
+  const uint32_t m = meta_of(x);
+  const double w0 = V[p], w1 = V[(size_t)NA + p], w2 = V[2 * (size_t)NA + p];
+  ctl[inst_of(m)].e_in += 0.5 * sa.mass[type_of(m) - 1] *
+                          ((v0 * v0 + v1 * v1 + v2 * v2) - (w0 * w0 + w1 * w1 + w2 * w2));
   x.x = x0; x.y = x1; x.z = x2;
   X[p] = x;
   V[p] = v0; V[(size_t)NA + p] = v1; V[2 * (size_t)NA + p] = v2;
@@ -2329,9 +2344,9 @@ void launch_md_pos(const DevBufs &d, const StepArgs &sa, int *gate, hipStream_t 
                        d.NA, d.V[d.cur], d.F[d.cur], d.Xi[d.cur], sa, d.pstride, d.partials,
                        d.ctl, d.Xb, d.disp);
   }
-  const int op[1] = {1};
-  const int dst[1] = {DCTL_IDX(maxdisp)};
-  finalize(d, 1, FIB_BLOCK, op, dst, s);
+  const int op[3] = {1, 0, 0};
+  const int dst[3] = {DCTL_IDX(maxdisp), DCTL_IDX(e_bath_step), DCTL_IDX(e_out_step)};
+  finalize(d, 3, FIB_BLOCK, op, dst, s);
   {
     KScope ks_(d, KT_POS);
     hipLaunchKernelGGL(k_disp2, dim3(d.NA / FIB_BLOCK), dim3(FIB_BLOCK), 0, s, d.X[d.cur], d.NA,
@@ -2424,11 +2439,11 @@ void launch_export_scalar(const DevBufs &d, int inst, const DInst &in, const dou
 }
 
 void launch_set_atom(const DevBufs &d, int p, const double x[3], const double v[3],
-                     hipStream_t s) {
+                     const StepArgs &sa, hipStream_t s) {
   {
     KScope ks_(d, KT_IO);
     hipLaunchKernelGGL(k_set_atom, dim3(1), dim3(FIB_WAVE), 0, s, d.NA, d.X[d.cur], d.V[d.cur], p,
-                       x[0], x[1], x[2], v[0], v[1], v[2]);
+                       x[0], x[1], x[2], v[0], v[1], v[2], sa, d.ctl);
   }
 }
 

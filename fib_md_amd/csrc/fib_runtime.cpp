@@ -860,7 +860,7 @@ extern "C" int32_t fib_knock(fib_handle *h, int32_t knc, const double xion[3],
   int p = -1;
   HIPCHK(h, hipMemcpy(&p, h->d.inv + in.atom_off + in.nsg + knc - 1, sizeof(int),
                       hipMemcpyDeviceToHost));
-  fibk::launch_set_atom(h->d, p, xion, vion, h->stream);
+  fibk::launch_set_atom(h->d, p, xion, vion, h->base, h->stream);
   HIPCHK(h, hipStreamSynchronize(h->stream));
   return FIB_OK;
 }
@@ -1030,7 +1030,7 @@ extern "C" int32_t fib_md_fire(fib_handle *h, int32_t inst, int32_t ion, const d
   int p = -1;
   HIPCHK(h, hipMemcpy(&p, h->d.inv + in.atom_off + in.nsg + ion - 1, sizeof(int),
                       hipMemcpyDeviceToHost));
-  fibk::launch_set_atom(h->d, p, x, v, h->stream);
+  fibk::launch_set_atom(h->d, p, x, v, h->md_sa, h->stream);
   in.impact = ion;
   HIPCHK(h, hipMemcpyAsync(&h->d.I[inst], &in, sizeof(DInst), hipMemcpyHostToDevice, h->stream));
   // the ion jumped: fresh lists, forces and dt
@@ -1091,6 +1091,27 @@ extern "C" int32_t fib_md_get_report(fib_handle *h, int32_t inst, fib_md_report 
   r->ions_removed = (int32_t)c.n_ga_removed;
   r->displaced = (int32_t)c.n_displaced;
   r->builds = (int32_t)c.builds;
+  r->e_bath = c.e_bath;
+  r->e_removed = c.e_out;
+  r->e_fired = c.e_in;
+  return FIB_OK;
+}
+
+extern "C" int32_t fib_md_energy(fib_handle *h, double *epot) {
+  if (!h || !epot) return FIB_ERR_ARG;
+  if (!h->md_on) return fail(h, FIB_ERR_STATE, "fib_md_energy: call fib_md_begin first");
+  // an evaluation with energies into the output buffer: the step's own
+  // forces (F of the current state) stay as they are
+  int32_t r = run_force(h, FIB_EVAL_FORCE | FIB_EVAL_ENERGY, 0, h->md_sa, h->d.Fout);
+  if (r != FIB_OK) return r;
+  HIPCHK(h, hipGetLastError());
+  std::vector<DCtl> c(h->inst.size());
+  HIPCHK(h, hipMemcpyAsync(c.data(), h->d.ctl, sizeof(DCtl) * c.size(), hipMemcpyDeviceToHost,
+                           h->stream));
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  for (size_t k = 0; k < c.size(); ++k) epot[k] = c[k].utot;
+  h->have_force = true;
+  h->last_flags = FIB_EVAL_FORCE | FIB_EVAL_ENERGY;
   return FIB_OK;
 }
 

@@ -249,6 +249,23 @@ class FibGPU:
         self._chk("fib_md_get_report", self.L.fib_md_get_report(self.h, int(inst), C.byref(r)))
         return {k: getattr(r, k) for k, _ in L.fib_md_report._fields_}
 
+    def md_energy(self):
+        """Potential energy (J) of each instance's current state."""
+        e = np.zeros(len(self.inst), dtype=np.float64)
+        self._chk("fib_md_energy", self.L.fib_md_energy(self.h, _p(e)))
+        return e
+
+    def md_ledger(self):
+        """Energy ledger per instance (J): kinetic + potential energy plus
+        what the shells took and removed atoms carried out, minus what fired
+        ions brought in.  Constant up to the integration error."""
+        ep = self.md_energy()
+        out = []
+        for k in range(len(self.inst)):
+            r = self.md_report(k)
+            out.append(r["ke"] + ep[k] + r["e_bath"] + r["e_removed"] - r["e_fired"])
+        return np.array(out)
+
     def synchronize(self):
         self._chk("fib_synchronize", self.L.fib_synchronize(self.h))
 

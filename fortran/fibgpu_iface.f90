@@ -70,6 +70,7 @@ module fibgpu
      real(c_double) :: time, dt, ion_z, ke
      integer(c_int64_t) :: steps
      integer(c_int32_t) :: removed, ions_removed, displaced, builds
+     real(c_double) :: e_bath, e_removed, e_fired
   end type fib_md_report
 
   interface
@@ -251,6 +252,12 @@ module fibgpu
        integer(c_int32_t), value :: inst
        type(fib_md_report), intent(out) :: r
      end function fib_md_get_report
+
+     integer(c_int32_t) function fib_md_energy(h, epot) bind(C, name="fib_md_energy")
+       import :: c_int32_t, c_ptr, c_double
+       type(c_ptr), value :: h
+       real(c_double), intent(out) :: epot(*)
+     end function fib_md_energy
   end interface
 
 contains

@@ -203,7 +203,8 @@ int32_t fib_temperature(fib_handle *h, int32_t inst, double *temp);
 
 /* ---- MD-step mode: impacts as the benchmark configurations define them ---
  * Velocity Verlet with a per-instance timestep dt = min(dt_max, dx_max /
- * |v|max), lists rebuilt when an atom has moved half the list skin, atoms
+ * |v|max), lists rebuilt when the two largest displacements since the
+ * build add up to more than the list skin, atoms
  * pinned through xsides (e.g. the bottom layers), atoms leaving the slab top
  * by more than remove_height taken out (sputtered), optional Berendsen
  * shells on the x/y sides and the bottom.  Every instance of the handle
@@ -243,8 +244,16 @@ typedef struct fib_md_report {
   int32_t ions_removed;   /* ions removed (backscattered)                       */
   int32_t displaced;      /* SW atoms farther than displaced_r from their site  */
   int32_t builds;         /* list builds so far                                 */
+  /* energy ledger since fib_md_begin (J): ke + potential + e_bath + e_removed
+   * - e_fired stays at its starting value up to the integration error */
+  double e_bath;          /* taken out by the Berendsen shells                  */
+  double e_removed;       /* kinetic energy of the atoms removed                */
+  double e_fired;         /* kinetic energy given to fired ions                 */
 } fib_md_report;
 int32_t fib_md_get_report(fib_handle *h, int32_t inst, fib_md_report *r);
+/* Potential energy (J) of every instance's current state, FP64 sums of the
+ * handle's precision: epot[k] for instance k.  The step's forces are kept. */
+int32_t fib_md_energy(fib_handle *h, double *epot);
 
 /* Device-side kernel timers (HIP events on the handle's stream around each
  * launch), accumulated since fib_timing_enable(h, mask); for benchmarks.

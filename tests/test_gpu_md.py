@@ -1,7 +1,8 @@
 """MD-step mode (the benchmark configurations' integrator): per-instance
-adaptive timestep dt = min(dt_max, dx_max / |v|max), half-skin list rebuilds
-gated on the device, pinned atoms, removal above the surface, batched
-instances, HIP-graph replay.
+adaptive timestep dt = min(dt_max, dx_max / |v|max), list rebuilds gated on
+the device (when the two largest displacements since the build exceed the
+skin), pinned atoms, removal above the surface, Berendsen shells, batched
+instances, HIP-graph replay, the energy ledger.
 
 Pins (no reference counterpart exists for this mode):
 - free flight of isolated atoms: x_n = x_0 + n dt v with the closed-form dt,
@@ -9,6 +10,8 @@ Pins (no reference counterpart exists for this mode):
 - removal of an atom crossing surface + 10 A at the step the closed form
   gives; pinned atoms stay at their sites at rest;
 - energy conservation (NVE, FP64) of a thermal slab with a live ion;
+- the energy ledger (kinetic + potential + shell and removal outflows -
+  ion inflow) stays constant with the shells and removal active;
 - batching: every instance of a batch follows bit for bit the trajectory it
   has alone (FP64 and mixed);
 - a captured step replayed as a graph equals the eager step, bit for bit.
@@ -158,3 +161,52 @@ def test_md_energy_conservation_fp64():
     r = g.md_report(0)
     assert r["removed"] == 0 and r["ions_removed"] == 0
     assert abs(e1 - e0) <= 2e-3 * ke0, (e0, e1, ke0)
+
+
+def _ledger_batch():
+    """Instance 0: a 2 keV ion into a slab whose side and bottom shells are
+    held at 300 K (the shells take energy out); instance 1: a surface atom
+    launched upward at 500 eV, which leaves the slab and is removed."""
+    ev = 1.602176634e-19
+    a = configs.si_slab(5, 11, nz=4)
+    x, v = configs.ion_launch(a["L"], a["surface_z"], 2000.0, 10.0, 30.0, np.random.default_rng(5))
+    a.update(ion_x=x, ion_v=v)
+    b = configs.si_slab(4, 12, nz=3)
+    S = b["X"][:b["nsg"]]
+    top = np.nonzero(S[:, 2] > b["surface_z"] - 0.1 * A)[0]
+    c = np.array([b["L"][0] / 2, b["L"][1] / 2])
+    k = top[np.argmin(((S[top, :2] - c) ** 2).sum(1))]
+    b["V"][k] = [0.0, 0.0, math.sqrt(2 * 500.0 * ev / configs.MASS_SI)]
+    return [a, b], ev
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_md_energy_ledger(precision):
+    """Kinetic + potential energy + what the shells took + what removed atoms
+    carried out - what the fired ion brought in stays constant.  Dropping or
+    flipping any term moves the ledger by tens to hundreds of eV; the
+    integration error over 800 steps of dx_max = 0.05 A is ~0.1 eV."""
+    inst, ev = _ledger_batch()
+    g = _gpu(inst, precision)
+    _load(g, inst)
+    g.md_begin([s["surface_z"] for s in inst], [s["bottom_z"] for s in inst],
+               shell_width=3 * A, ttar=300.0, tau=1e-14)
+    L0 = g.md_ledger()
+    g.md_fire(0, 1, inst[0]["ion_x"], inst[0]["ion_v"])
+    worst = np.zeros(2)
+    for _ in range(8):
+        g.md_run(100)
+        worst = np.maximum(worst, np.abs(g.md_ledger() - L0))
+    r0, r1 = g.md_report(0), g.md_report(1)
+    g.close()
+    print(f"\n{precision}: max |ledger drift| {worst / ev} eV; shells took {r0['e_bath'] / ev:.1f} eV,"
+          f" removed atoms carried {r0['e_removed'] / ev:.1f} / {r1['e_removed'] / ev:.1f} eV",
+          flush=True)
+    # the ion's speed comes from the reference's 1663.6334 sqrt(eV) m/s, which
+    # gives 2 keV to 1e-7 with the Ga mass of the parameter block
+    assert abs(r0["e_fired"] / ev - 2000.0) < 1e-6 * 2000.0 and r1["e_fired"] == 0.0
+    # the terms the ledger books are large next to its tolerance
+    assert r0["e_bath"] > 100.0 * ev
+    assert r1["removed"] == 1 and 480.0 * ev < r1["e_removed"] < 500.0 * ev
+    assert worst[0] < 5e-4 * r0["e_fired"]
+    assert worst[1] < 1e-3 * 500.0 * ev
