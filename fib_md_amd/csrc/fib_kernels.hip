@@ -490,84 +490,85 @@ __global__ void k_copyback(int NA, const double4 *__restrict__ X1, double4 *__re
 // search of j's row is needed.  The rank is counted from the acceptance
 // masks of the cell's earlier atoms: j accepts a exactly when a accepted j.
 //
-// One cell c of the build (one wave); see k_nbuild for the numbering of c.
-__device__ __forceinline__ void nbuild_cell(
-    const double4 *__restrict__ X, const DInst *__restrict__ I, int ninst, int c,
-    const int *__restrict__ cplan, const int *__restrict__ cstart, const int *__restrict__ bcnt,
-    double rv2, double rv2lj, int Wsw, int Wion, int *__restrict__ cnt_sw,
-    int *__restrict__ nbr_sw, int *__restrict__ cnt_ion, int *__restrict__ nbr_ion,
-    uint16_t *__restrict__ rk_sw, uint8_t *__restrict__ off_sw, double4 *s_x, int *s_j,
-    uint8_t *s_st, int *s_rs, int *s_off, uint32_t *s_acc) {
-  const int lane = threadIdx.x % FIB_WAVE;
-  int k, gcell;
-  if (cplan) {
-    // last instance whose prefix is <= c (gated-out instances have width 0)
-    int lo = 0, hi = ninst - 1;
-    while (lo < hi) {
-      const int mid = (lo + hi + 1) >> 1;
-      if (cplan[mid] <= c) lo = mid; else hi = mid - 1;
-    }
-    k = lo;
-    gcell = I[k].cell_real_off + (c - cplan[k]);
-  } else {
-    gcell = c;
-    k = find_inst_by_cell(I, ninst, gcell);
+// Neighbour cell index along one axis (d in {-1, 0, 1}): N > 2 wraps
+// periodically ((c + d + N) mod N with one add or subtract); N <= 2 does not
+// wrap and index N is the phantom layer holding atoms whose cell id came out
+// as N+1; a stencil cell outside [0, N] does not exist.
+__device__ __forceinline__ int stencil_axis(int c, int d, int n, bool &ok) {
+  int v = c + d;
+  if (n > 2) {
+    v = v < 0 ? v + n : (v >= n ? v - n : v);
+  } else if (v < 0 || v > n) {
+    ok = false;
   }
-  const DInst &in = I[k];
+  return v;
+}
+
+// One cell (cx, cy, cz) of instance `in`, built by one wave.
+__device__ __forceinline__ void nbuild_cell(
+    const double4 *__restrict__ X, const DInst &in, int cx, int cy, int cz, int sd0, int sd1,
+    int sd2, const int *__restrict__ cstart, const int *__restrict__ bcnt, double rv2,
+    double rv2lj, int Wsw, int Wion, int *__restrict__ cnt_sw, int *__restrict__ nbr_sw,
+    int *__restrict__ cnt_ion, int *__restrict__ nbr_ion, uint16_t *__restrict__ rk_sw,
+    uint8_t *__restrict__ off_sw, double4 *s_x, int *s_j, uint8_t *s_st, int *s_rs, int *s_off,
+    uint32_t *s_acc) {
+  const int lane = threadIdx.x % FIB_WAVE;
   // the previous cell of this wave is done with the shared arrays
   wave_lds_sync();
-  const int N0 = in.nc[0], N1 = in.nc[1];
-  const int loc = gcell - in.cell_real_off;
-  const int cx = loc % N0, cy = (loc / N0) % N1, cz = loc / (N0 * N1);
   const int bi = in.bucket_off + (cz * in.next[1] + cy) * in.next[0] + cx;
   const int i0 = cstart[bi], ni = bcnt[bi];
   if (ni == 0) return;
 
-  // 27 stencil ranges.  N > 2: periodic.  N <= 2: no wrap; index N is the
-  // phantom layer holding atoms whose cell id came out as N+1.
+  // lane s < 27 owns stencil cell s = (sd0, sd1, sd2) + 1 in base 3
   int rs = 0, rc = 0;
   if (lane < 27) {
-    int d[3] = {lane % 3 - 1, (lane / 3) % 3 - 1, lane / 9 - 1};
-    int c[3] = {cx, cy, cz};
     bool ok = true;
-    int e[3];
-    for (int a = 0; a < 3; ++a) {
-      int n = in.nc[a];
-      int v = c[a] + d[a];
-      if (n > 2) v = (v + n) % n;
-      else if (v < 0 || v > n) ok = false;
-      e[a] = v;
-    }
+    const int e0 = stencil_axis(cx, sd0, in.nc[0], ok);
+    const int e1 = stencil_axis(cy, sd1, in.nc[1], ok);
+    const int e2 = stencil_axis(cz, sd2, in.nc[2], ok);
     if (ok) {
-      int b = in.bucket_off + (e[2] * in.next[1] + e[1]) * in.next[0] + e[0];
+      const int b = in.bucket_off + (e2 * in.next[1] + e1) * in.next[0] + e0;
       rs = cstart[b];
       rc = bcnt[b];
     }
   }
-  // inclusive scan of rc over lanes
+  // inclusive scan of rc over lanes 0..31 (the stencil lanes)
   int sc = rc;
-  for (int o = 1; o < FIB_WAVE; o <<= 1) {
+  for (int o = 1; o < 32; o <<= 1) {
     int y = __shfl_up(sc, o);
     if (lane >= o) sc += y;
   }
-  if (lane < 27) { s_rs[lane] = rs; s_off[lane + 1] = sc; }
-  if (lane == 0) s_off[0] = 0;
   const int total = __shfl(sc, 26);
-  // lane s < 27 owns stencil cell s: candidates [my_lo, my_hi)
+  // stencil cell s covers candidates [my_lo, my_lo + rc)
   const int my_lo = sc - rc;
   const bool masks = total <= NB_ACC && ni <= 32;
   if (masks)
     for (int g = lane; g < total; g += FIB_WAVE) s_acc[g] = 0;
+  const bool one_chunk = total <= NB_CHUNK;
+  if (one_chunk) {
+    // each stencil lane writes its own run of candidates
+    for (int q = 0; __ballot(q < rc) != 0ull; ++q)
+      if (q < rc) {
+        s_j[my_lo + q] = rs + q;
+        s_st[my_lo + q] = (uint8_t)lane;
+      }
+  } else {
+    if (lane < 27) { s_rs[lane] = rs; s_off[lane + 1] = sc; }
+    if (lane == 0) s_off[0] = 0;
+  }
   wave_lds_sync();
 
   const double4 *Xc = X;
   // the cell's own atoms are stencil range 13: read them from the staged
   // chunk when it holds them (always, unless the halo is very dense)
-  const int own0 = s_off[13];
-  const bool own_staged = own0 + ni <= NB_CHUNK && total <= NB_CHUNK;
+  const int own0 = __shfl(my_lo, 13);
+  const bool own_staged = one_chunk;
   const double qb0 = 0.25 * in.lb[0], qb1 = 0.25 * in.lb[1], qb2 = 0.25 * in.lb[2];
   for (int ii = 0; ii < ni; ++ii) {
     const int i = i0 + ii;
+    int *const rowj = nbr_sw + (size_t)i * Wsw;
+    uint16_t *const rowk = rk_sw + (size_t)i * Wsw;
+    int *const rowi = nbr_ion + (size_t)i * Wion;
     int nsw = 0, nion = 0;
     int before = 0;  // lane s: accepted candidates before stencil cell s
     double4 xi;
@@ -575,21 +576,23 @@ __device__ __forceinline__ void nbuild_cell(
     for (int base = 0; base < total; base += NB_CHUNK) {
       const int n = min(NB_CHUNK, total - base);
       // (re)stage the chunk; a single chunk is loaded once per cell
-      if (ii == 0 || total > NB_CHUNK) {
-        wave_lds_sync();
-        for (int q = lane; q < n; q += FIB_WAVE) {
-          const int g = base + q;
-          // stencil range holding g: binary search over the 28 offsets
-          int lo = 0, hi = 26;
-          while (lo < hi) {
-            const int mid = (lo + hi + 1) >> 1;
-            if (s_off[mid] <= g) lo = mid; else hi = mid - 1;
+      if (ii == 0 || !one_chunk) {
+        if (!one_chunk) {
+          wave_lds_sync();
+          for (int q = lane; q < n; q += FIB_WAVE) {
+            const int g = base + q;
+            // stencil range holding g: binary search over the 28 offsets
+            int lo = 0, hi = 26;
+            while (lo < hi) {
+              const int mid = (lo + hi + 1) >> 1;
+              if (s_off[mid] <= g) lo = mid; else hi = mid - 1;
+            }
+            s_j[q] = s_rs[lo] + (g - s_off[lo]);
+            s_st[q] = (uint8_t)lo;
           }
-          const int j = s_rs[lo] + (g - s_off[lo]);
-          s_j[q] = j;
-          s_st[q] = (uint8_t)lo;
-          s_x[q] = Xc[j];
+          wave_lds_sync();
         }
+        for (int q = lane; q < n; q += FIB_WAVE) s_x[q] = Xc[s_j[q]];
         wave_lds_sync();
       }
       if (base == 0) {
@@ -616,7 +619,6 @@ __device__ __forceinline__ void nbuild_cell(
         const bool sw = cand && swp && r2 < rv2;
         const bool io = cand && !swp && r2 < rv2lj;
         uint64_t msw = __ballot(sw), mio = __ballot(io);
-        uint64_t below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
         if (sw) {
           // rank of i among the atoms of its cell that j lists (in cell order)
           int rank = 0;
@@ -632,15 +634,21 @@ __device__ __forceinline__ void nbuild_cell(
               if (type_of(meta_of(xa)) < 3 && ref_r2(xj, xa, in) < rv2) ++rank;
             }
           }
-          const int pos = nsw + __popcll(msw & below);
+          const int pos = nsw + (int)__builtin_amdgcn_mbcnt_hi(
+                                    (uint32_t)(msw >> 32),
+                                    __builtin_amdgcn_mbcnt_lo((uint32_t)msw, 0u));
           if (pos < Wsw) {
-            nbr_sw[(size_t)i * Wsw + pos] = j;
-            rk_sw[(size_t)i * Wsw + pos] = (uint16_t)((st << 8) | (rank & 0xff));
+            rowj[pos] = j;
+            rowk[pos] = (uint16_t)((st << 8) | (rank & 0xff));
           }
         }
-        if (io) {
-          int pos = nion + __popcll(mio & below);
-          if (pos < Wion) nbr_ion[(size_t)i * Wion + pos] = j;
+        if (mio) {
+          if (io) {
+            const int pos = nion + (int)__builtin_amdgcn_mbcnt_hi(
+                                       (uint32_t)(mio >> 32),
+                                       __builtin_amdgcn_mbcnt_lo((uint32_t)mio, 0u));
+            if (pos < Wion) rowi[pos] = j;
+          }
         }
         // stencil cell s starts at candidate my_lo: count the accepted ones
         // of this batch of 64 candidates that lie before it
@@ -661,6 +669,28 @@ __device__ __forceinline__ void nbuild_cell(
   }
 }
 
+// Instance k and its first cell number in the build's numbering of cells:
+// all real cells in instance order, or under a gate the real cells of the
+// gated instances only (prefix cplan, gated-out instances have width 0).
+__device__ __forceinline__ int locate_cell(const DInst *__restrict__ I, int ninst,
+                                           const int *__restrict__ cplan, int c, int &first) {
+  int lo = 0, hi = ninst - 1;
+  if (cplan) {
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (cplan[mid] <= c) lo = mid; else hi = mid - 1;
+    }
+    first = cplan[lo];
+  } else {
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (I[mid].cell_real_off <= c) lo = mid; else hi = mid - 1;
+    }
+    first = I[lo].cell_real_off;
+  }
+  return lo;
+}
+
 // Each wavefront builds NB_CELLS consecutive cells; a workgroup holds
 // NB_WAVES independent wavefronts (their own LDS, wave-level sync only), so
 // that the workgroup-per-CU limit does not cap the resident waves.  The cells
@@ -669,7 +699,8 @@ __device__ __forceinline__ void nbuild_cell(
 // the grid covers all real cells and the workgroups past the gated ones leave
 // at once (gated-out instances cost a launch of empty workgroups, no more).
 // The busy workgroups are remapped so that each XCD builds a contiguous run
-// of cells, whose halos then stay in its L2.
+// of cells, whose halos then stay in its L2.  A wave decodes its first cell
+// once and steps through the rest in x-fastest order.
 __global__ __launch_bounds__(NB_WAVES * FIB_WAVE) __attribute__((amdgpu_waves_per_eu(6)))
 void k_nbuild(
     const double4 *__restrict__ X, const DInst *__restrict__ I, int ninst, int total_cells,
@@ -691,10 +722,30 @@ void k_nbuild(
   if ((int)blockIdx.x >= nbusy) return;
   const int c0 = xcd_remap(blockIdx.x, nbusy) * CPB + w * NB_CELLS;
   const int c1 = min(ncell, c0 + NB_CELLS);
-  for (int c = c0; c < c1; ++c)
-    nbuild_cell(X, I, ninst, c, cplan, cstart, bcnt, rv2, rv2lj, Wsw, Wion, cnt_sw, nbr_sw,
-                cnt_ion, nbr_ion, rk_sw, off_sw, s_x[w], s_j[w], s_st[w], s_rs[w], s_off[w],
-                s_acc[w]);
+  const int lane = threadIdx.x % FIB_WAVE;
+  const int sd0 = lane % 3 - 1, sd1 = (lane / 3) % 3 - 1, sd2 = lane / 9 - 1;
+  int cend = c0, k = 0, cx = 0, cy = 0, cz = 0;
+  for (int c = c0; c < c1; ++c) {
+    if (c >= cend) {
+      // first cell, or the first of the next instance
+      int first;
+      k = locate_cell(I, ninst, cplan, c, first);
+      const DInst &in = I[k];
+      const int loc = c - first;
+      cx = loc % in.nc[0];
+      cy = (loc / in.nc[0]) % in.nc[1];
+      cz = loc / (in.nc[0] * in.nc[1]);
+      cend = first + in.nc[0] * in.nc[1] * in.nc[2];
+    }
+    const DInst &in = I[k];
+    nbuild_cell(X, in, cx, cy, cz, sd0, sd1, sd2, cstart, bcnt, rv2, rv2lj, Wsw, Wion, cnt_sw,
+                nbr_sw, cnt_ion, nbr_ion, rk_sw, off_sw, s_x[w], s_j[w], s_st[w], s_rs[w],
+                s_off[w], s_acc[w]);
+    if (++cx == in.nc[0]) {
+      cx = 0;
+      if (++cy == in.nc[1]) { cy = 0; ++cz; }
+    }
+  }
 }
 
 // Atoms outside every scanned cell (dropped, dead, or phantom i-atoms) get
