@@ -51,6 +51,13 @@ __device__ __forceinline__ bool gated_out(const int *gate, int inst) {
   return gate && !gate[1 + inst];
 }
 
+// Element loops of the list-build kernels run grid-stride over a capped grid
+// (GRID_CAP workgroups): a build skipped by its gate then costs a small launch
+// instead of one workgroup per 256 atoms.
+#define GRID_CAP 2048
+#define GRID_STRIDE(p, n) \
+  for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < (n); p += gridDim.x * blockDim.x)
+
 // ---------------------------------------------------------------------------
 // helpers
 __device__ __forceinline__ uint64_t wbits(const double4 &x) {
@@ -255,11 +262,10 @@ __global__ void k_bin(const double4 *__restrict__ X, int NA, const DInst *__rest
                       int *__restrict__ bkey, int *__restrict__ brank, int *__restrict__ bcnt,
                       DCtl *__restrict__ ctl, const int *__restrict__ gate) {
   GATE_OR_RETURN(gate);
-  int p = blockIdx.x * blockDim.x + threadIdx.x;
-  if (p >= NA) return;
+  GRID_STRIDE(p, NA) {
   double4 x = X[p];
   uint32_t m = meta_of(x);
-  if ((m & META_PAD) || gated_out(gate, inst_of(m))) { bkey[p] = -1; return; }
+  if ((m & META_PAD) || gated_out(gate, inst_of(m))) { bkey[p] = -1; continue; }
   const DInst &in = I[inst_of(m)];
   int local;
   if (m & META_DEAD) {
@@ -282,13 +288,13 @@ __global__ void k_bin(const double4 *__restrict__ X, int NA, const DInst *__rest
   int b = in.bucket_off + local;
   bkey[p] = b;
   brank[p] = atomicAdd(&bcnt[b], 1);
+  }
 }
 
 // Zero fill (used instead of a memset node so the step can be captured).
 __global__ void k_zero_i32(int *__restrict__ p, int n, const int *__restrict__ gate) {
   GATE_OR_RETURN(gate);
-  const int k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k < n) p[k] = 0;
+  GRID_STRIDE(k, n) p[k] = 0;
 }
 
 // Export of the reference cell ids in original atom order.
@@ -395,35 +401,32 @@ __global__ void k_scan_base(const int *__restrict__ out, const DInst *__restrict
 __global__ void k_scan_fix(int *__restrict__ out, int n, const DInst *__restrict__ I, int ninst,
                            const int *__restrict__ ibase, const int *__restrict__ gate) {
   GATE_OR_RETURN(gate);
-  const int b = blockIdx.x * blockDim.x + threadIdx.x;
-  if (b >= n) return;
-  out[b] -= ibase[find_inst_by_bucket(I, ninst, b)];
+  GRID_STRIDE(b, n) out[b] -= ibase[find_inst_by_bucket(I, ninst, b)];
 }
 
 __global__ void k_scatter(const int *__restrict__ bkey, const int *__restrict__ brank, int NA,
                           const int *__restrict__ cstart, int *__restrict__ perm,
                           const int *__restrict__ gate) {
   GATE_OR_RETURN(gate);
-  int p = blockIdx.x * blockDim.x + threadIdx.x;
-  if (p >= NA) return;
-  int b = bkey[p];
-  if (b < 0) return;
-  perm[cstart[b] + brank[p]] = p;
+  GRID_STRIDE(p, NA) {
+    const int b = bkey[p];
+    if (b >= 0) perm[cstart[b] + brank[p]] = p;
+  }
 }
 
 // Deterministic order inside a bucket: ascending previous position.
 __global__ void k_fixup(const int *__restrict__ cstart, const int *__restrict__ bcnt, int nb,
                         int *__restrict__ perm, const int *__restrict__ gate) {
   GATE_OR_RETURN(gate);
-  int b = blockIdx.x * blockDim.x + threadIdx.x;
-  if (b >= nb) return;
-  int n = bcnt[b];
-  if (n < 2) return;
-  int *a = perm + cstart[b];
-  for (int i = 1; i < n; ++i) {
-    int v = a[i], j = i - 1;
-    while (j >= 0 && a[j] > v) { a[j + 1] = a[j]; --j; }
-    a[j + 1] = v;
+  GRID_STRIDE(b, nb) {
+    const int n = bcnt[b];
+    if (n < 2) continue;
+    int *a = perm + cstart[b];
+    for (int i = 1; i < n; ++i) {
+      int v = a[i], j = i - 1;
+      while (j >= 0 && a[j] > v) { a[j + 1] = a[j]; --j; }
+      a[j + 1] = v;
+    }
   }
 }
 
@@ -434,10 +437,9 @@ __global__ void k_reorder(int NA, const int *__restrict__ perm, const DInst *__r
                           const double *__restrict__ Xi0, double *__restrict__ Xi1,
                           int *__restrict__ inv, const int *__restrict__ gate) {
   GATE_OR_RETURN(gate);
-  int p = blockIdx.x * blockDim.x + threadIdx.x;
-  if (p >= NA) return;
+  GRID_STRIDE(p, NA) {
   uint32_t mp = meta_of(X0[p]);
-  if ((mp & META_PAD) || gated_out(gate, inst_of(mp))) return;
+  if ((mp & META_PAD) || gated_out(gate, inst_of(mp))) continue;
   int s = perm[p];
   double4 x = X0[s];
   X1[p] = x;
@@ -449,6 +451,7 @@ __global__ void k_reorder(int NA, const int *__restrict__ perm, const DInst *__r
   }
   const DInst &in = I[inst_of(mp)];
   inv[in.atom_off + orig_of(x)] = p;
+  }
 }
 
 // In-place variant of the sort: copy the reordered arrays back, so the
@@ -459,15 +462,15 @@ __global__ void k_copyback(int NA, const double4 *__restrict__ X1, double4 *__re
                            const double *__restrict__ Xi1, double *__restrict__ Xi0,
                            const int *__restrict__ gate) {
   GATE_OR_RETURN(gate);
-  int p = blockIdx.x * blockDim.x + threadIdx.x;
-  if (p >= NA) return;
-  if (gated_out(gate, inst_of(meta_of(X0[p])))) return;
-  X0[p] = X1[p];
-  for (int c = 0; c < 3; ++c) {
-    const size_t a = (size_t)c * NA + p;
-    V0[a] = V1[a];
-    F0[a] = F1[a];
-    Xi0[a] = Xi1[a];
+  GRID_STRIDE(p, NA) {
+    if (gated_out(gate, inst_of(meta_of(X0[p])))) continue;
+    X0[p] = X1[p];
+    for (int c = 0; c < 3; ++c) {
+      const size_t a = (size_t)c * NA + p;
+      V0[a] = V1[a];
+      F0[a] = F1[a];
+      Xi0[a] = Xi1[a];
+    }
   }
 }
 
@@ -754,17 +757,15 @@ __global__ void k_clear_counts(int NA, int *__restrict__ cnt_sw, int *__restrict
                                uint8_t *__restrict__ off_sw, const double4 *__restrict__ X,
                                const int *__restrict__ gate) {
   GATE_OR_RETURN(gate);
-  if (gate && blockIdx.x * blockDim.x + threadIdx.x < NA &&
-      gated_out(gate, inst_of(meta_of(X[blockIdx.x * blockDim.x + threadIdx.x]))))
-    return;
-  int p = blockIdx.x * blockDim.x + threadIdx.x;
-  if (p >= NA) return;
-  cnt_sw[p] = 0;
-  cnt_ion[p] = 0;
-  // no row until k_nbuild writes one: offsets past any row (see k_force3)
-  uint4 *o = reinterpret_cast<uint4 *>(off_sw + (size_t)p * OFF_W);
-  o[0] = make_uint4(~0u, ~0u, ~0u, ~0u);
-  o[1] = make_uint4(~0u, ~0u, ~0u, ~0u);
+  GRID_STRIDE(p, NA) {
+    if (gate && gated_out(gate, inst_of(meta_of(X[p])))) continue;
+    cnt_sw[p] = 0;
+    cnt_ion[p] = 0;
+    // no row until k_nbuild writes one: offsets past any row (see k_force3)
+    uint4 *o = reinterpret_cast<uint4 *>(off_sw + (size_t)p * OFF_W);
+    o[0] = make_uint4(~0u, ~0u, ~0u, ~0u);
+    o[1] = make_uint4(~0u, ~0u, ~0u, ~0u);
+  }
 }
 
 // Longest SW / ion lists of the build (overflow check).
@@ -1060,31 +1061,17 @@ __global__ __launch_bounds__(FIB_BLOCK) void k_legs(
   if (sw_atom) {
     const int cs = min(cnt_sw[i], Wsw);
     const int4 *r4 = reinterpret_cast<const int4 *>(nbr_sw + row);
-    // the first 24 slots in one batch of index loads (all in flight), then
-    // the positions four at a time
-    int4 rq[6];
-#pragma unroll
-    for (int u = 0; u < 6; ++u) rq[u] = (4 * u < cs) ? r4[u] : make_int4(i, i, i, i);
-#pragma unroll
-    for (int u = 0; u < 6; ++u) {
-      const int k0 = 4 * u;
-      if (k0 >= cs) break;
-      const int jq[4] = {rq[u].x, (k0 + 1 < cs) ? rq[u].y : i, (k0 + 2 < cs) ? rq[u].z : i,
-                         (k0 + 3 < cs) ? rq[u].w : i};
-      double4 xq[4];
-#pragma unroll
-      for (int v = 0; v < 4; ++v) xq[v] = X[jq[v]];
-#pragma unroll
-      for (int v = 0; v < 4; ++v)
-        if (k0 + v < cs) pair(k0 + v, jq[v], xq[v]);
-    }
-    for (int k0 = 24; k0 < cs; k0 += 4) {
-      const int4 q = r4[k0 >> 2];
+    // four slots at a time: their indices, then their positions; the next
+    // four indices are loaded while the positions arrive
+    int4 rn = (0 < cs) ? r4[0] : make_int4(i, i, i, i);
+    for (int k0 = 0; k0 < cs; k0 += 4) {
+      const int4 q = rn;
       const int jq[4] = {q.x, (k0 + 1 < cs) ? q.y : i, (k0 + 2 < cs) ? q.z : i,
                          (k0 + 3 < cs) ? q.w : i};
       double4 xq[4];
 #pragma unroll
       for (int v = 0; v < 4; ++v) xq[v] = X[jq[v]];
+      if (k0 + 4 < cs) rn = r4[(k0 >> 2) + 1];
 #pragma unroll
       for (int v = 0; v < 4; ++v)
         if (k0 + v < cs) pair(k0 + v, jq[v], xq[v]);
@@ -1538,7 +1525,7 @@ __global__ void k_mark_build(int NA, int ninst, int Wsw, int Wion, const double4
                              double4 *__restrict__ Xb, float *__restrict__ disp,
                              DCtl *__restrict__ ctl, const int *__restrict__ gate) {
   GATE_OR_RETURN(gate);
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  GRID_STRIDE(i, max(NA, ninst)) {
   if (i < NA && !gated_out(gate, inst_of(meta_of(X[i])))) {
     Xb[i] = X[i];
     disp[i] = 0.0f;
@@ -1551,6 +1538,7 @@ __global__ void k_mark_build(int NA, int ninst, int Wsw, int Wion, const double4
     // a list longer than its width was cut: sticky, checked by the host
     if (ctl[i].maxcnt_sw > Wsw || ctl[i].maxcnt_ion > Wion) ctl[i].overflow = 1;
     ctl[i].builds += 1.0;
+  }
   }
 }
 
@@ -2145,6 +2133,8 @@ __global__ void k_set_atom(int NA, double4 *__restrict__ X, double *__restrict__
 // ---------------------------------------------------------------------------
 // launchers
 static inline int nblk(long n, int b) { return (int)((n + b - 1) / b); }
+// grid of a GRID_STRIDE kernel over n elements
+static inline int gcap(long n) { return std::max(1, std::min(nblk(n, FIB_BLOCK), GRID_CAP)); }
 
 // Times the launches issued in its scope under one key.
 struct KScope {
@@ -2201,12 +2191,12 @@ static void finalize_thermo(const DevBufs &d, hipStream_t s) {
 void launch_bin(const DevBufs &d, hipStream_t s) {
   {
     KScope ks_(d, KT_BIN);
-    hipLaunchKernelGGL(k_zero_i32, dim3(nblk(d.nbucket, FIB_BLOCK)), dim3(FIB_BLOCK), 0, s, d.bcnt,
+    hipLaunchKernelGGL(k_zero_i32, dim3(gcap(d.nbucket)), dim3(FIB_BLOCK), 0, s, d.bcnt,
                        d.nbucket, d.gate);
   }
   {
     KScope ks_(d, KT_BIN);
-    hipLaunchKernelGGL(k_bin, dim3(nblk(d.NA, FIB_BLOCK)), dim3(FIB_BLOCK), 0, s, d.X[d.cur], d.NA,
+    hipLaunchKernelGGL(k_bin, dim3(gcap(d.NA)), dim3(FIB_BLOCK), 0, s, d.X[d.cur], d.NA,
                        d.I, d.bkey, d.brank, d.bcnt, d.ctl, d.gate);
   }
 }
@@ -2234,7 +2224,7 @@ void launch_scan(const DevBufs &d, hipStream_t s) {
   }
   {
     KScope ks_(d, KT_SORT);
-    hipLaunchKernelGGL(k_scan_fix, dim3(nblk(d.nbucket, FIB_BLOCK)), dim3(FIB_BLOCK), 0, s,
+    hipLaunchKernelGGL(k_scan_fix, dim3(gcap(d.nbucket)), dim3(FIB_BLOCK), 0, s,
                        d.cstart, d.nbucket, d.I, d.ninst, d.ibase, d.gate);
   }
 }
@@ -2242,24 +2232,24 @@ void launch_scan(const DevBufs &d, hipStream_t s) {
 void launch_sort(const DevBufs &d, hipStream_t s, bool inplace) {
   {
     KScope ks_(d, KT_SORT);
-    hipLaunchKernelGGL(k_scatter, dim3(nblk(d.NA, FIB_BLOCK)), dim3(FIB_BLOCK), 0, s, d.bkey,
+    hipLaunchKernelGGL(k_scatter, dim3(gcap(d.NA)), dim3(FIB_BLOCK), 0, s, d.bkey,
                        d.brank, d.NA, d.cstart, d.perm, d.gate);
   }
   {
     KScope ks_(d, KT_SORT);
-    hipLaunchKernelGGL(k_fixup, dim3(nblk(d.nbucket, FIB_BLOCK)), dim3(FIB_BLOCK), 0, s, d.cstart,
+    hipLaunchKernelGGL(k_fixup, dim3(gcap(d.nbucket)), dim3(FIB_BLOCK), 0, s, d.cstart,
                        d.bcnt, d.nbucket, d.perm, d.gate);
   }
   const int c = d.cur, n = 1 - d.cur;
   {
     KScope ks_(d, KT_SORT);
-    hipLaunchKernelGGL(k_reorder, dim3(nblk(d.NA, FIB_BLOCK)), dim3(FIB_BLOCK), 0, s, d.NA, d.perm,
+    hipLaunchKernelGGL(k_reorder, dim3(gcap(d.NA)), dim3(FIB_BLOCK), 0, s, d.NA, d.perm,
                        d.I, d.X[c], d.X[n], d.V[c], d.V[n], d.F[c], d.F[n], d.Xi[c], d.Xi[n],
                        d.inv, d.gate);
   }
   if (inplace) {
     KScope ks_(d, KT_SORT);
-    hipLaunchKernelGGL(k_copyback, dim3(nblk(d.NA, FIB_BLOCK)), dim3(FIB_BLOCK), 0, s, d.NA, d.X[n],
+    hipLaunchKernelGGL(k_copyback, dim3(gcap(d.NA)), dim3(FIB_BLOCK), 0, s, d.NA, d.X[n],
                        d.X[c], d.V[n], d.V[c], d.F[n], d.F[c], d.Xi[n], d.Xi[c], d.gate);
   }
 }
@@ -2267,7 +2257,7 @@ void launch_sort(const DevBufs &d, hipStream_t s, bool inplace) {
 void launch_nbuild(const DevBufs &d, const FConst &fc, hipStream_t s) {
   {
     KScope ks_(d, KT_LISTAUX);
-    hipLaunchKernelGGL(k_clear_counts, dim3(nblk(d.NA, FIB_BLOCK)), dim3(FIB_BLOCK), 0, s, d.NA,
+    hipLaunchKernelGGL(k_clear_counts, dim3(gcap(d.NA)), dim3(FIB_BLOCK), 0, s, d.NA,
                        d.cnt_sw, d.cnt_ion, d.off_sw, d.X[d.cur], d.gate);
   }
   if (d.total_cells > 0) {
@@ -2434,7 +2424,7 @@ void launch_md_report(const DevBufs &d, const StepArgs &sa, hipStream_t s) {
 
 void launch_mark_build(const DevBufs &d, hipStream_t s) {
   KScope ks_(d, KT_LISTAUX);
-  hipLaunchKernelGGL(k_mark_build, dim3(nblk(std::max(d.NA, d.ninst), FIB_BLOCK)), dim3(FIB_BLOCK), 0,
+  hipLaunchKernelGGL(k_mark_build, dim3(gcap(std::max(d.NA, d.ninst))), dim3(FIB_BLOCK), 0,
                      s, d.NA, d.ninst, d.Wsw, d.Wion, d.X[d.cur], d.Xb, d.disp, d.ctl, d.gate);
   if (d.gate)
     hipLaunchKernelGGL(k_clear_gate, dim3(1), dim3(FIB_BLOCK), 0, s, (int *)d.gate, d.ninst);
