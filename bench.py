@@ -370,12 +370,22 @@ def md_bench(args):
     return out
 
 
-def rel_err(Fg, Fo, floor_frac=1e-3):
+def rel_err(Fg, Fo, floor_frac=1e-3, detail=False):
     d = np.sqrt(((Fg - Fo) ** 2).sum(1))
     m = np.sqrt((Fo ** 2).sum(1))
     frms = np.sqrt((m ** 2).mean())
     e = d / np.maximum(m, floor_frac * frms)
-    return float(e.max()), float(np.sqrt((e ** 2).mean())), float(np.linalg.norm(Fg - Fo) / np.linalg.norm(Fo))
+    out = (float(e.max()), float(np.sqrt((e ** 2).mean())),
+           float(np.linalg.norm(Fg - Fo) / np.linalg.norm(Fo)))
+    if detail:
+        # how the per-atom error is spread: atoms above 1e-5, the 99.99th
+        # percentile, and |F| of the worst atom against F_rms
+        k = int(np.argmax(e))
+        return out + ({"atoms_above_1e-5": int((e > 1e-5).sum()),
+                       "p9999_rel": float(np.quantile(e, 0.9999)),
+                       "worst_atom_F_over_Frms": float(m[k] / frms),
+                       "worst_atom_abs_err_over_Frms": float(d[k] / frms)},)
+    return out
 
 
 def cpu_baseline(ncell=24, reps=5):
@@ -468,7 +478,7 @@ def main():
         log(f"  {prec}: {res[prec]['ms_per_step']:.3f} ms/step, {res[prec]['rate']:.4g} atom-evals/s")
         for k, v in sorted(res[prec]["timing"].items(), key=lambda kv: -kv[1]["ms"]):
             log(f"    {k:10s} {v['avg_ms']:.4f} ms x {v['calls_per_step']:.0f}/step")
-    emax, erms, enorm = rel_err(res["mixed"]["F"], res["fp64"]["F"])
+    emax, erms, enorm, espread = rel_err(res["mixed"]["F"], res["fp64"]["F"], detail=True)
     m = res["mixed"]
     tm = m["timing"]
     # roofline of the longest single kernel of the mixed step
@@ -512,7 +522,7 @@ def main():
                  "kernels_avg_ms": {k: v["avg_ms"] for k, v in res["fp64"]["timing"].items()}},
         "mixed_over_fp64": m["rate"] / res["fp64"]["rate"],
         "force_error_mixed_vs_fp64": {"max_rel": emax, "rms_rel": erms, "norm_rel": enorm,
-                                      "floor": "1e-3 F_rms"},
+                                      "floor": "1e-3 F_rms", **espread},
         "energy_rel_diff": abs(m["utot"] - res["fp64"]["utot"]) / abs(res["fp64"]["utot"]),
         "kernels_avg_ms": {k: v["avg_ms"] for k, v in tm.items()},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS,

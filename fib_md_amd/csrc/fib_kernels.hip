@@ -257,37 +257,59 @@ __global__ void k_finalize2(const DInst *__restrict__ I, int nch, RedSpec rs,
 }
 
 // ---------------------------------------------------------------------------
-// binning
+// binning.  Atoms arrive mostly in cell order (the previous sort), so the
+// lanes of a wavefront form runs of equal buckets: each run takes its ranks
+// with one atomic add of its length.  Ranks only need to be distinct; k_fixup
+// orders every bucket by previous position.
 __global__ void k_bin(const double4 *__restrict__ X, int NA, const DInst *__restrict__ I,
                       int *__restrict__ bkey, int *__restrict__ brank, int *__restrict__ bcnt,
                       DCtl *__restrict__ ctl, const int *__restrict__ gate) {
   GATE_OR_RETURN(gate);
+  const int lane = threadIdx.x % FIB_WAVE;
+  const uint64_t le = (lane == FIB_WAVE - 1) ? ~0ull : ((2ull << lane) - 1ull);  // lanes <= me
+  // NA and the grid stride are multiples of the wavefront: the lanes of a
+  // wavefront run the loop together
   GRID_STRIDE(p, NA) {
-  double4 x = X[p];
-  uint32_t m = meta_of(x);
-  if ((m & META_PAD) || gated_out(gate, inst_of(m))) { bkey[p] = -1; continue; }
-  const DInst &in = I[inst_of(m)];
-  int local;
-  if (m & META_DEAD) {
-    local = in.nbucket - 1;
-  } else if (!(isfinite(x.x) && isfinite(x.y) && isfinite(x.z))) {
-    local = in.nbucket - 1;
-    atomicOr(&ctl[inst_of(m)].nonfinite, 1);
-  } else {
-    int c0 = ref_cell(x.x, in.ilb[0], in.lb[0], in.h[0]);
-    int c1 = ref_cell(x.y, in.ilb[1], in.lb[1], in.h[1]);
-    int c2 = ref_cell(x.z, in.ilb[2], in.lb[2], in.h[2]);
-    // Cell N+1 with N > 2 is overwritten by the periodic halo copy of
-    // chainlist (nlist.f90:53-64): such atoms drop out of every list.
-    bool drop = (c0 == in.nc[0] + 1 && in.nc[0] > 2) || (c1 == in.nc[1] + 1 && in.nc[1] > 2) ||
-                (c2 == in.nc[2] + 1 && in.nc[2] > 2) || c0 < 1 || c1 < 1 || c2 < 1 ||
-                c0 > in.nc[0] + 1 || c1 > in.nc[1] + 1 || c2 > in.nc[2] + 1;
-    if (drop) local = in.nbucket - 2;
-    else local = ((c2 - 1) * in.next[1] + (c1 - 1)) * in.next[0] + (c0 - 1);
-  }
-  int b = in.bucket_off + local;
-  bkey[p] = b;
-  brank[p] = atomicAdd(&bcnt[b], 1);
+    const double4 x = X[p];
+    const uint32_t m = meta_of(x);
+    int b = -1;
+    if (!((m & META_PAD) || gated_out(gate, inst_of(m)))) {
+      const DInst &in = I[inst_of(m)];
+      int local;
+      if (m & META_DEAD) {
+        local = in.nbucket - 1;
+      } else if (!(isfinite(x.x) && isfinite(x.y) && isfinite(x.z))) {
+        local = in.nbucket - 1;
+        atomicOr(&ctl[inst_of(m)].nonfinite, 1);
+      } else {
+        int c0 = ref_cell(x.x, in.ilb[0], in.lb[0], in.h[0]);
+        int c1 = ref_cell(x.y, in.ilb[1], in.lb[1], in.h[1]);
+        int c2 = ref_cell(x.z, in.ilb[2], in.lb[2], in.h[2]);
+        // Cell N+1 with N > 2 is overwritten by the periodic halo copy of
+        // chainlist (nlist.f90:53-64): such atoms drop out of every list.
+        bool drop = (c0 == in.nc[0] + 1 && in.nc[0] > 2) ||
+                    (c1 == in.nc[1] + 1 && in.nc[1] > 2) ||
+                    (c2 == in.nc[2] + 1 && in.nc[2] > 2) || c0 < 1 || c1 < 1 || c2 < 1 ||
+                    c0 > in.nc[0] + 1 || c1 > in.nc[1] + 1 || c2 > in.nc[2] + 1;
+        if (drop) local = in.nbucket - 2;
+        else local = ((c2 - 1) * in.next[1] + (c1 - 1)) * in.next[0] + (c0 - 1);
+      }
+      b = in.bucket_off + local;
+    }
+    // runs of equal buckets among the lanes; skipped lanes (-1) join none
+    const int prev = __shfl_up(b, 1);
+    const bool start = lane == 0 || prev != b || b < 0;
+    const uint64_t starts = __ballot(start);
+    const int lead = 63 - __clzll(starts & le);
+    int base = 0;
+    if (start && b >= 0) {
+      const uint64_t later = starts & ~le;
+      const int end = later ? __ffsll((unsigned long long)later) - 1 : FIB_WAVE;
+      base = atomicAdd(&bcnt[b], end - lane);
+    }
+    base = __shfl(base, lead);
+    bkey[p] = b;
+    if (b >= 0) brank[p] = base + (lane - lead);
   }
 }
 
