@@ -468,7 +468,7 @@ def main():
         b = torch.empty_like(a)
         b.copy_(a)
         torch.cuda.synchronize()
-        r = run_gpu(X, at, L, args.profile, 2, 1)
+        r = run_gpu(X, at, L, args.profile, 10, 2)
         log(json.dumps({k: v for k, v in r.items() if k in ("ms_per_step", "timing")}))
         return
     res = {}
