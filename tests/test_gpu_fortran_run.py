@@ -133,3 +133,46 @@ def test_fortran_driver_restart(tmp_path):
     lts = [int(ln.split()[0]) for ln in out.splitlines()
            if len(ln.split()) == 3 and ln.split()[0].isdigit()]
     assert lts and min(lts) > lt_rec, (lts[:3], lt_rec)
+
+
+CFG = os.path.join(ROOT, "fortran", "mdcfg_gpu")
+
+
+@pytest.mark.parametrize("prec", ["fp64", "mixed"])
+def test_fortran_config_driver(tmp_path, prec):
+    """The Fortran host's driver of the benchmark configurations' impacts
+    (fortran/mdcfg_gpu, MD-step mode): two replicas of a 4x4x4-cell slab,
+    two sequential 1 keV impacts each at 7 deg, 3 A Berendsen shells.  Pins:
+    the energy each impact brings in (1663.6334 sqrt(eV) m/s for Ga), the
+    energy ledger (constant to 1e-3 of the energy fired while the shells
+    work), dt within the configs' cap, and replica 1's frames in the
+    reference's data/ format (initial state and one per impact)."""
+    assert os.path.exists(CFG), "build fortran/mdcfg_gpu first"
+    os.makedirs(tmp_path / "data")
+    args = ["--cells", "4", "--nz", "4", "--replicas", "2", "--impacts", "2", "--steps", "400",
+            "--energy", "1000", "--polar", "7", "--azimuth", "30", "--shell", "3", "--seed", "11",
+            "--frames"] + (["--mixed"] if prec == "mixed" else [])
+    print("mdcfg_gpu " + " ".join(args), flush=True)
+    r = subprocess.run([CFG, *args], cwd=tmp_path, capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
+    rows = []
+    for ln in r.stdout.splitlines():
+        t = ln.split()
+        if t and t[0] == "impact":
+            rows.append({t[k]: float(t[k + 1]) for k in range(0, len(t) - 1, 2)})
+    assert [int(x["impact"]) for x in rows] == [1, 2], r.stdout
+    print(r.stdout, flush=True)
+    for m, x in enumerate(rows, start=1):
+        # the parameter block's Ga mass makes 1663.6334^2 eV/(2 e) 1 eV to 1e-7
+        assert abs(x["fired_eV"] - 1000.0 * m) < 1e-3 * m
+        assert abs(x["drift_max_eV"]) < 1e-3 * x["fired_eV"], x
+        assert 0 < x["dt_s"] <= 1e-15 and x["time_s"] > 0
+        assert x["yield"] >= 0
+    natm = 8 * 4 * 4 * 4 + 2
+    n, frames = parse(tmp_path / "data" / "mdrun2.xyz")
+    assert n == natm and len(frames) == 3
+    for hdr, fr in frames:
+        assert hdr.startswith("Time (ps) = ")
+        labels = [row[36:44].strip() for row in fr]
+        assert labels.count("Si") == natm - 2 and labels.count("Ga") == 2
+    assert int(open(tmp_path / "data" / "3dmol2.xyz").read().split("\n")[0]) == natm
