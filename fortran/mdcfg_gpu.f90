@@ -122,6 +122,7 @@ program mdcfg_gpu
   call ledger(led0)
   print '(A,I3,A,I9,A,I9)', ' replicas', nrep, '  atoms per replica', st%natm, &
        '  steps per impact', nsteps
+  flush(6)
   if (frames) then
      call open_outputs(1, ios)
      if (ios /= 0) then
@@ -158,6 +159,7 @@ program mdcfg_gpu
      print '(A,I5,A,F9.3,A,ES12.4,A,ES12.4,A,F12.3,A,ES12.4,A,ES12.4)', ' impact', m, &
           '  yield', ymean, '  drift_eV', dmean, '  drift_max_eV', dmax, '  fired_eV', fmean, &
           '  time_s', rep%time, '  dt_s', rep%dt
+     flush(6)  ! one line per impact, as it happens
      if (frames) then
         call check(fib_get_state(h, 0_c_int32_t, c_loc(Xo), c_loc(Vo), c_null_ptr), 'fib_get_state')
         st%X = Xo
