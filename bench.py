@@ -14,6 +14,8 @@ Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
        python bench.py --summarize-stats DIR OUT.txt CMD   (rocprofv3 --stats csv -> text)
        python bench.py --summarize-pmc FETCH_DIR WRITE_DIR OUT.txt CMD
                        (rocprofv3 --pmc csv -> text + profiles/pmc_traffic.json)
+       python bench.py --summarize-counters OUT.txt CMD DIR [DIR ...]
+                       (rocprofv3 --pmc SQ/GRBM counters -> text)
 """
 from __future__ import annotations
 
@@ -108,6 +110,46 @@ def summarize_pmc(dfetch, dwrite, out, cmd):
     open(out, "w").write("\n".join(lines) + "\n")
     with open(PMC_FILE, "w") as f:
         json.dump(res, f, indent=1)
+    print("\n".join(lines))
+
+
+def summarize_counters(out, cmd, *dirs):
+    """Per-kernel averages per dispatch of the SQ/GRBM counters of one or more
+    rocprofv3 --pmc runs (each counter summed over the hardware blocks that
+    report it), with VALU instructions per wavefront and per atom."""
+    import csv
+    from collections import defaultdict
+    per = defaultdict(lambda: defaultdict(list))
+    for d in dirs:
+        disp = defaultdict(float)
+        name = {}
+        for r in csv.DictReader(open(os.path.join(d, "run_counter_collection.csv"))):
+            key = (int(r["Dispatch_Id"]), r["Counter_Name"])
+            disp[key] += float(r["Counter_Value"])
+            name[key[0]] = short_name(r["Kernel_Name"])
+        for (k, c), v in disp.items():
+            per[name[k]][c].append(v)
+    avg = {k: {c: sum(v) / len(v) for c, v in cs.items()} for k, cs in per.items()}
+    natm = len(make_workload(64)[0])
+    counters = sorted({c for cs in avg.values() for c in cs})
+    lines = ["# rocprofv3 --pmc counters, average per dispatch (summed over the reporting blocks)",
+             f"# command: {cmd}", f"# atoms: {natm}", ""]
+    hot = [k for k in avg if any(t in k for t in ("k_nbuild", "k_tri", "k_legs", "k_force3"))]
+    for k in sorted(hot):
+        cs = avg[k]
+        lines.append(k)
+        for c in counters:
+            if c in cs:
+                lines.append(f"  {c:28s} {cs[c]:16.0f}")
+        if "SQ_INSTS_VALU" in cs and "SQ_WAVES" in cs and cs["SQ_WAVES"] > 0:
+            lines.append(f"  {'VALU instr per wavefront':28s} {cs['SQ_INSTS_VALU'] / cs['SQ_WAVES']:16.1f}")
+            lines.append(f"  {'VALU instr per atom':28s} {cs['SQ_INSTS_VALU'] / natm:16.1f}")
+        if "SQ_INSTS_VALU" in cs and cs.get("GRBM_GUI_ACTIVE", 0) > 0:
+            # GRBM_GUI_ACTIVE sums the busy cycles of the 8 XCDs; a CU issues
+            # at most one wave64 VALU instruction per cycle (4 SIMDs, 4 cycles each)
+            cu_cycles = 256 * cs["GRBM_GUI_ACTIVE"] / 8
+            lines.append(f"  {'VALU instr per CU cycle':28s} {cs['SQ_INSTS_VALU'] / cu_cycles:16.3f}")
+    open(out, "w").write("\n".join(lines) + "\n")
     print("\n".join(lines))
 
 
@@ -436,6 +478,8 @@ def main():
     ap.add_argument("--profile", choices=["fp64", "mixed", "md-fp64", "md-mixed"], default=None)
     ap.add_argument("--summarize-stats", nargs=3, metavar=("DIR", "OUT", "CMD"))
     ap.add_argument("--summarize-pmc", nargs=4, metavar=("FETCH_DIR", "WRITE_DIR", "OUT", "CMD"))
+    ap.add_argument("--summarize-counters", nargs="+", metavar="OUT CMD DIR",
+                    help="OUT CMD DIR [DIR ...]: SQ/GRBM counter table of rocprofv3 --pmc runs")
     args = ap.parse_args()
     if args.summarize_stats:
         summarize_stats(*args.summarize_stats)
@@ -443,9 +487,12 @@ def main():
     if args.summarize_pmc:
         summarize_pmc(*args.summarize_pmc)
         return
+    if args.summarize_counters:
+        summarize_counters(*args.summarize_counters)
+        return
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); "
-              "multi-GPU sharding is a design in DESIGN.md, not a measured path.", flush=True)
+              "the multi-GPU sharding is designed, not built or measured.", flush=True)
         sys.exit(2)
     import torch
     torch.cuda.init()

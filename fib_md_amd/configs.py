@@ -92,7 +92,7 @@ def force_microbench(n: int = 64, seed: int = 1002):
     return X, np.ones(len(X), np.int32), (L, L, L)
 
 
-# Ga launch speed per sqrt(eV) (knock.f90:28): v = 1663.6334 sqrt(E/eV) m/s
+# Ga launch speed per sqrt(eV) (knock.f90:29): v = 1663.6334 sqrt(E/eV) m/s
 GA_SPEED = 1663.6334
 # height of the unfired ion above the surface, inside the removal height so it
 # stays in the simulation (unfired ions carry no forces)
