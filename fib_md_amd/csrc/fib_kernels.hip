@@ -503,6 +503,7 @@ __global__ void k_copyback(int NA, const double4 *__restrict__ X1, double4 *__re
 #define NB_CELLS 8  // cells per k_nbuild wavefront
 #define NB_WAVES 4  // independent wavefronts per k_nbuild workgroup
 #define OFF_W 32    // bytes per atom of the stencil-offset table (27 used)
+#define NB_JBITS 27  // atom index bits beside the stencil cell in a staged w (fib_create keeps NA below)
 
 // Stencil bookkeeping for the force gather.  The list row of an atom is the
 // concatenation, in the fixed order s = 0..26 of the 27-cell stencil, of the
@@ -617,7 +618,16 @@ __device__ __forceinline__ void nbuild_cell(
           }
           wave_lds_sync();
         }
-        for (int q = lane; q < n; q += FIB_WAVE) s_x[q] = Xc[s_j[q]];
+        // the staged copy carries the candidate's index and stencil cell in
+        // the upper word of w (the original index, which the build does not
+        // use): one 32-byte read per candidate in the batches below
+        for (int q = lane; q < n; q += FIB_WAVE) {
+          const int j = s_j[q];
+          double4 x = Xc[j];
+          const uint64_t up = ((uint64_t)s_st[q] << NB_JBITS) | (uint64_t)j;
+          x.w = __longlong_as_double((long long)((up << 32) | (uint64_t)meta_of(x)));
+          s_x[q] = x;
+        }
         wave_lds_sync();
       }
       if (base == 0) {
@@ -627,10 +637,11 @@ __device__ __forceinline__ void nbuild_cell(
       for (int q0 = 0; q0 < n; q0 += FIB_WAVE) {
         const int q = q0 + lane;  // < NB_CHUNK: the stage arrays are read unguarded
         const int g = base + q;
-        const int j = s_j[q];
-        const bool cand = q < n && j != i;
         const double4 xj = s_x[q];
-        const int st = s_st[q];
+        const uint32_t up = (uint32_t)(wbits(xj) >> 32);
+        const int j = (int)(up & ((1u << NB_JBITS) - 1u));
+        const int st = (int)(up >> NB_JBITS);
+        const bool cand = q < n && j != i;
         double d0, d1, d2;
         ref_diff(xi, xj, d0, d1, d2);
         const bool far = cand && !(fabs(d0) < qb0 && fabs(d1) < qb1 && fabs(d2) < qb2);

@@ -354,7 +354,8 @@ extern "C" int32_t fib_create(const fib_params *p, int32_t ninst, const fib_inst
     natm_before += ds.natm;
     bucket_off += in.nbucket;
     cell_off += (long)in.nc[0] * in.nc[1] * in.nc[2];
-    if (atom_off > (1L << 30) || bucket_off > (1L << 30)) { delete h; return FIB_ERR_ARG; }
+    // the list build keeps atom indices in 27 bits beside a stencil index
+    if (atom_off >= (1L << 27) || bucket_off > (1L << 30)) { delete h; return FIB_ERR_ARG; }
   }
   DevBufs &d = h->d;
   d.NA = (int)atom_off;
