@@ -736,8 +736,11 @@ __device__ __forceinline__ int locate_cell(const DInst *__restrict__ I, int nins
 // at once (gated-out instances cost a launch of empty workgroups, no more).
 // The busy workgroups are remapped so that each XCD builds a contiguous run
 // of cells, whose halos then stay in its L2.  A wave decodes its first cell
-// once and steps through the rest in x-fastest order.
-__global__ __launch_bounds__(NB_WAVES * FIB_WAVE) __attribute__((amdgpu_waves_per_eu(6)))
+// once and steps through the rest in x-fastest order.  The occupancy hint
+// asks for seven waves per SIMD; allocation still ends at six (73 VGPRs),
+// but the schedule built for seven runs 3-4 % faster than the one for six
+// (same-box timing of both builds).
+__global__ __launch_bounds__(NB_WAVES * FIB_WAVE) __attribute__((amdgpu_waves_per_eu(7)))
 void k_nbuild(
     const double4 *__restrict__ X, const DInst *__restrict__ I, int ninst, int total_cells,
     const int *__restrict__ cstart, const int *__restrict__ bcnt, double rv2, double rv2lj,
