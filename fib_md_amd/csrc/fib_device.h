@@ -63,6 +63,10 @@ struct FConst {
                       // the build need a relative move above this to form a leg
   double mixed_xleg;  // mixed precision: three-body legs need r/sigma below
                       // this, i.e. E = exp(gamma/(r/sigma - a)) > e^-30
+  float rho[2][2];    // mixed precision: (float)(isig[a][b] / isig[0][0]), the
+                      // pair's length unit over sigma_Si (1 for Si-Si)
+  float fsw[2][2];    // mixed precision: (float)(epssig[a][b] / (eps[0][0] isig[0][0])),
+                      // the pair's eps/sigma in units of eps_Si/sigma_Si
 };
 
 // Pair-table entry for the reference's 3-point Lagrange interpolation

@@ -946,7 +946,7 @@ __device__ __forceinline__ void make_leg(const double4 &xc, const double4 &xn, c
     const float rinv = __builtin_amdgcn_rsqf(rr);
     const float rs = rr * rinv;
     const float inv_ra = __builtin_amdgcn_rcpf(rs - (float)fc.al);
-    const float rho = (float)(isg / fc.isig[0][0]);
+    const float rho = fc.rho[tc == 2 ? 1 : 0][tn == 2 ? 1 : 0];
     L.e0 = r0 * rinv; L.e1 = r1 * rinv; L.e2 = r2 * rinv;
     L.rinv = rinv * rho;
     L.E = __expf((float)fc.gamma * inv_ra);
@@ -1003,6 +1003,10 @@ __global__ __launch_bounds__(FIB_BLOCK) void k_legs(
   const double isgS = fc.isig[0][0];
   const double fscale = MIXED ? fc.eps[0][0] * isgS : 1.0;
   const double lscale = MIXED ? 1.0 / isgS : 1.0;
+  // mixed: SW pair force factor eps_ij/sigma_ij in units of eps_Si/sigma_Si
+  // and the pair's length unit over sigma_Si, divided on the host
+  const float fsA = fc.fsw[tr][0], fsB = fc.fsw[tr][1];
+  const float rhoA = fc.rho[tr][0], rhoB = fc.rho[tr][1];
   const size_t row = (size_t)i * Wsw;
 
   FAcc<MIXED> f;
@@ -1072,10 +1076,10 @@ __global__ __launch_bounds__(FIB_BLOCK) void k_legs(
       const float ir4 = ir2 * ir2;
       // phi2(r) = A (B r^-4 - 1) e^{1/(r-a)};  force on i = -dphi/dr e
       const float dphi = A * ex * (-4.0f * B * ir4 * rinv - (B * ir4 - 1.0f) * inv_ra * inv_ra);
-      const float fs = -dphi * rinv * (float)((jA ? esgA : esgB) / fscale);
+      const float fs = -dphi * rinv * (jA ? fsA : fsB);
       const float f0 = fs * r0, f1 = fs * r1, f2 = fs * r2;
       f.add(f0, f1, f2);
-      const float rho = (float)(isg / isgS);  // 1 for Si-Si
+      const float rho = jA ? rhoA : rhoB;  // 1 for Si-Si
       if (VIR) {
         const float q = 1.0f / rho;
         addW(r0 * q, r1 * q, r2 * q, 0.5f * f0, 0.5f * f1, 0.5f * f2);
@@ -1484,7 +1488,7 @@ __global__ __launch_bounds__(FIB_BLOCK) void k_extra(
           bool listed = false;
           for (int q = 0; q < cc; ++q) listed |= (rowc[q] == a);
           if (listed) continue;
-          const float rho = (float)(isg / isgS);
+          const float rho = fc.rho[(tc == 2) ? 1 : 0][(ta == 2) ? 1 : 0];
           const float inv_ra = __builtin_amdgcn_rcpf(rs - (float)fc.al);
           Lca.e0 = r0 * rinv; Lca.e1 = r1 * rinv; Lca.e2 = r2 * rinv;
           Lca.rinv = rinv * rho;

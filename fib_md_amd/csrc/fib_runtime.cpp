@@ -236,6 +236,11 @@ static void derive_constants(fib_handle *h) {
   c.izero = izero;
   c.cut_sw2 = c.cut_sw * c.cut_sw * (1.0 + 1e-6);
   c.mixed_xleg = std::min(c.cut_sw, c.al - c.gamma / 30.0);
+  for (int a = 0; a < 2; ++a)
+    for (int b = 0; b < 2; ++b) {
+      c.rho[a][b] = (float)(c.isig[a][b] / c.isig[0][0]);
+      c.fsw[a][b] = (float)(c.epssig[a][b] / (c.eps[0][0] * c.isig[0][0]));
+    }
   c.skin = rc - c.cut_sw * std::max(p.sigma_si, p.sigma_ge);
   {
     const double ug = std::max(c.umax * (double)(izero - 1) / (double)(nt - 1), c.cut_sw);
