@@ -205,8 +205,10 @@ def _md_handle(instances, precision, use_graph=True, params=None):
     return md_handle(instances, precision, params=params, use_graph=use_graph)
 
 
-def md_rate(instances, precision, steps, warmup=5, use_graph=True, params=None):
-    """Timed MD steps of a batch: atom-steps/s over all instances."""
+def md_rate(instances, precision, steps, warmup=5, use_graph=True, params=None,
+            keep_reports=False):
+    """Timed MD steps of a batch: atom-steps/s over all instances (with
+    keep_reports, also the per-instance cascade outputs at the end)."""
     import torch
     g = _md_handle(instances, precision, use_graph, params)
     g.md_run(warmup)
@@ -222,10 +224,48 @@ def md_rate(instances, precision, steps, warmup=5, use_graph=True, params=None):
     # list builds per instance during the timed steps (each instance counts
     # its own gated rebuilds)
     nb = [r["builds"] - b for r, b in zip(reps, b0)]
-    return dict(atom_steps_per_s=n * steps / t, ms_per_step=t * 1e3 / steps, atoms=n,
-                builds_per_instance_timed=float(np.mean(nb)),
-                removed=sum(r["removed"] for r in reps),
-                dt_min_s=min(r["dt"] for r in reps), sim_time_s=max(r["time"] for r in reps))
+    out = dict(atom_steps_per_s=n * steps / t, ms_per_step=t * 1e3 / steps, atoms=n,
+               builds_per_instance_timed=float(np.mean(nb)),
+               removed=sum(r["removed"] for r in reps),
+               dt_min_s=min(r["dt"] for r in reps), sim_time_s=max(r["time"] for r in reps))
+    if keep_reports:
+        out["per"] = _per_instance(instances, reps)
+    return out
+
+
+def _per_instance(instances, reps):
+    """Cascade outputs per instance: sputter yield (Si removed above the
+    surface), Ga projected range (surface - ion z, A; ions removed are left
+    out), displaced Si (> 1.5 A from the initial site)."""
+    return {
+        "yield": np.array([r["removed"] for r in reps], float),
+        "range_A": np.array([(s["surface_z"] - r["ion_z"]) * 1e10 if r["ion_z"] < 1e299
+                             else np.nan for s, r in zip(instances, reps)]),
+        "displaced": np.array([r["displaced"] for r in reps], float),
+        "ions_removed": int(sum(r["ions_removed"] for r in reps)),
+    }
+
+
+def _compare(per, steps, n_inst):
+    """Means, standard errors and the Welch z of mixed against FP64 per
+    output, and the paired differences (the same impacts in both)."""
+    out = {"steps": steps, "instances": n_inst,
+           "ions_removed": {p: per[p]["ions_removed"] for p in per}}
+    for key in ("yield", "range_A", "displaced"):
+        st = {}
+        for p in ("fp64", "mixed"):
+            a = per[p][key][np.isfinite(per[p][key])]
+            st[p] = float(a.mean())
+            st[p + "_sem"] = float(a.std(ddof=1) / np.sqrt(len(a)))
+        se = float(np.hypot(st["fp64_sem"], st["mixed_sem"]))
+        st["z_mixed_vs_fp64"] = (st["mixed"] - st["fp64"]) / se if se > 0 else 0.0
+        d = per["mixed"][key] - per["fp64"][key]
+        d = d[np.isfinite(d)]
+        sd = float(d.std(ddof=1) / np.sqrt(len(d))) if len(d) > 1 else 0.0
+        st["paired_mean_diff"] = float(d.mean()) if len(d) else 0.0
+        st["paired_z"] = st["paired_mean_diff"] / sd if sd > 0 else 0.0
+        out[key] = st
+    return out
 
 
 def dose_run(n_rep, ncell, impacts, steps_per_impact, precision):
@@ -280,47 +320,21 @@ def dose_run(n_rep, ncell, impacts, steps_per_impact, precision):
 
 def cascade_stats(instances, steps):
     """Per-instance cascade outputs after `steps` MD steps of the batch, FP64
-    and mixed: sputter yield (Si removed above the surface), Ga projected
-    range (surface - ion z, A; ions removed are left out), displaced Si
-    (> 1.5 A from the initial site).  Ensemble means, standard errors and the
-    Welch z of mixed against FP64 (cascades are chaotic: the statistics,
-    not single runs, must agree)."""
+    and mixed, compared (cascades are chaotic: the statistics, not single
+    runs, must agree)."""
     per = {}
     for prec in ("fp64", "mixed"):
         g = _md_handle(instances, prec)
         g.md_run(steps)
         reps = [g.md_report(k) for k in range(len(instances))]
         g.close()
-        per[prec] = {
-            "yield": np.array([r["removed"] for r in reps], float),
-            "range_A": np.array([(s["surface_z"] - r["ion_z"]) * 1e10 if r["ion_z"] < 1e299
-                                 else np.nan for s, r in zip(instances, reps)]),
-            "displaced": np.array([r["displaced"] for r in reps], float),
-            "ions_removed": int(sum(r["ions_removed"] for r in reps)),
-        }
-    out = {"steps": steps, "instances": len(instances),
-           "ions_removed": {p: per[p]["ions_removed"] for p in per}}
-    for key in ("yield", "range_A", "displaced"):
-        st = {}
-        for p in ("fp64", "mixed"):
-            a = per[p][key][np.isfinite(per[p][key])]
-            st[p] = float(a.mean())
-            st[p + "_sem"] = float(a.std(ddof=1) / np.sqrt(len(a)))
-        se = float(np.hypot(st["fp64_sem"], st["mixed_sem"]))
-        st["z_mixed_vs_fp64"] = (st["mixed"] - st["fp64"]) / se if se > 0 else 0.0
-        # the same impacts in both precisions: per-instance differences too
-        d = per["mixed"][key] - per["fp64"][key]
-        d = d[np.isfinite(d)]
-        sd = float(d.std(ddof=1) / np.sqrt(len(d))) if len(d) > 1 else 0.0
-        st["paired_mean_diff"] = float(d.mean()) if len(d) else 0.0
-        st["paired_z"] = st["paired_mean_diff"] / sd if sd > 0 else 0.0
-        out[key] = st
-    return out
+        per[prec] = _per_instance(instances, reps)
+    return _compare(per, steps, len(instances))
 
 
 def md_bench(args):
-    """MD-step throughput: the single 500 eV impact on an 8^3-cell slab, and
-    the batched ensemble (64 slabs of 10k-250k atoms, 1-10 keV, 0-60 deg),
+    """MD-step throughput: the single 500 eV impact on an 8^3-cell slab (alone,
+    and as a batch of replicas with their own seeds), and the batched ensemble (64 slabs of 10k-250k atoms, 1-10 keV, 0-60 deg),
     mixed-batched against the plain FP64 path (one instance at a time)."""
     from fib_md_amd import configs
     out = {}
@@ -335,6 +349,33 @@ def md_bench(args):
         out[f"single_impact_4097_{prec}"] = r
         log(f"  single impact {prec}: {r['ms_per_step']:.3f} ms/step, "
             f"{r['atom_steps_per_s']:.4g} atom-steps/s")
+    # the same configuration as a batch of replicas, each with its own seed
+    # (velocities and impact point), all 2000 steps timed: impacts/hour of
+    # the batched path, and the configuration's cascade statistics in both
+    # precisions
+    if args.replicas > 0:
+        reps = []
+        for k in range(args.replicas):
+            sk = configs.si_slab(8, 1001 + k)
+            x, v = configs.ion_launch(sk["L"], sk["surface_z"], 500.0, 0.0, 0.0,
+                                      np.random.default_rng(1001 + k))
+            sk.update(ion_x=x, ion_v=v)
+            reps.append(sk)
+        per = {}
+        for prec in ("fp64", "mixed"):
+            r = md_rate(reps, prec, STEPS_PER_IMPACT, keep_reports=True)
+            per[prec] = r.pop("per")
+            r["replicas"] = len(reps)
+            r["impacts_per_hour"] = 3600.0 * len(reps) / (r["ms_per_step"] * 1e-3 * STEPS_PER_IMPACT)
+            out[f"single_impact_4097_x{len(reps)}_{prec}"] = r
+            log(f"  single impact x{len(reps)} {prec}: {r['ms_per_step']:.3f} ms/step, "
+                f"{r['atom_steps_per_s']:.4g} atom-steps/s, {r['impacts_per_hour']:.4g} impacts/hour")
+        cs = _compare(per, STEPS_PER_IMPACT + 5, len(reps))
+        out[f"single_impact_4097_x{len(reps)}_stats"] = cs
+        for key in ("yield", "range_A", "displaced"):
+            st = cs[key]
+            log(f"  x{len(reps)} {key:9s} fp64 {st['fp64']:.3f} +- {st['fp64_sem']:.3f}, mixed "
+                f"{st['mixed']:.3f} +- {st['mixed_sem']:.3f}, z {st['z_mixed_vs_fp64']:.2f}")
     ens = configs.ensemble(args.ensemble)
     nat = sum(len(e["X"]) for e in ens)
     log(f"  ensemble: {len(ens)} instances, {nat} atoms")
@@ -466,6 +507,8 @@ def main():
     ap.add_argument("--no-md", action="store_true")
     ap.add_argument("--md-steps", type=int, default=100)
     ap.add_argument("--ensemble", type=int, default=64)
+    ap.add_argument("--replicas", type=int, default=1024,
+                    help="replicas of the single-impact configuration run as one batch")
     ap.add_argument("--stats-steps", type=int, default=2000,
                     help="MD steps of the cascade-statistics run (0: skip)")
     ap.add_argument("--big-steps", type=int, default=5000,
