@@ -65,6 +65,7 @@ struct DevBufs {
   double *V[2], *F[2], *Xi[2];
   int *inv;                  // [NA] atom_off + orig -> sorted position
   int *ibase;                // [ninst] scan offset of each instance's first bucket
+  int *binst;                // [NA / FIB_BLOCK] instance of each block of FIB_BLOCK atoms
   int *bkey, *brank, *bcnt, *cstart, *perm, *bsum;
   int *cnt_sw, *nbr_sw, *cnt_ion, *nbr_ion;
   uint16_t *rk_sw;           // [NA * Wsw]: (stencil index << 8) | rank of each row entry

@@ -50,6 +50,17 @@ namespace fibk {
 __device__ __forceinline__ bool gated_out(const int *gate, int inst) {
   return gate && !gate[1 + inst];
 }
+// Under a gate, the blocks of FIB_BLOCK atoms of instances that are not
+// rebuilt are skipped whole, before any per-atom load: every such block lies
+// inside one instance (binst), and the grid-stride loops that use this are
+// launched with FIB_BLOCK threads and advance by whole blocks, so all lanes
+// of a block take the same branch.
+__device__ __forceinline__ bool block_gated_out(const int *gate, const int *__restrict__ binst,
+                                                int p) {
+  if (!gate) return false;
+  const int b = __builtin_amdgcn_readfirstlane(p / FIB_BLOCK);
+  return !gate[1 + binst[b]];
+}
 
 // Element loops of the list-build kernels run grid-stride over a capped grid
 // (GRID_CAP workgroups): a build skipped by its gate then costs a small launch
@@ -263,17 +274,21 @@ __global__ void k_finalize2(const DInst *__restrict__ I, int nch, RedSpec rs,
 // orders every bucket by previous position.
 __global__ void k_bin(const double4 *__restrict__ X, int NA, const DInst *__restrict__ I,
                       int *__restrict__ bkey, int *__restrict__ brank, int *__restrict__ bcnt,
-                      DCtl *__restrict__ ctl, const int *__restrict__ gate) {
+                      DCtl *__restrict__ ctl, const int *__restrict__ binst,
+                      const int *__restrict__ gate) {
   GATE_OR_RETURN(gate);
   const int lane = threadIdx.x % FIB_WAVE;
   const uint64_t le = (lane == FIB_WAVE - 1) ? ~0ull : ((2ull << lane) - 1ull);  // lanes <= me
   // NA and the grid stride are multiples of the wavefront: the lanes of a
   // wavefront run the loop together
   GRID_STRIDE(p, NA) {
+    // atoms of instances that are not rebuilt keep no key (k_scatter skips
+    // the same blocks)
+    if (block_gated_out(gate, binst, p)) continue;
     const double4 x = X[p];
     const uint32_t m = meta_of(x);
     int b = -1;
-    if (!((m & META_PAD) || gated_out(gate, inst_of(m)))) {
+    if (!(m & META_PAD)) {
       const DInst &in = I[inst_of(m)];
       int local;
       if (m & META_DEAD) {
@@ -428,9 +443,10 @@ __global__ void k_scan_fix(int *__restrict__ out, int n, const DInst *__restrict
 
 __global__ void k_scatter(const int *__restrict__ bkey, const int *__restrict__ brank, int NA,
                           const int *__restrict__ cstart, int *__restrict__ perm,
-                          const int *__restrict__ gate) {
+                          const int *__restrict__ binst, const int *__restrict__ gate) {
   GATE_OR_RETURN(gate);
   GRID_STRIDE(p, NA) {
+    if (block_gated_out(gate, binst, p)) continue;
     const int b = bkey[p];
     if (b >= 0) perm[cstart[b] + brank[p]] = p;
   }
@@ -457,11 +473,13 @@ __global__ void k_reorder(int NA, const int *__restrict__ perm, const DInst *__r
                           const double *__restrict__ V0, double *__restrict__ V1,
                           const double *__restrict__ F0, double *__restrict__ F1,
                           const double *__restrict__ Xi0, double *__restrict__ Xi1,
-                          int *__restrict__ inv, const int *__restrict__ gate) {
+                          int *__restrict__ inv, const int *__restrict__ binst,
+                          const int *__restrict__ gate) {
   GATE_OR_RETURN(gate);
   GRID_STRIDE(p, NA) {
+  if (block_gated_out(gate, binst, p)) continue;
   uint32_t mp = meta_of(X0[p]);
-  if ((mp & META_PAD) || gated_out(gate, inst_of(mp))) continue;
+  if (mp & META_PAD) continue;
   int s = perm[p];
   double4 x = X0[s];
   X1[p] = x;
@@ -482,10 +500,10 @@ __global__ void k_copyback(int NA, const double4 *__restrict__ X1, double4 *__re
                            const double *__restrict__ V1, double *__restrict__ V0,
                            const double *__restrict__ F1, double *__restrict__ F0,
                            const double *__restrict__ Xi1, double *__restrict__ Xi0,
-                           const int *__restrict__ gate) {
+                           const int *__restrict__ binst, const int *__restrict__ gate) {
   GATE_OR_RETURN(gate);
   GRID_STRIDE(p, NA) {
-    if (gated_out(gate, inst_of(meta_of(X0[p])))) continue;
+    if (block_gated_out(gate, binst, p)) continue;
     X0[p] = X1[p];
     for (int c = 0; c < 3; ++c) {
       const size_t a = (size_t)c * NA + p;
@@ -790,11 +808,11 @@ void k_nbuild(
 // Atoms outside every scanned cell (dropped, dead, or phantom i-atoms) get
 // empty lists; k_nbuild only writes the atoms of real cells.
 __global__ void k_clear_counts(int NA, int *__restrict__ cnt_sw, int *__restrict__ cnt_ion,
-                               uint8_t *__restrict__ off_sw, const double4 *__restrict__ X,
+                               uint8_t *__restrict__ off_sw, const int *__restrict__ binst,
                                const int *__restrict__ gate) {
   GATE_OR_RETURN(gate);
   GRID_STRIDE(p, NA) {
-    if (gate && gated_out(gate, inst_of(meta_of(X[p])))) continue;
+    if (block_gated_out(gate, binst, p)) continue;
     cnt_sw[p] = 0;
     cnt_ion[p] = 0;
     // no row until k_nbuild writes one: offsets past any row (see k_force3)
@@ -1563,10 +1581,12 @@ __global__ __launch_bounds__(FIB_BLOCK) void k_extra(
 // After a list build: positions of the build, displacement bookkeeping reset.
 __global__ void k_mark_build(int NA, int ninst, int Wsw, int Wion, const double4 *__restrict__ X,
                              double4 *__restrict__ Xb, float *__restrict__ disp,
-                             DCtl *__restrict__ ctl, const int *__restrict__ gate) {
+                             DCtl *__restrict__ ctl, const int *__restrict__ binst,
+                             const int *__restrict__ gate) {
   GATE_OR_RETURN(gate);
   GRID_STRIDE(i, max(NA, ninst)) {
-  if (i < NA && !gated_out(gate, inst_of(meta_of(X[i])))) {
+  // NA is a multiple of FIB_BLOCK: the lanes of a block are all below it or all above
+  if (i < NA && !block_gated_out(gate, binst, i)) {
     Xb[i] = X[i];
     disp[i] = 0.0f;
   }
@@ -2237,7 +2257,7 @@ void launch_bin(const DevBufs &d, hipStream_t s) {
   {
     KScope ks_(d, KT_BIN);
     hipLaunchKernelGGL(k_bin, dim3(gcap(d.NA)), dim3(FIB_BLOCK), 0, s, d.X[d.cur], d.NA,
-                       d.I, d.bkey, d.brank, d.bcnt, d.ctl, d.gate);
+                       d.I, d.bkey, d.brank, d.bcnt, d.ctl, d.binst, d.gate);
   }
 }
 
@@ -2273,7 +2293,7 @@ void launch_sort(const DevBufs &d, hipStream_t s, bool inplace) {
   {
     KScope ks_(d, KT_SORT);
     hipLaunchKernelGGL(k_scatter, dim3(gcap(d.NA)), dim3(FIB_BLOCK), 0, s, d.bkey,
-                       d.brank, d.NA, d.cstart, d.perm, d.gate);
+                       d.brank, d.NA, d.cstart, d.perm, d.binst, d.gate);
   }
   {
     KScope ks_(d, KT_SORT);
@@ -2285,12 +2305,13 @@ void launch_sort(const DevBufs &d, hipStream_t s, bool inplace) {
     KScope ks_(d, KT_SORT);
     hipLaunchKernelGGL(k_reorder, dim3(gcap(d.NA)), dim3(FIB_BLOCK), 0, s, d.NA, d.perm,
                        d.I, d.X[c], d.X[n], d.V[c], d.V[n], d.F[c], d.F[n], d.Xi[c], d.Xi[n],
-                       d.inv, d.gate);
+                       d.inv, d.binst, d.gate);
   }
   if (inplace) {
     KScope ks_(d, KT_SORT);
     hipLaunchKernelGGL(k_copyback, dim3(gcap(d.NA)), dim3(FIB_BLOCK), 0, s, d.NA, d.X[n],
-                       d.X[c], d.V[n], d.V[c], d.F[n], d.F[c], d.Xi[n], d.Xi[c], d.gate);
+                       d.X[c], d.V[n], d.V[c], d.F[n], d.F[c], d.Xi[n], d.Xi[c], d.binst,
+                       d.gate);
   }
 }
 
@@ -2298,7 +2319,7 @@ void launch_nbuild(const DevBufs &d, const FConst &fc, hipStream_t s) {
   {
     KScope ks_(d, KT_LISTAUX);
     hipLaunchKernelGGL(k_clear_counts, dim3(gcap(d.NA)), dim3(FIB_BLOCK), 0, s, d.NA,
-                       d.cnt_sw, d.cnt_ion, d.off_sw, d.X[d.cur], d.gate);
+                       d.cnt_sw, d.cnt_ion, d.off_sw, d.binst, d.gate);
   }
   if (d.total_cells > 0) {
     KScope ks_(d, KT_NBUILD);
@@ -2465,7 +2486,8 @@ void launch_md_report(const DevBufs &d, const StepArgs &sa, hipStream_t s) {
 void launch_mark_build(const DevBufs &d, hipStream_t s) {
   KScope ks_(d, KT_LISTAUX);
   hipLaunchKernelGGL(k_mark_build, dim3(gcap(std::max(d.NA, d.ninst))), dim3(FIB_BLOCK), 0,
-                     s, d.NA, d.ninst, d.Wsw, d.Wion, d.X[d.cur], d.Xb, d.disp, d.ctl, d.gate);
+                     s, d.NA, d.ninst, d.Wsw, d.Wion, d.X[d.cur], d.Xb, d.disp, d.ctl, d.binst,
+                     d.gate);
   if (d.gate)
     hipLaunchKernelGGL(k_clear_gate, dim3(1), dim3(FIB_BLOCK), 0, s, (int *)d.gate, d.ninst);
 }

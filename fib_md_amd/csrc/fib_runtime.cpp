@@ -261,7 +261,7 @@ static void derive_constants(fib_handle *h) {
 
 static int32_t free_bufs(fib_handle *h) {
   DevBufs &d = h->d;
-  void *ptrs[] = {d.I, d.ctl, d.ibase, d.X[0], d.X[1], d.V[0], d.V[1], d.F[0], d.F[1], d.Xi[0], d.Xi[1],
+  void *ptrs[] = {d.I, d.ctl, d.ibase, d.binst, d.X[0], d.X[1], d.V[0], d.V[1], d.F[0], d.F[1], d.Xi[0], d.Xi[1],
                   d.inv, d.bkey, d.brank, d.bcnt, d.cstart, d.perm, d.bsum, d.cnt_sw, d.nbr_sw,
                   d.cnt_ion, d.nbr_ion, d.rk_sw, d.off_sw, d.G, d.RA, d.amask, d.cnt_leg, d.Xb, d.disp, d.EX, d.EXself, d.exmask, d.Fpart, d.upart, d.Fout, d.uout,
                   d.partials, d.fin_tmp, d.tab, d.xfer};
@@ -391,6 +391,7 @@ extern "C" int32_t fib_create(const fib_params *p, int32_t ninst, const fib_inst
   }
   ALLOC(d.inv, NA);
   ALLOC(d.ibase, (size_t)ninst);
+  ALLOC(d.binst, (size_t)NA / FIB_BLOCK);
   ALLOC(d.bkey, NA);
   ALLOC(d.brank, NA);
   ALLOC(d.bcnt, (size_t)d.nbucket);
@@ -431,6 +432,12 @@ extern "C" int32_t fib_create(const fib_params *p, int32_t ninst, const fib_inst
     return rc_;
   }
   hipError_t e = hipMemcpy(d.I, h->inst.data(), sizeof(DInst) * ninst, hipMemcpyHostToDevice);
+  if (e == hipSuccess) {
+    std::vector<int> binst((size_t)NA / FIB_BLOCK);
+    for (int k = 0; k < ninst; ++k)
+      for (int b = 0; b < h->inst[k].nblocks; ++b) binst[(size_t)h->inst[k].block_off + b] = k;
+    e = hipMemcpy(d.binst, binst.data(), sizeof(int) * binst.size(), hipMemcpyHostToDevice);
+  }
   if (e == hipSuccess) e = hipMemset(d.ctl, 0, sizeof(DCtl) * ninst);
   if (e == hipSuccess) e = hipMemset(d.cnt_sw, 0, sizeof(int) * NA);
   if (e == hipSuccess) e = hipMemset(d.cnt_ion, 0, sizeof(int) * NA);
