@@ -1711,15 +1711,21 @@ __global__ __launch_bounds__(FIB_BLOCK) void k_disp2(const double4 *__restrict__
 // Rebuild gates of the step: each instance whose two largest displacements
 // since the build add up to more than the skin (only then can a pair the
 // build left out have come within the cutoff), and whether there is any.
-__global__ void k_md_decide(int ninst, const DInst *__restrict__ I, DCtl *__restrict__ ctl,
-                            double skin, int *__restrict__ gate) {
-  // one wave; behind the gates, the prefix over instances of the real cells
-  // of the gated ones (the cell numbering of a gated k_nbuild)
-  const int lane = threadIdx.x;
+#define DECIDE_T 1024  // threads of k_md_decide: instances in flight at once
+__global__ __launch_bounds__(DECIDE_T) void k_md_decide(int ninst, const DInst *__restrict__ I,
+                                                      DCtl *__restrict__ ctl, double skin,
+                                                      int *__restrict__ gate) {
+  // one workgroup; behind the gates, the prefix over instances of the real
+  // cells of the gated ones (the cell numbering of a gated k_nbuild)
+  __shared__ int s_w[DECIDE_T / FIB_WAVE];
+  __shared__ int s_any;
+  const int t = threadIdx.x, lane = t % FIB_WAVE, wv = t / FIB_WAVE;
   int *cplan = gate + 1 + ninst;
-  int any = 0, carry = 0;
-  for (int k0 = 0; k0 < ninst; k0 += FIB_WAVE) {
-    const int k = k0 + lane;
+  if (t == 0) s_any = 0;
+  __syncthreads();
+  int carry = 0;
+  for (int k0 = 0; k0 < ninst; k0 += DECIDE_T) {
+    const int k = k0 + t;
     int g = 0, w = 0;
     if (k < ninst) {
       const DCtl &c = ctl[k];
@@ -1729,18 +1735,24 @@ __global__ void k_md_decide(int ninst, const DInst *__restrict__ I, DCtl *__rest
       gate[1 + k] = g;
       if (g) w = I[k].nc[0] * I[k].nc[1] * I[k].nc[2];
     }
+    // inclusive scan of w: in the wave, then over the waves' totals
     int sc = w;
     for (int o = 1; o < FIB_WAVE; o <<= 1) {
       const int y = __shfl_up(sc, o);
       if (lane >= o) sc += y;
     }
-    if (k < ninst) cplan[k] = carry + sc - w;
-    carry += __shfl(sc, FIB_WAVE - 1);
-    any |= __ballot(g) != 0ull;
+    if (lane == FIB_WAVE - 1) s_w[wv] = sc;
+    if (__ballot(g) != 0ull && lane == 0) s_any = 1;
+    __syncthreads();
+    int before = carry;
+    for (int v = 0; v < wv; ++v) before += s_w[v];
+    if (k < ninst) cplan[k] = before + sc - w;
+    for (int v = 0; v < DECIDE_T / FIB_WAVE; ++v) carry += s_w[v];
+    __syncthreads();  // s_w is rewritten by the next chunk
   }
-  if (lane == 0) {
+  if (t == 0) {
     cplan[ninst] = carry;
-    gate[0] = any;
+    gate[0] = s_any;
   }
 }
 
@@ -2457,7 +2469,7 @@ void launch_md_pos(const DevBufs &d, const StepArgs &sa, int *gate, hipStream_t 
   const int op2[2] = {1, 0};
   const int dst2[2] = {DCTL_IDX(maxdisp2), DCTL_IDX(nmaxdisp)};
   finalize(d, 2, FIB_BLOCK, op2, dst2, s);
-  hipLaunchKernelGGL(k_md_decide, dim3(1), dim3(FIB_WAVE), 0, s, d.ninst, d.I, d.ctl, sa.skin,
+  hipLaunchKernelGGL(k_md_decide, dim3(1), dim3(DECIDE_T), 0, s, d.ninst, d.I, d.ctl, sa.skin,
                      gate);
 }
 
