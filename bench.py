@@ -332,6 +332,21 @@ def cascade_stats(instances, steps):
     return _compare(per, steps, len(instances))
 
 
+def single_impact_replicas(n):
+    """n replicas of the single-impact configuration (8^3-cell Si(100) slab,
+    500 eV Ga at normal incidence), replica k with seed 1001 + k for its
+    velocities and impact point."""
+    from fib_md_amd import configs
+    reps = []
+    for k in range(n):
+        sk = configs.si_slab(8, 1001 + k)
+        x, v = configs.ion_launch(sk["L"], sk["surface_z"], 500.0, 0.0, 0.0,
+                                  np.random.default_rng(1001 + k))
+        sk.update(ion_x=x, ion_v=v)
+        reps.append(sk)
+    return reps
+
+
 def md_bench(args):
     """MD-step throughput: the single 500 eV impact on an 8^3-cell slab (alone,
     and as a batch of replicas with their own seeds), and the batched ensemble (64 slabs of 10k-250k atoms, 1-10 keV, 0-60 deg),
@@ -354,13 +369,7 @@ def md_bench(args):
     # the batched path, and the configuration's cascade statistics in both
     # precisions
     if args.replicas > 0:
-        reps = []
-        for k in range(args.replicas):
-            sk = configs.si_slab(8, 1001 + k)
-            x, v = configs.ion_launch(sk["L"], sk["surface_z"], 500.0, 0.0, 0.0,
-                                      np.random.default_rng(1001 + k))
-            sk.update(ion_x=x, ion_v=v)
-            reps.append(sk)
+        reps = single_impact_replicas(args.replicas)
         per = {}
         for prec in ("fp64", "mixed"):
             r = md_rate(reps, prec, STEPS_PER_IMPACT, keep_reports=True)
@@ -518,7 +527,7 @@ def main():
                     help="sequential impacts per replica of the shortened dose run (0: skip)")
     ap.add_argument("--dose-steps", type=int, default=1500,
                     help="MD steps per impact of the dose run")
-    ap.add_argument("--profile", choices=["fp64", "mixed", "md-fp64", "md-mixed"], default=None)
+    ap.add_argument("--profile", choices=["fp64", "mixed", "md-fp64", "md-mixed", "md-replicas"], default=None)
     ap.add_argument("--summarize-stats", nargs=3, metavar=("DIR", "OUT", "CMD"))
     ap.add_argument("--summarize-pmc", nargs=4, metavar=("FETCH_DIR", "WRITE_DIR", "OUT", "CMD"))
     ap.add_argument("--summarize-counters", nargs="+", metavar="OUT CMD DIR",
@@ -550,6 +559,11 @@ def main():
         from fib_md_amd import configs
         ens = configs.ensemble(args.ensemble)
         r = md_rate(ens, args.profile[3:], 60, warmup=2)
+        log(json.dumps(r))
+        return
+    if args.profile == "md-replicas":
+        # replicas of the single-impact configuration, mixed, a few MD steps
+        r = md_rate(single_impact_replicas(args.replicas), "mixed", 60, warmup=2)
         log(json.dumps(r))
         return
     if args.profile:
