@@ -53,6 +53,15 @@ def crystal_with_ions(n=4, sigma=0.1e-10, seed=7, nions=3):
     return Xa, at, (L, L, L), len(X), nions
 
 
+def with_ge(at, frac=0.3, seed=5):
+    """A copy of `at` with a random fraction of the Si atoms made Ge (type 2)."""
+    rng = np.random.default_rng(seed)
+    at = np.array(at, np.int32)
+    si = np.nonzero(at == 1)[0]
+    at[rng.choice(si, int(frac * len(si)), replace=False)] = 2
+    return at
+
+
 def reference_slab(nc=(4, 4, 3), nions=2, seed=11):
     """The reference's geometry: pureepi lattice in a box 10x the slab in z,
     ions in storage at ionz (prms.f90:484-557, data.f90:223-260)."""

@@ -20,12 +20,15 @@ def _sets_equal(o, g):
     return bad2, badlj
 
 
-@pytest.mark.parametrize("case", ["bulk", "ions", "slab", "small"])
+@pytest.mark.parametrize("case", ["bulk", "ions", "slab", "small", "sige"])
 def test_cells_and_lists_bit_exact(oracle_lib, case):
     if case == "bulk":
         X, at, L, nsg, nlj = gc.crystal_with_ions(4, 0.12e-10, 3, 0)
     elif case == "ions":
         X, at, L, nsg, nlj = gc.crystal_with_ions(4, 0.12e-10, 5, 4)
+    elif case == "sige":
+        X, at, L, nsg, nlj = gc.crystal_with_ions(4, 0.12e-10, 6, 3)
+        at = gc.with_ge(at, 0.4, 7)
     elif case == "slab":
         X, at, L, nsg, nlj = gc.reference_slab((4, 4, 3), 3)
     else:
@@ -55,12 +58,16 @@ def test_binning_edge_positions(oracle_lib):
     assert not bad2 and not badlj
 
 
-@pytest.mark.parametrize("case", ["bulk", "ions", "slab"])
+@pytest.mark.parametrize("case", ["bulk", "ions", "slab", "sige"])
 def test_force_fp64(oracle_lib, case):
     if case == "bulk":
         X, at, L, nsg, nlj = gc.crystal_with_ions(5, 0.12e-10, 21, 0)
     elif case == "ions":
         X, at, L, nsg, nlj = gc.crystal_with_ions(4, 0.12e-10, 23, 4)
+    elif case == "sige":
+        # Si/Ge mixing of the pair and three-body terms, Ge-Ga Moliere pairs
+        X, at, L, nsg, nlj = gc.crystal_with_ions(4, 0.12e-10, 25, 4)
+        at = gc.with_ge(at, 0.4, 9)
     else:
         X, at, L, nsg, nlj = gc.reference_slab((4, 4, 3), 3)
     impact = nlj  # all ions fired
@@ -98,6 +105,25 @@ def test_force_mixed_vs_fp64(oracle_lib):
     r = g.force(0, energy=True, virial=True)
     emax, erms, enorm = gc.rel_err(r["F"], Fo)
     print(f"mixed: max {emax:.3e} rms {erms:.3e} norm {enorm:.3e}")
+    assert emax <= 1e-5
+    assert abs(r["utot"] - Uo) <= 1e-6 * abs(Uo)
+    assert np.abs(r["virial"] - Wo).max() <= 1e-5 * np.abs(Wo).max()
+
+
+def test_force_mixed_sige_with_ions(oracle_lib):
+    """Mixed path on a Si/Ge crystal with fired Ga: the species-mixed SW
+    terms (length unit and eps/sigma of each pair, Ge legs in the three-body
+    pass) and the FP32 Moliere pairs against the oracle's FP64, to the same
+    bounds as the pure-Si case."""
+    X, at, L, nsg, nlj = gc.crystal_with_ions(5, 0.12e-10, 27, 4)
+    at = gc.with_ge(at, 0.4, 11)
+    o = gc.oracle_system(X, at, L, nsg, nlj, impact=nlj)
+    Fo, Wo = o.si_force(X, virial=True)
+    Uo, _ = o.si_potential(X)
+    g = gc.gpu_system(X, at, L, nsg, nlj, precision="mixed")
+    r = g.force(nlj, energy=True, virial=True)
+    emax, erms, enorm = gc.rel_err(r["F"], Fo)
+    print(f"mixed SiGe+Ga: max {emax:.3e} rms {erms:.3e} norm {enorm:.3e}")
     assert emax <= 1e-5
     assert abs(r["utot"] - Uo) <= 1e-6 * abs(Uo)
     assert np.abs(r["virial"] - Wo).max() <= 1e-5 * np.abs(Wo).max()

@@ -22,6 +22,7 @@ from fib_md_amd import configs
 from tests import gpu_common as gc
 
 EPS_SI = 3.473928e-19
+EPS_GE = 3.085e-19  # prms.f90 Ge SW energy
 SIGMA = 2.0951e-10
 EV = 1.60217646e-19
 COULOMB_EV_A = 14.399645  # e^2/(4 pi eps0) in eV*A (CODATA)
@@ -64,6 +65,35 @@ def test_perfect_diamond_energy_and_forces(oracle_lib):
     assert np.abs(F).max() <= 1e-9 * EPS_SI / SIGMA
     # 92 connected triples per atom in the buffered crystal graph
     assert o.geti("nbrs") == 92 * N
+
+
+def test_perfect_diamond_sige_bond_energies(oracle_lib):
+    """Random Si/Ge occupation of the perfect diamond at the SW minimum
+    (sigma_Ge = sigma_Si): every bond sits at the pair minimum and every angle
+    is tetrahedral, so all forces vanish and each bond carries -eps_ij, with
+    the mixing eps_SiGe = (eps_Si + eps_Ge) / 2 (prms.f90:452-480).  The per-atom
+    energy is half of its four bonds."""
+    a = 4 * 2 ** (1 / 6) * SIGMA / math.sqrt(3)
+    n = 3
+    X = configs.diamond(n, n, n, a)
+    N = len(X)
+    L = n * a
+    at = gc.with_ge(np.ones(N, np.int32), 0.4, 17)
+    eps = {1: EPS_SI, 2: EPS_GE}
+    o = gc.oracle_system(X, at, (L,) * 3, N, 0)
+    U, u = o.si_potential(X)
+    # nearest neighbours by brute force (distance a sqrt(3)/4)
+    d = X[:, None, :] - X[None, :, :]
+    d -= np.round(d / L) * L
+    r = np.sqrt((d * d).sum(2))
+    nn = np.abs(r - a * math.sqrt(3) / 4) < 1e-3 * a
+    assert (nn.sum(1) == 4).all()
+    e_ij = np.array([[0.5 * (eps[int(at[i])] + eps[int(at[j])]) for j in range(N)] for i in range(N)])
+    u_ref = -0.5 * (e_ij * nn).sum(1)
+    np.testing.assert_allclose(u, u_ref, rtol=1e-9)
+    assert abs(U - u_ref.sum()) <= 1e-9 * abs(u_ref.sum())
+    F = o.si_force(X)
+    assert np.abs(F).max() <= 1e-9 * EPS_SI / SIGMA
 
 
 def test_moliere_coulomb_limit_and_firsov(oracle_lib):
@@ -112,8 +142,11 @@ def _perturbed_with_ions(seed=3):
     return gc.crystal_with_ions(3, 0.15e-10, seed, 2)
 
 
-def test_force_is_minus_gradient(oracle_lib):
+@pytest.mark.parametrize("species", ["si", "sige"])
+def test_force_is_minus_gradient(oracle_lib, species):
     X, at, L, nsg, nlj = _perturbed_with_ions()
+    if species == "sige":
+        at = gc.with_ge(at, 0.4, 23)
     o = gc.oracle_system(X, at, L, nsg, nlj, impact=nlj)
     F = o.si_force(X)
     h = 1e-15
