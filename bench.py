@@ -22,6 +22,7 @@ from __future__ import annotations
 import argparse
 import json
 import os
+import subprocess
 import sys
 import time
 
@@ -480,11 +481,10 @@ def rel_err(Fg, Fo, floor_frac=1e-3, detail=False):
     return out
 
 
-def cpu_baseline(ncell=24, reps=5):
-    """The oracle (scalar C restatement of si_nlist/si_force/si_potential) on a
-    bounded sample of the same workload, timed on one host core."""
-    from oracle.oracle import Oracle, build
-    build()
+def _cpu_eval_rate(ncell, reps):
+    """One process: the oracle's si_nlist + si_force + si_potential on the
+    ncell^3-cell sample, median of `reps` evaluations (atoms, seconds)."""
+    from oracle.oracle import Oracle
     X, at, L = make_workload(ncell, seed=1002)
     n = len(X)
     o = Oracle()
@@ -499,11 +499,36 @@ def cpu_baseline(ncell=24, reps=5):
         o.si_force(X)
         o.si_potential(X)
         ts.append(time.perf_counter() - t0)
-    t = float(np.median(ts))
-    return dict(value=n / t, unit="atom-evals/s", cores=1, kind="port",
+    return n, float(np.median(ts))
+
+
+CPU_PROCS = 16  # the host cores of one GPU's share of the machine
+
+
+def cpu_baseline(ncell=24, reps=5, procs=CPU_PROCS):
+    """The oracle (scalar C restatement of si_nlist/si_force/si_potential) on a
+    bounded sample of the same workload: one process on one core, then
+    `procs` independent processes at once (one per core, as the reference's
+    MPI ranks would use the host), each timing its own copy of the sample."""
+    from oracle.oracle import build
+    build()
+    n, t1 = _cpu_eval_rate(ncell, reps)
+    procs = max(1, min(procs, os.cpu_count() or 1))
+    cmd = [sys.executable, os.path.abspath(__file__), "--cpu-worker", str(ncell), str(reps)]
+    ps = [subprocess.Popen(cmd, stdout=subprocess.PIPE, text=True) for _ in range(procs)]
+    rates = []
+    for p in ps:
+        out, _ = p.communicate()
+        if p.returncode != 0:
+            raise RuntimeError(f"cpu worker failed ({p.returncode})")
+        w = json.loads(out.strip().splitlines()[-1])
+        rates.append(w["atoms"] / w["seconds"])
+    return dict(value=float(sum(rates)), unit="atom-evals/s", cores=procs, kind="port",
+                single_core_value=n / t1,
                 sample=f"{ncell}^3-cell crystal ({n} atoms, same noise and 20 A amorphous "
-                       f"sphere), si_nlist+si_force+si_potential, median of {reps}, "
-                       f"{t:.2f} s per evaluation")
+                       f"sphere), si_nlist+si_force+si_potential, median of {reps}; "
+                       f"{procs} independent processes at once, one per core, rates summed "
+                       f"(one core alone: {t1:.2f} s per evaluation)")
 
 
 def main():
@@ -529,10 +554,16 @@ def main():
                     help="MD steps per impact of the dose run")
     ap.add_argument("--profile", choices=["fp64", "mixed", "md-fp64", "md-mixed", "md-replicas"], default=None)
     ap.add_argument("--summarize-stats", nargs=3, metavar=("DIR", "OUT", "CMD"))
+    ap.add_argument("--cpu-worker", nargs=2, metavar=("NCELL", "REPS"), help=argparse.SUPPRESS)
     ap.add_argument("--summarize-pmc", nargs=4, metavar=("FETCH_DIR", "WRITE_DIR", "OUT", "CMD"))
     ap.add_argument("--summarize-counters", nargs="+", metavar="OUT CMD DIR",
                     help="OUT CMD DIR [DIR ...]: SQ/GRBM counter table of rocprofv3 --pmc runs")
     args = ap.parse_args()
+    if args.cpu_worker:
+        # one CPU baseline process (no GPU use): atoms and median seconds
+        n, t = _cpu_eval_rate(int(args.cpu_worker[0]), int(args.cpu_worker[1]))
+        print(json.dumps({"atoms": n, "seconds": t}), flush=True)
+        return
     if args.summarize_stats:
         summarize_stats(*args.summarize_stats)
         return
