@@ -382,6 +382,19 @@ def md_bench(args):
                 f"{r['atom_steps_per_s']:.4g} atom-steps/s, {r['impacts_per_hour']:.4g} impacts/hour")
         cs = _compare(per, STEPS_PER_IMPACT + 5, len(reps))
         out[f"single_impact_4097_x{len(reps)}_stats"] = cs
+        # the list radius factor chosen by the autotuner for this batch
+        from fib_md_amd import autotune
+        from fib_md_amd.gpu import default_params
+        lf, table = autotune.tune_listfac(reps, "mixed")
+        pt = default_params()
+        pt.listfac = lf
+        r = md_rate(reps, "mixed", STEPS_PER_IMPACT, params=pt)
+        r.update(replicas=len(reps), listfac=lf,
+                 tuning_ms_per_step={f"{a:.2f}": b for a, b, _ in table})
+        r["impacts_per_hour"] = 3600.0 * len(reps) / (r["ms_per_step"] * 1e-3 * STEPS_PER_IMPACT)
+        out[f"single_impact_4097_x{len(reps)}_mixed_autotuned"] = r
+        log(f"  single impact x{len(reps)} mixed, autotuned listfac {lf:.2f}: "
+            f"{r['ms_per_step']:.3f} ms/step, {r['impacts_per_hour']:.4g} impacts/hour")
         for key in ("yield", "range_A", "displaced"):
             st = cs[key]
             log(f"  x{len(reps)} {key:9s} fp64 {st['fp64']:.3f} +- {st['fp64_sem']:.3f}, mixed "
