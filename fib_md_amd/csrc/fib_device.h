@@ -21,6 +21,7 @@
 #define META_FROZEN 0x4u      // xsides > 0: pinned to Xi every step
 #define META_DEAD 0x8u        // removed from the simulation
 #define META_PAD 0x10u        // padding slot
+#define META_LEAVING 0x20u    // removed in the current MD step (with META_DEAD)
 #define META_INST_SHIFT 8
 
 // Per-instance geometry and bookkeeping (device copy in an array).
@@ -117,7 +118,8 @@ struct DCtl {
   int32_t nonfinite;    // set by k_bin on a NaN/Inf position
   int32_t extra_any;    // k_extra found vertex terms outside a centre's list
   int32_t overflow;     // a neighbour list exceeded its width (sticky)
-  int32_t pad;
+  int32_t leave_overflow;  // more atoms left in one MD step than the removal
+                           // list holds (sticky)
 };
 
 // Double index of a DCtl field (for the reduction finalize kernel).

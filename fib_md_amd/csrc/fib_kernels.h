@@ -7,6 +7,7 @@
 
 // Workgroup size of the per-instance reductions (k_finalize).
 #define FIB_FIN_T 1024
+#define FIB_LEAVE_CAP 256  // atoms that may leave the simulation in one MD step
 
 // Per-step arguments of the integrate kernels (onestep, timeint.f90:158-242).
 struct StepArgs {
@@ -115,7 +116,10 @@ void launch_force_b(const DevBufs &d, const FConst &fc, bool mixed, bool energy,
                     const StepArgs &sa, double *Fout, hipStream_t s);
 void launch_pos(const DevBufs &d, const StepArgs &sa, hipStream_t s);
 // MD-step mode: head of the step (kick, drift, pins, removal) + rebuild gate
-void launch_md_pos(const DevBufs &d, const StepArgs &sa, int *gate, hipStream_t s);
+// gate: [2 + 2*ninst] rebuild gates and cell prefix, then the removal list
+// [1 + FIB_LEAVE_CAP] (count, atoms removed in the step)
+void launch_md_pos(const DevBufs &d, const FConst &fc, const StepArgs &sa, int *gate,
+                   hipStream_t s);
 // MD-step mode: time, next dt, shell factor (after the MODE 2 force pass)
 void launch_md_ctl(const DevBufs &d, const StepArgs &sa, hipStream_t s);
 // MD-step mode: dt from the current |v|max (reset = also time, steps, factor)

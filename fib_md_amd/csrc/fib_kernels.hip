@@ -1634,7 +1634,8 @@ __global__ __launch_bounds__(FIB_BLOCK) void k_md_pos(double4 *__restrict__ X,
                                                       int pstride, double *__restrict__ partials,
                                                       const DCtl *__restrict__ ctl,
                                                       const double4 *__restrict__ Xb,
-                                                      float *__restrict__ disp) {
+                                                      float *__restrict__ disp,
+                                                      int *__restrict__ leave) {
   const int lb = xcd_remap(blockIdx.x, gridDim.x);
   const int i = lb * FIB_BLOCK + threadIdx.x;
   double4 x = X[i];
@@ -1668,11 +1669,15 @@ __global__ __launch_bounds__(FIB_BLOCK) void k_md_pos(double4 *__restrict__ X,
     for (int c = 0; c < 3; ++c) xx[c] = fmod(xx[c] + 2. * in.lb[c], in.lb[c]);
     if (xx[2] - in.surf_z > sa.remove_dz && xx[2] < in.lb[2] - sa.remove_dz) {
       // sputtered (or backscattered): out of the simulation from now on,
-      // with its kinetic energy (beyond every cutoff, it has no potential)
-      mi |= META_DEAD;
+      // with its kinetic energy here; its interaction energy with the atoms
+      // that stay (an atom of a sputtered cluster is still bound) is booked
+      // by k_md_decide from the removal list
+      mi |= META_DEAD | META_LEAVING;
       red[2] = 0.5 * m * (v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
       for (int c = 0; c < 3; ++c) v[c] = 0.0;
       x.w = __longlong_as_double((long long)((wbits(x) & ~0xffffffffull) | mi));
+      const int slot = atomicAdd(&leave[0], 1);
+      if (slot < FIB_LEAVE_CAP) leave[1 + slot] = i;
     }
     x.x = xx[0]; x.y = xx[1]; x.z = xx[2];
     X[i] = x;
@@ -1711,15 +1716,178 @@ __global__ __launch_bounds__(FIB_BLOCK) void k_disp2(const double4 *__restrict__
 // Rebuild gates of the step: each instance whose two largest displacements
 // since the build add up to more than the skin (only then can a pair the
 // build left out have come within the cutoff), and whether there is any.
-#define DECIDE_T 1024  // threads of k_md_decide: instances in flight at once
-__global__ __launch_bounds__(DECIDE_T) void k_md_decide(int ninst, const DInst *__restrict__ I,
-                                                      DCtl *__restrict__ ctl, double skin,
-                                                      int *__restrict__ gate) {
-  // one workgroup; behind the gates, the prefix over instances of the real
-  // cells of the gated ones (the cell numbering of a gated k_nbuild)
+// Interaction energy (FP64, J) of atom i, removed in this step, with the
+// atoms that stay: every SW pair and three-body term and every Moliere pair
+// it takes part in (si.f90:278-368 forms), each term divided by the number
+// of its atoms that leave in this step, so that a term is booked once.  The
+// lists are those of the last build, which hold every pair in range.
+__device__ double leaving_energy(int i, const double4 *__restrict__ X,
+                                 const DInst *__restrict__ I, const FConst &fc, int Wsw,
+                                 int Wion, const int *__restrict__ cnt_sw,
+                                 const int *__restrict__ nbr_sw, const int *__restrict__ cnt_ion,
+                                 const int *__restrict__ nbr_ion) {
+  // the parameters in scalars: indexing the parameter block by species (or
+  // taking its address in a lambda) would copy it to scratch memory
+  const double isg00 = fc.isig[0][0], isg01 = fc.isig[0][1], isg11 = fc.isig[1][1];
+  const double eps00 = fc.eps[0][0], eps01 = fc.eps[0][1], eps11 = fc.eps[1][1];
+  const double lp0 = fc.leps[0][0][0], lp1 = fc.leps[0][0][1], lp2 = fc.leps[0][1][1],
+               lp3 = fc.leps[1][1][1];
+  const double swA = fc.A, swB = fc.B, al = fc.al, gam = fc.gamma, cut = fc.cut_sw;
+  const double z1 = fc.zmol[0], z2 = fc.zmol[1], z3 = fc.zmol[2];
+  const double a13 = fc.aamol[0][2], a23 = fc.aamol[1][2], a33 = fc.aamol[2][2];
+  const double c0 = fc.molc[0], c1 = fc.molc[1], c2 = fc.molc[2];
+  const double b0 = fc.molb[0], b1 = fc.molb[1], b2 = fc.molb[2];
+  const double ie0 = fc.ie0, ec = fc.ec, rlj2 = fc.rlj2;
+  auto gone = [](uint32_t m) { return (m & META_DEAD) && !(m & META_LEAVING); };
+  auto leaves = [](uint32_t m) { return (m & META_LEAVING) ? 1 : 0; };
+  auto isg = [=](int ta, int tb) {
+    const int g = (ta == 2) + (tb == 2);
+    return g == 0 ? isg00 : g == 1 ? isg01 : isg11;
+  };
+  auto legE = [=](int ta, int tb, double r) {  // exp(gamma / (r/sigma_ab - a)), 0 out of range
+    const double rs = r * isg(ta, tb);
+    return rs < cut ? exp(gam / (rs - al)) : 0.0;
+  };
+  auto leps = [=](int ta, int tb, int tc) {  // by the number of Ge atoms of the term
+    const int g = (ta == 2) + (tb == 2) + (tc == 2);
+    return g == 0 ? lp0 : g == 1 ? lp1 : g == 2 ? lp2 : lp3;
+  };
+  const double4 xi = X[i];
+  const uint32_t mi = meta_of(xi);
+  const DInst &in = I[inst_of(mi)];
+  const int ti = type_of(mi);
+  double e = 0.0;
+  if (ti < 3) {
+    const int ci = min(cnt_sw[i], Wsw);
+    const int *ri = nbr_sw + (size_t)i * Wsw;
+    for (int a = 0; a < ci; ++a) {
+      const int j = ri[a];
+      const double4 xj = X[j];
+      const uint32_t mj = meta_of(xj);
+      if (gone(mj)) continue;
+      const int tj = type_of(mj);
+      double d0, d1, d2;  // x_i - x_j
+      min_image_fast(xi, xj, in, d0, d1, d2);
+      const double rij = sqrt(d0 * d0 + d1 * d1 + d2 * d2);
+      const double rs = rij * isg(ti, tj);
+      if (!(rs < cut)) continue;
+      const double rs2 = rs * rs;
+      const int g = (ti == 2) + (tj == 2);
+      const double eij = g == 0 ? eps00 : g == 1 ? eps01 : eps11;
+      const double phi = eij * swA * (swB / (rs2 * rs2) - 1.0) * exp(1.0 / (rs - al));
+      e += phi / (1 + leaves(mj));
+      const double Eij = legE(ti, tj, rij);
+      // terms centred on j with legs to i and to another k of j's list
+      const int cj = min(cnt_sw[j], Wsw);
+      const int *rj = nbr_sw + (size_t)j * Wsw;
+      for (int b = 0; b < cj; ++b) {
+        const int k = rj[b];
+        if (k == i) continue;
+        const double4 xk = X[k];
+        const uint32_t mk = meta_of(xk);
+        if (gone(mk)) continue;
+        const int tk = type_of(mk);
+        double f0, f1, f2;  // x_k - x_j
+        min_image_fast(xk, xj, in, f0, f1, f2);
+        const double rjk = sqrt(f0 * f0 + f1 * f1 + f2 * f2);
+        const double Ejk = legE(tj, tk, rjk);
+        if (Ejk == 0.0) continue;
+        const double cs = (d0 * f0 + d1 * f1 + d2 * f2) / (rij * rjk);
+        const double D = cs + 1.0 / 3.0;
+        e += leps(ti, tj, tk) * Eij * Ejk * D * D / (1 + leaves(mj) + leaves(mk));
+      }
+      // terms centred on i with legs to j and to a later k of i's list
+      for (int b = a + 1; b < ci; ++b) {
+        const int k = ri[b];
+        const double4 xk = X[k];
+        const uint32_t mk = meta_of(xk);
+        if (gone(mk)) continue;
+        const int tk = type_of(mk);
+        double f0, f1, f2;  // x_i - x_k
+        min_image_fast(xi, xk, in, f0, f1, f2);
+        const double rik = sqrt(f0 * f0 + f1 * f1 + f2 * f2);
+        const double Eik = legE(ti, tk, rik);
+        if (Eik == 0.0) continue;
+        const double cs = (d0 * f0 + d1 * f1 + d2 * f2) / (rij * rik);
+        const double D = cs + 1.0 / 3.0;
+        e += leps(ti, tj, tk) * Eij * Eik * D * D / (1 + leaves(mj) + leaves(mk));
+      }
+    }
+  }
+  // Moliere pairs (si.f90:426-446): both atoms fired, r < 2.5 sigma; one of
+  // the two is Ga (the ion list holds no Si-Si or Si-Ge pair)
+  const bool ifired = !(ti == 3 && orig_of(xi) >= in.nsg + in.impact);
+  const int cn = min(cnt_ion[i], Wion);
+  const int *rn = nbr_ion + (size_t)i * Wion;
+  for (int a = 0; ifired && a < cn; ++a) {
+    const int j = rn[a];
+    const double4 xj = X[j];
+    const uint32_t mj = meta_of(xj);
+    if (gone(mj)) continue;
+    const int tj = type_of(mj);
+    if (tj == 3 && orig_of(xj) >= in.nsg + in.impact) continue;
+    double d0, d1, d2;
+    min_image_fast(xi, xj, in, d0, d1, d2);
+    const double r2 = d0 * d0 + d1 * d1 + d2 * d2;
+    if (!(r2 < rlj2)) continue;
+    const double r = sqrt(r2);
+    const int other = ti == 3 ? tj : ti;  // the partner of the Ga atom
+    const double aa = other == 1 ? a13 : other == 2 ? a23 : a33;
+    const double zi = ti == 1 ? z1 : ti == 2 ? z2 : z3, zj = tj == 1 ? z1 : tj == 2 ? z2 : z3;
+    const double zz = ie0 * zi * zj * ec * ec;
+    const double x = r / aa;
+    const double sa = c0 * exp(-b0 * x) + c1 * exp(-b1 * x) + c2 * exp(-b2 * x);
+    e += zz / r * sa / (1 + leaves(mj));
+  }
+  return e;
+}
+
+#define DECIDE_T 256  // threads of k_md_decide (a small block keeps the removal path in registers)
+static_assert(FIB_LEAVE_CAP == DECIDE_T, "one thread per listed removal");
+__global__ __launch_bounds__(DECIDE_T) void k_md_decide(
+    int ninst, const DInst *__restrict__ I, DCtl *__restrict__ ctl, double skin,
+    int *__restrict__ gate, int *__restrict__ leave, double4 *__restrict__ X, const FConst fc,
+    int Wsw, int Wion, const int *__restrict__ cnt_sw, const int *__restrict__ nbr_sw,
+    const int *__restrict__ cnt_ion, const int *__restrict__ nbr_ion) {
   __shared__ int s_w[DECIDE_T / FIB_WAVE];
   __shared__ int s_any;
+  __shared__ int s_list[FIB_LEAVE_CAP];
+  __shared__ double s_e[FIB_LEAVE_CAP];
   const int t = threadIdx.x, lane = t % FIB_WAVE, wv = t / FIB_WAVE;
+  // atoms removed in this step: their interaction energy with the atoms that
+  // stay goes to the energy carried out, summed in atom order (the list is
+  // filled in arbitrary order and sorted here), so the ledger is the same
+  // for an instance alone or in any batch
+  const int nleave = leave[0];
+  if (nleave > 0) {
+    const int n = min(nleave, FIB_LEAVE_CAP);
+    s_list[t] = t < n ? leave[1 + t] : 0x7fffffff;
+    __syncthreads();
+    for (int k = 2; k <= FIB_LEAVE_CAP; k <<= 1)
+      for (int j = k >> 1; j > 0; j >>= 1) {
+        const int u = t ^ j;
+        if (u > t) {
+          const int a = s_list[t], b = s_list[u];
+          if ((a > b) == ((t & k) == 0)) { s_list[t] = b; s_list[u] = a; }
+        }
+        __syncthreads();
+      }
+    if (t < n)
+      s_e[t] = leaving_energy(s_list[t], X, I, fc, Wsw, Wion, cnt_sw, nbr_sw, cnt_ion, nbr_ion);
+    __syncthreads();
+    if (t == 0) {
+      for (int q = 0; q < n; ++q) ctl[inst_of(meta_of(X[s_list[q]]))].e_out_step += s_e[q];
+      if (nleave > FIB_LEAVE_CAP) ctl[0].leave_overflow = 1;
+      leave[0] = 0;
+    }
+    __syncthreads();
+    if (t < n) {
+      double4 &x = X[s_list[t]];
+      x.w = __longlong_as_double((long long)(wbits(x) & ~(uint64_t)META_LEAVING));
+    }
+  }
+  // one workgroup; behind the gates, the prefix over instances of the real
+  // cells of the gated ones (the cell numbering of a gated k_nbuild)
   int *cplan = gate + 1 + ninst;
   if (t == 0) s_any = 0;
   __syncthreads();
@@ -2451,12 +2619,14 @@ void launch_pos(const DevBufs &d, const StepArgs &sa, hipStream_t s) {
   finalize(d, 4, FIB_BLOCK, op, dst, s);
 }
 
-void launch_md_pos(const DevBufs &d, const StepArgs &sa, int *gate, hipStream_t s) {
+void launch_md_pos(const DevBufs &d, const FConst &fc, const StepArgs &sa, int *gate,
+                   hipStream_t s) {
+  int *leave = gate + 2 + 2 * d.ninst;
   {
     KScope ks_(d, KT_POS);
     hipLaunchKernelGGL(k_md_pos, dim3(d.NA / FIB_BLOCK), dim3(FIB_BLOCK), 0, s, d.X[d.cur], d.I,
                        d.NA, d.V[d.cur], d.F[d.cur], d.Xi[d.cur], sa, d.pstride, d.partials,
-                       d.ctl, d.Xb, d.disp);
+                       d.ctl, d.Xb, d.disp, leave);
   }
   const int op[3] = {1, 0, 0};
   const int dst[3] = {DCTL_IDX(maxdisp), DCTL_IDX(e_bath_step), DCTL_IDX(e_out_step)};
@@ -2470,7 +2640,8 @@ void launch_md_pos(const DevBufs &d, const StepArgs &sa, int *gate, hipStream_t 
   const int dst2[2] = {DCTL_IDX(maxdisp2), DCTL_IDX(nmaxdisp)};
   finalize(d, 2, FIB_BLOCK, op2, dst2, s);
   hipLaunchKernelGGL(k_md_decide, dim3(1), dim3(DECIDE_T), 0, s, d.ninst, d.I, d.ctl, sa.skin,
-                     gate);
+                     gate, leave, d.X[d.cur], fc, d.Wsw, d.Wion, d.cnt_sw, d.nbr_sw, d.cnt_ion,
+                     d.nbr_ion);
 }
 
 void launch_md_ctl(const DevBufs &d, const StepArgs &sa, hipStream_t s) {

@@ -969,7 +969,7 @@ extern "C" int32_t fib_step(fib_handle *h, const fib_step_ctl *ctl, fib_step_rep
 static void enqueue_md_step(fib_handle *h) {
   DevBufs &d = h->d;
   const StepArgs &sa = h->md_sa;
-  fibk::launch_md_pos(d, sa, h->md_gate, h->stream);
+  fibk::launch_md_pos(d, h->fc, sa, h->md_gate, h->stream);
   // list rebuild, skipped on the device unless the gate is set; the sort is
   // in place so every array stays where the step (or its graph) expects it
   d.gate = h->md_gate;
@@ -1004,8 +1004,10 @@ extern "C" int32_t fib_md_begin(fib_handle *h, const fib_md_params *p, const dou
     return fail(h, FIB_ERR_ARG, "fib_md_begin: bad parameters");
   if (!h->have_state) return fail(h, FIB_ERR_STATE, "fib_md_begin: set the state first");
   if (!h->md_gate) {
-    HIPCHK(h, hipMalloc(&h->md_gate, sizeof(int) * (2 * h->inst.size() + 2)));
-    HIPCHK(h, hipMemset(h->md_gate, 0, sizeof(int) * (2 * h->inst.size() + 2)));
+    // gates and cell prefix [2 + 2*ninst], then the removal list [1 + FIB_LEAVE_CAP]
+    const size_t ng = 2 * h->inst.size() + 2 + 1 + FIB_LEAVE_CAP;
+    HIPCHK(h, hipMalloc(&h->md_gate, sizeof(int) * ng));
+    HIPCHK(h, hipMemset(h->md_gate, 0, sizeof(int) * ng));
   }
   for (size_t k = 0; k < h->inst.size(); ++k) {
     h->inst[k].surf_z = surface_z[k];
@@ -1081,6 +1083,9 @@ extern "C" int32_t fib_md_run(fib_handle *h, int32_t nsteps) {
   HIPCHK(h, hipMemcpy(c.data(), h->d.ctl, sizeof(DCtl) * c.size(), hipMemcpyDeviceToHost));
   for (auto &x : c) {
     if (x.overflow) return fail(h, FIB_ERR_OVERFLOW, "fib_md_run: a neighbour list outgrew its width");
+    if (x.leave_overflow)
+      return fail(h, FIB_ERR_OVERFLOW, "fib_md_run: more atoms left the simulation in one step "
+                                       "than the removal list holds");
     if (x.nonfinite) return fail(h, FIB_ERR_NONFINITE, "fib_md_run: non-finite position");
   }
   return FIB_OK;

@@ -210,3 +210,54 @@ def test_md_energy_ledger(precision):
     assert r1["removed"] == 1 and 480.0 * ev < r1["e_removed"] < 500.0 * ev
     assert worst[0] < 5e-4 * r0["e_fired"]
     assert worst[1] < 1e-3 * 500.0 * ev
+
+
+def _dimer_batch():
+    """Two slabs, each with a Si dimer at the pair minimum 8 A above the
+    surface, flying up at 2 km/s: instance 0 horizontal (both atoms leave in
+    the same step), instance 1 vertical (the top atom leaves while still
+    bound to the other, which leaves later on its own)."""
+    r0 = 2 ** (1 / 6) * 2.0951e-10
+    out = []
+    for k, kind in enumerate(("horizontal", "vertical")):
+        s = configs.si_slab(4, 21 + k, nz=3)
+        nsg = s["nsg"]
+        c = np.array([s["L"][0] / 2, s["L"][1] / 2, s["surface_z"] + 8 * A])
+        d = np.array([r0, 0.0, 0.0]) if kind == "horizontal" else np.array([0.0, 0.0, r0])
+        s["X"] = np.vstack([s["X"][:nsg], c - d / 2, c + d / 2, s["X"][nsg:]])
+        s["V"] = np.vstack([s["V"][:nsg], [[0.0, 0.0, 2000.0]] * 2, s["V"][nsg:]])
+        s["atype"] = np.concatenate([s["atype"][:nsg], [1, 1], s["atype"][nsg:]]).astype(np.int32)
+        s["xsides"] = np.concatenate([s["xsides"][:nsg], [0, 0], s["xsides"][nsg:]]).astype(np.int32)
+        s["nsg"] = nsg + 2
+        out.append(s)
+    return out
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_md_ledger_bound_atoms_leave(precision):
+    """Atoms removed while still bound carry their interaction energy out:
+    a dimer at the SW pair minimum (-eps bond) leaving whole or atom by atom
+    keeps the ledger constant, and the energy booked as removed is the
+    dimer's kinetic energy minus eps (closed form).  Booking only the
+    kinetic energy would move each ledger by eps = 2.17 eV."""
+    ev = 1.602176634e-19
+    eps = 3.473928e-19
+    inst = _dimer_batch()
+    g = _gpu(inst, precision)
+    _load(g, inst)
+    g.md_begin([s["surface_z"] for s in inst], [s["bottom_z"] for s in inst])
+    L0 = g.md_ledger()
+    worst = np.zeros(2)
+    for _ in range(4):
+        g.md_run(50)
+        worst = np.maximum(worst, np.abs(g.md_ledger() - L0))
+    reps = [g.md_report(k) for k in range(2)]
+    g.close()
+    ke = 2 * 0.5 * configs.MASS_SI * 2000.0 ** 2
+    print(f"\n{precision}: max |ledger drift| {worst / ev} eV; removed "
+          f"{[r['removed'] for r in reps]}, carried {[r['e_removed'] / ev for r in reps]} eV "
+          f"(closed form {(ke - eps) / ev:.4f})", flush=True)
+    for r in reps:
+        assert r["removed"] == 2
+        assert abs(r["e_removed"] - (ke - eps)) < 1e-3 * eps
+    assert np.all(worst < 0.02 * eps)
