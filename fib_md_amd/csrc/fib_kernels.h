@@ -120,6 +120,9 @@ void launch_pos(const DevBufs &d, const StepArgs &sa, hipStream_t s);
 // [1 + FIB_LEAVE_CAP] (count, atoms removed in the step)
 void launch_md_pos(const DevBufs &d, const FConst &fc, const StepArgs &sa, int *gate,
                    hipStream_t s);
+// adds the interaction energy of the atom with unsorted index q to its
+// instance's energy brought in (fired ions)
+void launch_fire_energy(const DevBufs &d, const FConst &fc, int q, hipStream_t s);
 // MD-step mode: time, next dt, shell factor (after the MODE 2 force pass)
 void launch_md_ctl(const DevBufs &d, const StepArgs &sa, hipStream_t s);
 // MD-step mode: dt from the current |v|max (reset = also time, steps, factor)

@@ -1716,12 +1716,13 @@ __global__ __launch_bounds__(FIB_BLOCK) void k_disp2(const double4 *__restrict__
 // Rebuild gates of the step: each instance whose two largest displacements
 // since the build add up to more than the skin (only then can a pair the
 // build left out have come within the cutoff), and whether there is any.
-// Interaction energy (FP64, J) of atom i, removed in this step, with the
-// atoms that stay: every SW pair and three-body term and every Moliere pair
-// it takes part in (si.f90:278-368 forms), each term divided by the number
-// of its atoms that leave in this step, so that a term is booked once.  The
-// lists are those of the last build, which hold every pair in range.
-__device__ double leaving_energy(int i, const double4 *__restrict__ X,
+// Interaction energy (FP64, J) of atom i with the atoms that stay in the
+// simulation: every SW pair and three-body term and every Moliere pair it
+// takes part in (si.f90:278-368 forms), each term divided by the number of
+// its atoms that leave in the current step, so that a term is booked once.
+// The lists are those of the last build, which hold every pair in range.
+// Used for atoms removed in a step and for ions where they are fired.
+__device__ double interaction_energy(int i, const double4 *__restrict__ X,
                                  const DInst *__restrict__ I, const FConst &fc, int Wsw,
                                  int Wion, const int *__restrict__ cnt_sw,
                                  const int *__restrict__ nbr_sw, const int *__restrict__ cnt_ion,
@@ -1873,7 +1874,8 @@ __global__ __launch_bounds__(DECIDE_T) void k_md_decide(
         __syncthreads();
       }
     if (t < n)
-      s_e[t] = leaving_energy(s_list[t], X, I, fc, Wsw, Wion, cnt_sw, nbr_sw, cnt_ion, nbr_ion);
+      s_e[t] = interaction_energy(s_list[t], X, I, fc, Wsw, Wion, cnt_sw, nbr_sw, cnt_ion,
+                                  nbr_ion);
     __syncthreads();
     if (t == 0) {
       for (int q = 0; q < n; ++q) ctl[inst_of(meta_of(X[s_list[q]]))].e_out_step += s_e[q];
@@ -1922,6 +1924,20 @@ __global__ __launch_bounds__(DECIDE_T) void k_md_decide(
     cplan[ninst] = carry;
     gate[0] = s_any;
   }
+}
+
+// A fired ion brings its interaction energy with the atoms near its start
+// point in with its kinetic energy (k_set_atom books the latter); q is its
+// unsorted index atom_off + orig, evaluated after the lists were rebuilt.
+__global__ void k_fire_energy(int q, const int *__restrict__ inv, const double4 *__restrict__ X,
+                              const DInst *__restrict__ I, const FConst fc, int Wsw, int Wion,
+                              const int *__restrict__ cnt_sw, const int *__restrict__ nbr_sw,
+                              const int *__restrict__ cnt_ion, const int *__restrict__ nbr_ion,
+                              DCtl *__restrict__ ctl) {
+  if (threadIdx.x != 0) return;
+  const int p = inv[q];
+  const double e = interaction_energy(p, X, I, fc, Wsw, Wion, cnt_sw, nbr_sw, cnt_ion, nbr_ion);
+  ctl[inst_of(meta_of(X[p]))].e_in += e;
 }
 
 // After the step's second half kick: advance time, choose the next dt and
@@ -2642,6 +2658,11 @@ void launch_md_pos(const DevBufs &d, const FConst &fc, const StepArgs &sa, int *
   hipLaunchKernelGGL(k_md_decide, dim3(1), dim3(DECIDE_T), 0, s, d.ninst, d.I, d.ctl, sa.skin,
                      gate, leave, d.X[d.cur], fc, d.Wsw, d.Wion, d.cnt_sw, d.nbr_sw, d.cnt_ion,
                      d.nbr_ion);
+}
+
+void launch_fire_energy(const DevBufs &d, const FConst &fc, int q, hipStream_t s) {
+  hipLaunchKernelGGL(k_fire_energy, dim3(1), dim3(FIB_WAVE), 0, s, q, d.inv, d.X[d.cur], d.I, fc,
+                     d.Wsw, d.Wion, d.cnt_sw, d.nbr_sw, d.cnt_ion, d.nbr_ion, d.ctl);
 }
 
 void launch_md_ctl(const DevBufs &d, const StepArgs &sa, hipStream_t s) {

@@ -1048,9 +1048,11 @@ extern "C" int32_t fib_md_fire(fib_handle *h, int32_t inst, int32_t ion, const d
   fibk::launch_set_atom(h->d, p, x, v, h->md_sa, h->stream);
   in.impact = ion;
   HIPCHK(h, hipMemcpyAsync(&h->d.I[inst], &in, sizeof(DInst), hipMemcpyHostToDevice, h->stream));
-  // the ion jumped: fresh lists, forces and dt
+  // the ion jumped: fresh lists, forces and dt; the ledger takes the
+  // ion's potential energy at its start point with its kinetic energy
   int32_t r = build_lists(h);
   if (r != FIB_OK) return r;
+  fibk::launch_fire_energy(h->d, h->fc, in.atom_off + in.nsg + ion - 1, h->stream);
   return md_refresh(h, false);
 }
 

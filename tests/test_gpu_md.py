@@ -261,3 +261,36 @@ def test_md_ledger_bound_atoms_leave(precision):
         assert r["removed"] == 2
         assert abs(r["e_removed"] - (ke - eps)) < 1e-3 * eps
     assert np.all(worst < 0.02 * eps)
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_md_ledger_ion_fired_near_atoms(precision):
+    """An ion fired 1.6 A above a surface atom starts inside the Moliere range
+    with ~15 eV of potential energy.  The ledger books that energy with the
+    ion's kinetic energy where it is fired, and stays constant through the
+    collision; booking only the kinetic energy would move it by the start
+    energy."""
+    ev = 1.602176634e-19
+    s = configs.si_slab(4, 31, nz=3)
+    S = s["X"][:s["nsg"]]
+    top = np.nonzero(S[:, 2] > s["surface_z"] - 0.1 * A)[0]
+    c = np.array([s["L"][0] / 2, s["L"][1] / 2])
+    k = top[np.argmin(((S[top, :2] - c) ** 2).sum(1))]
+    x = S[k] + np.array([0.0, 0.0, 1.6 * A])
+    v = np.array([0.0, 0.0, -configs.GA_SPEED * math.sqrt(200.0)])
+    g = _gpu([s], precision)
+    _load(g, [s])
+    g.md_begin([s["surface_z"]], [s["bottom_z"]])
+    L0 = g.md_ledger()[0]
+    g.md_fire(0, 1, x, v)
+    worst = 0.0
+    for _ in range(4):
+        g.md_run(100)
+        worst = max(worst, abs(g.md_ledger()[0] - L0))
+    r = g.md_report(0)
+    g.close()
+    start = r["e_fired"] / ev - 200.0
+    print(f"\n{precision}: start energy {start:.3f} eV, max |ledger drift| {worst / ev:.4f} eV",
+          flush=True)
+    assert start > 5.0
+    assert worst < 0.05 * start * ev
