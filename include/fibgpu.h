@@ -247,8 +247,10 @@ typedef struct fib_md_report {
   /* energy ledger since fib_md_begin (J): ke + potential + e_bath + e_removed
    * - e_fired stays at its starting value up to the integration error */
   double e_bath;          /* taken out by the Berendsen shells                  */
-  double e_removed;       /* kinetic energy of the atoms removed                */
-  double e_fired;         /* kinetic energy given to fired ions                 */
+  double e_removed;       /* carried out by removed atoms: kinetic energy plus
+                             interaction energy with the atoms that stay        */
+  double e_fired;         /* brought in by fired ions: kinetic energy plus
+                             interaction energy at the start point              */
 } fib_md_report;
 int32_t fib_md_get_report(fib_handle *h, int32_t inst, fib_md_report *r);
 /* Potential energy (J) of every instance's current state, FP64 sums of the
