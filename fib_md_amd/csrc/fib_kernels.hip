@@ -42,7 +42,8 @@ namespace fibk {
 // rebuilds" and gate[1 + k] = "instance k rebuilds".  With gate[0] zero they
 // return at once (block-uniform), so a build can be enqueued -- or captured
 // in a graph -- and skipped on the device; atoms, cells and lists of an
-// instance whose own flag is zero are left untouched.
+// instance whose own flag is zero are left untouched.  k_md_decide writes
+// every flag at the start of each MD step, before any kernel reads them.
 #define GATE_OR_RETURN(gate) \
   do {                       \
     if ((gate) && !(gate)[0]) return; \
@@ -1602,11 +1603,6 @@ __global__ void k_mark_build(int NA, int ninst, int Wsw, int Wion, const double4
   }
 }
 
-// End of a gated build: clear the gate (its own kernel, so that every block
-// of the build kernels has read it).
-__global__ void k_clear_gate(int *__restrict__ gate, int ninst) {
-  for (int k = threadIdx.x; k <= ninst; k += blockDim.x) gate[k] = 0;
-}
 
 // ---------------------------------------------------------------------------
 // MD-step mode: velocity Verlet with a per-instance adaptive timestep
@@ -2692,8 +2688,6 @@ void launch_mark_build(const DevBufs &d, hipStream_t s) {
   hipLaunchKernelGGL(k_mark_build, dim3(gcap(std::max(d.NA, d.ninst))), dim3(FIB_BLOCK), 0,
                      s, d.NA, d.ninst, d.Wsw, d.Wion, d.X[d.cur], d.Xb, d.disp, d.ctl, d.binst,
                      d.gate);
-  if (d.gate)
-    hipLaunchKernelGGL(k_clear_gate, dim3(1), dim3(FIB_BLOCK), 0, s, (int *)d.gate, d.ninst);
 }
 
 void launch_sums(const DevBufs &d, const StepArgs &sa, hipStream_t s) {
