@@ -67,9 +67,11 @@ def summarize_stats(d, out, cmd):
     print("\n".join(lines))
 
 
-def summarize_pmc(dfetch, dwrite, out, cmd):
+def summarize_pmc(dfetch, dwrite, out, cmd, atoms=None):
     """Per-dispatch FETCH_SIZE / WRITE_SIZE (KiB) of two rocprofv3 --pmc runs,
-    scaled by what the counters read for the 256 MiB copy."""
+    scaled by what the counters read for the 256 MiB copy.  Bytes per atom
+    are per atom of the microbench, or of `atoms` when given (then the
+    microbench traffic file that `value`'s roofline reads is left alone)."""
     import csv
     from collections import defaultdict
 
@@ -92,8 +94,8 @@ def summarize_pmc(dfetch, dwrite, out, cmd):
     fetch, fcopy = per_kernel(dfetch, "FETCH_SIZE")
     write, wcopy = per_kernel(dwrite, "WRITE_SIZE")
     ff, wf = COPY_KIB / fcopy, COPY_KIB / wcopy
-    X, at, L = make_workload(64)
-    res = {"command": cmd, "atoms": len(X), "copy_fetch_kib": fcopy, "copy_write_kib": wcopy,
+    n = int(atoms) if atoms is not None else len(make_workload(64)[0])
+    res = {"command": cmd, "atoms": n, "copy_fetch_kib": fcopy, "copy_write_kib": wcopy,
            "fetch_factor": ff, "write_factor": wf, "kernels": {}}
     lines = ["# HBM traffic per dispatch from rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE (KiB)",
              f"# command: {cmd}",
@@ -105,12 +107,13 @@ def summarize_pmc(dfetch, dwrite, out, cmd):
             continue
         tb = (fetch[k] * ff + write[k] * wf) * 1024.0
         res["kernels"][k] = {"fetch_kib": fetch[k], "write_kib": write[k], "traffic_bytes": tb,
-                             "bytes_per_atom": tb / len(X)}
+                             "bytes_per_atom": tb / n}
         lines.append(f"{k[:36]:36s} {fetch[k]:12.0f} {write[k]:12.0f} {tb / 1e6:12.1f} "
-                     f"{tb / len(X):8.1f}")
+                     f"{tb / n:8.1f}")
     open(out, "w").write("\n".join(lines) + "\n")
-    with open(PMC_FILE, "w") as f:
-        json.dump(res, f, indent=1)
+    if atoms is None:
+        with open(PMC_FILE, "w") as f:
+            json.dump(res, f, indent=1)
     print("\n".join(lines))
 
 
@@ -568,7 +571,7 @@ def main():
     ap.add_argument("--profile", choices=["fp64", "mixed", "md-fp64", "md-mixed", "md-replicas"], default=None)
     ap.add_argument("--summarize-stats", nargs=3, metavar=("DIR", "OUT", "CMD"))
     ap.add_argument("--cpu-worker", nargs=2, metavar=("NCELL", "REPS"), help=argparse.SUPPRESS)
-    ap.add_argument("--summarize-pmc", nargs=4, metavar=("FETCH_DIR", "WRITE_DIR", "OUT", "CMD"))
+    ap.add_argument("--summarize-pmc", nargs="+", metavar="FETCH_DIR WRITE_DIR OUT CMD [ATOMS]")
     ap.add_argument("--summarize-counters", nargs="+", metavar="OUT CMD DIR",
                     help="OUT CMD DIR [DIR ...]: SQ/GRBM counter table of rocprofv3 --pmc runs")
     args = ap.parse_args()
@@ -581,7 +584,7 @@ def main():
         summarize_stats(*args.summarize_stats)
         return
     if args.summarize_pmc:
-        summarize_pmc(*args.summarize_pmc)
+        summarize_pmc(*args.summarize_pmc[:5])
         return
     if args.summarize_counters:
         summarize_counters(*args.summarize_counters)
@@ -598,6 +601,13 @@ def main():
     X, at, L = make_workload(args.ncell)
     n = len(X)
     log(f"workload: {n} atoms, box {L[0]*1e10:.2f} A, generated in {time.perf_counter()-t0:.1f} s")
+    if args.profile:
+        # a plain copy of known size calibrates the FETCH/WRITE counters
+        a = torch.empty(256 * 1024 * 1024 // 4, dtype=torch.float32, device="cuda").fill_(1.0)
+        b = torch.empty_like(a)
+        b.copy_(a)
+        torch.cuda.synchronize()
+        del a, b
     if args.profile in ("md-fp64", "md-mixed"):
         # the batched ensemble, a few MD steps (for rocprofv3)
         from fib_md_amd import configs
@@ -611,11 +621,6 @@ def main():
         log(json.dumps(r))
         return
     if args.profile:
-        # a plain copy of known size calibrates the FETCH/WRITE counters
-        a = torch.empty(256 * 1024 * 1024 // 4, dtype=torch.float32, device="cuda").fill_(1.0)
-        b = torch.empty_like(a)
-        b.copy_(a)
-        torch.cuda.synchronize()
         r = run_gpu(X, at, L, args.profile, 10, 2)
         log(json.dumps({k: v for k, v in r.items() if k in ("ms_per_step", "timing")}))
         return
