@@ -294,3 +294,44 @@ def test_md_ledger_ion_fired_near_atoms(precision):
           flush=True)
     assert start > 5.0
     assert worst < 0.05 * start * ev
+
+
+def test_md_batch_equals_single_many_instances():
+    """300 instances, more than the rebuild decision's 256 threads: it scans
+    them in chunks and numbers the cells of the gated instances across the
+    chunks.  Ions go into every other instance only (the dt rule moves the
+    fastest atom 0.05 A per step whatever the ion energy, so fired instances
+    rebuild every few steps and the others, thermal only, seldom): the gated
+    sets differ from step to step, and instances on both sides of the chunk
+    boundary match their runs alone, bit for bit."""
+    batch = []
+    for k in range(300):
+        s = configs.si_slab(3, 900 + k, nz=2)
+        x, v = configs.ion_launch(s["L"], s["surface_z"], 500.0, 0.0, 0.0,
+                                  np.random.default_rng(900 + k))
+        s.update(ion_x=x, ion_v=v)
+        batch.append(s)
+
+    def run(insts, fire):
+        g = _gpu(insts, "mixed")
+        _load(g, insts)
+        g.md_begin([s["surface_z"] for s in insts], [s["bottom_z"] for s in insts])
+        for k, s in enumerate(insts):
+            if fire[k]:
+                g.md_fire(k, 1, s["ion_x"], s["ion_v"])
+        b0 = [g.md_report(k)["builds"] for k in range(len(insts))]
+        g.md_run(80)
+        out = [(*g.state(inst=k), g.md_report(k)) for k in range(len(insts))]
+        g.close()
+        return out, [o[3]["builds"] - b for o, b in zip(out, b0)]
+
+    fire = [k % 2 == 0 for k in range(300)]
+    together, gated = run(batch, fire)
+    print(f"\ngated rebuilds in 80 steps: fired {sorted(set(gated[0::2]))}, "
+          f"thermal {sorted(set(gated[1::2]))}", flush=True)
+    assert min(gated[0::2]) > max(gated[1::2])
+    for k in (0, 255, 256, 299):
+        alone, _ = run([batch[k]], [fire[k]])
+        np.testing.assert_array_equal(together[k][0], alone[0][0])
+        np.testing.assert_array_equal(together[k][1], alone[0][1])
+        assert together[k][3]["time"] == alone[0][3]["time"]
