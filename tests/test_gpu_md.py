@@ -335,3 +335,19 @@ def test_md_batch_equals_single_many_instances():
         np.testing.assert_array_equal(together[k][0], alone[0][0])
         np.testing.assert_array_equal(together[k][1], alone[0][1])
         assert together[k][3]["time"] == alone[0][3]["time"]
+
+
+def test_md_removal_list_overflow_is_reported():
+    """More atoms leaving in one step than the removal list holds (256): the
+    run reports it instead of booking part of them.  Forced from the host: a
+    surface height below the slab puts every atom above the removal height at
+    the first step."""
+    from fib_md_amd.gpu import GPUError
+    s = configs.si_slab(6, 41, nz=3)
+    assert s["nsg"] > 256
+    g = _gpu([s], "mixed")
+    _load(g, [s])
+    g.md_begin([s["bottom_z"] - 20 * A], [s["bottom_z"]], remove_height=1 * A)
+    with pytest.raises(GPUError, match="left the simulation"):
+        g.md_run(1)
+    g.close()
