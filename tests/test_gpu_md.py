@@ -351,3 +351,48 @@ def test_md_removal_list_overflow_is_reported():
     with pytest.raises(GPUError, match="left the simulation"):
         g.md_run(1)
     g.close()
+
+
+def test_md_ledger_bound_trimer_leaves():
+    """A linear Si trimer (pair-minimum spacing, cos = -1: a three-body term of
+    ~0.6 eV) leaving atom by atom (vertical) or all in one step (horizontal,
+    the term split three ways): the ledger stays constant, so the removal
+    energy carries the three-body term with the right share."""
+    ev = 1.602176634e-19
+    r0 = 2 ** (1 / 6) * 2.0951e-10
+    inst = []
+    for k, axis in enumerate((2, 0)):
+        s = configs.si_slab(4, 51 + k, nz=3)
+        nsg = s["nsg"]
+        c = np.array([s["L"][0] / 2, s["L"][1] / 2, s["surface_z"] + 7 * A])
+        d = np.zeros(3)
+        d[axis] = r0
+        T = np.stack([c - d, c, c + d])
+        s["X"] = np.vstack([s["X"][:nsg], T, s["X"][nsg:]])
+        s["V"] = np.vstack([s["V"][:nsg], [[0.0, 0.0, 2000.0]] * 3, s["V"][nsg:]])
+        s["atype"] = np.concatenate([s["atype"][:nsg], [1, 1, 1], s["atype"][nsg:]]).astype(np.int32)
+        s["xsides"] = np.concatenate([s["xsides"][:nsg], [0, 0, 0], s["xsides"][nsg:]]).astype(np.int32)
+        s["nsg"] = nsg + 3
+        inst.append(s)
+    g = _gpu(inst, "mixed")
+    _load(g, inst)
+    g.md_begin([s["surface_z"] for s in inst], [s["bottom_z"] for s in inst])
+    L0 = g.md_ledger()
+    worst = np.zeros(2)
+    for _ in range(12):
+        g.md_run(50)
+        worst = np.maximum(worst, np.abs(g.md_ledger() - L0))
+    reps = [g.md_report(k) for k in range(2)]
+    g.close()
+    print(f"\nmax |ledger drift| {worst / ev} eV; removed {[r['removed'] for r in reps]}, "
+          f"carried {[r['e_removed'] / ev for r in reps]} eV", flush=True)
+    # closed form of what leaves: the trimer's kinetic energy, two bonds at the
+    # pair minimum (-eps each) and the vertex term at the middle atom,
+    # lambda eps E(r0)^2 (cos + 1/3)^2 with cos = -1, E = exp(gamma/(r0/sigma - a))
+    eps = 3.473928e-19
+    E = math.exp(1.2 / (2 ** (1 / 6) - 1.8))
+    total = 3 * 0.5 * configs.MASS_SI * 2000.0 ** 2 - 2 * eps + 21.0 * eps * E * E * 4.0 / 9.0
+    assert all(r["removed"] == 3 for r in reps)
+    assert np.all(worst < 0.05 * ev)
+    for r in reps:
+        assert abs(r["e_removed"] - total) < 0.05 * ev
