@@ -422,7 +422,8 @@ contains
   ! Restart records (writerestart, out.f90:34-59): D/restart.in_old and
   ! D/restart.in<impact:6>_<lt:9>, unformatted sequential, records
   ! (lt, restart_time), (X, V, mass, atype), (impact) -- as the reference
-  ! writes them, restart_time being the clock value the run started from.
+  ! writes them, restart_time being the simulated clock of step lt (onestep
+  ! sets restart_time = time every step, timeint.f90:165-166).
   ! Beside each goes D/restart.rng<same suffix> with what the reference does
   ! not keep: the ran1 state and the impact spots, so that a restart carries
   ! on the random sequence where the run left it.

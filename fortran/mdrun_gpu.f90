@@ -174,7 +174,9 @@ program mdrun_gpu
         if (rep%output_due /= 0 .and. s%res_out > 0) then
            if (mod(lt, int(s%res_out, c_int64_t)) == 0) then
               call fetch()
-              call write_restart(s, st, int(lt), restart_time, impact)
+              ! the record carries the clock of this step, as onestep sets
+              ! restart_time = time every step (timeint.f90:165-166)
+              call write_restart(s, st, int(lt), time, impact)
            end if
         end if
         if (rep%atoms_due /= 0 .and. s%atm_out_i > 0) then

@@ -64,6 +64,15 @@ struct orc {
   /* ---- timeint locals kept between TI iterations ---- */
   int32_t lt;
   double time, temp;
+  /* ---- config mode (no reference counterpart): removed atoms, positions at
+   * the last list build, step control and energy ledger ---- */
+  int32_t *dead;  /* NULL until config mode starts; 1 = removed */
+  double *Xb;
+  orc_md_params md;
+  double surf_z, bot_z, skin;
+  double md_dt, md_lambda, md_time, md_vmax, md_ke;
+  double e_bath, e_out, e_in;
+  int64_t md_steps, builds;
 };
 
 static const double kB = 1.381E-23;        /* prms.f90:151 */
@@ -95,6 +104,7 @@ void orc_free(orc *o) {
   free(o->nss); free(o->nsslj); free(o->nlist2); free(o->nlistlj);
   free(o->nlist3);
   free(o->full2_cnt); free(o->fulllj_cnt); free(o->full2); free(o->fulllj);
+  free(o->dead); free(o->Xb);
   free(o);
 }
 
@@ -177,6 +187,8 @@ static void alloc_atoms(orc *o) {
   o->mass = (double *)calloc(n, sizeof(double));
   o->atype = (int32_t *)calloc(n, sizeof(int32_t));
   o->xsides = (int32_t *)calloc(n, sizeof(int32_t));
+  free(o->dead); free(o->Xb);
+  o->dead = NULL; o->Xb = NULL;
 }
 
 /* Text round trips of the reference's intermediate files. */
@@ -619,6 +631,8 @@ int orc_si_nlist(orc *o, const double *X) {
 #define H(a, b, c) HOC[((size_t)(c) * S2 + (b)) * S1 + (a)]
   orc_atomcell(o, X, ic);
   for (int i = 1; i <= natm; ++i) {
+    /* config mode: a removed atom is in no cell and so in no list */
+    if (o->dead && o->dead[i - 1]) continue;
     int a = XA(ic, i - 1, 0), b = XA(ic, i - 1, 1), c = XA(ic, i - 1, 2);
     LL[i] = H(a, b, c);
     H(a, b, c) = i;
@@ -781,6 +795,13 @@ static void vir_add(double *w, const double r[3], const double f[3]) {
     for (int b = 0; b < 3; ++b) w[3 * a + b] += r[a] * f[b];
 }
 
+/* Atoms a force or energy term leaves out: ions not fired yet (index above
+ * Nsg+impact, si.f90:415,428,457) and, in config mode, removed atoms, which
+ * stay in the lists of the last build until the next one. */
+static int skipped(const orc *o, int i1, int lim) {
+  return i1 > lim || (o->dead && o->dead[i1 - 1]);
+}
+
 /* si_force, si.f90:384-552.  The virial (absent in the reference) sums
  * r_ij (x) f_i for pairs and -x_ij (x) f_j - x_ik (x) f_k for triplets. */
 void orc_si_force(orc *o, const double *Xp, double *F, double *W) {
@@ -793,7 +814,7 @@ void orc_si_force(orc *o, const double *Xp, double *F, double *W) {
     int i = i1 - 1;
     for (int l = o->nss[i]; l <= o->nss[natm + i]; ++l) {
       int j1 = o->nlist2[l - 1], j = j1 - 1;
-      if (i1 > lim || j1 > lim) continue;
+      if (skipped(o, i1, lim) || skipped(o, j1, lim)) continue;
       double xx[3];
       min_image(o, Xp, i, j, xx);
       double rij = sqrt(xx[0] * xx[0] + xx[1] * xx[1] + xx[2] * xx[2]);
@@ -810,7 +831,7 @@ void orc_si_force(orc *o, const double *Xp, double *F, double *W) {
     }
     for (int l = o->nsslj[i]; l <= o->nsslj[natm + i]; ++l) {
       int j1 = o->nlistlj[l - 1], j = j1 - 1;
-      if (i1 > lim || j1 > lim) continue;
+      if (skipped(o, i1, lim) || skipped(o, j1, lim)) continue;
       double xx[3];
       min_image(o, Xp, i, j, xx);
       double rij2 = xx[0] * xx[0] + xx[1] * xx[1] + xx[2] * xx[2];
@@ -840,7 +861,7 @@ void orc_si_force(orc *o, const double *Xp, double *F, double *W) {
   for (int l = 0; l < o->nbrs; ++l) {
     int i1 = o->nlist3[3 * (size_t)l], j1 = o->nlist3[3 * (size_t)l + 1],
         k1 = o->nlist3[3 * (size_t)l + 2];
-    if (i1 > lim || j1 > lim || k1 > lim) continue;
+    if (skipped(o, i1, lim) || skipped(o, j1, lim) || skipped(o, k1, lim)) continue;
     int i = i1 - 1, j = j1 - 1, k = k1 - 1;
     int ti = o->atype[i] - 1, tj = o->atype[j] - 1, tk = o->atype[k] - 1;
     double isigij = o->isigma[ti][tj], isigik = o->isigma[ti][tk],
@@ -929,14 +950,18 @@ static double hfun(const orc *o, double u, double v, double w) {
   return 0.;
 }
 
-/* si_potential, si.f90:278-368 */
-void orc_si_potential(orc *o, const double *X, double *utot, double *u) {
+/* si_potential, si.f90:278-368.  lim < 0 keeps every listed term, as the
+ * reference does; config mode passes Nsg+impact and leaves out the terms
+ * the force leaves out (unfired ions, removed atoms). */
+static void potential_impl(orc *o, const double *X, double *utot, double *u, int lim) {
   const int natm = o->natm;
   double U = 0.;
   memset(u, 0, (size_t)natm * sizeof(double));
+#define SKIP(a) (lim >= 0 && skipped(o, (a) + 1, lim))
   for (int i = 0; i < natm; ++i) {
     for (int l = o->nss[i]; l <= o->nss[natm + i]; ++l) {
       int j = o->nlist2[l - 1] - 1;
+      if (SKIP(i) || SKIP(j)) continue;
       double xx[3];
       min_image(o, X, i, j, xx);
       double rij = sqrt(xx[0] * xx[0] + xx[1] * xx[1] + xx[2] * xx[2]);
@@ -952,6 +977,7 @@ void orc_si_potential(orc *o, const double *X, double *utot, double *u) {
     }
     for (int l = o->nsslj[i]; l <= o->nsslj[natm + i]; ++l) {
       int j = o->nlistlj[l - 1] - 1;
+      if (SKIP(i) || SKIP(j)) continue;
       double xx[3];
       min_image(o, X, i, j, xx);
       double rij2 = xx[0] * xx[0] + xx[1] * xx[1] + xx[2] * xx[2];
@@ -972,6 +998,7 @@ void orc_si_potential(orc *o, const double *X, double *utot, double *u) {
   for (int l = 0; l < o->nbrs; ++l) {
     int i = o->nlist3[3 * (size_t)l] - 1, j = o->nlist3[3 * (size_t)l + 1] - 1,
         k = o->nlist3[3 * (size_t)l + 2] - 1;
+    if (SKIP(i) || SKIP(j) || SKIP(k)) continue;
     int ti = o->atype[i] - 1, tj = o->atype[j] - 1, tk = o->atype[k] - 1;
     double xij[3], xik[3], xjk[3];
     min_image(o, X, i, j, xij);
@@ -994,7 +1021,12 @@ void orc_si_potential(orc *o, const double *X, double *utot, double *u) {
     u[k] = u[k] + ul;
     U = U + 3. * ul;
   }
+#undef SKIP
   *utot = U;
+}
+
+void orc_si_potential(orc *o, const double *X, double *utot, double *u) {
+  potential_impl(o, X, utot, u, -1);
 }
 
 /* ---------------------------------------------------------------------- */
@@ -1169,4 +1201,235 @@ int orc_ti_iter(orc *o, int32_t whichctrl, double ttar) {
   o->lt = o->lt + 1;
   orc_onestep(o, whichctrl, ttar, o->lt);
   return 0;
+}
+
+/* ---------------------------------------------------------------------- */
+/* Config mode (the benchmark configurations' integrator).  It has no
+ * reference counterpart; it reuses the reference's force (si_force) and
+ * list build (si_nlist), with removed atoms out of the lists and terms.
+ * Per step, in this order:
+ *   1. each live atom: Berendsen factor of the previous step on the shells
+ *      (x/y sides and bottom, SW atoms, not pinned), half kick, drift,
+ *      pinned atoms back to Xi with V = 0, rebox, removal above the surface
+ *      (its kinetic energy to the ledger), displacement since the build;
+ *   2. interaction energy of the atoms removed in this step with the atoms
+ *      that stay, from the lists of the last build: U(with them) - U(without);
+ *   3. rebuild when the largest plus the second largest displacement since
+ *      the build (as float32 lengths) exceed the skin;
+ *   4. forces, second half kick (pinned atoms stay at rest), |v|max, kinetic
+ *      energy and the shells' sum m v^2 and count;
+ *   5. time += dt, dt = min(dt_max, dx_max/|v|max), the shells' factor
+ *      sqrt(1 + dt/tau (Ttar/T_shell - 1)) for the next step. */
+
+static int md_in_shell(const orc *o, const double x[3]) {
+  const double w = o->md.shell_width;
+  return x[0] < w || x[0] > o->lb[0] - w || x[1] < w || x[1] > o->lb[1] - w ||
+         x[2] < o->bot_z + w;
+}
+
+static double md_potential(orc *o) {
+  double U;
+  double *u = (double *)malloc(sizeof(double) * (size_t)o->natm);
+  potential_impl(o, o->X, &U, u, o->nsg + o->impact);
+  free(u);
+  return U;
+}
+
+static void md_build(orc *o) {
+  orc_si_nlist(o, o->X);
+  memcpy(o->Xb, o->X, 3 * (size_t)o->natm * sizeof(double));
+  o->builds++;
+}
+
+/* forces of the current state, |v|max, kinetic energy; dt from |v|max */
+static void md_refresh(orc *o) {
+  orc_si_force(o, o->X, o->F, NULL);
+  double vmax = 0., ke = 0.;
+  for (int i = 0; i < o->natm; ++i) {
+    if (o->dead[i]) continue;
+    double v2 = XA(o->V, i, 0) * XA(o->V, i, 0) + XA(o->V, i, 1) * XA(o->V, i, 1) +
+                XA(o->V, i, 2) * XA(o->V, i, 2);
+    double vm = sqrt(v2);
+    if (vm > vmax) vmax = vm;
+    ke += 0.5 * o->mass[i] * v2;
+  }
+  o->md_vmax = vmax;
+  o->md_ke = ke;
+  o->md_dt = vmax > 0. ? fmin(o->md.dt_max, o->md.dx_max / vmax) : o->md.dt_max;
+}
+
+int orc_md_begin(orc *o, const orc_md_params *p, double surface_z, double bottom_z) {
+  const size_t n = (size_t)o->natm;
+  o->md = *p;
+  o->surf_z = surface_z;
+  o->bot_z = bottom_z;
+  o->skin = o->rc - 0.999 * o->al * o->sigmaSi;
+  if (!o->dead) o->dead = (int32_t *)calloc(n, sizeof(int32_t));
+  if (!o->Xb) o->Xb = (double *)calloc(3 * n, sizeof(double));
+  o->builds = 0;
+  md_build(o);
+  o->md_time = 0.;
+  o->md_steps = 0;
+  o->md_lambda = 1.;
+  o->e_bath = o->e_out = o->e_in = 0.;
+  md_refresh(o);
+  return 0;
+}
+
+int orc_md_fire(orc *o, int32_t ion, const double x[3], const double v[3]) {
+  if (!o->dead || ion < 1 || ion > o->nlj) return 1;
+  const int i = o->nsg + ion - 1;
+  const double m = o->mass[i];
+  double w2 = 0., v2 = 0.;
+  for (int c = 0; c < 3; ++c) {
+    w2 += XA(o->V, i, c) * XA(o->V, i, c);
+    v2 += v[c] * v[c];
+    XA(o->X, i, c) = x[c];
+    XA(o->V, i, c) = v[c];
+  }
+  o->e_in += 0.5 * m * (v2 - w2);
+  md_build(o);
+  /* the ion's interaction energy at its start point */
+  o->impact = ion - 1;
+  double u0 = md_potential(o);
+  o->impact = ion;
+  double u1 = md_potential(o);
+  o->e_in += u1 - u0;
+  md_refresh(o);
+  return 0;
+}
+
+int orc_md_step(orc *o) {
+  const int n = o->natm;
+  const double dt = o->md_dt, lam = o->md_lambda;
+  double e_bath = 0., e_out = 0.;
+  float *disp = (float *)calloc((size_t)n, sizeof(float));
+  int32_t *leaving = (int32_t *)calloc((size_t)n, sizeof(int32_t));
+  int nleave = 0;
+  double dmax = 0.;
+  for (int i = 0; i < n; ++i) {
+    if (o->dead[i]) continue;
+    const int t = o->atype[i];
+    const double m = o->mass[i];
+    const int pinned = o->xsides[i] > 0;
+    double x[3], v[3];
+    for (int c = 0; c < 3; ++c) { x[c] = XA(o->X, i, c); v[c] = XA(o->V, i, c); }
+    if (o->md.shell_width > 0. && t < 3 && !pinned && md_in_shell(o, x)) {
+      e_bath += 0.5 * m * (v[0] * v[0] + v[1] * v[1] + v[2] * v[2]) * (1. - lam * lam);
+      for (int c = 0; c < 3; ++c) v[c] = v[c] * lam;
+    }
+    for (int c = 0; c < 3; ++c) {
+      v[c] = v[c] + 0.5 * XA(o->F, i, c) * dt / m;
+      x[c] = x[c] + v[c] * dt;
+    }
+    if (pinned)
+      for (int c = 0; c < 3; ++c) { x[c] = XA(o->Xi, i, c); v[c] = 0.; }
+    for (int c = 0; c < 3; ++c) x[c] = fmod(x[c] + 2. * o->lb[c], o->lb[c]);
+    if (x[2] - o->surf_z > o->md.remove_height && x[2] < o->lb[2] - o->md.remove_height) {
+      o->dead[i] = 1;
+      leaving[i] = 1;
+      ++nleave;
+      e_out += 0.5 * m * (v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+      v[0] = v[1] = v[2] = 0.;
+    }
+    for (int c = 0; c < 3; ++c) { XA(o->X, i, c) = x[c]; XA(o->V, i, c) = v[c]; }
+    double e[3];
+    for (int c = 0; c < 3; ++c) {
+      double d = x[c] - XA(o->Xb, i, c);
+      e[c] = d - (double)fnint(d * o->ilb[c]) * o->lb[c];
+    }
+    disp[i] = (float)sqrt(e[0] * e[0] + e[1] * e[1] + e[2] * e[2]);
+    if ((double)disp[i] > dmax) dmax = (double)disp[i];
+  }
+  /* second largest displacement, or the largest when two atoms share it */
+  double d2 = 0.;
+  int nmax = 0;
+  for (int i = 0; i < n; ++i) {
+    double d = (double)disp[i];
+    if (d == dmax) ++nmax;
+    else if (d < dmax && d > d2) d2 = d;
+  }
+  if (nmax >= 2) d2 = dmax;
+  if (nleave > 0) {
+    /* U of the state with the leaving atoms still in, minus U without them */
+    for (int i = 0; i < n; ++i) if (leaving[i]) o->dead[i] = 0;
+    double u_with = md_potential(o);
+    for (int i = 0; i < n; ++i) if (leaving[i]) o->dead[i] = 1;
+    double u_without = md_potential(o);
+    e_out += u_with - u_without;
+  }
+  free(leaving);
+  free(disp);
+  if (dmax + d2 > o->skin) md_build(o);
+  orc_si_force(o, o->X, o->Fp, NULL);
+  double vmax = 0., ke = 0., ksh = 0., nsh = 0.;
+  for (int i = 0; i < n; ++i) {
+    if (o->dead[i]) continue;
+    const int t = o->atype[i];
+    const double m = o->mass[i];
+    const int pinned = o->xsides[i] > 0;
+    double v[3], x[3];
+    for (int c = 0; c < 3; ++c) {
+      v[c] = pinned ? 0. : XA(o->V, i, c) + 0.5 * XA(o->Fp, i, c) * dt / m;
+      XA(o->V, i, c) = v[c];
+      x[c] = XA(o->X, i, c);
+    }
+    double v2 = v[0] * v[0] + v[1] * v[1] + v[2] * v[2];
+    double vm = sqrt(v2);
+    if (vm > vmax) vmax = vm;
+    ke += 0.5 * m * v2;
+    if (o->md.shell_width > 0. && t < 3 && !pinned && md_in_shell(o, x)) {
+      ksh += m * v2;
+      nsh += 1.;
+    }
+  }
+  memcpy(o->F, o->Fp, 3 * (size_t)n * sizeof(double));
+  o->md_vmax = vmax;
+  o->md_ke = ke;
+  o->md_time += dt;
+  o->md_steps++;
+  o->e_bath += e_bath;
+  o->e_out += e_out;
+  const double ndt = vmax > 0. ? fmin(o->md.dt_max, o->md.dx_max / vmax) : o->md.dt_max;
+  o->md_dt = ndt;
+  double l = 1.;
+  if (o->md.shell_width > 0. && nsh > 0. && ksh > 0.) {
+    double ts = ksh / (3. * nsh * kB);
+    l = sqrt(1. + ndt / o->md.tau * (o->md.ttar / ts - 1.));
+  }
+  o->md_lambda = l;
+  return 0;
+}
+
+void orc_md_report(orc *o, double *out) {
+  double rem = 0., rem_ga = 0., disp = 0., live = 0., ionz = -1e300;
+  const double r2 = o->md.displaced_r * o->md.displaced_r;
+  for (int i = 0; i < o->natm; ++i) {
+    const int t = o->atype[i];
+    if (o->dead[i]) {
+      if (t < 3) rem += 1.; else rem_ga += 1.;
+    } else {
+      live += 1.;
+      if (t < 3) {
+        double e[3];
+        for (int c = 0; c < 3; ++c) {
+          double d = XA(o->X, i, c) - XA(o->Xi, i, c);
+          e[c] = d - (double)fnint(d * o->ilb[c]) * o->lb[c];
+        }
+        if (e[0] * e[0] + e[1] * e[1] + e[2] * e[2] > r2) disp += 1.;
+      }
+    }
+    if (t == 3 && o->impact > 0 && i == o->nsg + o->impact - 1)
+      ionz = o->dead[i] ? 1e300 : XA(o->X, i, 2);
+  }
+  out[0] = o->md_time; out[1] = o->md_dt; out[2] = o->md_lambda; out[3] = o->md_ke;
+  out[4] = (double)o->md_steps; out[5] = (double)o->builds;
+  out[6] = rem; out[7] = rem_ga; out[8] = disp;
+  out[9] = o->e_bath; out[10] = o->e_out; out[11] = o->e_in;
+  out[12] = ionz; out[13] = o->md_vmax; out[14] = md_potential(o); out[15] = live;
+}
+
+void orc_md_dead(const orc *o, int32_t *dead) {
+  if (o->dead) memcpy(dead, o->dead, sizeof(int32_t) * (size_t)o->natm);
+  else memset(dead, 0, sizeof(int32_t) * (size_t)o->natm);
 }

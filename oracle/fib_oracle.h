@@ -139,6 +139,32 @@ void orc_ti_begin(orc *o, int32_t whichctrl);
  * time > impact*dti (lt was incremented, as in the reference). */
 int orc_ti_iter(orc *o, int32_t whichctrl, double ttar);
 
+/* --- config mode: the integrator of the benchmark configurations --------
+ * No reference counterpart (the reference's own loop is onestep above).
+ * Velocity Verlet with dt = min(dt_max, dx_max / |v|max) chosen after each
+ * step, atoms with xsides > 0 pinned to Xi, atoms more than remove_height
+ * above surface_z removed (out of every list and term from then on),
+ * optional Berendsen shells on the x/y sides and the bottom, lists rebuilt
+ * when the two largest displacements since the build exceed the list skin
+ * rc - 0.999 a sigma, and an FP64 energy ledger.  One instance; the GPU's
+ * batches are checked against single instances on the GPU itself. */
+typedef struct orc_md_params {
+  double dt_max, dx_max, remove_height, shell_width, ttar, tau, displaced_r;
+} orc_md_params;
+/* Enter config mode: lists of the current state, forces, first dt; clock,
+ * step count and ledger reset. */
+int orc_md_begin(orc *o, const orc_md_params *p, double surface_z, double bottom_z);
+/* Fire ion `ion` (1-based among the Nlj ions) from x with velocity v. */
+int orc_md_fire(orc *o, int32_t ion, const double x[3], const double v[3]);
+/* One step. */
+int orc_md_step(orc *o);
+/* Report: out[0..15] = time, dt, lambda, ke, steps, builds, removed SW
+ * atoms, removed ions, displaced SW atoms, e_bath, e_removed, e_fired,
+ * height of the last fired ion (1e300 once removed), vmax, potential
+ * energy of the current state, number of live atoms. */
+void orc_md_report(orc *o, double *out);
+void orc_md_dead(const orc *o, int32_t *dead);
+
 #ifdef __cplusplus
 }
 #endif

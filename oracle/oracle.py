@@ -41,6 +41,17 @@ class Siga(C.Structure):
     ]
 
 
+class MdParams(C.Structure):
+    """Config-mode step parameters (orc_md_params)."""
+
+    _fields_ = [(k, C.c_double) for k in ("dt_max", "dx_max", "remove_height", "shell_width",
+                                         "ttar", "tau", "displaced_r")]
+
+
+MD_REPORT = ("time", "dt", "lam", "ke", "steps", "builds", "removed", "ions_removed",
+             "displaced", "e_bath", "e_removed", "e_fired", "ion_z", "vmax", "epot", "live")
+
+
 def build() -> str:
     """Compile the oracle library in place (gcc, seconds)."""
     subprocess.run(["make", "-s", "-C", HERE], check=True)
@@ -94,6 +105,11 @@ def lib():
             "orc_onestep": ([vp, i32, f64, i32], C.c_int),
             "orc_tint_init": ([vp], None), "orc_ti_begin": ([vp, i32], None),
             "orc_ti_iter": ([vp, i32, f64], C.c_int),
+            "orc_md_begin": ([vp, P(MdParams), f64, f64], C.c_int),
+            "orc_md_fire": ([vp, i32, vp, vp], C.c_int),
+            "orc_md_step": ([vp], C.c_int),
+            "orc_md_report": ([vp, vp], None),
+            "orc_md_dead": ([vp, vp], None),
         }
         for name, (args, res) in sig.items():
             fn = getattr(L, name)
@@ -315,6 +331,42 @@ class Oracle:
 
     def frzsptrdatms(self):
         self.L.orc_frzsptrdatms(self.h)
+
+    # ---- config mode (the benchmark configurations' integrator) ----------------
+    def md_begin(self, surface_z, bottom_z, dt_max=1e-15, dx_max=0.05e-10, remove_height=10e-10,
+                 shell_width=0.0, ttar=300.0, tau=1e-13, displaced_r=1.0e-10):
+        p = MdParams(dt_max, dx_max, remove_height, shell_width, ttar, tau, displaced_r)
+        if self.L.orc_md_begin(self.h, C.byref(p), float(surface_z), float(bottom_z)) != 0:
+            raise ValueError("orc_md_begin failed")
+
+    def md_fire(self, ion, x, v):
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        v = np.ascontiguousarray(v, dtype=np.float64)
+        if self.L.orc_md_fire(self.h, int(ion), _p(x), _p(v)) != 0:
+            raise ValueError("orc_md_fire failed")
+
+    def md_step(self, n=1):
+        for _ in range(n):
+            self.L.orc_md_step(self.h)
+
+    def md_report(self) -> dict:
+        out = np.zeros(16)
+        self.L.orc_md_report(self.h, _p(out))
+        return dict(zip(MD_REPORT, (float(v) for v in out)))
+
+    def md_dead(self):
+        d = np.zeros(self.natm, np.int32)
+        self.L.orc_md_dead(self.h, _p(d))
+        return d
+
+    def adjusttemp(self, temp, ttar):
+        self.L.orc_adjusttemp(self.h, float(temp), float(ttar))
+
+    def freezsides(self):
+        self.L.orc_freezsides(self.h)
+
+    def rebox(self):
+        self.L.orc_rebox(self.h)
 
 
 def siga_from_values(v: dict) -> Siga:

@@ -95,33 +95,79 @@ def test_fortran_driver_matches_oracle(tmp_path, oracle_lib):
     assert int(snap[0]) == n
 
 
+def _restart_header(path):
+    """First record of a restart file (out.f90:46): int32 lt, real(8) clock,
+    between 4-byte sequential-record markers."""
+    raw = open(path, "rb").read(24)
+    n = int(np.frombuffer(raw[0:4], np.int32)[0])
+    assert n == 12, n
+    lt = int(np.frombuffer(raw[4:8], np.int32)[0])
+    t = float(np.frombuffer(raw[8:16], np.float64)[0])
+    return lt, t
+
+
+def _set(text, key, value):
+    lines = text.split("\n")
+    hit = [k for k, ln in enumerate(lines) if ln.split("\\")[-1].strip().startswith(key)]
+    assert len(hit) == 1, (key, hit)
+    lines[hit[0]] = f"{value}    \\ {key}"
+    return "\n".join(lines)
+
+
 def test_fortran_driver_restart(tmp_path):
-    """Restart records on the res_out cadence (out.f90:129-150) and a
-    whichic-0 start from one (data.f90:71-93, timeint.f90:51-111): the
-    restarted run skips equilibration, resumes lt and the impact count from
-    the record, and fires only the ions not fired yet."""
+    """Restart records on the res_out cadence (out.f90:34-59, 129-150) and a
+    whichic-0 start from one (data.f90:71-93, timeint.f90:50-111).  The run
+    writes a record every step; the last record of impact 1 is the state at
+    the last step of that impact.  A run restarted from it (the record plus
+    its ran1 companion file) skips equilibration, keeps the step counter and
+    the simulated clock of the record, fires only ion 2, from the same spot,
+    and ends impact 2 with the uninterrupted run's final frame: same clock,
+    positions within 2e-4 A and KE within 1e-3 (the two runs rebuild their
+    lists from different atom orders, so the FP64 force sums round
+    differently).  Cadences of 1 step make both runs rebuild, freeze
+    sputtered atoms and test the regime on every step, although the
+    restarted run's step counter is one behind (the reference's TI loop
+    counts one more step when an impact ends, timeint.f90:140-145)."""
     assert os.path.exists(EXE), "build fortran/mdrun_gpu first"
     text = open(SIGA).read()
-    assert "-5000    \\ res_out" in text
-    text = text.replace("-5000    \\ res_out", "25    \\ res_out")
-    (tmp_path / "siga.in").write_text(text)
-    for d in ("data", "D", "F"):
-        os.makedirs(tmp_path / d)
-    print(f"mdrun_gpu --seed {SEED} (res_out 25) in {tmp_path}", flush=True)
-    r = subprocess.run([EXE, "--seed", str(SEED)], cwd=tmp_path, capture_output=True, text=True,
+    text = _set(text, "res_out", 1)
+    for k in ("ntlist", "atlist", "dslist"):
+        for s in ("_i", "_r"):
+            text = _set(text, k + s, 1)
+    a = tmp_path / "a"
+    b = tmp_path / "b"
+    for top in (a, b):
+        for d in ("data", "D", "F"):
+            os.makedirs(top / d)
+    (a / "siga.in").write_text(text)
+    print(f"mdrun_gpu --seed {SEED} (res_out 1) in {a}", flush=True)
+    r = subprocess.run([EXE, "--seed", str(SEED)], cwd=a, capture_output=True, text=True,
                        timeout=300)
     assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
-    names = sorted(os.listdir(tmp_path / "D"))
+    names = sorted(os.listdir(a / "D"))
     assert "restart.in_old" in names and "restart.rng_old" in names
-    first = [f for f in names if f.startswith("restart.in000001_")]
-    assert first, names
-    lt_rec = int(first[0].split("_")[1])
-    assert lt_rec % 25 == 0
-    text = text.replace("4    \\ whichic", "0    \\ whichic").replace(
-        "0    \\ startimpact", "1    \\ startimpact")
-    (tmp_path / "siga.in").write_text(text)
-    print("mdrun_gpu restarted from impact 1", flush=True)
-    r2 = subprocess.run([EXE, "--seed", str(SEED)], cwd=tmp_path, capture_output=True, text=True,
+    rec1 = sorted((f for f in names if f.startswith("restart.in000001_")),
+                  key=lambda f: int(f.split("_")[1]))
+    assert len(rec1) > 10, names[:5]
+    last = rec1[-1]
+    lt_rec = int(last.split("_")[1])
+    lt_h, t_rec = _restart_header(a / "D" / last)
+    assert lt_h == lt_rec
+    # the record's clock is the clock of its step: impact 1's final frame
+    # (written after the TI loop exits, no step in between) shows it
+    n, fa = parse(a / "data" / "mdrun2.xyz")
+    times_a = [float(h[12:24]) for h, _ in fa]
+    assert any(abs(t_rec * 1e12 - t) < 1e-5 for t in times_a), (t_rec, times_a)
+    assert t_rec > EXPECTED["dti"] and t_rec <= EXPECTED["dti"] + 1.01 * EXPECTED["ts_r"]
+    # earlier records carry earlier clocks
+    lt_e, t_e = _restart_header(a / "D" / rec1[-11])
+    assert lt_e == lt_rec - 10 and t_e < t_rec
+    for stem in ("restart.in", "restart.rng"):
+        shutil.copy(a / "D" / last.replace("restart.in", stem), b / "D")
+    rtext = _set(_set(text, "whichic", 0), "startimpact", 1)
+    (b / "siga.in").write_text(rtext)
+    print(f"mdrun_gpu restarted from {last}", flush=True)
+    r2 = subprocess.run([EXE, "--seed", str(SEED)], cwd=b, capture_output=True, text=True,
                         timeout=300)
     assert r2.returncode == 0, r2.stdout[-2000:] + r2.stderr[-2000:]
     out = r2.stdout
@@ -129,10 +175,22 @@ def test_fortran_driver_restart(tmp_path):
     assert "Initializing target temperature" not in out
     knocks = [ln for ln in out.splitlines() if "KNOCK" in ln]
     assert len(knocks) == 1 and knocks[0].split()[-1] == str(nsg + 2), knocks
-    # progress lines of the restarted run continue the step counter
-    lts = [int(ln.split()[0]) for ln in out.splitlines()
-           if len(ln.split()) == 3 and ln.split()[0].isdigit()]
-    assert lts and min(lts) > lt_rec, (lts[:3], lt_rec)
+    # progress lines of the restarted run continue the step counter and clock
+    prog = [ln.split() for ln in out.splitlines()
+            if len(ln.split()) == 3 and ln.split()[0].isdigit()]
+    assert prog and int(prog[0][0]) > lt_rec, (prog[:3], lt_rec)
+    assert float(prog[0][1]) > t_rec * 1e12
+    n2, fb = parse(b / "data" / "mdrun2.xyz")
+    assert n2 == n
+    (ha, ra), (hb, rb) = fa[-1], fb[-1]
+    assert ha[:24] == hb[:24], (ha[:24], hb[:24])
+    for i in range(n):
+        xa = [float(ra[i][12 * c:12 * c + 12]) for c in range(3)]
+        xb = [float(rb[i][12 * c:12 * c + 12]) for c in range(3)]
+        assert np.allclose(xa, xb, rtol=0, atol=2e-4), (i, ra[i], rb[i])
+        assert ra[i][36:52] == rb[i][36:52]
+        ka, kb = float(ra[i][52:66]), float(rb[i][52:66])
+        assert abs(ka - kb) <= 1e-3 * abs(ka) + 1e-30, (i, ra[i], rb[i])
 
 
 CFG = os.path.join(ROOT, "fortran", "mdcfg_gpu")
