@@ -115,6 +115,9 @@ SIGNATURES = {
     "fib_timer_name": ([i32], C.c_char_p),
     "fib_md_begin": ([vp, P(fib_md_params), vp, vp], i32),
     "fib_md_fire": ([vp, i32, i32, vp, vp], i32),
+    "fib_md_fire_many": ([vp, i32, vp, vp, vp, vp], i32),
+    "fib_set_list_width": ([vp, i32, i32], i32),
+    "fib_get_list_width": ([vp, vp, vp], i32),
     "fib_md_run": ([vp, i32], i32),
     "fib_md_get_report": ([vp, i32, P(fib_md_report)], i32),
     "fib_md_energy": ([vp, vp], i32),

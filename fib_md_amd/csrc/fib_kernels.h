@@ -93,6 +93,8 @@ struct DevBufs {
   const KTimer *kt;          // timing hook (may be null)
   const int *gate;           // non-null: [2 + 2*ninst] build gates and the cell prefix of
                              // the gated instances, see k_md_decide
+  int *halt;                 // MD-step mode: set when a gated build overflowed its
+                             // width; the step's later kernels then return at once
 };
 
 namespace fibk {
@@ -103,8 +105,18 @@ void launch_sort(const DevBufs &d, hipStream_t s, bool inplace = false);
 // list build + longest-list reduction (DCtl::maxcnt_*)
 void launch_nbuild(const DevBufs &d, const FConst &fc, hipStream_t s);
 void launch_cellid(const DevBufs &d, int inst, int32_t *dout, hipStream_t s);
-// positions of the new build (Xb), displacement bookkeeping reset
-void launch_mark_build(const DevBufs &d, hipStream_t s);
+// positions of the new build (Xb), displacement bookkeeping reset; count =
+// false repeats a build after the lists grew (the build count stays)
+void launch_mark_build(const DevBufs &d, hipStream_t s, bool count = true);
+// ELL rows into a wider layout: dst[p * wnew + k] = src[p * wold + k]
+void launch_copy_rows(int NA, int wold, const int *src, int wnew, int *dst, hipStream_t s);
+void launch_copy_rows(int NA, int wold, const uint16_t *src, int wnew, uint16_t *dst,
+                      hipStream_t s);
+// clears the leaving mark of every atom (after a removal-list overflow)
+void launch_clear_leaving(const DevBufs &d, hipStream_t s);
+// lists = true: clears the list-overflow flags and the halt flag; false: the
+// removal-list overflow flags
+void launch_clear_overflow(const DevBufs &d, hipStream_t s, bool lists = true);
 // passes 1-2 (k_legs, k_tri): own pair terms and legs, centre-term shares
 // into G, energy and virial reductions
 void launch_force_a(const DevBufs &d, const FConst &fc, bool mixed, bool energy, bool virial,

@@ -48,6 +48,14 @@ namespace fibk {
   do {                       \
     if ((gate) && !(gate)[0]) return; \
   } while (0)
+// MD-step mode: once a list build of a step overflowed its width, `halt` is
+// set and every later kernel of the step (and of the steps after it, when
+// they were already enqueued or captured) returns at once, so the host can
+// grow the lists and finish the step from the state the overflow left.
+#define HALT_OR_RETURN(halt) \
+  do {                       \
+    if ((halt) && *(halt)) return; \
+  } while (0)
 __device__ __forceinline__ bool gated_out(const int *gate, int inst) {
   return gate && !gate[1 + inst];
 }
@@ -221,8 +229,10 @@ __global__ __launch_bounds__(FIN_T) void k_finalize(const DInst *__restrict__ I,
                                                     int pstride, RedSpec rs,
                                                     DCtl *__restrict__ ctl,
                                                     const int *__restrict__ gate,
-                                                    double *__restrict__ tmp) {
+                                                    double *__restrict__ tmp,
+                                                    const int *__restrict__ halt) {
   GATE_OR_RETURN(gate);
+  HALT_OR_RETURN(halt);
   __shared__ double red[FIN_T / FIB_WAVE][NV];
   const int inst = blockIdx.x, ch = blockIdx.y, nch = gridDim.y;
   const int blk0 = I[inst].atom_off / rs.bs, nblk = I[inst].natm_pad / rs.bs;
@@ -257,8 +267,9 @@ __global__ __launch_bounds__(FIN_T) void k_finalize(const DInst *__restrict__ I,
 template <int NV>
 __global__ void k_finalize2(const DInst *__restrict__ I, int nch, RedSpec rs,
                             DCtl *__restrict__ ctl, const int *__restrict__ gate,
-                            const double *__restrict__ tmp) {
+                            const double *__restrict__ tmp, const int *__restrict__ halt) {
   GATE_OR_RETURN(gate);
+  HALT_OR_RETURN(halt);
   const int inst = blockIdx.x, k = threadIdx.x;
   if (k >= NV) return;
   const int nci = (I[inst].natm_pad / rs.bs + FIN_T - 1) / FIN_T;
@@ -1003,7 +1014,8 @@ __global__ __launch_bounds__(FIB_BLOCK) void k_legs(
     const int *__restrict__ nbr_sw, const int *__restrict__ cnt_ion,
     const int *__restrict__ nbr_ion, int *__restrict__ cnt_leg, uint64_t *__restrict__ amask,
     double *__restrict__ Fpart, double *__restrict__ upart, int pstride,
-    double *__restrict__ partials) {
+    double *__restrict__ partials, const int *__restrict__ halt) {
+  HALT_OR_RETURN(halt);
   typedef typename Prec<MIXED>::T T;
   const int lb = xcd_remap(blockIdx.x, gridDim.x);
   const int i = lb * FIB_BLOCK + threadIdx.x;
@@ -1229,7 +1241,9 @@ __global__ __launch_bounds__(FIB_BLOCK) void k_tri(
     int Wsw, const int *__restrict__ cnt_leg, const int *__restrict__ nbr_sw,
     const uint16_t *__restrict__ rk_sw, const uint8_t *__restrict__ off_sw,
     const uint64_t *__restrict__ amask, typename Prec<MIXED>::T4 *__restrict__ G,
-    uint32_t *__restrict__ RA, int pstride, double *__restrict__ partials) {
+    uint32_t *__restrict__ RA, int pstride, double *__restrict__ partials,
+    const int *__restrict__ halt) {
+  HALT_OR_RETURN(halt);
   typedef typename Prec<MIXED>::T T;
   typedef typename Prec<MIXED>::T4 T4;
   constexpr int NS = TriStage<MIXED>::N;
@@ -1425,7 +1439,8 @@ __global__ __launch_bounds__(FIB_BLOCK) void k_extra(
     const int *__restrict__ cnt_leg, const uint64_t *__restrict__ amask,
     const float *__restrict__ disp, DCtl *__restrict__ ctl, typename Prec<MIXED>::T4 *__restrict__ EX,
     typename Prec<MIXED>::T4 *__restrict__ EXself, uint64_t *__restrict__ exmask, int pstride,
-    double *__restrict__ partials) {
+    double *__restrict__ partials, const int *__restrict__ halt) {
+  HALT_OR_RETURN(halt);
   typedef typename Prec<MIXED>::T T;
   typedef typename Prec<MIXED>::T4 T4;
   const int lb = xcd_remap(blockIdx.x, gridDim.x);
@@ -1583,7 +1598,7 @@ __global__ __launch_bounds__(FIB_BLOCK) void k_extra(
 __global__ void k_mark_build(int NA, int ninst, int Wsw, int Wion, const double4 *__restrict__ X,
                              double4 *__restrict__ Xb, float *__restrict__ disp,
                              DCtl *__restrict__ ctl, const int *__restrict__ binst,
-                             const int *__restrict__ gate) {
+                             const int *__restrict__ gate, int *__restrict__ halt, double inc) {
   GATE_OR_RETURN(gate);
   GRID_STRIDE(i, max(NA, ninst)) {
   // NA is a multiple of FIB_BLOCK: the lanes of a block are all below it or all above
@@ -1596,9 +1611,14 @@ __global__ void k_mark_build(int NA, int ninst, int Wsw, int Wion, const double4
     ctl[i].maxdisp2 = 0.0;
     ctl[i].nmaxdisp = 0.0;
     ctl[i].extra_any = 0;
-    // a list longer than its width was cut: sticky, checked by the host
-    if (ctl[i].maxcnt_sw > Wsw || ctl[i].maxcnt_ion > Wion) ctl[i].overflow = 1;
-    ctl[i].builds += 1.0;
+    // a list longer than its width was cut: sticky, checked by the host;
+    // in MD-step mode the rest of the step halts until the host has grown
+    // the lists and rebuilt them (inc = 0 then: the same build once more)
+    if (ctl[i].maxcnt_sw > Wsw || ctl[i].maxcnt_ion > Wion) {
+      ctl[i].overflow = 1;
+      if (halt) *halt = 1;
+    }
+    ctl[i].builds += inc;
   }
   }
 }
@@ -1631,7 +1651,9 @@ __global__ __launch_bounds__(FIB_BLOCK) void k_md_pos(double4 *__restrict__ X,
                                                       const DCtl *__restrict__ ctl,
                                                       const double4 *__restrict__ Xb,
                                                       float *__restrict__ disp,
-                                                      int *__restrict__ leave) {
+                                                      int *__restrict__ leave,
+                                                      const int *__restrict__ halt) {
+  HALT_OR_RETURN(halt);
   const int lb = xcd_remap(blockIdx.x, gridDim.x);
   const int i = lb * FIB_BLOCK + threadIdx.x;
   double4 x = X[i];
@@ -1697,7 +1719,9 @@ __global__ __launch_bounds__(FIB_BLOCK) void k_md_pos(double4 *__restrict__ X,
 __global__ __launch_bounds__(FIB_BLOCK) void k_disp2(const double4 *__restrict__ X, int NA,
                                                      const float *__restrict__ disp,
                                                      const DCtl *__restrict__ ctl, int pstride,
-                                                     double *__restrict__ partials) {
+                                                     double *__restrict__ partials,
+                                                     const int *__restrict__ halt) {
+  HALT_OR_RETURN(halt);
   const int lb = xcd_remap(blockIdx.x, gridDim.x);
   const int i = lb * FIB_BLOCK + threadIdx.x;
   // every block lies inside one instance (its first atom names it)
@@ -1845,7 +1869,13 @@ __global__ __launch_bounds__(DECIDE_T) void k_md_decide(
     int ninst, const DInst *__restrict__ I, DCtl *__restrict__ ctl, double skin,
     int *__restrict__ gate, int *__restrict__ leave, double4 *__restrict__ X, const FConst fc,
     int Wsw, int Wion, const int *__restrict__ cnt_sw, const int *__restrict__ nbr_sw,
-    const int *__restrict__ cnt_ion, const int *__restrict__ nbr_ion) {
+    const int *__restrict__ cnt_ion, const int *__restrict__ nbr_ion,
+    const int *__restrict__ halt) {
+  // halted: no build in this step either (the host restores the flag)
+  if (halt && *halt) {
+    if (threadIdx.x == 0) gate[0] = 0;
+    return;
+  }
   __shared__ int s_w[DECIDE_T / FIB_WAVE];
   __shared__ int s_any;
   __shared__ int s_list[FIB_LEAVE_CAP];
@@ -1938,7 +1968,9 @@ __global__ void k_fire_energy(int q, const int *__restrict__ inv, const double4 
 
 // After the step's second half kick: advance time, choose the next dt and
 // the next Berendsen factor of the shells.
-__global__ void k_md_ctl(int ninst, DCtl *__restrict__ ctl, StepArgs sa) {
+__global__ void k_md_ctl(int ninst, DCtl *__restrict__ ctl, StepArgs sa,
+                         const int *__restrict__ halt) {
+  HALT_OR_RETURN(halt);
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= ninst) return;
   DCtl &c = ctl[k];
@@ -1968,6 +2000,7 @@ __global__ void k_md_init(int ninst, DCtl *__restrict__ ctl, StepArgs sa, int re
     c.cfg_steps = 0.0;
     c.cfg_lambda = 1.0;
     c.e_bath = c.e_out = c.e_in = 0.0;
+    c.leave_overflow = 0;
   }
 }
 
@@ -2018,7 +2051,9 @@ __global__ __launch_bounds__(FIB_BLOCK) void k_force3(
     double escale, double *__restrict__ Fout, double *__restrict__ uout, double *__restrict__ V,
     StepArgs sa, int pstride, double *__restrict__ partials, const DCtl *__restrict__ ctl,
     const typename Prec<MIXED>::T4 *__restrict__ EX,
-    const typename Prec<MIXED>::T4 *__restrict__ EXself, const uint64_t *__restrict__ exmask) {
+    const typename Prec<MIXED>::T4 *__restrict__ EXself, const uint64_t *__restrict__ exmask,
+    const int *__restrict__ halt) {
+  HALT_OR_RETURN(halt);
   typedef typename Prec<MIXED>::T4 T4;
   const int lb = xcd_remap(blockIdx.x, gridDim.x);
   const int i = lb * FIB_BLOCK + threadIdx.x;
@@ -2404,10 +2439,10 @@ template <int NV>
 static void fin_launch(const DevBufs &d, const RedSpec &rs, int nch, const int *gate,
                        hipStream_t s) {
   hipLaunchKernelGGL(k_finalize<NV>, dim3(d.ninst, nch), dim3(FIN_T), 0, s, d.I, d.partials,
-                     d.pstride, rs, d.ctl, gate, d.fin_tmp);
+                     d.pstride, rs, d.ctl, gate, d.fin_tmp, d.halt);
   if (nch > 1)
     hipLaunchKernelGGL(k_finalize2<NV>, dim3(d.ninst), dim3(FIB_WAVE), 0, s, d.I, nch, rs, d.ctl,
-                       gate, d.fin_tmp);
+                       gate, d.fin_tmp, d.halt);
 }
 
 static void finalize(const DevBufs &d, int nv, int bs, const int *op, const int *dst,
@@ -2546,14 +2581,14 @@ static void force_ab(const DevBufs &d, const FConst &fc, hipStream_t s) {
     hipLaunchKernelGGL((k_legs<MIXED, EN, VIR>), dim3(d.NA / FIB_BLOCK), dim3(FIB_BLOCK), 0, s,
                        d.X[d.cur], d.I, fc, d.tab, d.NA, d.Wsw, d.Wion, d.cnt_sw, d.nbr_sw,
                        d.cnt_ion, d.nbr_ion, d.cnt_leg, d.amask, d.Fpart, d.upart, d.pstride,
-                       d.partials);
+                       d.partials, d.halt);
   }
   {
     KScope ks_(d, KT_TRI);
     hipLaunchKernelGGL((k_tri<MIXED, EN, VIR>), dim3(d.NA / FIB_BLOCK), dim3(FIB_BLOCK), 0, s,
                        d.X[d.cur], d.I, fc, d.NA, d.Wsw, d.cnt_leg, d.nbr_sw, d.rk_sw,
                        d.off_sw, d.amask, (T4 *)d.G, d.RA, d.pstride,
-                       d.partials + (size_t)FIB_NRED * d.pstride);
+                       d.partials + (size_t)FIB_NRED * d.pstride, d.halt);
   }
   {
     KScope ks_(d, KT_EXTRA);
@@ -2561,7 +2596,7 @@ static void force_ab(const DevBufs &d, const FConst &fc, hipStream_t s) {
                        d.X[d.cur], d.I, fc, d.NA, d.Wsw, d.cnt_sw, d.nbr_sw, d.cnt_leg, d.amask,
                        d.disp, d.ctl, (T4 *)d.EX,
                        (T4 *)d.EXself, d.exmask, d.pstride,
-                       d.partials + (size_t)2 * FIB_NRED * d.pstride);
+                       d.partials + (size_t)2 * FIB_NRED * d.pstride, d.halt);
   }
   if (EN || VIR) {
     int op[10], dst[10];
@@ -2583,7 +2618,7 @@ static void force_c(const DevBufs &d, const FConst &fc, const StepArgs &sa, doub
                        (const T4 *)d.G, d.RA, d.cnt_leg, d.cnt_sw, d.Fpart,
                        d.upart, fscale, escale, Fo,
                        d.uout, d.V[d.cur], sa, d.pstride, d.partials, d.ctl, (const T4 *)d.EX,
-                       (const T4 *)d.EXself, d.exmask);
+                       (const T4 *)d.EXself, d.exmask, d.halt);
   }
   if (MODE == 1) finalize_thermo(d, s);
   if (MODE == 2) {
@@ -2638,7 +2673,7 @@ void launch_md_pos(const DevBufs &d, const FConst &fc, const StepArgs &sa, int *
     KScope ks_(d, KT_POS);
     hipLaunchKernelGGL(k_md_pos, dim3(d.NA / FIB_BLOCK), dim3(FIB_BLOCK), 0, s, d.X[d.cur], d.I,
                        d.NA, d.V[d.cur], d.F[d.cur], d.Xi[d.cur], sa, d.pstride, d.partials,
-                       d.ctl, d.Xb, d.disp, leave);
+                       d.ctl, d.Xb, d.disp, leave, d.halt);
   }
   const int op[3] = {1, 0, 0};
   const int dst[3] = {DCTL_IDX(maxdisp), DCTL_IDX(e_bath_step), DCTL_IDX(e_out_step)};
@@ -2646,14 +2681,14 @@ void launch_md_pos(const DevBufs &d, const FConst &fc, const StepArgs &sa, int *
   {
     KScope ks_(d, KT_POS);
     hipLaunchKernelGGL(k_disp2, dim3(d.NA / FIB_BLOCK), dim3(FIB_BLOCK), 0, s, d.X[d.cur], d.NA,
-                       d.disp, d.ctl, d.pstride, d.partials);
+                       d.disp, d.ctl, d.pstride, d.partials, d.halt);
   }
   const int op2[2] = {1, 0};
   const int dst2[2] = {DCTL_IDX(maxdisp2), DCTL_IDX(nmaxdisp)};
   finalize(d, 2, FIB_BLOCK, op2, dst2, s);
   hipLaunchKernelGGL(k_md_decide, dim3(1), dim3(DECIDE_T), 0, s, d.ninst, d.I, d.ctl, sa.skin,
                      gate, leave, d.X[d.cur], fc, d.Wsw, d.Wion, d.cnt_sw, d.nbr_sw, d.cnt_ion,
-                     d.nbr_ion);
+                     d.nbr_ion, d.halt);
 }
 
 void launch_fire_energy(const DevBufs &d, const FConst &fc, int q, hipStream_t s) {
@@ -2663,7 +2698,7 @@ void launch_fire_energy(const DevBufs &d, const FConst &fc, int q, hipStream_t s
 
 void launch_md_ctl(const DevBufs &d, const StepArgs &sa, hipStream_t s) {
   hipLaunchKernelGGL(k_md_ctl, dim3(nblk(d.ninst, FIB_BLOCK)), dim3(FIB_BLOCK), 0, s, d.ninst,
-                     d.ctl, sa);
+                     d.ctl, sa, d.halt);
 }
 
 void launch_md_init(const DevBufs &d, const StepArgs &sa, bool reset, hipStream_t s) {
@@ -2683,11 +2718,62 @@ void launch_md_report(const DevBufs &d, const StepArgs &sa, hipStream_t s) {
   finalize(d, 4, FIB_BLOCK, op, dst, s);
 }
 
-void launch_mark_build(const DevBufs &d, hipStream_t s) {
+void launch_mark_build(const DevBufs &d, hipStream_t s, bool count) {
   KScope ks_(d, KT_LISTAUX);
   hipLaunchKernelGGL(k_mark_build, dim3(gcap(std::max(d.NA, d.ninst))), dim3(FIB_BLOCK), 0,
                      s, d.NA, d.ninst, d.Wsw, d.Wion, d.X[d.cur], d.Xb, d.disp, d.ctl, d.binst,
-                     d.gate);
+                     d.gate, d.halt, count ? 1.0 : 0.0);
+}
+
+// Rows of an ELL list copied into a wider layout (the list grew).
+template <typename T>
+__global__ void k_copy_rows(int NA, int wold, const T *__restrict__ src, int wnew,
+                            T *__restrict__ dst) {
+  GRID_STRIDE(p, NA) {
+    for (int k = 0; k < wold; ++k) dst[(size_t)p * wnew + k] = src[(size_t)p * wold + k];
+  }
+}
+
+void launch_copy_rows(int NA, int wold, const int *src, int wnew, int *dst, hipStream_t s) {
+  hipLaunchKernelGGL(k_copy_rows<int>, dim3(gcap(NA)), dim3(FIB_BLOCK), 0, s, NA, wold, src,
+                     wnew, dst);
+}
+void launch_copy_rows(int NA, int wold, const uint16_t *src, int wnew, uint16_t *dst,
+                      hipStream_t s) {
+  hipLaunchKernelGGL(k_copy_rows<uint16_t>, dim3(gcap(NA)), dim3(FIB_BLOCK), 0, s, NA, wold, src,
+                     wnew, dst);
+}
+
+// After a removal-list overflow: no atom stays marked as leaving.
+__global__ void k_clear_leaving(int NA, double4 *__restrict__ X) {
+  GRID_STRIDE(p, NA) {
+    const double4 x = X[p];
+    if (meta_of(x) & META_LEAVING) {
+      double4 y = x;
+      y.w = __longlong_as_double((long long)(wbits(x) & ~(uint64_t)META_LEAVING));
+      X[p] = y;
+    }
+  }
+}
+
+// Clears the list-overflow flags and the halt flag (after the lists grew),
+// or the removal-list overflow flags.
+__global__ void k_clear_overflow(int ninst, DCtl *__restrict__ ctl, int *__restrict__ halt,
+                                 int lists) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < ninst) {
+    if (lists) ctl[k].overflow = 0; else ctl[k].leave_overflow = 0;
+  }
+  if (k == 0 && lists && halt) *halt = 0;
+}
+
+void launch_clear_overflow(const DevBufs &d, hipStream_t s, bool lists) {
+  hipLaunchKernelGGL(k_clear_overflow, dim3(nblk(d.ninst, FIB_BLOCK)), dim3(FIB_BLOCK), 0, s,
+                     d.ninst, d.ctl, d.halt, lists ? 1 : 0);
+}
+
+void launch_clear_leaving(const DevBufs &d, hipStream_t s) {
+  hipLaunchKernelGGL(k_clear_leaving, dim3(gcap(d.NA)), dim3(FIB_BLOCK), 0, s, d.NA, d.X[d.cur]);
 }
 
 void launch_sums(const DevBufs &d, const StepArgs &sa, hipStream_t s) {

@@ -303,6 +303,38 @@ static int32_t alloc_lists(fib_handle *h, int Wsw, int Wion) {
   return FIB_OK;
 }
 
+// Wider lists that keep what the current ones hold (rows copied into the
+// new layout): instances that are not rebuilt keep their lists.  The
+// per-evaluation arrays (G, RA, EX) are reallocated empty.
+static int32_t regrow_lists(fib_handle *h, int Wsw, int Wion) {
+  DevBufs &d = h->d;
+  const size_t NA = (size_t)d.NA;
+  if ((uint64_t)Wsw * NA >= (1ull << 32))
+    return fail(h, FIB_ERR_ARG, "too many atoms for one handle: atoms x list width >= 2^32");
+  int *nsw = nullptr, *nion = nullptr;
+  uint16_t *rk = nullptr;
+  HIPCHK(h, dalloc(&nsw, (size_t)Wsw * NA));
+  HIPCHK(h, dalloc(&nion, (size_t)Wion * NA));
+  HIPCHK(h, dalloc(&rk, (size_t)Wsw * NA));
+  fibk::launch_copy_rows(d.NA, d.Wsw, d.nbr_sw, Wsw, nsw, h->stream);
+  fibk::launch_copy_rows(d.NA, d.Wsw, d.rk_sw, Wsw, rk, h->stream);
+  fibk::launch_copy_rows(d.NA, d.Wion, d.nbr_ion, Wion, nion, h->stream);
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  void *old[] = {d.nbr_sw, d.nbr_ion, d.rk_sw, d.G, d.RA, d.EX};
+  for (void *q : old)
+    if (q) (void)hipFree(q);
+  d.nbr_sw = nsw; d.nbr_ion = nion; d.rk_sw = rk;
+  d.G = nullptr; d.RA = nullptr; d.EX = nullptr;
+  d.Wsw = Wsw;
+  d.Wion = Wion;
+  const size_t gsz = h->prec == FIB_PREC_MIXED ? sizeof(float4) : sizeof(double4);
+  HIPCHK(h, hipMalloc(&d.G, gsz * (size_t)Wsw * NA));
+  HIPCHK(h, dalloc(&d.RA, (size_t)Wsw * NA));
+  HIPCHK(h, hipMalloc(&d.EX, gsz * (size_t)Wsw * NA));
+  h->graph_valid = false;  // the graph refers to the old arrays
+  return FIB_OK;
+}
+
 extern "C" int32_t fib_create(const fib_params *p, int32_t ninst, const fib_instance_desc *desc,
                               int32_t precision, fib_handle **out) {
   if (!out) return FIB_ERR_ARG;
@@ -996,6 +1028,75 @@ static int32_t md_refresh(fib_handle *h, bool reset) {
   return FIB_OK;
 }
 
+// Layout of md_gate: [0] any gate, [1 + k] gate of instance k, [1 + n + k]
+// cell prefix of the gated instances (n + 1 entries), then the removal list
+// [count, FIB_LEAVE_CAP atoms], then the halt flag.
+static size_t md_leave_idx(const fib_handle *h) { return 2 * h->inst.size() + 2; }
+static size_t md_halt_idx(const fib_handle *h) { return md_leave_idx(h) + 1 + FIB_LEAVE_CAP; }
+
+static int32_t md_halted(fib_handle *h, bool &halted) {
+  int v = 0;
+  HIPCHK(h, hipMemcpy(&v, h->d.halt, sizeof(int), hipMemcpyDeviceToHost));
+  halted = v != 0;
+  return FIB_OK;
+}
+
+// Gives up on a list that cannot grow: the halt flag is cleared so that the
+// handle's other calls work; the state is that of the halted step.
+static int32_t md_abort(fib_handle *h, const std::string &msg) {
+  fibk::launch_clear_overflow(h->d, h->stream, true);
+  (void)hipStreamSynchronize(h->stream);
+  return fail(h, FIB_ERR_OVERFLOW, msg);
+}
+
+// A gated build overflowed its list width (the halt flag is set, the stream
+// idle).  Grow the widths to fit the longest list, copy the lists of the
+// instances not rebuilt into the wider layout, repeat the gated list build
+// (binning and sort are done; positions are where the build found them) and,
+// when the build belonged to an MD step, finish that step.
+static int32_t md_recover(fib_handle *h, bool finish_step) {
+  DevBufs &d = h->d;
+  for (int attempt = 0; attempt < 4; ++attempt) {
+    std::vector<DCtl> c(h->inst.size());
+    HIPCHK(h, hipMemcpy(c.data(), d.ctl, sizeof(DCtl) * c.size(), hipMemcpyDeviceToHost));
+    int msw = 0, mion = 0;
+    for (auto &x : c) {
+      msw = std::max(msw, (int)x.maxcnt_sw);
+      mion = std::max(mion, (int)x.maxcnt_ion);
+    }
+    int nsw = d.Wsw, nion = d.Wion;
+    while (nsw < msw) nsw *= 2;
+    while (nion < mion) nion *= 2;
+    if (nsw > 64) return md_abort(h, "SW neighbor list exceeds 64 entries per atom");
+    if (nsw != d.Wsw || nion != d.Wion) {
+      int32_t r = regrow_lists(h, nsw, nion);
+      if (r != FIB_OK) return r;
+    }
+    fibk::launch_clear_overflow(d, h->stream);
+    const int one = 1;
+    HIPCHK(h, hipMemcpyAsync(h->md_gate, &one, sizeof(int), hipMemcpyHostToDevice, h->stream));
+    d.gate = h->md_gate;
+    fibk::launch_nbuild(d, h->fc, h->stream);
+    fibk::launch_mark_build(d, h->stream, false);
+    d.gate = nullptr;
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    bool halted = false;
+    int32_t r = md_halted(h, halted);
+    if (r != FIB_OK) return r;
+    if (halted) continue;
+    if (finish_step) {
+      const bool mixed = h->prec == FIB_PREC_MIXED;
+      fibk::launch_force_a(d, h->fc, mixed, false, false, h->stream);
+      fibk::launch_force_b(d, h->fc, mixed, false, 2, h->md_sa, d.F[d.cur], h->stream);
+      fibk::launch_md_ctl(d, h->md_sa, h->stream);
+      HIPCHK(h, hipStreamSynchronize(h->stream));
+    }
+    h->graph_valid = false;
+    return FIB_OK;
+  }
+  return md_abort(h, "neighbor list did not fit after regrowth");
+}
+
 extern "C" int32_t fib_md_begin(fib_handle *h, const fib_md_params *p, const double *surface_z,
                                 const double *bottom_z) {
   if (!h || !p || !surface_z || !bottom_z) return FIB_ERR_ARG;
@@ -1004,8 +1105,7 @@ extern "C" int32_t fib_md_begin(fib_handle *h, const fib_md_params *p, const dou
     return fail(h, FIB_ERR_ARG, "fib_md_begin: bad parameters");
   if (!h->have_state) return fail(h, FIB_ERR_STATE, "fib_md_begin: set the state first");
   if (!h->md_gate) {
-    // gates and cell prefix [2 + 2*ninst], then the removal list [1 + FIB_LEAVE_CAP]
-    const size_t ng = 2 * h->inst.size() + 2 + 1 + FIB_LEAVE_CAP;
+    const size_t ng = md_halt_idx(h) + 1;
     HIPCHK(h, hipMalloc(&h->md_gate, sizeof(int) * ng));
     HIPCHK(h, hipMemset(h->md_gate, 0, sizeof(int) * ng));
   }
@@ -1026,6 +1126,10 @@ extern "C" int32_t fib_md_begin(fib_handle *h, const fib_md_params *p, const dou
   sa.tau = p->tau > 0 ? p->tau : 1.0;
   sa.disp_r = p->displaced_r;
   sa.skin = h->fc.skin;
+  // a fresh start: no removal pending, nothing halted
+  HIPCHK(h, hipMemset(h->md_gate + md_leave_idx(h), 0, sizeof(int)));
+  HIPCHK(h, hipMemset(h->md_gate + md_halt_idx(h), 0, sizeof(int)));
+  h->d.halt = h->md_gate + md_halt_idx(h);
   if (!h->have_lists) {
     int32_t r = build_lists(h);
     if (r != FIB_OK) return r;
@@ -1035,61 +1139,170 @@ extern "C" int32_t fib_md_begin(fib_handle *h, const fib_md_params *p, const dou
   return md_refresh(h, true);
 }
 
+extern "C" int32_t fib_md_fire_many(fib_handle *h, int32_t count, const int32_t *inst,
+                                    const int32_t *ion, const double *x, const double *v) {
+  if (!h || count < 0 || (count > 0 && (!inst || !ion || !x || !v))) return FIB_ERR_ARG;
+  if (!h->md_on) return fail(h, FIB_ERR_STATE, "fib_md_fire: call fib_md_begin first");
+  if (count == 0) return FIB_OK;
+  const int n = (int)h->inst.size();
+  std::vector<int> fired(n, 0);
+  for (int q = 0; q < count; ++q) {
+    if (bad_inst(h, inst[q])) return fail(h, FIB_ERR_ARG, "fib_md_fire: bad instance");
+    if (ion[q] < 1 || ion[q] > h->inst[inst[q]].nlj)
+      return fail(h, FIB_ERR_ARG, "fib_md_fire: ion out of range");
+    if (fired[inst[q]]++) return fail(h, FIB_ERR_ARG, "fib_md_fire: one ion per instance and call");
+  }
+  DevBufs &d = h->d;
+  // sorted positions of the ions (the device's inverse map), then place them
+  std::vector<int> pos(count);
+  for (int q = 0; q < count; ++q) {
+    const DInst &in = h->inst[inst[q]];
+    HIPCHK(h, hipMemcpy(&pos[q], d.inv + in.atom_off + in.nsg + ion[q] - 1, sizeof(int),
+                        hipMemcpyDeviceToHost));
+  }
+  for (int q = 0; q < count; ++q) {
+    fibk::launch_set_atom(d, pos[q], x + 3 * (size_t)q, v + 3 * (size_t)q, h->md_sa, h->stream);
+    DInst &in = h->inst[inst[q]];
+    in.impact = ion[q];
+    HIPCHK(h, hipMemcpyAsync(&d.I[inst[q]], &in, sizeof(DInst), hipMemcpyHostToDevice,
+                             h->stream));
+  }
+  // the ions jumped: rebuild the fired instances only (gated, in place, so
+  // the other instances keep their order, lists and rebuild bookkeeping)
+  std::vector<int> g(2 + 2 * (size_t)n, 0);
+  int acc = 0;
+  g[0] = 1;
+  for (int k = 0; k < n; ++k) {
+    g[1 + k] = fired[k];
+    g[1 + n + k] = acc;
+    if (fired[k]) acc += h->inst[k].nc[0] * h->inst[k].nc[1] * h->inst[k].nc[2];
+  }
+  g[1 + 2 * (size_t)n] = acc;
+  HIPCHK(h, hipMemcpyAsync(h->md_gate, g.data(), sizeof(int) * g.size(), hipMemcpyHostToDevice,
+                           h->stream));
+  d.gate = h->md_gate;
+  fibk::launch_bin(d, h->stream);
+  fibk::launch_scan(d, h->stream);
+  fibk::launch_sort(d, h->stream, true);
+  fibk::launch_nbuild(d, h->fc, h->stream);
+  fibk::launch_mark_build(d, h->stream);
+  d.gate = nullptr;
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  bool halted = false;
+  int32_t r = md_halted(h, halted);
+  if (r != FIB_OK) return r;
+  if (halted) {
+    r = md_recover(h, false);
+    if (r != FIB_OK) return r;
+  }
+  ++h->n_builds;
+  h->have_lists = true;
+  // the ledger takes each ion's potential energy at its start point with its
+  // kinetic energy
+  for (int q = 0; q < count; ++q) {
+    const DInst &in = h->inst[inst[q]];
+    fibk::launch_fire_energy(d, h->fc, in.atom_off + in.nsg + ion[q] - 1, h->stream);
+  }
+  return md_refresh(h, false);
+}
+
 extern "C" int32_t fib_md_fire(fib_handle *h, int32_t inst, int32_t ion, const double x[3],
                                const double v[3]) {
   if (!h || !x || !v) return FIB_ERR_ARG;
-  if (!h->md_on) return fail(h, FIB_ERR_STATE, "fib_md_fire: call fib_md_begin first");
-  if (bad_inst(h, inst)) return fail(h, FIB_ERR_ARG, "fib_md_fire: bad instance");
-  DInst &in = h->inst[inst];
-  if (ion < 1 || ion > in.nlj) return fail(h, FIB_ERR_ARG, "fib_md_fire: ion out of range");
-  int p = -1;
-  HIPCHK(h, hipMemcpy(&p, h->d.inv + in.atom_off + in.nsg + ion - 1, sizeof(int),
-                      hipMemcpyDeviceToHost));
-  fibk::launch_set_atom(h->d, p, x, v, h->md_sa, h->stream);
-  in.impact = ion;
-  HIPCHK(h, hipMemcpyAsync(&h->d.I[inst], &in, sizeof(DInst), hipMemcpyHostToDevice, h->stream));
-  // the ion jumped: fresh lists, forces and dt; the ledger takes the
-  // ion's potential energy at its start point with its kinetic energy
-  int32_t r = build_lists(h);
-  if (r != FIB_OK) return r;
-  fibk::launch_fire_energy(h->d, h->fc, in.atom_off + in.nsg + ion - 1, h->stream);
-  return md_refresh(h, false);
+  return fib_md_fire_many(h, 1, &inst, &ion, x, v);
+}
+
+static int32_t md_steps_done(fib_handle *h, double &steps) {
+  DCtl c;
+  HIPCHK(h, hipMemcpy(&c, h->d.ctl, sizeof(DCtl), hipMemcpyDeviceToHost));
+  steps = c.cfg_steps;
+  return FIB_OK;
 }
 
 extern "C" int32_t fib_md_run(fib_handle *h, int32_t nsteps) {
   if (!h || nsteps < 0) return FIB_ERR_ARG;
   if (!h->md_on) return fail(h, FIB_ERR_STATE, "fib_md_run: call fib_md_begin first");
   if (nsteps == 0) return FIB_OK;
-  if (h->md.use_graph) {
-    if (!h->graph_valid) {
-      if (h->graph_exec) { (void)hipGraphExecDestroy(h->graph_exec); h->graph_exec = nullptr; }
-      if (h->graph) { (void)hipGraphDestroy(h->graph); h->graph = nullptr; }
-      const KTimer *kt = h->d.kt;
-      h->d.kt = nullptr;  // no timing events inside the graph
-      HIPCHK(h, hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
-      enqueue_md_step(h);
-      hipError_t e = hipStreamEndCapture(h->stream, &h->graph);
-      h->d.kt = kt;
-      HIPCHK(h, e);
-      HIPCHK(h, hipGraphInstantiate(&h->graph_exec, h->graph, nullptr, nullptr, 0));
-      h->graph_valid = true;
+  // every instance steps together: instance 0's count is the batch's
+  double s0 = 0;
+  int32_t r = md_steps_done(h, s0);
+  if (r != FIB_OK) return r;
+  const double target = s0 + nsteps;
+  for (int rounds = 0;; ++rounds) {
+    double now = 0;
+    r = md_steps_done(h, now);
+    if (r != FIB_OK) return r;
+    const int left = (int)(target - now + 0.5);
+    if (left <= 0) break;
+    if (h->md.use_graph) {
+      if (!h->graph_valid) {
+        if (h->graph_exec) { (void)hipGraphExecDestroy(h->graph_exec); h->graph_exec = nullptr; }
+        if (h->graph) { (void)hipGraphDestroy(h->graph); h->graph = nullptr; }
+        const KTimer *kt = h->d.kt;
+        h->d.kt = nullptr;  // no timing events inside the graph
+        HIPCHK(h, hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
+        enqueue_md_step(h);
+        hipError_t e = hipStreamEndCapture(h->stream, &h->graph);
+        h->d.kt = kt;
+        HIPCHK(h, e);
+        HIPCHK(h, hipGraphInstantiate(&h->graph_exec, h->graph, nullptr, nullptr, 0));
+        h->graph_valid = true;
+      }
+      for (int k = 0; k < left; ++k) HIPCHK(h, hipGraphLaunch(h->graph_exec, h->stream));
+    } else {
+      for (int k = 0; k < left; ++k) enqueue_md_step(h);
     }
-    for (int k = 0; k < nsteps; ++k) HIPCHK(h, hipGraphLaunch(h->graph_exec, h->stream));
-  } else {
-    for (int k = 0; k < nsteps; ++k) enqueue_md_step(h);
+    HIPCHK(h, hipGetLastError());
+    HIPCHK(h, hipStreamSynchronize(h->stream));
+    bool halted = false;
+    r = md_halted(h, halted);
+    if (r != FIB_OK) return r;
+    if (!halted) break;
+    // a gated build of some step overflowed: the steps after it did nothing;
+    // grow the lists, finish that step and run the rest
+    r = md_recover(h, true);
+    if (r != FIB_OK) return r;
+    if (rounds > 64) return md_abort(h, "fib_md_run: lists keep overflowing");
   }
-  HIPCHK(h, hipGetLastError());
-  HIPCHK(h, hipStreamSynchronize(h->stream));
   h->have_force = false;
   std::vector<DCtl> c(h->inst.size());
   HIPCHK(h, hipMemcpy(c.data(), h->d.ctl, sizeof(DCtl) * c.size(), hipMemcpyDeviceToHost));
   for (auto &x : c) {
     if (x.overflow) return fail(h, FIB_ERR_OVERFLOW, "fib_md_run: a neighbour list outgrew its width");
-    if (x.leave_overflow)
-      return fail(h, FIB_ERR_OVERFLOW, "fib_md_run: more atoms left the simulation in one step "
-                                       "than the removal list holds");
     if (x.nonfinite) return fail(h, FIB_ERR_NONFINITE, "fib_md_run: non-finite position");
   }
+  for (auto &x : c)
+    if (x.leave_overflow) {
+      // the ledger missed the interaction energy of the atoms past the
+      // removal list's capacity in that step; no atom stays marked leaving
+      fibk::launch_clear_leaving(h->d, h->stream);
+      fibk::launch_clear_overflow(h->d, h->stream, false);
+      HIPCHK(h, hipStreamSynchronize(h->stream));
+      return fail(h, FIB_ERR_OVERFLOW, "fib_md_run: more atoms left the simulation in one step "
+                                       "than the removal list holds (" +
+                                       std::to_string(FIB_LEAVE_CAP) + "); the energy ledger "
+                                       "misses their interaction energy");
+    }
+  return FIB_OK;
+}
+
+extern "C" int32_t fib_set_list_width(fib_handle *h, int32_t wsw, int32_t wion) {
+  if (!h) return FIB_ERR_ARG;
+  if (wsw < 4 || wsw > 64 || wsw % 4 || wion < 4 || wion % 4)
+    return fail(h, FIB_ERR_ARG, "fib_set_list_width: widths must be multiples of 4, SW at most 64");
+  if (h->md_on) return fail(h, FIB_ERR_STATE, "fib_set_list_width: not in MD-step mode");
+  int32_t r = alloc_lists(h, wsw, wion);
+  if (r != FIB_OK) return r;
+  h->have_lists = false;
+  h->have_force = false;
+  h->graph_valid = false;
+  return FIB_OK;
+}
+
+extern "C" int32_t fib_get_list_width(const fib_handle *h, int32_t *wsw, int32_t *wion) {
+  if (!h || !wsw || !wion) return FIB_ERR_ARG;
+  *wsw = h->d.Wsw;
+  *wion = h->d.Wion;
   return FIB_OK;
 }
 

@@ -241,6 +241,25 @@ class FibGPU:
         v = np.ascontiguousarray(v, dtype=np.float64)
         self._chk("fib_md_fire", self.L.fib_md_fire(self.h, int(inst), int(ion), _p(x), _p(v)))
 
+    def md_fire_many(self, insts, ions, xs, vs):
+        """Fire ions[q] of instance insts[q] from xs[q] with velocity vs[q],
+        all at once (one list rebuild of the fired instances)."""
+        inst = np.ascontiguousarray(insts, dtype=np.int32)
+        ion = np.ascontiguousarray(ions, dtype=np.int32)
+        x = np.ascontiguousarray(xs, dtype=np.float64).reshape(-1, 3)
+        v = np.ascontiguousarray(vs, dtype=np.float64).reshape(-1, 3)
+        assert len(inst) == len(ion) == len(x) == len(v)
+        self._chk("fib_md_fire_many", self.L.fib_md_fire_many(self.h, len(inst), _p(inst), _p(ion),
+                                                              _p(x), _p(v)))
+
+    def set_list_width(self, wsw, wion):
+        self._chk("fib_set_list_width", self.L.fib_set_list_width(self.h, int(wsw), int(wion)))
+
+    def list_width(self):
+        a, b = C.c_int32(), C.c_int32()
+        self._chk("fib_get_list_width", self.L.fib_get_list_width(self.h, C.byref(a), C.byref(b)))
+        return a.value, b.value
+
     def md_run(self, nsteps):
         self._chk("fib_md_run", self.L.fib_md_run(self.h, int(nsteps)))
 

@@ -231,7 +231,17 @@ int32_t fib_md_begin(fib_handle *h, const fib_md_params *p, const double *surfac
  * refreshed and the instance's dt follows the ion's speed. */
 int32_t fib_md_fire(fib_handle *h, int32_t inst, int32_t ion, const double x[3],
                     const double v[3]);
-/* Advance every instance by nsteps steps of its own dt. */
+/* Fire one ion in each of `count` distinct instances at once: inst[q],
+ * ion[q], x[3q..3q+2], v[3q..3q+2].  Only the fired instances' lists are
+ * rebuilt; the others keep their state bit for bit. */
+int32_t fib_md_fire_many(fib_handle *h, int32_t count, const int32_t *inst,
+                         const int32_t *ion, const double *x, const double *v);
+/* Advance every instance by nsteps steps of its own dt.  A list build that
+ * outgrows the list width halts the step on the device; the lists are then
+ * grown, the build repeated and the step finished, so the trajectory is the
+ * one wider lists give, bit for bit.  More than 256 atoms leaving in one
+ * step returns FIB_ERR_OVERFLOW: the step is complete, but the ledger
+ * misses those atoms' interaction energy. */
 int32_t fib_md_run(fib_handle *h, int32_t nsteps);
 
 typedef struct fib_md_report {
@@ -256,6 +266,15 @@ int32_t fib_md_get_report(fib_handle *h, int32_t inst, fib_md_report *r);
 /* Potential energy (J) of every instance's current state, FP64 sums of the
  * handle's precision: epot[k] for instance k.  The step's forces are kept. */
 int32_t fib_md_energy(fib_handle *h, double *epot);
+
+/* ELL list widths (entries per atom): SW list (a multiple of 4, at most 64)
+ * and ion list (a multiple of 4).  Lists grow by themselves; setting a width
+ * reserves room up front (or shrinks it), and drops the current lists.  Not
+ * in MD-step mode.  No reference counterpart: the reference sizes its lists
+ * once (init_nlist, si.f90:63-74) and only prints on overflow
+ * (si.f90:167-169,264-272). */
+int32_t fib_set_list_width(fib_handle *h, int32_t wsw, int32_t wion);
+int32_t fib_get_list_width(const fib_handle *h, int32_t *wsw, int32_t *wion);
 
 /* Device-side kernel timers (HIP events on the handle's stream around each
  * launch), accumulated since fib_timing_enable(h, mask); for benchmarks.

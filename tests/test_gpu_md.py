@@ -126,11 +126,70 @@ def test_md_batch_equals_single(precision):
         np.testing.assert_array_equal(Xb, Xa)
         np.testing.assert_array_equal(Vb, Va)
         assert rb["time"] == ra["time"]
-        # each fib_md_fire rebuilds every instance; the gated rebuilds during
-        # the run are the instance's own
-        assert rb["builds"] - ra["builds"] == len(batch) - 1
+        # a fired ion rebuilds its own instance only
+        assert rb["builds"] == ra["builds"]
     # the ions went in and the lists were rebuilt on the way
     assert all(r[3]["builds"] >= 2 for r in together)
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_md_fire_many_equals_fires_one_by_one(precision):
+    """Firing the ions of a batch in one call (one rebuild of the fired
+    instances) gives the same trajectories, bit for bit, as one call per ion;
+    instances that are not fired keep their lists and rebuild count."""
+    batch = _slab_batch()
+    out = []
+    for many in (True, False):
+        g = _gpu(batch, precision)
+        _load(g, batch)
+        g.md_begin([s["surface_z"] for s in batch], [s["bottom_z"] for s in batch])
+        b0 = [g.md_report(k)["builds"] for k in range(3)]
+        fire = [0, 2]
+        if many:
+            g.md_fire_many(fire, [1, 1], [batch[k]["ion_x"] for k in fire],
+                           [batch[k]["ion_v"] for k in fire])
+        else:
+            for k in fire:
+                g.md_fire(k, 1, batch[k]["ion_x"], batch[k]["ion_v"])
+        assert [g.md_report(k)["builds"] - b0[k] for k in range(3)] == [1, 0, 1]
+        g.md_run(80)
+        out.append([(*g.state(inst=k)[:2], g.md_report(k)) for k in range(3)])
+        g.close()
+    for a, b in zip(*out):
+        np.testing.assert_array_equal(a[0], b[0])
+        np.testing.assert_array_equal(a[1], b[1])
+        assert a[2] == b[2]
+
+
+@pytest.mark.parametrize("use_graph", [True, False])
+def test_md_list_overflow_grows_and_replays(use_graph):
+    """Lists narrower than a cascade needs: a 0 K slab (16 neighbours per
+    bulk atom within the list radius) with SW lists 16 wide and ion lists 4
+    wide, hit by a 3 keV ion.  The fire-time build and gated builds during the
+    run overflow; the step halts on the device, the host grows the lists,
+    repeats the build and finishes the step.  The trajectory equals, bit for
+    bit, that of lists wide enough from the start."""
+    s = configs.si_slab(4, 31, nz=4, T=0.0)
+    x, v = configs.ion_launch(s["L"], s["surface_z"], 3000.0, 10.0, 20.0,
+                              np.random.default_rng(31))
+    res = []
+    for w in ((16, 4), (64, 64)):
+        g = _gpu([s], "fp64")
+        g.set_list_width(*w)
+        _load(g, [s])
+        g.md_begin([s["surface_z"]], [0.0], use_graph=use_graph if w[0] == 16 else True)
+        assert g.list_width() == w
+        g.md_fire(0, 1, x, v)
+        for _ in range(5):
+            g.md_run(60)
+        res.append((*g.state()[:2], g.md_report(0), g.list_width()))
+        g.close()
+    (Xa, Va, ra, wa), (Xb, Vb, rb, wb) = res
+    print(f"narrow lists grew to {wa}; builds {ra['builds']}", flush=True)
+    assert wa[0] > 16 and wa[1] > 4 and wb == (64, 64)
+    np.testing.assert_array_equal(Xa, Xb)
+    np.testing.assert_array_equal(Va, Vb)
+    assert ra == rb
 
 
 def test_md_graph_equals_eager():
