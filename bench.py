@@ -1,19 +1,32 @@
 #!/usr/bin/env python3
-"""Throughput of the FIB_MD force path on one MI355X.
+"""Throughput of the FIB_MD force-and-integrate path on one MI355X.
 
-Workload (the force microbenchmark): a 64x64x64-cell periodic Si crystal
-(2,097,152 sites), every atom displaced by N(0, 0.1 A), a central 20 A sphere
-randomised at 2.28 g/cc.  One "step" is one neighbour-list rebuild plus one
-SW force + energy + virial evaluation of every atom.  Reported value: atom
-evaluations per second of the mixed-precision path (FP32 terms, compensated
-FP32 force sums, FP64 positions and energies); the plain FP64 path and the
-force error of mixed vs FP64 are reported beside it.
+Headline workload (the cumulative-dose configuration): 16 replica Si(100)
+slabs of 40x40x40 cells (8,192,000 Si + 16 Ga) in one batched handle, 3 A
+Berendsen side and bottom shells at 300 K, each replica's first 2 keV
+normal-incidence Ga ion fired at once.  One "step" is one MD step of every
+replica: half kick, drift, pins, removal, gated list rebuilds, SW + Moliere
+forces, second half kick, per-replica adaptive dt.  Reported value:
+atom-steps/s of the mixed-precision path (FP32 terms, compensated FP32
+force sums, FP64 integration and energies); the FP64 path runs beside it.
+Roofline: 240 algorithmic bytes per atom-step (read X, V, F, type and 16
+list indices, write X, V, F) over the step's HIP-event time.
+
+Extras in the same JSON line: the force microbenchmark (64^3-cell crystal,
+one list build + force/energy/virial, FP64 vs mixed force error), the single
+500 eV impact (alone and as 1024 replicas), the 64-instance ensemble, the
+30 keV impact on 2.3 M atoms and a two-impact dose series, with mixed-vs-FP64
+force error and energy drift per configuration; a CPU baseline of the
+oracle's MD step on a bounded sample.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
-       python bench.py --profile fp64|mixed   (few steps + a 256 MiB copy, for rocprofv3)
+       python bench.py --profile fp64|mixed|md-fp64|md-mixed|md-replicas|dose-mixed|dose-fp64
+                       (few steps + a 256 MiB copy, for rocprofv3)
        python bench.py --summarize-stats DIR OUT.txt CMD   (rocprofv3 --stats csv -> text)
-       python bench.py --summarize-pmc FETCH_DIR WRITE_DIR OUT.txt CMD
+       python bench.py --summarize-pmc FETCH_DIR WRITE_DIR OUT.txt CMD [ATOMS]
                        (rocprofv3 --pmc csv -> text + profiles/pmc_traffic.json)
+       python bench.py --summarize-step-pmc FETCH_DIR WRITE_DIR OUT.txt CMD
+                       (rocprofv3 --pmc csv of a dose profile -> HBM bytes per MD step)
        python bench.py --summarize-counters OUT.txt CMD DIR [DIR ...]
                        (rocprofv3 --pmc SQ/GRBM counters -> text)
 """
@@ -39,6 +52,15 @@ HBM_PEAK_GBS = 8000.0
 # F and u (32 B); k_tri only moves intermediate data (0 B).
 KERNEL_BYTES = {"k_nbuild": 88.0, "k_legs": 92.0, "k_force3": 32.0, "k_tri": 0.0}
 PMC_FILE = os.path.join(ROOT, "profiles", "pmc_traffic.json")
+# Algorithmic bytes per atom-step of the MD step: integrate 124 B (read X, V,
+# F 72 B and type 4 B, write X, V 48 B) + force 116 B (read X 24 B, type 4 B,
+# 16 list indices 64 B, write F 24 B).  Per kernel: k_pos (k_md_pos) carries
+# the integrate's 124 B, k_legs reads X, type and list (92 B), k_force3
+# writes F (24 B); the other kernels move intermediate data only.
+STEP_BYTES = 240.0
+KERNEL_BYTES_MD = {"k_pos": 124.0, "k_legs": 92.0, "k_force3": 24.0}
+STEP_PMC_FILE = os.path.join(ROOT, "profiles", "md_dose_step_traffic.json")
+DOSE_REPLICAS, DOSE_CELLS = 16, 40
 
 
 COPY_KIB = 256 * 1024  # the plain copy of --profile mode: 256 MiB read, 256 MiB written
@@ -157,6 +179,68 @@ def summarize_counters(out, cmd, *dirs):
     print("\n".join(lines))
 
 
+def summarize_step_pmc(dfetch, dwrite, out, cmd):
+    """HBM bytes per MD step of a dose profile (--profile dose-*): every
+    dispatch from the first step's k_md_pos on belongs to the steps; their
+    FETCH_SIZE + WRITE_SIZE (scaled by the 256 MiB copy's readings) summed
+    and divided by the number of steps (k_md_pos dispatches)."""
+    import csv
+    from collections import defaultdict
+
+    def per_dispatch(d, counter):
+        rows = list(csv.DictReader(open(os.path.join(d, "run_counter_collection.csv"))))
+        disp, name = defaultdict(float), {}
+        for r in rows:
+            if r["Counter_Name"] != counter:
+                continue
+            k = int(r["Dispatch_Id"])
+            disp[k] += float(r["Counter_Value"])
+            name[k] = short_name(r["Kernel_Name"])
+        return disp, name
+
+    fd, fn = per_dispatch(dfetch, "FETCH_SIZE")
+    wd, wn = per_dispatch(dwrite, "WRITE_SIZE")
+    fcopy = max(v for k, v in fd.items() if fn[k] == "__amd_rocclr_copyBuffer")
+    wcopy = max(v for k, v in wd.items() if wn[k] == "__amd_rocclr_copyBuffer")
+    ff, wf = COPY_KIB / fcopy, COPY_KIB / wcopy
+
+    def steps_part(disp, name, fac):
+        first = min(k for k in disp if name[k] == "k_md_pos")
+        nsteps = sum(1 for k in disp if name[k] == "k_md_pos")
+        per = defaultdict(float)
+        for k, v in disp.items():
+            if k >= first:
+                per[name[k]] += v * fac * 1024.0
+        return {k: v / nsteps for k, v in per.items()}, nsteps
+
+    fper, nsteps = steps_part(fd, fn, ff)
+    wper, _ = steps_part(wd, wn, wf)
+    atoms = DOSE_REPLICAS * (8 * DOSE_CELLS ** 3 + 1)
+    kern = {}
+    for k in set(fper) | set(wper):
+        b = fper.get(k, 0.0) + wper.get(k, 0.0)
+        kern[k] = {"fetch_bytes": fper.get(k, 0.0), "write_bytes": wper.get(k, 0.0),
+                   "bytes_per_step": b, "bytes_per_atom_step": b / atoms}
+    tot = sum(v["bytes_per_step"] for v in kern.values())
+    res = {"command": cmd, "atoms": atoms, "steps": nsteps, "fetch_factor": ff,
+           "write_factor": wf, "traffic_per_step_bytes": tot,
+           "traffic_per_atom_step": tot / atoms, "kernels": kern}
+    lines = ["# HBM traffic per MD step of the cumulative-dose batch, rocprofv3 --pmc "
+             "FETCH_SIZE and WRITE_SIZE", f"# command: {cmd}",
+             f"# calibration: 256 MiB copy reads FETCH_SIZE {fcopy:.0f} KiB (factor {ff:.3f}), "
+             f"WRITE_SIZE {wcopy:.0f} KiB (factor {wf:.3f})",
+             f"# {nsteps} steps of {atoms} atoms; every dispatch from the first step's k_md_pos on",
+             "", f"{'kernel':36s} {'MB/step':>10s} {'B/atom-step':>12s}"]
+    for k in sorted(kern, key=lambda k: -kern[k]["bytes_per_step"]):
+        v = kern[k]
+        lines.append(f"{k[:36]:36s} {v['bytes_per_step'] / 1e6:10.1f} {v['bytes_per_atom_step']:12.1f}")
+    lines.append(f"{'total':36s} {tot / 1e6:10.1f} {tot / atoms:12.1f}")
+    open(out, "w").write("\n".join(lines) + "\n")
+    with open(STEP_PMC_FILE, "w") as f:
+        json.dump(res, f, indent=1)
+    print("\n".join(lines))
+
+
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
@@ -189,9 +273,8 @@ def run_gpu(X, at, L, precision, steps, warmup):
     torch.cuda.synchronize()
     t1 = time.perf_counter()
     tm = g.timing()
-    timing_steps = steps
     for v in tm.values():
-        v["calls_per_step"] = v["calls"] / timing_steps
+        v["calls_per_step"] = v["calls"] / steps
     res = g.forces(energy=False, virial=True)
     g.timing(enable=False)
     out = dict(seconds=t1 - t0, ms_per_step=(t1 - t0) * 1e3 / steps,
@@ -202,6 +285,8 @@ def run_gpu(X, at, L, precision, steps, warmup):
 
 
 STEPS_PER_IMPACT = 2000  # the single-impact configuration's step count
+STEP_KERNELS = ("k_pos", "k_disp2", "k_bin", "sort", "k_nbuild", "list_aux", "k_legs", "k_tri",
+                "k_extra", "k_force3", "reduce")
 
 
 def _md_handle(instances, precision, use_graph=True, params=None):
@@ -209,12 +294,28 @@ def _md_handle(instances, precision, use_graph=True, params=None):
     return md_handle(instances, precision, params=params, use_graph=use_graph)
 
 
+def _kernel_times(g, steps):
+    """Per-kernel HIP-event times over `steps` steps run as ordinary launches
+    (the trajectory is the graph's, bit for bit): {name: avg ms per launch,
+    launches per step}."""
+    g.md_use_graph(False)
+    g.timing(enable=True, only=STEP_KERNELS)
+    g.md_run(steps)
+    tm = g.timing()
+    g.timing(enable=False)
+    g.md_use_graph(True)
+    return {k: {"avg_ms": v["avg_ms"], "per_step": v["calls"] / steps} for k, v in tm.items()}
+
+
 def md_rate(instances, precision, steps, warmup=5, use_graph=True, params=None,
-            keep_reports=False):
+            keep_reports=False, ledger=False, snapshot=False):
     """Timed MD steps of a batch: atom-steps/s over all instances (with
-    keep_reports, also the per-instance cascade outputs at the end)."""
+    keep_reports, also the per-instance cascade outputs at the end; with
+    ledger, the energy ledger's drift over warmup + timed steps; with
+    snapshot, the final state of every instance)."""
     import torch
     g = _md_handle(instances, precision, use_graph, params)
+    L0 = g.md_ledger() if ledger else None
     g.md_run(warmup)
     b0 = [g.md_report(k)["builds"] for k in range(len(instances))]
     torch.cuda.synchronize()
@@ -224,19 +325,55 @@ def md_rate(instances, precision, steps, warmup=5, use_graph=True, params=None,
     t = time.perf_counter() - t0
     n = sum(len(s["X"]) for s in instances)
     reps = [g.md_report(k) for k in range(len(instances))]
-    g.close()
     # list builds per instance during the timed steps (each instance counts
     # its own gated rebuilds)
     nb = [r["builds"] - b for r, b in zip(reps, b0)]
     out = dict(atom_steps_per_s=n * steps / t, ms_per_step=t * 1e3 / steps, atoms=n,
                builds_per_instance_timed=float(np.mean(nb)),
                removed=sum(r["removed"] for r in reps),
-               dt_min_s=min(r["dt"] for r in reps), sim_time_s=max(r["time"] for r in reps))
+               dt_min_s=min(r["dt_min"] for r in reps), dt_end_s=min(r["dt"] for r in reps),
+               sim_time_s=max(r["time"] for r in reps))
+    if ledger:
+        d = (g.md_ledger() - L0) / EV
+        out["ledger_drift_eV"] = d.tolist()
+        out["fired_eV"] = [r["e_fired"] / EV for r in reps]
+    if snapshot:
+        out["snapshot"] = [(g.state(k)[0], s["atype"], s["L"], s["nsg"], s["nlj"], 1)
+                           for k, s in enumerate(instances)]
+    g.close()
     if keep_reports:
         out["per"] = _per_instance(instances, reps)
     return out
 
 
+def force_error(snapshot):
+    """Forces of the same configurations (one batch of instances: X, atype,
+    Lb, nsg, nlj, fired ions) in FP64 and mixed precision: max / RMS per-atom
+    relative error (floor 1e-3 F_rms) and the norm ratio over all atoms."""
+    from fib_md_amd import FibGPU
+    F = {}
+    for prec in ("fp64", "mixed"):
+        g = FibGPU([(len(X), nsg, nlj, L) for X, _, L, nsg, nlj, _ in snapshot], precision=prec)
+        for k, (X, at, L, nsg, nlj, imp) in enumerate(snapshot):
+            g.set_state(X, atype=at, inst=k)
+            g.set_impact(imp, inst=k)
+        g.build_nlist()
+        g.compute()
+        F[prec] = np.vstack([g.forces(k)["F"] for k in range(len(snapshot))])
+        g.close()
+    emax, erms, enorm, spread = rel_err(F["mixed"], F["fp64"], detail=True)
+    return {"max_rel": emax, "rms_rel": erms, "norm_rel": enorm, "floor": "1e-3 F_rms", **spread}
+
+
+def drift_summary(rf, rm):
+    """Energy-ledger drift (eV) of the mixed and the FP64 run of the same
+    impacts, and mixed - FP64 instance by instance."""
+    a, b = np.array(rm["ledger_drift_eV"]), np.array(rf["ledger_drift_eV"])
+    return {"mixed_mean_eV": float(a.mean()), "mixed_maxabs_eV": float(np.abs(a).max()),
+            "fp64_mean_eV": float(b.mean()), "fp64_maxabs_eV": float(np.abs(b).max()),
+            "mixed_minus_fp64_mean_eV": float((a - b).mean()),
+            "mixed_minus_fp64_maxabs_eV": float(np.abs(a - b).max()),
+            "fired_eV_mean": float(np.mean(rm["fired_eV"]))}
 def _per_instance(instances, reps):
     """Cascade outputs per instance: sputter yield (Si removed above the
     surface), Ga projected range (surface - ion z, A; ions removed are left
@@ -272,6 +409,56 @@ def _compare(per, steps, n_inst):
     return out
 
 
+def _dose_handle(precision, n_rep=DOSE_REPLICAS, ncell=DOSE_CELLS, impacts=1):
+    """The cumulative-dose batch: `n_rep` replicas of ncell^3 cells with
+    `impacts` parked ions each, 3 A Berendsen side and bottom shells at 300 K
+    (tau 0.1 ps), sputtered atoms removed, Ga retained."""
+    from fib_md_amd import FibGPU, configs
+    reps, shots = configs.dose_replicas(n_rep, ncell, impacts)
+    g = FibGPU([(len(s["X"]), s["nsg"], s["nlj"], s["L"]) for s in reps], precision=precision)
+    for k, s in enumerate(reps):
+        g.set_state(s["X"], V=s["V"], atype=s["atype"], xsides=s["xsides"], Xi=s["X"], inst=k)
+    g.md_begin([s["surface_z"] for s in reps], [s["bottom_z"] for s in reps],
+               shell_width=3e-10, ttar=300.0, tau=1e-13)
+    return g, reps, shots
+
+
+def _fire_all(g, shots, m):
+    """Impact m (0-based) of every replica at once (one gated rebuild)."""
+    n = len(shots)
+    g.md_fire_many(list(range(n)), [m + 1] * n, [shots[k][m][0] for k in range(n)],
+                   [shots[k][m][1] for k in range(n)])
+
+
+def dose_rate(precision, steps, warmup, kernel_steps=20):
+    """The headline: the cumulative-dose batch right after its first impact,
+    `warmup` untimed steps, then `steps` timed steps (wall clock around them,
+    and HIP events around each step's graph launch on the library's
+    stream); per-kernel HIP-event times over `kernel_steps` further steps."""
+    import torch
+    g, reps, shots = _dose_handle(precision)
+    _fire_all(g, shots, 0)
+    g.md_run(warmup)
+    b0 = [g.md_report(k)["builds"] for k in range(len(reps))]
+    torch.cuda.synchronize()
+    g.timing(enable=True, only=("md_step",))
+    t0 = time.perf_counter()
+    g.md_run(steps)
+    torch.cuda.synchronize()
+    t = time.perf_counter() - t0
+    ev = g.timing()["md_step"]
+    g.timing(enable=False)
+    rr = [g.md_report(k) for k in range(len(reps))]
+    kern = _kernel_times(g, kernel_steps) if kernel_steps > 0 else {}
+    g.close()
+    n = sum(len(s["X"]) for s in reps)
+    return dict(atoms=n, replicas=len(reps), steps=steps, warmup=warmup,
+                atom_steps_per_s=n * steps / t, ms_per_step=t * 1e3 / steps,
+                step_ms_hip_events=ev["avg_ms"], step_events=ev["calls"],
+                builds_per_replica_timed=float(np.mean([r["builds"] - b for r, b in zip(rr, b0)])),
+                dt_min_s=min(r["dt_min"] for r in rr), kernels=kern)
+
+
 def dose_run(n_rep, ncell, impacts, steps_per_impact, precision):
     """Cumulative dose: `n_rep` replicas of ncell^3 cells, `impacts` sequential
     2 keV normal Ga impacts each (all replicas fire together), the Berendsen
@@ -283,22 +470,14 @@ def dose_run(n_rep, ncell, impacts, steps_per_impact, precision):
     The timed region holds the firing and the steps; reports and the
     ledger's energy evaluations lie outside it."""
     import torch
-    from fib_md_amd import configs
-    reps, shots = configs.dose_replicas(n_rep, ncell, impacts)
-    from fib_md_amd import FibGPU
-    g = FibGPU([(len(s["X"]), s["nsg"], s["nlj"], s["L"]) for s in reps], precision=precision)
-    for k, s in enumerate(reps):
-        g.set_state(s["X"], V=s["V"], atype=s["atype"], xsides=s["xsides"], Xi=s["X"], inst=k)
-    g.md_begin([s["surface_z"] for s in reps], [s["bottom_z"] for s in reps],
-               shell_width=3e-10, ttar=300.0, tau=1e-13)
+    g, reps, shots = _dose_handle(precision, n_rep, ncell, impacts)
     L0 = g.md_ledger()
     yields, ke, drift, fired, bath = [], [], [], [], []
     t = 0.0
     for m in range(impacts):
         torch.cuda.synchronize()
         t0 = time.perf_counter()
-        for k in range(n_rep):
-            g.md_fire(k, m + 1, *shots[k][m])
+        _fire_all(g, shots, m)
         g.md_run(steps_per_impact)
         torch.cuda.synchronize()
         t += time.perf_counter() - t0
@@ -325,15 +504,18 @@ def dose_run(n_rep, ncell, impacts, steps_per_impact, precision):
 def cascade_stats(instances, steps):
     """Per-instance cascade outputs after `steps` MD steps of the batch, FP64
     and mixed, compared (cascades are chaotic: the statistics, not single
-    runs, must agree)."""
-    per = {}
+    runs, must agree); the ledger drift of both runs and the force error of
+    mixed against FP64 on the FP64 run's final configurations."""
+    per, runs = {}, {}
     for prec in ("fp64", "mixed"):
-        g = _md_handle(instances, prec)
-        g.md_run(steps)
-        reps = [g.md_report(k) for k in range(len(instances))]
-        g.close()
-        per[prec] = _per_instance(instances, reps)
-    return _compare(per, steps, len(instances))
+        r = md_rate(instances, prec, steps, warmup=0, keep_reports=True, ledger=True,
+                    snapshot=(prec == "fp64"))
+        per[prec] = r.pop("per")
+        runs[prec] = r
+    out = _compare(per, steps, len(instances))
+    out["energy_drift_vs_fp64"] = drift_summary(runs["fp64"], runs["mixed"])
+    out["force_error_mixed_vs_fp64"] = force_error(runs["fp64"].pop("snapshot"))
+    return out
 
 
 def single_impact_replicas(n):
@@ -352,22 +534,34 @@ def single_impact_replicas(n):
 
 
 def md_bench(args):
-    """MD-step throughput: the single 500 eV impact on an 8^3-cell slab (alone,
-    and as a batch of replicas with their own seeds), and the batched ensemble (64 slabs of 10k-250k atoms, 1-10 keV, 0-60 deg),
-    mixed-batched against the plain FP64 path (one instance at a time)."""
+    """MD-step throughput beside the headline: the single 500 eV impact on an
+    8^3-cell slab (alone, and as a batch of replicas with their own seeds),
+    the batched ensemble (64 slabs of 10k-250k atoms, 1-10 keV, 0-60 deg),
+    mixed-batched against the plain FP64 path (one instance at a time), the
+    30 keV impact on 2.3 M atoms and a shortened dose series; mixed-vs-FP64
+    force error and energy drift per configuration."""
     from fib_md_amd import configs
     out = {}
     # single impact: 8x8x8 cells, 500 eV normal incidence, 300 K, seed 1001
     s = configs.si_slab(8, 1001)
     x, v = configs.ion_launch(s["L"], s["surface_z"], 500.0, 0.0, 0.0, np.random.default_rng(1001))
     s.update(ion_x=x, ion_v=v)
+    runs = {}
     for prec in ("fp64", "mixed"):
         # the whole impact: all 2000 steps of the configuration are timed
-        r = md_rate([s], prec, STEPS_PER_IMPACT)
+        r = md_rate([s], prec, STEPS_PER_IMPACT, ledger=True, snapshot=(prec == "fp64"))
         r["impacts_per_hour"] = 3600.0 / (r["ms_per_step"] * 1e-3 * STEPS_PER_IMPACT)
-        out[f"single_impact_4097_{prec}"] = r
+        runs[prec] = r
         log(f"  single impact {prec}: {r['ms_per_step']:.3f} ms/step, "
-            f"{r['atom_steps_per_s']:.4g} atom-steps/s")
+            f"{r['atom_steps_per_s']:.4g} atom-steps/s, ledger drift "
+            f"{r['ledger_drift_eV'][0]:.3g} eV of {r['fired_eV'][0]:.0f} eV")
+    fe = force_error(runs["fp64"].pop("snapshot"))
+    for prec in ("fp64", "mixed"):
+        out[f"single_impact_4097_{prec}"] = runs[prec]
+    out["single_impact_4097_energy_drift_vs_fp64"] = drift_summary(runs["fp64"], runs["mixed"])
+    out["single_impact_4097_force_error_mixed_vs_fp64"] = fe
+    log(f"  single impact force error mixed vs fp64 after 2005 steps: max {fe['max_rel']:.3g}, "
+        f"rms {fe['rms_rel']:.3g}")
     # the same configuration as a batch of replicas, each with its own seed
     # (velocities and impact point), all 2000 steps timed: impacts/hour of
     # the batched path, and the configuration's cascade statistics in both
@@ -451,21 +645,32 @@ def md_bench(args):
             st = cs[key]
             log(f"  {key:9s} fp64 {st['fp64']:.3f} +- {st['fp64_sem']:.3f}, mixed "
                 f"{st['mixed']:.3f} +- {st['mixed_sem']:.3f}, z {st['z_mixed_vs_fp64']:.2f}")
+        ed, fe = cs["energy_drift_vs_fp64"], cs["force_error_mixed_vs_fp64"]
+        log(f"  ensemble drift mixed {ed['mixed_mean_eV']:.3g} eV, fp64 {ed['fp64_mean_eV']:.3g} eV;"
+            f" force error max {fe['max_rel']:.3g}, rms {fe['rms_rel']:.3g}")
     # the single large impact: 60 x 60 x 80 cells (2,304,000 atoms), 30 keV at
     # 7 deg; impacts/hour at 5000 steps per impact
     if args.big_steps > 0:
         s3 = configs.big_impact()
+        runs = {}
         for prec in ("mixed", "fp64"):
-            r = md_rate([s3], prec, args.big_steps)
+            r = md_rate([s3], prec, args.big_steps, ledger=True, snapshot=(prec == "fp64"))
             r["impacts_per_hour"] = 3600.0 / (r["ms_per_step"] * 1e-3 * 5000)
-            out[f"big_impact_2304001_{prec}"] = r
+            runs[prec] = r
             log(f"  30 keV impact, 2.3 M atoms, {prec}: {r['ms_per_step']:.3f} ms/step, "
-                f"{r['atom_steps_per_s']:.4g} atom-steps/s, dt down to {r['dt_min_s'] * 1e15:.4f} fs")
+                f"{r['atom_steps_per_s']:.4g} atom-steps/s, smallest dt {r['dt_min_s'] * 1e15:.4f} fs, "
+                f"ledger drift {r['ledger_drift_eV'][0]:.3g} eV")
+        fe = force_error(runs["fp64"].pop("snapshot"))
+        for prec in ("mixed", "fp64"):
+            out[f"big_impact_2304001_{prec}"] = runs[prec]
+        out["big_impact_2304001_energy_drift_vs_fp64"] = drift_summary(runs["fp64"], runs["mixed"])
+        out["big_impact_2304001_force_error_mixed_vs_fp64"] = fe
+        log(f"  30 keV force error mixed vs fp64: max {fe['max_rel']:.3g}, rms {fe['rms_rel']:.3g}")
     # cumulative dose, shortened: 16 replicas of 40^3 cells, a few impacts
     if args.dose_impacts > 0:
         out["dose"] = {}
         for prec in ("mixed", "fp64"):
-            r = dose_run(16, 40, args.dose_impacts, args.dose_steps, prec)
+            r = dose_run(DOSE_REPLICAS, DOSE_CELLS, args.dose_impacts, args.dose_steps, prec)
             out["dose"][prec] = r
             log(f"  dose {prec}: {r['atoms']} atoms, {r['atom_steps_per_s']:.4g} atom-steps/s, "
                 f"cumulative yield {['%.2f' % y for y in r['cumulative_yield_mean']]}, ledger drift "
@@ -497,15 +702,46 @@ def rel_err(Fg, Fo, floor_frac=1e-3, detail=False):
     return out
 
 
+CPU_PROCS = 16  # the host cores of one GPU's share of the machine
+MASS = {1: 46.637063e-27, 2: 120.6049350e-27, 3: 1.157777417e-25}
+
+
+def _cpu_md_rate(ncell, steps, seed):
+    """One process: the oracle's MD step (orc_md_step, the configurations'
+    integrator around the reference's si_force and si_nlist) on one replica
+    of the dose configuration's kind, an ncell^3-cell Si(100) slab with 3 A
+    shells at 300 K, right after its 2 keV normal impact.  (atoms, seconds)."""
+    from fib_md_amd import configs
+    from oracle.oracle import Oracle
+    s = configs.si_slab(ncell, seed, nions=1)
+    x, v = configs.ion_launch(s["L"], s["surface_z"], 2000.0, 0.0, 0.0,
+                              np.random.default_rng(seed + 104729))
+    n = len(s["X"])
+    o = Oracle()
+    o.set_geometry(n, s["nsg"], s["nlj"], s["L"])
+    o.set_state(X=s["X"], V=s["V"], atype=s["atype"],
+                mass=np.array([MASS[int(t)] for t in s["atype"]]))
+    o.init_nlist()
+    o.mktables()
+    o.si_nlist(s["X"])
+    o.set_xsides(s["xsides"])
+    o.set_xi(s["X"])
+    o.md_begin(s["surface_z"], 0.0, shell_width=3e-10, ttar=300.0, tau=1e-13)
+    o.md_fire(1, x, v)
+    t0 = time.perf_counter()
+    o.md_step(steps)
+    return n, time.perf_counter() - t0
+
+
 def _cpu_eval_rate(ncell, reps):
     """One process: the oracle's si_nlist + si_force + si_potential on the
-    ncell^3-cell sample, median of `reps` evaluations (atoms, seconds)."""
+    ncell^3-cell microbench sample, median of `reps` evaluations."""
     from oracle.oracle import Oracle
     X, at, L = make_workload(ncell, seed=1002)
     n = len(X)
     o = Oracle()
     o.set_geometry(n, n, 0, L)
-    o.set_state(X=X, V=np.zeros((n, 3)), atype=at, mass=np.full(n, 46.637063e-27))
+    o.set_state(X=X, V=np.zeros((n, 3)), atype=at, mass=np.full(n, MASS[1]))
     o.init_nlist()
     o.mktables()
     ts = []
@@ -518,43 +754,132 @@ def _cpu_eval_rate(ncell, reps):
     return n, float(np.median(ts))
 
 
-CPU_PROCS = 16  # the host cores of one GPU's share of the machine
-
-
-def cpu_baseline(ncell=24, reps=5, procs=CPU_PROCS):
-    """The oracle (scalar C restatement of si_nlist/si_force/si_potential) on a
-    bounded sample of the same workload: one process on one core, then
-    `procs` independent processes at once (one per core, as the reference's
-    MPI ranks would use the host), each timing its own copy of the sample."""
-    from oracle.oracle import build
-    build()
-    n, t1 = _cpu_eval_rate(ncell, reps)
-    procs = max(1, min(procs, os.cpu_count() or 1))
-    cmd = [sys.executable, os.path.abspath(__file__), "--cpu-worker", str(ncell), str(reps)]
-    ps = [subprocess.Popen(cmd, stdout=subprocess.PIPE, text=True) for _ in range(procs)]
-    rates = []
+def _parallel_workers(argv_list):
+    """Run the CPU workers at once (one per core), each printing a JSON line
+    of atoms and seconds; returns their results."""
+    cmd = [sys.executable, os.path.abspath(__file__)]
+    ps = [subprocess.Popen(cmd + a, stdout=subprocess.PIPE, text=True) for a in argv_list]
+    out = []
     for p in ps:
-        out, _ = p.communicate()
+        o, _ = p.communicate()
         if p.returncode != 0:
             raise RuntimeError(f"cpu worker failed ({p.returncode})")
-        w = json.loads(out.strip().splitlines()[-1])
-        rates.append(w["atoms"] / w["seconds"])
+        out.append(json.loads(o.strip().splitlines()[-1]))
+    return out
+
+
+def cpu_baseline(ncell=16, steps=50, procs=CPU_PROCS):
+    """The oracle's MD step (a scalar C restatement) on a bounded sample of
+    the headline workload: `procs` replicas (seeds 3000 + r) of an
+    ncell^3-cell slab, each in its own process on its own core (as the
+    reference's MPI ranks would use the host), `steps` steps after the
+    impact; atom-steps/s summed over the processes."""
+    from oracle.oracle import build
+    build()
+    procs = max(1, min(procs, os.cpu_count() or 1))
+    res = _parallel_workers([["--cpu-md-worker", str(ncell), str(steps), str(3000 + r)]
+                             for r in range(procs)])
+    rates = [w["atoms"] * steps / w["seconds"] for w in res]
+    n = res[0]["atoms"]
+    return dict(value=float(sum(rates)), unit="atom-steps/s", cores=procs, kind="port",
+                per_core_mean=float(np.mean(rates)),
+                sample=f"{procs} replicas (seeds 3000+r) of a {ncell}^3-cell Si(100) slab "
+                       f"({n} atoms, 3 A shells at 300 K), {steps} MD steps each right after "
+                       f"a 2 keV normal Ga impact, the oracle's orc_md_step around the "
+                       f"reference's si_force/si_nlist; one process per core at once, "
+                       f"rates summed ({np.mean([w['seconds'] for w in res]):.1f} s per process)")
+
+
+def cpu_eval_baseline(ncell=24, reps=3, procs=CPU_PROCS):
+    """The microbench's CPU line: the oracle's si_nlist + si_force +
+    si_potential on a 24^3-cell sample, `procs` processes at once."""
+    procs = max(1, min(procs, os.cpu_count() or 1))
+    res = _parallel_workers([["--cpu-worker", str(ncell), str(reps)] for _ in range(procs)])
+    rates = [w["atoms"] / w["seconds"] for w in res]
     return dict(value=float(sum(rates)), unit="atom-evals/s", cores=procs, kind="port",
-                single_core_value=n / t1,
-                sample=f"{ncell}^3-cell crystal ({n} atoms, same noise and 20 A amorphous "
-                       f"sphere), si_nlist+si_force+si_potential, median of {reps}; "
-                       f"{procs} independent processes at once, one per core, rates summed "
-                       f"(one core alone: {t1:.2f} s per evaluation)")
+                sample=f"{ncell}^3-cell microbench sample ({res[0]['atoms']} atoms), "
+                       f"si_nlist+si_force+si_potential, median of {reps}; {procs} processes")
+
+
+def microbench(args):
+    """The force-kernel microbenchmark (64^3 cells, 2.1 M atoms): FP64 and
+    mixed list build + force/energy/virial, force error of mixed vs FP64,
+    roofline of its longest kernel."""
+    X, at, L = make_workload(args.ncell)
+    n = len(X)
+    res = {}
+    for prec in ("fp64", "mixed"):
+        res[prec] = run_gpu(X, at, L, prec, max(args.steps, 10), args.warmup)
+        log(f"  microbench {prec}: {res[prec]['ms_per_step']:.3f} ms/step, "
+            f"{res[prec]['rate']:.4g} atom-evals/s")
+    emax, erms, enorm, espread = rel_err(res["mixed"]["F"], res["fp64"]["F"], detail=True)
+    tm = res["mixed"]["timing"]
+    dom = max((k for k in tm if KERNEL_BYTES.get(k, 0) > 0), key=lambda k: tm[k]["avg_ms"])
+    dur = tm[dom]["avg_ms"]
+    ach = KERNEL_BYTES[dom] * n / (dur * 1e-3) / 1e9
+    traffic = None
+    if os.path.exists(PMC_FILE):
+        pmc = json.load(open(PMC_FILE))
+        for name, k in pmc.get("kernels", {}).items():
+            if name.split("<")[0] == dom and pmc.get("atoms") == n:
+                traffic = k["traffic_bytes"]
+    return {"atoms": n, "box_A": L[0] * 1e10,
+            "mixed": {"value": res["mixed"]["rate"], "ms_per_step": res["mixed"]["ms_per_step"],
+                      "kernels_avg_ms": {k: v["avg_ms"] for k, v in tm.items()}},
+            "fp64": {"value": res["fp64"]["rate"], "ms_per_step": res["fp64"]["ms_per_step"],
+                     "kernels_avg_ms": {k: v["avg_ms"] for k, v in res["fp64"]["timing"].items()}},
+            "unit": "atom-evals/s", "mixed_over_fp64": res["mixed"]["rate"] / res["fp64"]["rate"],
+            "force_error_mixed_vs_fp64": {"max_rel": emax, "rms_rel": erms, "norm_rel": enorm,
+                                          "floor": "1e-3 F_rms", **espread},
+            "energy_rel_diff": abs(res["mixed"]["utot"] - res["fp64"]["utot"]) /
+            abs(res["fp64"]["utot"]),
+            "roofline": {"bound": "hbm", "kernel": dom, "achieved": ach, "peak": HBM_PEAK_GBS,
+                         "unit": "GB/s", "frac": ach / HBM_PEAK_GBS, "traffic": traffic,
+                         "bytes_per_atom": KERNEL_BYTES[dom], "avg_launch_ms": dur}}
+
+
+def step_roofline(h):
+    """Roofline of the headline step: 240 algorithmic bytes per atom-step over
+    the step's HIP-event time (a captured graph of the step's kernels, one
+    launch per step), the measured HBM traffic per step when a dose profile
+    of this workload was summarized, and each kernel's share."""
+    n = h["atoms"]
+    ms = h["step_ms_hip_events"]
+    ach = STEP_BYTES * n / (ms * 1e-3) / 1e9
+    traffic = None
+    if os.path.exists(STEP_PMC_FILE):
+        pmc = json.load(open(STEP_PMC_FILE))
+        if pmc.get("atoms") == n:
+            traffic = pmc["traffic_per_step_bytes"]
+    kern = {}
+    for k, v in h["kernels"].items():
+        b = KERNEL_BYTES_MD.get(k, 0.0)
+        t = v["avg_ms"] * v["per_step"]
+        kern[k] = {"ms_per_step": t, "launches_per_step": v["per_step"],
+                   "bytes_per_atom": b,
+                   "achieved_GBs": b * n / (v["avg_ms"] * 1e-3) / 1e9 if b > 0 else 0.0}
+    dom = max((k for k in kern if kern[k]["bytes_per_atom"] > 0),
+              key=lambda k: kern[k]["ms_per_step"], default=None)
+    out = {"bound": "hbm", "kernel": "md_step (one captured graph per step)", "achieved": ach,
+           "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": ach / HBM_PEAK_GBS, "traffic": traffic,
+           "bytes_per_atom_step": STEP_BYTES, "avg_launch_ms": ms, "kernels": kern}
+    if dom is not None:
+        d = kern[dom]
+        out["dominant_kernel"] = {"kernel": dom, "achieved": d["achieved_GBs"],
+                                  "frac": d["achieved_GBs"] / HBM_PEAK_GBS,
+                                  "bytes_per_atom": d["bytes_per_atom"],
+                                  "avg_launch_ms": h["kernels"][dom]["avg_ms"]}
+    return out
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=10)
-    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--ncell", type=int, default=64)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--no-md", action="store_true")
+    ap.add_argument("--no-md", action="store_true", help="headline only (no extras)")
     ap.add_argument("--md-steps", type=int, default=100)
     ap.add_argument("--ensemble", type=int, default=64)
     ap.add_argument("--replicas", type=int, default=1024,
@@ -568,10 +893,14 @@ def main():
                     help="sequential impacts per replica of the shortened dose run (0: skip)")
     ap.add_argument("--dose-steps", type=int, default=1500,
                     help="MD steps per impact of the dose run")
-    ap.add_argument("--profile", choices=["fp64", "mixed", "md-fp64", "md-mixed", "md-replicas"], default=None)
+    ap.add_argument("--profile", choices=["fp64", "mixed", "md-fp64", "md-mixed", "md-replicas",
+                                          "dose-mixed", "dose-fp64"], default=None)
     ap.add_argument("--summarize-stats", nargs=3, metavar=("DIR", "OUT", "CMD"))
     ap.add_argument("--cpu-worker", nargs=2, metavar=("NCELL", "REPS"), help=argparse.SUPPRESS)
+    ap.add_argument("--cpu-md-worker", nargs=3, metavar=("NCELL", "STEPS", "SEED"),
+                    help=argparse.SUPPRESS)
     ap.add_argument("--summarize-pmc", nargs="+", metavar="FETCH_DIR WRITE_DIR OUT CMD [ATOMS]")
+    ap.add_argument("--summarize-step-pmc", nargs=4, metavar=("FETCH_DIR", "WRITE_DIR", "OUT", "CMD"))
     ap.add_argument("--summarize-counters", nargs="+", metavar="OUT CMD DIR",
                     help="OUT CMD DIR [DIR ...]: SQ/GRBM counter table of rocprofv3 --pmc runs")
     args = ap.parse_args()
@@ -580,11 +909,19 @@ def main():
         n, t = _cpu_eval_rate(int(args.cpu_worker[0]), int(args.cpu_worker[1]))
         print(json.dumps({"atoms": n, "seconds": t}), flush=True)
         return
+    if args.cpu_md_worker:
+        a = args.cpu_md_worker
+        n, t = _cpu_md_rate(int(a[0]), int(a[1]), int(a[2]))
+        print(json.dumps({"atoms": n, "seconds": t}), flush=True)
+        return
     if args.summarize_stats:
         summarize_stats(*args.summarize_stats)
         return
     if args.summarize_pmc:
         summarize_pmc(*args.summarize_pmc[:5])
+        return
+    if args.summarize_step_pmc:
+        summarize_step_pmc(*args.summarize_step_pmc)
         return
     if args.summarize_counters:
         summarize_counters(*args.summarize_counters)
@@ -597,10 +934,6 @@ def main():
     torch.cuda.init()
     from fib_md_amd import _lib
     _lib.lib()
-    t0 = time.perf_counter()
-    X, at, L = make_workload(args.ncell)
-    n = len(X)
-    log(f"workload: {n} atoms, box {L[0]*1e10:.2f} A, generated in {time.perf_counter()-t0:.1f} s")
     if args.profile:
         # a plain copy of known size calibrates the FETCH/WRITE counters
         a = torch.empty(256 * 1024 * 1024 // 4, dtype=torch.float32, device="cuda").fill_(1.0)
@@ -608,6 +941,14 @@ def main():
         b.copy_(a)
         torch.cuda.synchronize()
         del a, b
+    if args.profile in ("dose-mixed", "dose-fp64"):
+        # the headline batch: first impact, then 30 steps (for rocprofv3)
+        g, reps, shots = _dose_handle(args.profile[5:])
+        _fire_all(g, shots, 0)
+        g.md_run(30)
+        g.close()
+        log(f"dose profile {args.profile}: 30 steps")
+        return
     if args.profile in ("md-fp64", "md-mixed"):
         # the batched ensemble, a few MD steps (for rocprofv3)
         from fib_md_amd import configs
@@ -621,44 +962,43 @@ def main():
         log(json.dumps(r))
         return
     if args.profile:
+        X, at, L = make_workload(args.ncell)
         r = run_gpu(X, at, L, args.profile, 10, 2)
         log(json.dumps({k: v for k, v in r.items() if k in ("ms_per_step", "timing")}))
         return
-    res = {}
-    for prec in ("fp64", "mixed"):
-        log(f"running {prec} ...")
-        res[prec] = run_gpu(X, at, L, prec, args.steps, args.warmup)
-        log(f"  {prec}: {res[prec]['ms_per_step']:.3f} ms/step, {res[prec]['rate']:.4g} atom-evals/s")
-        for k, v in sorted(res[prec]["timing"].items(), key=lambda kv: -kv[1]["ms"]):
-            log(f"    {k:10s} {v['avg_ms']:.4f} ms x {v['calls_per_step']:.0f}/step")
-    emax, erms, enorm, espread = rel_err(res["mixed"]["F"], res["fp64"]["F"], detail=True)
-    m = res["mixed"]
-    tm = m["timing"]
-    # roofline of the longest single kernel of the mixed step
-    dom = max((k for k in tm if k in KERNEL_BYTES and KERNEL_BYTES[k] > 0),
-              key=lambda k: tm[k]["avg_ms"])
-    dur_ms = tm[dom]["avg_ms"]
-    bpa = KERNEL_BYTES[dom]
-    achieved = bpa * n / (dur_ms * 1e-3) / 1e9
-    traffic = None
-    if os.path.exists(PMC_FILE):
-        with open(PMC_FILE) as f:
-            pmc = json.load(f)
-        for name, k in pmc.get("kernels", {}).items():
-            if name.split("<")[0] == dom and pmc.get("atoms") == n:
-                traffic = k["traffic_bytes"]
-    md = None
+    # ---- headline: the cumulative-dose batch, mixed and FP64 --------------
+    head = {}
+    for prec in ("mixed", "fp64"):
+        log(f"dose batch {prec}: {args.warmup} warmup + {args.steps} timed steps ...")
+        head[prec] = dose_rate(prec, args.steps, args.warmup)
+        h = head[prec]
+        log(f"  {prec}: {h['ms_per_step']:.3f} ms/step (HIP events {h['step_ms_hip_events']:.3f}), "
+            f"{h['atom_steps_per_s']:.4g} atom-steps/s, {h['builds_per_replica_timed']:.1f} "
+            f"builds per replica")
+        for k, v in sorted(h["kernels"].items(), key=lambda kv: -kv[1]["avg_ms"] * kv[1]["per_step"]):
+            log(f"    {k:10s} {v['avg_ms'] * v['per_step']:.4f} ms/step ({v['per_step']:.0f} launches)")
+    m = head["mixed"]
+    roof = step_roofline(m)
+    log(f"  roofline: {roof['achieved']:.0f} GB/s of {STEP_BYTES:.0f} B/atom-step = "
+        f"{100 * roof['frac']:.2f} % of HBM peak")
+    extras = {}
     if not args.no_md:
-        log("MD steps ...")
-        md = md_bench(args)
+        log("force microbench ...")
+        extras["microbench"] = microbench(args)
+        log("MD configurations ...")
+        extras["md_steps"] = md_bench(args)
     cpu = None
     if not args.no_cpu_baseline:
-        log("cpu baseline ...")
+        log("cpu baseline (oracle MD step) ...")
         cpu = cpu_baseline()
+        log(f"  cpu: {cpu['value']:.4g} atom-steps/s on {cpu['cores']} cores")
+        if not args.no_md:
+            extras["microbench"]["cpu_baseline"] = cpu_eval_baseline()
+    n = m["atoms"]
     line = {
         "metric": "atom-steps/s on 1 MI355X, mixed vs FP64 (SW+Moliere); Ga impacts/hour",
-        "value": m["rate"],
-        "unit": "atom-evals/s (one list rebuild + SW force/energy/virial per atom)",
+        "value": m["atom_steps_per_s"],
+        "unit": "atom-steps/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
@@ -666,23 +1006,22 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32 terms + f64 positions/energies (mixed)",
+        "dtype": "f32 terms + compensated f32 force sums; f64 integration and energies (mixed)",
         "data": "synthetic",
-        "config": {"workload": "force microbench: 64^3-cell Si crystal, N(0,0.1 A) noise, "
-                               "20 A sphere at 2.28 g/cc; list rebuild + force/energy/virial",
-                   "atoms": n, "box_A": L[0] * 1e10, "parallelism": "single GPU"},
-        "fp64": {"value": res["fp64"]["rate"], "ms_per_step": res["fp64"]["ms_per_step"],
-                 "kernels_avg_ms": {k: v["avg_ms"] for k, v in res["fp64"]["timing"].items()}},
-        "mixed_over_fp64": m["rate"] / res["fp64"]["rate"],
-        "force_error_mixed_vs_fp64": {"max_rel": emax, "rms_rel": erms, "norm_rel": enorm,
-                                      "floor": "1e-3 F_rms", **espread},
-        "energy_rel_diff": abs(m["utot"] - res["fp64"]["utot"]) / abs(res["fp64"]["utot"]),
-        "kernels_avg_ms": {k: v["avg_ms"] for k, v in tm.items()},
-        "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
-                     "bytes_per_atom": bpa, "avg_launch_ms": dur_ms},
-        "md_steps": md,
+        "config": {"workload": f"cumulative dose: {DOSE_REPLICAS} replica Si(100) slabs of "
+                               f"{DOSE_CELLS}^3 cells in one batch, first 2 keV normal Ga impact "
+                               f"of every replica, 3 A Berendsen shells at 300 K; "
+                               f"{args.warmup} warmup + {args.steps} timed MD steps",
+                   "atoms": n, "replicas": DOSE_REPLICAS, "parallelism": "single GPU"},
+        "fp64": {"value": head["fp64"]["atom_steps_per_s"],
+                 "ms_per_step": head["fp64"]["ms_per_step"],
+                 "step_ms_hip_events": head["fp64"]["step_ms_hip_events"]},
+        "mixed_over_fp64": m["atom_steps_per_s"] / head["fp64"]["atom_steps_per_s"],
+        "step_ms_hip_events": m["step_ms_hip_events"],
+        "builds_per_replica_timed": m["builds_per_replica_timed"],
+        "roofline": roof,
         "cpu_baseline": cpu,
+        **extras,
     }
     print(json.dumps(line), flush=True)
 

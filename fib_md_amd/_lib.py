@@ -80,7 +80,8 @@ class fib_md_report(C.Structure):
     _fields_ = [("time", C.c_double), ("dt", C.c_double), ("ion_z", C.c_double),
                 ("ke", C.c_double), ("steps", C.c_int64), ("removed", C.c_int32),
                 ("ions_removed", C.c_int32), ("displaced", C.c_int32), ("builds", C.c_int32),
-                ("e_bath", C.c_double), ("e_removed", C.c_double), ("e_fired", C.c_double)]
+                ("e_bath", C.c_double), ("e_removed", C.c_double), ("e_fired", C.c_double),
+                ("dt_min", C.c_double)]
 
 
 class fib_timing(C.Structure):
@@ -116,6 +117,7 @@ SIGNATURES = {
     "fib_md_begin": ([vp, P(fib_md_params), vp, vp], i32),
     "fib_md_fire": ([vp, i32, i32, vp, vp], i32),
     "fib_md_fire_many": ([vp, i32, vp, vp, vp, vp], i32),
+    "fib_md_use_graph": ([vp, i32], i32),
     "fib_set_list_width": ([vp, i32, i32], i32),
     "fib_get_list_width": ([vp, vp, vp], i32),
     "fib_md_run": ([vp, i32], i32),

@@ -33,8 +33,8 @@ def md_handle(instances, precision, params=None, use_graph=True):
         g.set_state(s["X"], V=s["V"], atype=s["atype"], xsides=s["xsides"], Xi=s["X"], inst=k)
     g.md_begin([s["surface_z"] for s in instances], [s["bottom_z"] for s in instances],
                use_graph=use_graph)
-    for k, s in enumerate(instances):
-        g.md_fire(k, 1, s["ion_x"], s["ion_v"])
+    g.md_fire_many(list(range(len(instances))), [1] * len(instances),
+                   [s["ion_x"] for s in instances], [s["ion_v"] for s in instances])
     return g
 
 

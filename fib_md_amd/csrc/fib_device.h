@@ -102,6 +102,7 @@ struct DCtl {
   double cfg_time;      // simulated time (s)
   double cfg_lambda;    // Berendsen factor of the thermostat shells, next step
   double cfg_steps;     // steps taken
+  double cfg_dt_min;    // smallest timestep taken since fib_md_begin (s)
   double n_removed;     // SW atoms removed above the surface (sputtered)
   double n_ga_removed;  // ions removed (backscattered)
   double n_displaced;   // SW atoms farther than the threshold from their site

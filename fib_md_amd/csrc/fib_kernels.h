@@ -40,6 +40,8 @@ enum KKey {
   KT_POS,       // k_pos
   KT_REDUCE,    // k_finalize, k_sums
   KT_IO,        // import / export / set-atom / cell-id kernels
+  KT_DISP2,     // k_disp2
+  KT_STEP,      // one whole MD step (a graph launch or the step's launches)
   KT_N
 };
 

@@ -1976,6 +1976,7 @@ __global__ void k_md_ctl(int ninst, DCtl *__restrict__ ctl, StepArgs sa,
   DCtl &c = ctl[k];
   c.cfg_time += c.cfg_dt;
   c.cfg_steps += 1.0;
+  c.cfg_dt_min = fmin(c.cfg_dt_min, c.cfg_dt);
   c.e_bath += c.e_bath_step;
   c.e_out += c.e_out_step;
   const double dt = c.vmax > 0.0 ? fmin(sa.dt_max, sa.dx_max / c.vmax) : sa.dt_max;
@@ -1999,6 +2000,7 @@ __global__ void k_md_init(int ninst, DCtl *__restrict__ ctl, StepArgs sa, int re
     c.cfg_time = 0.0;
     c.cfg_steps = 0.0;
     c.cfg_lambda = 1.0;
+    c.cfg_dt_min = 1.0e300;
     c.e_bath = c.e_out = c.e_in = 0.0;
     c.leave_overflow = 0;
   }
@@ -2679,7 +2681,7 @@ void launch_md_pos(const DevBufs &d, const FConst &fc, const StepArgs &sa, int *
   const int dst[3] = {DCTL_IDX(maxdisp), DCTL_IDX(e_bath_step), DCTL_IDX(e_out_step)};
   finalize(d, 3, FIB_BLOCK, op, dst, s);
   {
-    KScope ks_(d, KT_POS);
+    KScope ks_(d, KT_DISP2);
     hipLaunchKernelGGL(k_disp2, dim3(d.NA / FIB_BLOCK), dim3(FIB_BLOCK), 0, s, d.X[d.cur], d.NA,
                        d.disp, d.ctl, d.pstride, d.partials, d.halt);
   }

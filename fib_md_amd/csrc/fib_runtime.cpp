@@ -78,7 +78,7 @@ struct Timer {
 
 static const char *const kTimerNames[KT_N] = {
     "k_bin", "sort", "k_nbuild", "list_aux", "k_legs",
-    "k_tri", "k_force3", "k_extra", "k_pos", "reduce", "io"};
+    "k_tri", "k_force3", "k_extra", "k_pos", "reduce", "io", "k_disp2", "md_step"};
 static_assert(KT_N <= FIB_NTIMERS, "timer table");
 
 struct RefCtl {
@@ -1248,9 +1248,17 @@ extern "C" int32_t fib_md_run(fib_handle *h, int32_t nsteps) {
         HIPCHK(h, hipGraphInstantiate(&h->graph_exec, h->graph, nullptr, nullptr, 0));
         h->graph_valid = true;
       }
-      for (int k = 0; k < left; ++k) HIPCHK(h, hipGraphLaunch(h->graph_exec, h->stream));
+      for (int k = 0; k < left; ++k) {
+        const int t = h->tim.begin(KT_STEP);
+        HIPCHK(h, hipGraphLaunch(h->graph_exec, h->stream));
+        h->tim.end(t, KT_STEP);
+      }
     } else {
-      for (int k = 0; k < left; ++k) enqueue_md_step(h);
+      for (int k = 0; k < left; ++k) {
+        const int t = h->tim.begin(KT_STEP);
+        enqueue_md_step(h);
+        h->tim.end(t, KT_STEP);
+      }
     }
     HIPCHK(h, hipGetLastError());
     HIPCHK(h, hipStreamSynchronize(h->stream));
@@ -1283,6 +1291,13 @@ extern "C" int32_t fib_md_run(fib_handle *h, int32_t nsteps) {
                                        std::to_string(FIB_LEAVE_CAP) + "); the energy ledger "
                                        "misses their interaction energy");
     }
+  return FIB_OK;
+}
+
+extern "C" int32_t fib_md_use_graph(fib_handle *h, int32_t on) {
+  if (!h) return FIB_ERR_ARG;
+  if (!h->md_on) return fail(h, FIB_ERR_STATE, "fib_md_use_graph: call fib_md_begin first");
+  h->md.use_graph = on ? 1 : 0;
   return FIB_OK;
 }
 
@@ -1327,6 +1342,7 @@ extern "C" int32_t fib_md_get_report(fib_handle *h, int32_t inst, fib_md_report 
   r->e_bath = c.e_bath;
   r->e_removed = c.e_out;
   r->e_fired = c.e_in;
+  r->dt_min = c.cfg_dt_min < 1e299 ? c.cfg_dt_min : 0.0;
   return FIB_OK;
 }
 

@@ -252,6 +252,9 @@ class FibGPU:
         self._chk("fib_md_fire_many", self.L.fib_md_fire_many(self.h, len(inst), _p(inst), _p(ion),
                                                               _p(x), _p(v)))
 
+    def md_use_graph(self, on):
+        self._chk("fib_md_use_graph", self.L.fib_md_use_graph(self.h, 1 if on else 0))
+
     def set_list_width(self, wsw, wion):
         self._chk("fib_set_list_width", self.L.fib_set_list_width(self.h, int(wsw), int(wion)))
 

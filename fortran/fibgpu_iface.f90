@@ -70,7 +70,7 @@ module fibgpu
      real(c_double) :: time, dt, ion_z, ke
      integer(c_int64_t) :: steps
      integer(c_int32_t) :: removed, ions_removed, displaced, builds
-     real(c_double) :: e_bath, e_removed, e_fired
+     real(c_double) :: e_bath, e_removed, e_fired, dt_min
   end type fib_md_report
 
   interface

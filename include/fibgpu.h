@@ -231,6 +231,10 @@ int32_t fib_md_begin(fib_handle *h, const fib_md_params *p, const double *surfac
  * refreshed and the instance's dt follows the ion's speed. */
 int32_t fib_md_fire(fib_handle *h, int32_t inst, int32_t ion, const double x[3],
                     const double v[3]);
+/* Replay the step as a captured HIP graph (on) or as ordinary launches
+ * (off; per-kernel timers only see ordinary launches).  The trajectory is
+ * the same either way, bit for bit. */
+int32_t fib_md_use_graph(fib_handle *h, int32_t on);
 /* Fire one ion in each of `count` distinct instances at once: inst[q],
  * ion[q], x[3q..3q+2], v[3q..3q+2].  Only the fired instances' lists are
  * rebuilt; the others keep their state bit for bit. */
@@ -261,6 +265,7 @@ typedef struct fib_md_report {
                              interaction energy with the atoms that stay        */
   double e_fired;         /* brought in by fired ions: kinetic energy plus
                              interaction energy at the start point              */
+  double dt_min;          /* s, smallest timestep taken (0 before the first)    */
 } fib_md_report;
 int32_t fib_md_get_report(fib_handle *h, int32_t inst, fib_md_report *r);
 /* Potential energy (J) of every instance's current state, FP64 sums of the
