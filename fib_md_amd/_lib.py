@@ -13,6 +13,7 @@ HEADER = os.path.join(os.path.dirname(HERE), "include", "fibgpu.h")
 FIB_OK, FIB_ERR_ARG, FIB_ERR_HIP, FIB_ERR_OVERFLOW = 0, 1, 2, 3
 FIB_ERR_TABLE, FIB_ERR_NONFINITE, FIB_ERR_STATE, FIB_ERR_NODEVICE = 4, 5, 6, 7
 FIB_PREC_FP64, FIB_PREC_MIXED = 0, 1
+FIB_NOALLOC_MAX_ATOMS = 16384  # include/fibgpu.h
 FIB_EVAL_FORCE, FIB_EVAL_ENERGY, FIB_EVAL_VIRIAL = 1, 2, 4
 FIB_TI_EQUILIBRATE, FIB_TI_IMPACT, FIB_TI_FREE = 0, 1, 2
 
@@ -118,6 +119,7 @@ SIGNATURES = {
     "fib_md_fire": ([vp, i32, i32, vp, vp], i32),
     "fib_md_fire_many": ([vp, i32, vp, vp, vp, vp], i32),
     "fib_md_use_graph": ([vp, i32], i32),
+    "fib_force_noalloc": ([P(fib_params), i32, i32, i32, i32, vp, i32, vp, vp, vp, vp, vp], i32),
     "fib_set_list_width": ([vp, i32, i32], i32),
     "fib_get_list_width": ([vp, vp, vp], i32),
     "fib_md_run": ([vp, i32], i32),

@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "../../include/fibgpu.h"
 #include "fib_device.h"
 
 // Workgroup size of the per-instance reductions (k_finalize).
@@ -114,6 +115,11 @@ void launch_mark_build(const DevBufs &d, hipStream_t s, bool count = true);
 void launch_copy_rows(int NA, int wold, const int *src, int wnew, int *dst, hipStream_t s);
 void launch_copy_rows(int NA, int wold, const uint16_t *src, int wnew, uint16_t *dst,
                       hipStream_t s);
+// handle-free evaluations: instance record, zero fill (bytes a multiple of
+// 4), energy total of instance 0 into a device double
+void launch_put_inst(const DevBufs &d, const DInst &in, hipStream_t s);
+void launch_fill_zero(void *p, size_t bytes, hipStream_t s);
+void launch_copy_utot(const DevBufs &d, double *out, hipStream_t s);
 // clears the leaving mark of every atom (after a removal-list overflow)
 void launch_clear_leaving(const DevBufs &d, hipStream_t s);
 // lists = true: clears the list-overflow flags and the halt flag; false: the
@@ -156,3 +162,10 @@ void launch_export_scalar(const DevBufs &d, int inst, const DInst &in, const dou
 void launch_set_atom(const DevBufs &d, int p, const double x[3], const double v[3],
                      const StepArgs &sa, hipStream_t s);
 }  // namespace fibk
+
+// Handle-free force evaluation with every array inside `arena` (device
+// memory of arena_bytes); allocates nothing (fib_runtime.cpp).
+int32_t fib_arena_force(const fib_params *p, int32_t precision, int32_t natm, int32_t nsg,
+                        int32_t nlj, const double lb[3], int32_t impact, const double *X,
+                        const int32_t *atype, double *F, double *utot, void *arena,
+                        size_t arena_bytes, int32_t max_atoms, hipStream_t s);

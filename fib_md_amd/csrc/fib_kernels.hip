@@ -2774,6 +2774,32 @@ void launch_clear_overflow(const DevBufs &d, hipStream_t s, bool lists) {
                      d.ninst, d.ctl, d.halt, lists ? 1 : 0);
 }
 
+// Start state of a handle-free evaluation, written by kernels: the instance
+// record (by value), zeroed bookkeeping, the energy total out.
+__global__ void k_put_inst(DInst in, DInst *__restrict__ I) {
+  if (blockIdx.x == 0 && threadIdx.x == 0) I[0] = in;
+}
+__global__ void k_fill_zero(uint32_t *__restrict__ p, size_t nwords) {
+  for (size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x; k < nwords;
+       k += (size_t)gridDim.x * blockDim.x)
+    p[k] = 0u;
+}
+__global__ void k_copy_utot(const DCtl *__restrict__ ctl, double *__restrict__ out) {
+  if (blockIdx.x == 0 && threadIdx.x == 0) out[0] = ctl[0].utot;
+}
+
+void launch_put_inst(const DevBufs &d, const DInst &in, hipStream_t s) {
+  hipLaunchKernelGGL(k_put_inst, dim3(1), dim3(FIB_WAVE), 0, s, in, d.I);
+}
+void launch_fill_zero(void *p, size_t bytes, hipStream_t s) {
+  const size_t nw = bytes / 4;  // every array of the arena is a multiple of 4 bytes
+  hipLaunchKernelGGL(k_fill_zero, dim3(gcap((long)std::min<size_t>(nw, 1u << 30))),
+                     dim3(FIB_BLOCK), 0, s, static_cast<uint32_t *>(p), nw);
+}
+void launch_copy_utot(const DevBufs &d, double *out, hipStream_t s) {
+  hipLaunchKernelGGL(k_copy_utot, dim3(1), dim3(FIB_WAVE), 0, s, d.ctl, out);
+}
+
 void launch_clear_leaving(const DevBufs &d, hipStream_t s) {
   hipLaunchKernelGGL(k_clear_leaving, dim3(gcap(d.NA)), dim3(FIB_BLOCK), 0, s, d.NA, d.X[d.cur]);
 }

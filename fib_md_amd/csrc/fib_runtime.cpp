@@ -335,6 +335,41 @@ static int32_t regrow_lists(fib_handle *h, int Wsw, int Wion) {
   return FIB_OK;
 }
 
+// Derived layout of one instance (cells, buckets, padding) after the ones
+// before it; false when the description is inconsistent.
+static bool layout_instance(double rc, const fib_instance_desc &ds, long &atom_off,
+                            long &bucket_off, long &cell_off, long &natm_before, DInst &in) {
+  if (ds.natm < 1 || ds.nsg < 0 || ds.nlj < 0 || ds.nsg + ds.nlj != ds.natm ||
+      !(ds.lb[0] > 0 && ds.lb[1] > 0 && ds.lb[2] > 0))
+    return false;
+  std::memset(&in, 0, sizeof(in));
+  for (int a = 0; a < 3; ++a) {
+    in.lb[a] = ds.lb[a];
+    in.ilb[a] = 1. / ds.lb[a];
+    const int n = (int)(ds.lb[a] / rc);
+    in.nc[a] = n > 1 ? n : 1;
+    in.h[a] = ds.lb[a] / (double)in.nc[a];
+    in.next[a] = in.nc[a] <= 2 ? in.nc[a] + 1 : in.nc[a];
+  }
+  in.natm = ds.natm;
+  in.nsg = ds.nsg;
+  in.nlj = ds.nlj;
+  in.natm_pad = (ds.natm + FIB_BLOCK - 1) / FIB_BLOCK * FIB_BLOCK;
+  in.atom_off = (int)atom_off;
+  in.pad_before = (int)(atom_off - natm_before);
+  in.bucket_off = (int)bucket_off;
+  in.nbucket = in.next[0] * in.next[1] * in.next[2] + 2;
+  in.cell_real_off = (int)cell_off;
+  in.block_off = (int)(atom_off / FIB_BLOCK);
+  in.nblocks = in.natm_pad / FIB_BLOCK;
+  atom_off += in.natm_pad;
+  natm_before += ds.natm;
+  bucket_off += in.nbucket;
+  cell_off += (long)in.nc[0] * in.nc[1] * in.nc[2];
+  // the list build keeps atom indices in 27 bits beside a stencil index
+  return atom_off < (1L << 27) && bucket_off <= (1L << 30);
+}
+
 extern "C" int32_t fib_create(const fib_params *p, int32_t ninst, const fib_instance_desc *desc,
                               int32_t precision, fib_handle **out) {
   if (!out) return FIB_ERR_ARG;
@@ -360,39 +395,10 @@ extern "C" int32_t fib_create(const fib_params *p, int32_t ninst, const fib_inst
   h->inst.resize(ninst);
   long atom_off = 0, bucket_off = 0, cell_off = 0, natm_before = 0;
   for (int k = 0; k < ninst; ++k) {
-    const fib_instance_desc &ds = desc[k];
-    if (ds.natm < 1 || ds.nsg < 0 || ds.nlj < 0 || ds.nsg + ds.nlj != ds.natm ||
-        !(ds.lb[0] > 0 && ds.lb[1] > 0 && ds.lb[2] > 0)) {
+    if (!layout_instance(rc, desc[k], atom_off, bucket_off, cell_off, natm_before, h->inst[k])) {
       delete h;
       return FIB_ERR_ARG;
     }
-    DInst &in = h->inst[k];
-    std::memset(&in, 0, sizeof(in));
-    for (int a = 0; a < 3; ++a) {
-      in.lb[a] = ds.lb[a];
-      in.ilb[a] = 1. / ds.lb[a];
-      const int n = (int)(ds.lb[a] / rc);
-      in.nc[a] = n > 1 ? n : 1;
-      in.h[a] = ds.lb[a] / (double)in.nc[a];
-      in.next[a] = in.nc[a] <= 2 ? in.nc[a] + 1 : in.nc[a];
-    }
-    in.natm = ds.natm;
-    in.nsg = ds.nsg;
-    in.nlj = ds.nlj;
-    in.natm_pad = (ds.natm + FIB_BLOCK - 1) / FIB_BLOCK * FIB_BLOCK;
-    in.atom_off = (int)atom_off;
-    in.pad_before = (int)(atom_off - natm_before);
-    in.bucket_off = (int)bucket_off;
-    in.nbucket = in.next[0] * in.next[1] * in.next[2] + 2;
-    in.cell_real_off = (int)cell_off;
-    in.block_off = (int)(atom_off / FIB_BLOCK);
-    in.nblocks = in.natm_pad / FIB_BLOCK;
-    atom_off += in.natm_pad;
-    natm_before += ds.natm;
-    bucket_off += in.nbucket;
-    cell_off += (long)in.nc[0] * in.nc[1] * in.nc[2];
-    // the list build keeps atom indices in 27 bits beside a stencil index
-    if (atom_off >= (1L << 27) || bucket_off > (1L << 30)) { delete h; return FIB_ERR_ARG; }
   }
   DevBufs &d = h->d;
   d.NA = (int)atom_off;
@@ -516,8 +522,11 @@ extern "C" int32_t fib_destroy(fib_handle *h) {
   return FIB_OK;
 }
 
+// Message of the calling thread's last failed handle-free call.
+static thread_local std::string g_free_err = "null handle";
+
 extern "C" const char *fib_last_error(const fib_handle *h) {
-  return h ? h->err.c_str() : "null handle";
+  return h ? h->err.c_str() : g_free_err.c_str();
 }
 
 extern "C" int32_t fib_get_cells(const fib_handle *h, int32_t inst, int32_t nc[3]) {
@@ -1391,5 +1400,170 @@ extern "C" int32_t fib_synchronize(fib_handle *h) {
   if (!h) return FIB_ERR_ARG;
   HIPCHK(h, hipStreamSynchronize(h->stream));
   HIPCHK(h, hipGetLastError());
+  return FIB_OK;
+}
+
+
+// ---------------------------------------------------------------------------
+// Handle-free force evaluation inside a caller-provided device arena.  The
+// arrays of one handle (one instance) are carved out of `arena` and the
+// usual pipeline runs: import, binning and sort, list build, force passes,
+// export in the caller's atom order.  No device memory is allocated.  Used
+// by fib_force_noalloc (fib_noalloc.hip), which brings its own arena.
+
+namespace {
+// Host copy of the pair table of the last parameter block, kept for the
+// asynchronous upload (its memory must outlive the call).
+struct TableCache {
+  fib_params P{};
+  bool have = false;
+  std::vector<TabEntry> tab;
+  FConst fc{};
+};
+thread_local TableCache g_tab;
+
+struct Arena {
+  char *base;
+  size_t size, used = 0;
+  template <typename T>
+  T *take(size_t n) {
+    const size_t b = (used + 255) & ~(size_t)255;
+    const size_t e = b + std::max<size_t>(n, 1) * sizeof(T);
+    if (e > size) return nullptr;
+    used = e;
+    return reinterpret_cast<T *>(base + b);
+  }
+};
+}  // namespace
+
+int32_t fib_arena_force(const fib_params *p, int32_t precision, int32_t natm, int32_t nsg,
+                        int32_t nlj, const double lb[3], int32_t impact, const double *X,
+                        const int32_t *atype, double *F, double *utot, void *arena,
+                        size_t arena_bytes, int32_t max_atoms, hipStream_t s) {
+  auto bad = [](int32_t code, const std::string &m) { g_free_err = m; return code; };
+  if (!p || !lb || !X || !atype || !F || !arena)
+    return bad(FIB_ERR_ARG, "fib_force_noalloc: null argument");
+  if (precision != FIB_PREC_FP64 && precision != FIB_PREC_MIXED)
+    return bad(FIB_ERR_ARG, "fib_force_noalloc: bad precision");
+  if (natm > max_atoms)
+    return bad(FIB_ERR_ARG, "fib_force_noalloc: " + std::to_string(natm) + " atoms, at most " +
+                                std::to_string(max_atoms) + " without allocation");
+  if (impact < 0 || impact > nlj) return bad(FIB_ERR_ARG, "fib_force_noalloc: bad impact");
+  // constants and the pair table: derived on the host (no device memory)
+  if (!g_tab.have || std::memcmp(&g_tab.P, p, sizeof(fib_params)) != 0) {
+    fib_handle tmp;
+    tmp.P = *p;
+    derive_constants(&tmp);
+    g_tab.P = *p;
+    g_tab.tab = tmp.tab;
+    g_tab.fc = tmp.fc;
+    g_tab.have = true;
+  }
+  const double smax = std::max(p->sigma_ge, p->sigma_si);
+  const double rc = p->listfac * p->sw_a * smax;
+  fib_instance_desc ds{};
+  ds.natm = natm;
+  ds.nsg = nsg;
+  ds.nlj = nlj;
+  for (int a = 0; a < 3; ++a) ds.lb[a] = lb[a];
+  DInst in;
+  long atom_off = 0, bucket_off = 0, cell_off = 0, natm_before = 0;
+  if (!layout_instance(rc, ds, atom_off, bucket_off, cell_off, natm_before, in))
+    return bad(FIB_ERR_ARG, "fib_force_noalloc: bad atom counts or box");
+  in.impact = impact;
+  DevBufs d{};
+  d.NA = (int)atom_off;
+  d.ninst = 1;
+  d.nbucket = (int)bucket_off;
+  d.total_cells = (int)cell_off;
+  d.Wsw = 32;
+  d.Wion = 64;
+  d.pstride = d.NA / FIB_WAVE + 1;
+  d.max_pad = in.natm_pad;
+  const size_t NA = (size_t)d.NA;
+  const bool mixed = precision == FIB_PREC_MIXED;
+  const size_t g4 = mixed ? sizeof(float4) : sizeof(double4);
+  Arena A{static_cast<char *>(arena), arena_bytes};
+  d.I = A.take<DInst>(1);
+  d.ctl = A.take<DCtl>(1);
+  for (int c = 0; c < 2; ++c) {
+    d.X[c] = A.take<double4>(NA);
+    d.V[c] = A.take<double>(3 * NA);
+    d.F[c] = A.take<double>(3 * NA);
+    d.Xi[c] = A.take<double>(3 * NA);
+  }
+  d.inv = A.take<int>(NA);
+  d.ibase = A.take<int>(1);
+  d.binst = A.take<int>(NA / FIB_BLOCK);
+  d.bkey = A.take<int>(NA);
+  d.brank = A.take<int>(NA);
+  d.bcnt = A.take<int>((size_t)d.nbucket);
+  d.cstart = A.take<int>((size_t)d.nbucket + 1);
+  d.perm = A.take<int>(NA);
+  d.bsum = A.take<int>((size_t)d.nbucket / 2048 + 2);
+  d.cnt_sw = A.take<int>(NA);
+  d.cnt_ion = A.take<int>(NA);
+  d.amask = A.take<uint64_t>(NA);
+  d.cnt_leg = A.take<int>(NA);
+  d.off_sw = A.take<uint8_t>(32 * NA);
+  d.Xb = A.take<double4>(NA);
+  d.disp = A.take<float>(NA);
+  d.exmask = A.take<uint64_t>(NA);
+  d.EXself = A.take<char>(g4 * NA);
+  d.Fpart = A.take<double>(3 * NA);
+  d.upart = A.take<double>(NA);
+  d.Fout = A.take<double>(3 * NA);
+  d.uout = A.take<double>(NA);
+  d.partials = A.take<double>((size_t)d.pstride * FIB_NRED * 3);
+  d.fin_tmp = A.take<double>((d.max_pad / FIB_WAVE / FIB_FIN_T + 1) * FIB_NRED);
+  d.tab = A.take<TabEntry>(g_tab.tab.size());
+  d.nbr_sw = A.take<int>((size_t)d.Wsw * NA);
+  d.nbr_ion = A.take<int>((size_t)d.Wion * NA);
+  d.rk_sw = A.take<uint16_t>((size_t)d.Wsw * NA);
+  d.G = A.take<char>(g4 * d.Wsw * NA);
+  d.RA = A.take<uint32_t>((size_t)d.Wsw * NA);
+  d.EX = A.take<char>(g4 * d.Wsw * NA);
+  if (!d.EX)
+    return bad(FIB_ERR_ARG, "fib_force_noalloc: the configuration needs more than the " +
+                                std::to_string(arena_bytes >> 20) + " MiB arena (" +
+                                std::to_string(d.nbucket) + " cell buckets)");
+  // start state: instance record and zeroed bookkeeping, written by kernels
+  // (no memset or host copy of the arrays)
+  fibk::launch_put_inst(d, in, s);
+  fibk::launch_fill_zero(d.binst, NA / FIB_BLOCK * sizeof(int), s);
+  fibk::launch_fill_zero(d.ctl, sizeof(DCtl), s);
+  fibk::launch_fill_zero(d.cnt_sw, NA * sizeof(int), s);
+  fibk::launch_fill_zero(d.cnt_ion, NA * sizeof(int), s);
+  fibk::launch_fill_zero(d.amask, NA * sizeof(uint64_t), s);
+  fibk::launch_fill_zero(d.cnt_leg, NA * sizeof(int), s);
+  fibk::launch_fill_zero(d.exmask, NA * sizeof(uint64_t), s);
+  fibk::launch_fill_zero(d.off_sw, 32 * NA, s);
+  fibk::launch_fill_zero(d.disp, NA * sizeof(float), s);
+  if (!mixed) {
+    hipError_t e = hipMemcpyAsync(d.tab, g_tab.tab.data(), sizeof(TabEntry) * g_tab.tab.size(),
+                                  hipMemcpyHostToDevice, s);
+    if (e != hipSuccess) return bad(FIB_ERR_HIP, std::string("hipMemcpyAsync: ") + hipGetErrorString(e));
+  }
+  fibk::launch_import(d, 0, in, X, nullptr, atype, nullptr, nullptr, s);
+  fibk::launch_bin(d, s);
+  fibk::launch_scan(d, s);
+  fibk::launch_sort(d, s);
+  d.cur = 1 - d.cur;
+  fibk::launch_nbuild(d, g_tab.fc, s);
+  // the widths cannot grow here: a longer list is an error
+  DCtl c;
+  hipError_t e = hipMemcpyAsync(&c, d.ctl, sizeof(DCtl), hipMemcpyDeviceToHost, s);
+  if (e == hipSuccess) e = hipStreamSynchronize(s);
+  if (e != hipSuccess) return bad(FIB_ERR_HIP, std::string("fib_force_noalloc: ") + hipGetErrorString(e));
+  if (c.nonfinite) return bad(FIB_ERR_NONFINITE, "fib_force_noalloc: non-finite position");
+  if (c.maxcnt_sw > d.Wsw || c.maxcnt_ion > d.Wion)
+    return bad(FIB_ERR_OVERFLOW, "fib_force_noalloc: a neighbour list is longer than its "
+                                 "fixed width (SW 32, ion 64)");
+  fibk::launch_force_a(d, g_tab.fc, mixed, utot != nullptr, false, s);
+  fibk::launch_force_b(d, g_tab.fc, mixed, false, 0, StepArgs{}, d.Fout, s);
+  fibk::launch_export_vec(d, 0, in, d.Fout, F, s);
+  if (utot) fibk::launch_copy_utot(d, utot, s);
+  e = hipGetLastError();
+  if (e != hipSuccess) return bad(FIB_ERR_HIP, std::string("fib_force_noalloc: ") + hipGetErrorString(e));
   return FIB_OK;
 }

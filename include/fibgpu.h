@@ -142,6 +142,27 @@ int32_t fib_get_forces(fib_handle *h, int32_t inst, double *F, double *u,
 int32_t fib_force(fib_handle *h, int32_t inst, int32_t impact, double *F,
                   double *u, double *utot, double virial[9]);
 
+/* Force (and, with utot non-null, the potential energy) of one
+ * configuration without allocating device memory -- for callers that
+ * cannot call hipMalloc (code under HIP graph capture, tight serving
+ * loops).  It has no counterpart in the reference; it is fib_force of a
+ * one-instance handle without the handle: every intermediate array lives in
+ * a fixed device array of FIB_NOALLOC_BYTES that each GPU holds for the life
+ * of the process.  X (natm,3) and atype (natm) are device arrays in the
+ * reference's layout, F (natm,3) a device array, utot a device double or
+ * NULL; stream is a hipStream_t or NULL.  At most FIB_NOALLOC_MAX_ATOMS
+ * atoms (and as many cells as the array holds after them); list widths are
+ * fixed (SW 32, ion 64): a longer list returns FIB_ERR_OVERFLOW.  The call
+ * returns once the lists are checked; the forces follow on the stream.
+ * Calls that overlap on one GPU race for the array.  fib_last_error(NULL)
+ * gives the message of the calling thread's last failure. */
+#define FIB_NOALLOC_BYTES (64u << 20)
+#define FIB_NOALLOC_MAX_ATOMS 16384
+int32_t fib_force_noalloc(const fib_params *p, int32_t precision, int32_t natm,
+                          int32_t nsg, int32_t nlj, const double lb[3],
+                          int32_t impact, const double *X, const int32_t *atype,
+                          double *F, double *utot, void *stream);
+
 /* ---- time integration: the reference's onestep/TI loop ----------------
  * Step control mirrors prms/timeint module state: Ts and the cadences switch
  * from the *_i to the *_r set when the largest kinetic energy drops below
