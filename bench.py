@@ -409,13 +409,18 @@ def _compare(per, steps, n_inst):
     return out
 
 
-def _dose_handle(precision, n_rep=DOSE_REPLICAS, ncell=DOSE_CELLS, impacts=1):
+def _dose_handle(precision, n_rep=DOSE_REPLICAS, ncell=DOSE_CELLS, impacts=1, listfac=None):
     """The cumulative-dose batch: `n_rep` replicas of ncell^3 cells with
     `impacts` parked ions each, 3 A Berendsen side and bottom shells at 300 K
     (tau 0.1 ps), sputtered atoms removed, Ga retained."""
     from fib_md_amd import FibGPU, configs
+    from fib_md_amd.gpu import default_params
     reps, shots = configs.dose_replicas(n_rep, ncell, impacts)
-    g = FibGPU([(len(s["X"]), s["nsg"], s["nlj"], s["L"]) for s in reps], precision=precision)
+    p = default_params()
+    if listfac is not None:
+        p.listfac = float(listfac)
+    g = FibGPU([(len(s["X"]), s["nsg"], s["nlj"], s["L"]) for s in reps], precision=precision,
+               params=p)
     for k, s in enumerate(reps):
         g.set_state(s["X"], V=s["V"], atype=s["atype"], xsides=s["xsides"], Xi=s["X"], inst=k)
     g.md_begin([s["surface_z"] for s in reps], [s["bottom_z"] for s in reps],
@@ -430,13 +435,38 @@ def _fire_all(g, shots, m):
                    [shots[k][m][1] for k in range(n)])
 
 
-def dose_rate(precision, steps, warmup, kernel_steps=20):
+LISTFAC_CANDIDATES = (1.10, 1.15, 1.20, 1.25)
+
+
+def tune_dose_listfac(precision, steps=40, warmup=10):
+    """The project's autotuner on the headline batch itself: the list radius
+    factor (cell size and skin; physics unchanged, every pair within the
+    cutoff is listed either way) that runs the batch's steps fastest, each
+    candidate timed over `steps` steps after its first impact.  Untimed
+    set-up of the headline run."""
+    import torch
+    table = []
+    for lf in LISTFAC_CANDIDATES:
+        g, reps, shots = _dose_handle(precision, listfac=lf)
+        _fire_all(g, shots, 0)
+        g.md_run(warmup)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        g.md_run(steps)
+        torch.cuda.synchronize()
+        table.append((lf, (time.perf_counter() - t0) * 1e3 / steps))
+        g.close()
+    best = min(table, key=lambda r: r[1])[0]
+    return best, {f"{a:.2f}": b for a, b in table}
+
+
+def dose_rate(precision, steps, warmup, kernel_steps=20, listfac=None):
     """The headline: the cumulative-dose batch right after its first impact,
     `warmup` untimed steps, then `steps` timed steps (wall clock around them,
     and HIP events around each step's graph launch on the library's
     stream); per-kernel HIP-event times over `kernel_steps` further steps."""
     import torch
-    g, reps, shots = _dose_handle(precision)
+    g, reps, shots = _dose_handle(precision, listfac=listfac)
     _fire_all(g, shots, 0)
     g.md_run(warmup)
     b0 = [g.md_report(k)["builds"] for k in range(len(reps))]
@@ -453,6 +483,7 @@ def dose_rate(precision, steps, warmup, kernel_steps=20):
     g.close()
     n = sum(len(s["X"]) for s in reps)
     return dict(atoms=n, replicas=len(reps), steps=steps, warmup=warmup,
+                listfac=listfac if listfac is not None else 1.1,
                 atom_steps_per_s=n * steps / t, ms_per_step=t * 1e3 / steps,
                 step_ms_hip_events=ev["avg_ms"], step_events=ev["calls"],
                 builds_per_replica_timed=float(np.mean([r["builds"] - b for r, b in zip(rr, b0)])),
@@ -880,6 +911,8 @@ def main():
     ap.add_argument("--ncell", type=int, default=64)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-md", action="store_true", help="headline only (no extras)")
+    ap.add_argument("--no-tune", action="store_true",
+                    help="headline with the reference's list factor 1.1 (no autotuning)")
     ap.add_argument("--md-steps", type=int, default=100)
     ap.add_argument("--ensemble", type=int, default=64)
     ap.add_argument("--replicas", type=int, default=1024,
@@ -967,10 +1000,20 @@ def main():
         log(json.dumps({k: v for k, v in r.items() if k in ("ms_per_step", "timing")}))
         return
     # ---- headline: the cumulative-dose batch, mixed and FP64 --------------
-    head = {}
+    head, tuned, default_lf = {}, {}, {}
     for prec in ("mixed", "fp64"):
-        log(f"dose batch {prec}: {args.warmup} warmup + {args.steps} timed steps ...")
-        head[prec] = dose_rate(prec, args.steps, args.warmup)
+        lf, table = (None, {}) if args.no_tune else tune_dose_listfac(prec)
+        tuned[prec] = {"listfac": lf if lf is not None else 1.1, "ms_per_step": table}
+        log(f"dose batch {prec}: list factor {tuned[prec]['listfac']:.2f} (autotuner: {table}); "
+            f"{args.warmup} warmup + {args.steps} timed steps ...")
+        head[prec] = dose_rate(prec, args.steps, args.warmup, listfac=lf)
+        if lf is not None and abs(lf - 1.1) > 1e-9:
+            d = dose_rate(prec, args.steps, args.warmup, kernel_steps=0)
+            default_lf[prec] = {"atom_steps_per_s": d["atom_steps_per_s"],
+                                "ms_per_step": d["ms_per_step"]}
+        else:
+            default_lf[prec] = {"atom_steps_per_s": head[prec]["atom_steps_per_s"],
+                                "ms_per_step": head[prec]["ms_per_step"]}
         h = head[prec]
         log(f"  {prec}: {h['ms_per_step']:.3f} ms/step (HIP events {h['step_ms_hip_events']:.3f}), "
             f"{h['atom_steps_per_s']:.4g} atom-steps/s, {h['builds_per_replica_timed']:.1f} "
@@ -1011,11 +1054,15 @@ def main():
         "config": {"workload": f"cumulative dose: {DOSE_REPLICAS} replica Si(100) slabs of "
                                f"{DOSE_CELLS}^3 cells in one batch, first 2 keV normal Ga impact "
                                f"of every replica, 3 A Berendsen shells at 300 K; "
-                               f"{args.warmup} warmup + {args.steps} timed MD steps",
+                               f"{args.warmup} warmup + {args.steps} timed MD steps; list "
+                               f"factor {m['listfac']:.2f} from the autotuner",
                    "atoms": n, "replicas": DOSE_REPLICAS, "parallelism": "single GPU"},
         "fp64": {"value": head["fp64"]["atom_steps_per_s"],
                  "ms_per_step": head["fp64"]["ms_per_step"],
-                 "step_ms_hip_events": head["fp64"]["step_ms_hip_events"]},
+                 "step_ms_hip_events": head["fp64"]["step_ms_hip_events"],
+                 "listfac": head["fp64"]["listfac"]},
+        "autotuner": {"knob": "list radius factor (cell size and skin)", **tuned},
+        "reference_listfac_1.10": default_lf,
         "mixed_over_fp64": m["atom_steps_per_s"] / head["fp64"]["atom_steps_per_s"],
         "step_ms_hip_events": m["step_ms_hip_events"],
         "builds_per_replica_timed": m["builds_per_replica_timed"],

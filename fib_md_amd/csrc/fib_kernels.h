@@ -127,8 +127,10 @@ void launch_clear_leaving(const DevBufs &d, hipStream_t s);
 void launch_clear_overflow(const DevBufs &d, hipStream_t s, bool lists = true);
 // passes 1-2 (k_legs, k_tri): own pair terms and legs, centre-term shares
 // into G, energy and virial reductions
+// extra = false leaves out the stale-list pass when no term can be missing
+// (the MD step, whose rebuild gate keeps every pair in range listed)
 void launch_force_a(const DevBufs &d, const FConst &fc, bool mixed, bool energy, bool virial,
-                    hipStream_t s);
+                    hipStream_t s, bool extra = true);
 // pass 3 (k_force3): gather of G; mode 0: store forces into Fout; mode 1:
 // velocity update with Fout the persistent force array F (timeint.f90:231-236);
 // mode 2: MD-step mode second half kick

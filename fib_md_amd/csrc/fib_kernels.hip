@@ -2576,7 +2576,7 @@ void launch_cellid(const DevBufs &d, int inst, int32_t *dout, hipStream_t s) {
 }
 
 template <bool MIXED, bool EN, bool VIR>
-static void force_ab(const DevBufs &d, const FConst &fc, hipStream_t s) {
+static void force_ab(const DevBufs &d, const FConst &fc, hipStream_t s, bool extra) {
   typedef typename Prec<MIXED>::T4 T4;
   {
     KScope ks_(d, KT_LEGS);
@@ -2592,7 +2592,9 @@ static void force_ab(const DevBufs &d, const FConst &fc, hipStream_t s) {
                        d.off_sw, d.amask, (T4 *)d.G, d.RA, d.pstride,
                        d.partials + (size_t)FIB_NRED * d.pstride, d.halt);
   }
-  {
+  // k_extra is launched whenever its partials are reduced (energy, virial)
+  // or stale-list terms can exist; the MD step's gate rules the latter out
+  if (extra || EN || VIR) {
     KScope ks_(d, KT_EXTRA);
     hipLaunchKernelGGL((k_extra<MIXED, EN, VIR>), dim3(d.NA / FIB_BLOCK), dim3(FIB_BLOCK), 0, s,
                        d.X[d.cur], d.I, fc, d.NA, d.Wsw, d.cnt_sw, d.nbr_sw, d.cnt_leg, d.amask,
@@ -2631,13 +2633,13 @@ static void force_c(const DevBufs &d, const FConst &fc, const StepArgs &sa, doub
 }
 
 void launch_force_a(const DevBufs &d, const FConst &fc, bool mixed, bool energy, bool virial,
-                    hipStream_t s) {
+                    hipStream_t s, bool extra) {
   if (mixed) {
-    if (energy) { if (virial) force_ab<true, true, true>(d, fc, s); else force_ab<true, true, false>(d, fc, s); }
-    else { if (virial) force_ab<true, false, true>(d, fc, s); else force_ab<true, false, false>(d, fc, s); }
+    if (energy) { if (virial) force_ab<true, true, true>(d, fc, s, extra); else force_ab<true, true, false>(d, fc, s, extra); }
+    else { if (virial) force_ab<true, false, true>(d, fc, s, extra); else force_ab<true, false, false>(d, fc, s, extra); }
   } else {
-    if (energy) { if (virial) force_ab<false, true, true>(d, fc, s); else force_ab<false, true, false>(d, fc, s); }
-    else { if (virial) force_ab<false, false, true>(d, fc, s); else force_ab<false, false, false>(d, fc, s); }
+    if (energy) { if (virial) force_ab<false, true, true>(d, fc, s, extra); else force_ab<false, true, false>(d, fc, s, extra); }
+    else { if (virial) force_ab<false, false, true>(d, fc, s, extra); else force_ab<false, false, false>(d, fc, s, extra); }
   }
 }
 

@@ -1021,7 +1021,10 @@ static void enqueue_md_step(fib_handle *h) {
   fibk::launch_mark_build(d, h->stream);
   d.gate = nullptr;
   const bool mixed = h->prec == FIB_PREC_MIXED;
-  fibk::launch_force_a(d, h->fc, mixed, false, false, h->stream);
+  // no stale-list pass: a step either rebuilt an instance's lists or left
+  // its two largest displacements within the skin, so every pair in range
+  // is listed and k_extra would find nothing
+  fibk::launch_force_a(d, h->fc, mixed, false, false, h->stream, false);
   fibk::launch_force_b(d, h->fc, mixed, false, 2, sa, d.F[d.cur], h->stream);
   fibk::launch_md_ctl(d, sa, h->stream);
 }
@@ -1095,7 +1098,7 @@ static int32_t md_recover(fib_handle *h, bool finish_step) {
     if (halted) continue;
     if (finish_step) {
       const bool mixed = h->prec == FIB_PREC_MIXED;
-      fibk::launch_force_a(d, h->fc, mixed, false, false, h->stream);
+      fibk::launch_force_a(d, h->fc, mixed, false, false, h->stream, false);
       fibk::launch_force_b(d, h->fc, mixed, false, 2, h->md_sa, d.F[d.cur], h->stream);
       fibk::launch_md_ctl(d, h->md_sa, h->stream);
       HIPCHK(h, hipStreamSynchronize(h->stream));
