@@ -530,8 +530,12 @@ __global__ void k_copyback(int NA, const double4 *__restrict__ X1, double4 *__re
 // neighbour build: one 64-lane block per real cell
 #define NB_CHUNK 128
 #define NB_ACC 384  // candidates whose acceptance masks are kept for the ranks
+#ifndef NB_CELLS
 #define NB_CELLS 8  // cells per k_nbuild wavefront
+#endif
+#ifndef NB_WAVES
 #define NB_WAVES 4  // independent wavefronts per k_nbuild workgroup
+#endif
 #define OFF_W 32    // bytes per atom of the stencil-offset table (27 used)
 #define NB_JBITS 27  // atom index bits beside the stencil cell in a staged w (fib_create keeps NA below)
 
@@ -1230,10 +1234,18 @@ __global__ __launch_bounds__(FIB_BLOCK) void k_legs(
 // terms, which read every leg of the centre once per leg, read LDS; no leg
 // record goes through HBM.  Runs end on atom boundaries and hold at most
 // TriStage::N legs.
+#ifndef TRI_BATCH
 #define TRI_BATCH 4  // legs per lane whose neighbour loads are in flight together
+#endif
+#ifndef TRI_NS_MIXED
+#define TRI_NS_MIXED 256  // legs per stage run (mixed)
+#endif
+#ifndef TRI_NS_FP64
+#define TRI_NS_FP64 256  // legs per stage run (FP64); 128 measured 10 % slower on the dose batch
+#endif
 template <bool MIXED>
 struct TriStage {
-  static constexpr int N = MIXED ? 256 : 128;  // legs per run: 4 KB of each table per wave
+  static constexpr int N = MIXED ? TRI_NS_MIXED : TRI_NS_FP64;  // legs per run
 };
 template <bool MIXED, bool EN, bool VIR>
 __global__ __launch_bounds__(FIB_BLOCK) void k_tri(
