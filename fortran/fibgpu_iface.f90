@@ -240,6 +240,15 @@ module fibgpu
        real(c_double), intent(in) :: x(3), v(3)
      end function fib_md_fire
 
+     integer(c_int32_t) function fib_md_fire_many(h, count, inst, ion, x, v) &
+          bind(C, name="fib_md_fire_many")
+       import :: c_int32_t, c_double, c_ptr
+       type(c_ptr), value :: h
+       integer(c_int32_t), value :: count
+       integer(c_int32_t), intent(in) :: inst(*), ion(*)
+       real(c_double), intent(in) :: x(3,*), v(3,*)
+     end function fib_md_fire_many
+
      integer(c_int32_t) function fib_md_run(h, nsteps) bind(C, name="fib_md_run")
        import :: c_int32_t, c_ptr
        type(c_ptr), value :: h

@@ -21,9 +21,14 @@
 ! With --frames, replica 1 is written to data/mdrun2.xyz and data/3dmol2.xyz
 ! in the reference's format, at the start and after each impact.
 !
+! All replicas fire their next ion in one call (fib_md_fire_many: one list
+! rebuild of the fired replicas).  --listfac sets the list radius factor
+! (cell size and skin; the autotuner's choice for the dose batch is 1.15).
+!
 ! Usage: mdcfg_gpu [--cells N] [--nz NZ] [--replicas R] [--impacts M]
 !                  [--steps S] [--energy EV] [--polar DEG] [--azimuth DEG]
-!                  [--temp K] [--shell A] [--seed N] [--mixed] [--frames]
+!                  [--temp K] [--shell A] [--seed N] [--listfac F] [--mixed]
+!                  [--frames]
 !------------------------------------------------------------------------------
 program mdcfg_gpu
   use iso_c_binding
@@ -37,7 +42,7 @@ program mdcfg_gpu
 
   integer :: n, nz, nrep, nimp, nsteps, k, nargs, r, m, ios
   integer(c_int32_t) :: prec, rc
-  real(dp) :: energy, polar, azim, tempk, shell, surf, th, ph, spd
+  real(dp) :: energy, polar, azim, tempk, shell, surf, th, ph, spd, listfac
   real(dp) :: x(3), v(3), ymean, dmean, dmax, fmean, temp
   logical :: frames
   character(len=64) :: arg
@@ -51,10 +56,13 @@ program mdcfg_gpu
   real(dp), allocatable, target :: X0(:,:), V0(:,:), Xo(:,:), Vo(:,:)
   integer(c_int32_t), allocatable, target :: at(:), xs(:)
   real(dp), allocatable, target :: surf_z(:), bot_z(:), epot(:), led0(:), led(:)
+  real(dp), allocatable :: xf(:,:), vf(:,:)
+  integer(c_int32_t), allocatable :: finst(:), fion(:)
   integer(c_int64_t) :: lt
 
   n = 4; nz = 4; nrep = 2; nimp = 2; nsteps = 1000
   energy = 2000.0_dp; polar = 0.0_dp; azim = 0.0_dp; tempk = 300.0_dp; shell = 0.0_dp
+  listfac = 0.0_dp
   prec = FIB_PREC_FP64
   frames = .false.
   ranseed = -4242
@@ -70,7 +78,7 @@ program mdcfg_gpu
      case ('--cells', '--nz', '--replicas', '--impacts', '--steps', '--seed')
         k = k + 1
         call int_arg(trim(arg))
-     case ('--energy', '--polar', '--azimuth', '--temp', '--shell')
+     case ('--energy', '--polar', '--azimuth', '--temp', '--shell', '--listfac')
         k = k + 1
         call real_arg(trim(arg))
      case default
@@ -97,6 +105,7 @@ program mdcfg_gpu
   surf_z = surf
   bot_z = 0.0_dp
   call fib_default_params(prm)
+  if (listfac > 0.0_dp) prm%listfac = listfac
   rc = fib_create(prm, int(nrep, c_int32_t), desc, prec, h)
   if (rc /= FIB_OK) then
      write(0, *) 'mdcfg_gpu: fib_create failed, status ', rc
@@ -138,11 +147,15 @@ program mdcfg_gpu
   ph = azim * pi_ref() / 180.0_dp
   spd = ga_speed * sqrt(energy)
   v = [spd * sin(th) * cos(ph), spd * sin(th) * sin(ph), -spd * cos(th)]
+  allocate(xf(3,nrep), vf(3,nrep), finst(nrep), fion(nrep))
   do m = 1, nimp
      do r = 1, nrep
-        x = [ran1() * st%lb(1), ran1() * st%lb(2), surf + 5.0_dp * ang]
-        call check(fib_md_fire(h, int(r - 1, c_int32_t), int(m, c_int32_t), x, v), 'fib_md_fire')
+        xf(:,r) = [ran1() * st%lb(1), ran1() * st%lb(2), surf + 5.0_dp * ang]
+        vf(:,r) = v
+        finst(r) = int(r - 1, c_int32_t)
+        fion(r) = int(m, c_int32_t)
      end do
+     call check(fib_md_fire_many(h, int(nrep, c_int32_t), finst, fion, xf, vf), 'fib_md_fire_many')
      call check(fib_md_run(h, int(nsteps, c_int32_t)), 'fib_md_run')
      call ledger(led)
      ymean = 0; fmean = 0
@@ -212,6 +225,7 @@ contains
     case ('--azimuth'); azim = val
     case ('--temp');    tempk = val
     case ('--shell');   shell = val
+    case ('--listfac'); listfac = val
     end select
   end subroutine real_arg
 
