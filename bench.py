@@ -22,7 +22,9 @@ oracle's MD step on a bounded sample.
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
        python bench.py --profile fp64|mixed|md-fp64|md-mixed|md-replicas|dose-mixed|dose-fp64
                        (few steps + a 256 MiB copy, for rocprofv3)
-       python bench.py --summarize-stats DIR OUT.txt CMD   (rocprofv3 --stats csv -> text)
+       python bench.py --summarize-stats DIR OUT.txt CMD [LAST_STEPS]
+                       (rocprofv3 --stats csv -> text; LAST_STEPS: the kernel trace's
+                       dispatches of the last MD steps only)
        python bench.py --summarize-pmc FETCH_DIR WRITE_DIR OUT.txt CMD [ATOMS]
                        (rocprofv3 --pmc csv -> text + profiles/pmc_traffic.json)
        python bench.py --summarize-step-pmc FETCH_DIR WRITE_DIR OUT.txt CMD
@@ -61,6 +63,14 @@ STEP_BYTES = 240.0
 KERNEL_BYTES_MD = {"k_pos": 124.0, "k_legs": 92.0, "k_force3": 24.0}
 STEP_PMC_FILE = os.path.join(ROOT, "profiles", "md_dose_step_traffic.json")
 DOSE_REPLICAS, DOSE_CELLS = 16, 40
+# untimed steps after the first impact is fired, before the headline's
+# warm-up and timed steps: the slabs start on perfect lattice sites, and it
+# takes ~20 fs (about 250 steps at the ion's small dt) until thermal motion
+# brings second neighbours within the three-body range and the list
+# rebuilds reach their rate; from then on a step costs the same for the
+# rest of the impact (about 20 % more than the first steps)
+DOSE_SETTLE = 250
+PROFILE_STEPS = 30
 
 
 COPY_KIB = 256 * 1024  # the plain copy of --profile mode: 256 MiB read, 256 MiB written
@@ -75,10 +85,34 @@ def short_name(n: str) -> str:
     return n
 
 
-def summarize_stats(d, out, cmd):
-    """Kernel table of a rocprofv3 --kernel-trace --stats --output-format csv run."""
+def _last_steps_stats(d, nsteps):
+    """Per-kernel rows like run_kernel_stats.csv, from the kernel trace's
+    dispatches of the last `nsteps` MD steps (from the nsteps-th last
+    k_md_pos dispatch on)."""
     import csv
-    rows = list(csv.DictReader(open(os.path.join(d, "run_kernel_stats.csv"))))
+    from collections import defaultdict
+    tr = list(csv.DictReader(open(os.path.join(d, "run_kernel_trace.csv"))))
+    pos = sorted(int(r["Dispatch_Id"]) for r in tr if short_name(r["Kernel_Name"]) == "k_md_pos")
+    first = pos[-nsteps]
+    dur = defaultdict(list)
+    for r in tr:
+        if int(r["Dispatch_Id"]) >= first:
+            dur[r["Kernel_Name"]].append(int(r["End_Timestamp"]) - int(r["Start_Timestamp"]))
+    tot = sum(sum(v) for v in dur.values())
+    rows = [{"Name": k, "Calls": len(v), "AverageNs": sum(v) / len(v), "MinNs": min(v),
+             "MaxNs": max(v), "Percentage": 100.0 * sum(v) / tot} for k, v in dur.items()]
+    return sorted(rows, key=lambda r: -r["Percentage"])
+
+
+def summarize_stats(d, out, cmd, last_steps=None):
+    """Kernel table of a rocprofv3 --kernel-trace --stats --output-format csv
+    run; with last_steps, only the dispatches of the last MD steps (from the
+    kernel trace)."""
+    import csv
+    if last_steps:
+        rows = _last_steps_stats(d, int(last_steps))
+    else:
+        rows = list(csv.DictReader(open(os.path.join(d, "run_kernel_stats.csv"))))
     lines = [f"# rocprofv3 kernel statistics", f"# command: {cmd}", "",
              f"{'kernel':44s} {'calls':>6s} {'avg_us':>10s} {'min_us':>10s} {'max_us':>10s} {'pct':>6s}"]
     for r in rows:
@@ -181,9 +215,9 @@ def summarize_counters(out, cmd, *dirs):
 
 def summarize_step_pmc(dfetch, dwrite, out, cmd):
     """HBM bytes per MD step of a dose profile (--profile dose-*): every
-    dispatch from the first step's k_md_pos on belongs to the steps; their
-    FETCH_SIZE + WRITE_SIZE (scaled by the 256 MiB copy's readings) summed
-    and divided by the number of steps (k_md_pos dispatches)."""
+    dispatch of its last PROFILE_STEPS steps (from that step's k_md_pos on);
+    their FETCH_SIZE + WRITE_SIZE (scaled by the 256 MiB copy's readings)
+    summed and divided by the number of steps."""
     import csv
     from collections import defaultdict
 
@@ -205,8 +239,10 @@ def summarize_step_pmc(dfetch, dwrite, out, cmd):
     ff, wf = COPY_KIB / fcopy, COPY_KIB / wcopy
 
     def steps_part(disp, name, fac):
-        first = min(k for k in disp if name[k] == "k_md_pos")
-        nsteps = sum(1 for k in disp if name[k] == "k_md_pos")
+        # the profiled steps: the last PROFILE_STEPS of the run
+        pos = sorted(k for k in disp if name[k] == "k_md_pos")
+        nsteps = min(PROFILE_STEPS, len(pos))
+        first = pos[-nsteps]
         per = defaultdict(float)
         for k, v in disp.items():
             if k >= first:
@@ -229,7 +265,8 @@ def summarize_step_pmc(dfetch, dwrite, out, cmd):
              "FETCH_SIZE and WRITE_SIZE", f"# command: {cmd}",
              f"# calibration: 256 MiB copy reads FETCH_SIZE {fcopy:.0f} KiB (factor {ff:.3f}), "
              f"WRITE_SIZE {wcopy:.0f} KiB (factor {wf:.3f})",
-             f"# {nsteps} steps of {atoms} atoms; every dispatch from the first step's k_md_pos on",
+             f"# the last {nsteps} steps of {atoms} atoms (after {DOSE_SETTLE} steps of settling); "
+             "every dispatch from the first of them's k_md_pos on",
              "", f"{'kernel':36s} {'MB/step':>10s} {'B/atom-step':>12s}"]
     for k in sorted(kern, key=lambda k: -kern[k]["bytes_per_step"]):
         v = kern[k]
@@ -449,7 +486,7 @@ def tune_dose_listfac(precision, steps=40, warmup=10):
     for lf in LISTFAC_CANDIDATES:
         g, reps, shots = _dose_handle(precision, listfac=lf)
         _fire_all(g, shots, 0)
-        g.md_run(warmup)
+        g.md_run(DOSE_SETTLE + warmup)
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         g.md_run(steps)
@@ -461,14 +498,15 @@ def tune_dose_listfac(precision, steps=40, warmup=10):
 
 
 def dose_rate(precision, steps, warmup, kernel_steps=20, listfac=None):
-    """The headline: the cumulative-dose batch right after its first impact,
-    `warmup` untimed steps, then `steps` timed steps (wall clock around them,
-    and HIP events around each step's graph launch on the library's
-    stream); per-kernel HIP-event times over `kernel_steps` further steps."""
+    """The headline: the cumulative-dose batch during its first impact,
+    DOSE_SETTLE + `warmup` untimed steps after the fire, then `steps` timed
+    steps (wall clock around them, and HIP events around each step's graph
+    launch on the library's stream); per-kernel HIP-event times over
+    `kernel_steps` further steps."""
     import torch
     g, reps, shots = _dose_handle(precision, listfac=listfac)
     _fire_all(g, shots, 0)
-    g.md_run(warmup)
+    g.md_run(DOSE_SETTLE + warmup)
     b0 = [g.md_report(k)["builds"] for k in range(len(reps))]
     torch.cuda.synchronize()
     g.timing(enable=True, only=("md_step",))
@@ -737,11 +775,13 @@ CPU_PROCS = 16  # the host cores of one GPU's share of the machine
 MASS = {1: 46.637063e-27, 2: 120.6049350e-27, 3: 1.157777417e-25}
 
 
-def _cpu_md_rate(ncell, steps, seed):
+def _cpu_md_rate(ncell, steps, seed, settle=0):
     """One process: the oracle's MD step (orc_md_step, the configurations'
     integrator around the reference's si_force and si_nlist) on one replica
     of the dose configuration's kind, an ncell^3-cell Si(100) slab with 3 A
-    shells at 300 K, right after its 2 keV normal impact.  (atoms, seconds)."""
+    shells at 300 K, its 2 keV normal impact fired and `settle` untimed steps
+    taken (the lattice thermalized, as in the GPU headline).  (atoms,
+    seconds of the `steps` timed steps)."""
     from fib_md_amd import configs
     from oracle.oracle import Oracle
     s = configs.si_slab(ncell, seed, nions=1)
@@ -759,6 +799,8 @@ def _cpu_md_rate(ncell, steps, seed):
     o.set_xi(s["X"])
     o.md_begin(s["surface_z"], 0.0, shell_width=3e-10, ttar=300.0, tau=1e-13)
     o.md_fire(1, x, v)
+    if settle:
+        o.md_step(settle)
     t0 = time.perf_counter()
     o.md_step(steps)
     return n, time.perf_counter() - t0
@@ -799,24 +841,27 @@ def _parallel_workers(argv_list):
     return out
 
 
-def cpu_baseline(ncell=16, steps=50, procs=CPU_PROCS):
+def cpu_baseline(ncell=12, steps=60, procs=CPU_PROCS, settle=None):
     """The oracle's MD step (a scalar C restatement) on a bounded sample of
     the headline workload: `procs` replicas (seeds 3000 + r) of an
     ncell^3-cell slab, each in its own process on its own core (as the
-    reference's MPI ranks would use the host), `steps` steps after the
-    impact; atom-steps/s summed over the processes."""
+    reference's MPI ranks would use the host), `steps` timed steps after the
+    impact and the same untimed settling steps as the GPU headline;
+    atom-steps/s summed over the processes."""
+    settle = DOSE_SETTLE if settle is None else settle
     from oracle.oracle import build
     build()
     procs = max(1, min(procs, os.cpu_count() or 1))
-    res = _parallel_workers([["--cpu-md-worker", str(ncell), str(steps), str(3000 + r)]
-                             for r in range(procs)])
+    res = _parallel_workers([["--cpu-md-worker", str(ncell), str(steps), str(3000 + r),
+                              str(settle)] for r in range(procs)])
     rates = [w["atoms"] * steps / w["seconds"] for w in res]
     n = res[0]["atoms"]
     return dict(value=float(sum(rates)), unit="atom-steps/s", cores=procs, kind="port",
                 per_core_mean=float(np.mean(rates)),
                 sample=f"{procs} replicas (seeds 3000+r) of a {ncell}^3-cell Si(100) slab "
-                       f"({n} atoms, 3 A shells at 300 K), {steps} MD steps each right after "
-                       f"a 2 keV normal Ga impact, the oracle's orc_md_step around the "
+                       f"({n} atoms, 3 A shells at 300 K), a 2 keV normal Ga impact fired, "
+                       f"{settle} untimed steps, then {steps} timed MD steps, the oracle's "
+                       f"orc_md_step around the "
                        f"reference's si_force/si_nlist; one process per core at once, "
                        f"rates summed ({np.mean([w['seconds'] for w in res]):.1f} s per process)")
 
@@ -928,9 +973,11 @@ def main():
                     help="MD steps per impact of the dose run")
     ap.add_argument("--profile", choices=["fp64", "mixed", "md-fp64", "md-mixed", "md-replicas",
                                           "dose-mixed", "dose-fp64"], default=None)
-    ap.add_argument("--summarize-stats", nargs=3, metavar=("DIR", "OUT", "CMD"))
+    ap.add_argument("--summarize-stats", nargs="+", metavar="DIR OUT CMD [LAST_STEPS]")
+    ap.add_argument("--listfac", type=float, default=None,
+                    help="list radius factor of the dose profile (default: the reference's)")
     ap.add_argument("--cpu-worker", nargs=2, metavar=("NCELL", "REPS"), help=argparse.SUPPRESS)
-    ap.add_argument("--cpu-md-worker", nargs=3, metavar=("NCELL", "STEPS", "SEED"),
+    ap.add_argument("--cpu-md-worker", nargs=4, metavar=("NCELL", "STEPS", "SEED", "SETTLE"),
                     help=argparse.SUPPRESS)
     ap.add_argument("--summarize-pmc", nargs="+", metavar="FETCH_DIR WRITE_DIR OUT CMD [ATOMS]")
     ap.add_argument("--summarize-step-pmc", nargs=4, metavar=("FETCH_DIR", "WRITE_DIR", "OUT", "CMD"))
@@ -944,7 +991,7 @@ def main():
         return
     if args.cpu_md_worker:
         a = args.cpu_md_worker
-        n, t = _cpu_md_rate(int(a[0]), int(a[1]), int(a[2]))
+        n, t = _cpu_md_rate(int(a[0]), int(a[1]), int(a[2]), int(a[3]))
         print(json.dumps({"atoms": n, "seconds": t}), flush=True)
         return
     if args.summarize_stats:
@@ -975,12 +1022,15 @@ def main():
         torch.cuda.synchronize()
         del a, b
     if args.profile in ("dose-mixed", "dose-fp64"):
-        # the headline batch: first impact, then 30 steps (for rocprofv3)
-        g, reps, shots = _dose_handle(args.profile[5:])
+        # the headline batch at the headline's list factor: first impact,
+        # the settling steps, then PROFILE_STEPS steps (for rocprofv3; the
+        # summaries take the last PROFILE_STEPS steps)
+        g, reps, shots = _dose_handle(args.profile[5:], listfac=args.listfac)
         _fire_all(g, shots, 0)
-        g.md_run(30)
+        g.md_run(DOSE_SETTLE)
+        g.md_run(PROFILE_STEPS)
         g.close()
-        log(f"dose profile {args.profile}: 30 steps")
+        log(f"dose profile {args.profile}: {DOSE_SETTLE} + {PROFILE_STEPS} steps")
         return
     if args.profile in ("md-fp64", "md-mixed"):
         # the batched ensemble, a few MD steps (for rocprofv3)
@@ -1054,6 +1104,7 @@ def main():
         "config": {"workload": f"cumulative dose: {DOSE_REPLICAS} replica Si(100) slabs of "
                                f"{DOSE_CELLS}^3 cells in one batch, first 2 keV normal Ga impact "
                                f"of every replica, 3 A Berendsen shells at 300 K; "
+                               f"{DOSE_SETTLE} steps after the fire (thermalized lattice), "
                                f"{args.warmup} warmup + {args.steps} timed MD steps; list "
                                f"factor {m['listfac']:.2f} from the autotuner",
                    "atoms": n, "replicas": DOSE_REPLICAS, "parallelism": "single GPU"},

@@ -123,6 +123,10 @@ SIGNATURES = {
     "fib_set_list_width": ([vp, i32, i32], i32),
     "fib_get_list_width": ([vp, vp, vp], i32),
     "fib_md_run": ([vp, i32], i32),
+    "fib_md_run_begin": ([vp, i32], i32),
+    "fib_md_run_end": ([vp], i32),
+    "fib_snapshot_begin": ([vp, i32, i32], i32),
+    "fib_snapshot_wait": ([vp, i32, vp, vp], i32),
     "fib_md_get_report": ([vp, i32, P(fib_md_report)], i32),
     "fib_md_energy": ([vp, vp], i32),
     "fib_synchronize": ([vp], i32),

@@ -122,6 +122,18 @@ struct fib_handle {
   Timer tim;
   StepArgs base{};
   int n_builds = 0;
+  // fib_md_run_begin / fib_md_run_end: steps enqueued, not yet checked
+  bool run_pending = false;
+  double run_target = 0;
+  // asynchronous snapshots: per slot a device staging buffer (X then V,
+  // (natm,3) each) and its pinned host copy, filled on copy_stream
+  struct Snap {
+    double *dev = nullptr, *host = nullptr;
+    size_t cap = 0, n = 0;  // doubles held; atoms of the pending snapshot
+    hipEvent_t ready = nullptr, copied = nullptr;
+    bool pending = false;
+  } snap[FIB_SNAPSHOT_SLOTS];
+  hipStream_t copy_stream = nullptr;
 };
 
 #define HIPCHK(h, x)                                                              \
@@ -512,6 +524,16 @@ extern "C" int32_t fib_create(const fib_params *p, int32_t ninst, const fib_inst
 extern "C" int32_t fib_destroy(fib_handle *h) {
   if (!h) return FIB_ERR_ARG;
   (void)hipStreamSynchronize(h->stream);
+  if (h->copy_stream) {
+    (void)hipStreamSynchronize(h->copy_stream);
+    (void)hipStreamDestroy(h->copy_stream);
+  }
+  for (auto &sn : h->snap) {
+    if (sn.dev) (void)hipFree(sn.dev);
+    if (sn.host) (void)hipHostFree(sn.host);
+    if (sn.ready) (void)hipEventDestroy(sn.ready);
+    if (sn.copied) (void)hipEventDestroy(sn.copied);
+  }
   for (auto e : h->tim.pool) (void)hipEventDestroy(e);
   if (h->graph_exec) (void)hipGraphExecDestroy(h->graph_exec);
   if (h->graph) (void)hipGraphDestroy(h->graph);
@@ -537,6 +559,14 @@ extern "C" int32_t fib_get_cells(const fib_handle *h, int32_t inst, int32_t nc[3
 
 static bool bad_inst(const fib_handle *h, int32_t inst) {
   return !h || inst < 0 || inst >= (int)h->inst.size();
+}
+
+// Steps enqueued by fib_md_run_begin are not checked yet: calls that read
+// or change the state must wait for fib_md_run_end.
+static bool run_in_flight(fib_handle *h, const char *who) {
+  if (!h || !h->run_pending) return false;
+  h->err = std::string(who) + ": steps in flight; call fib_md_run_end first";
+  return true;
 }
 
 extern "C" int32_t fib_set_state(fib_handle *h, int32_t inst, const double *X, const double *V,
@@ -584,6 +614,7 @@ extern "C" int32_t fib_set_state(fib_handle *h, int32_t inst, const double *X, c
 
 extern "C" int32_t fib_get_state(fib_handle *h, int32_t inst, double *X, double *V, double *F) {
   if (bad_inst(h, inst)) return fail(h, FIB_ERR_ARG, "fib_get_state: bad instance");
+  if (run_in_flight(h, "fib_get_state")) return FIB_ERR_STATE;
   if (!h->have_state) return fail(h, FIB_ERR_STATE, "fib_get_state: no state");
   const DInst &in = h->inst[inst];
   const size_t n = (size_t)in.natm;
@@ -1112,6 +1143,7 @@ static int32_t md_recover(fib_handle *h, bool finish_step) {
 extern "C" int32_t fib_md_begin(fib_handle *h, const fib_md_params *p, const double *surface_z,
                                 const double *bottom_z) {
   if (!h || !p || !surface_z || !bottom_z) return FIB_ERR_ARG;
+  if (run_in_flight(h, "fib_md_begin")) return FIB_ERR_STATE;
   if (!(p->dt_max > 0) || !(p->dx_max > 0) || !(p->remove_height > 0) || p->shell_width < 0 ||
       (p->shell_width > 0 && !(p->tau > 0 && p->ttar > 0)) || p->displaced_r < 0)
     return fail(h, FIB_ERR_ARG, "fib_md_begin: bad parameters");
@@ -1155,6 +1187,7 @@ extern "C" int32_t fib_md_fire_many(fib_handle *h, int32_t count, const int32_t 
                                     const int32_t *ion, const double *x, const double *v) {
   if (!h || count < 0 || (count > 0 && (!inst || !ion || !x || !v))) return FIB_ERR_ARG;
   if (!h->md_on) return fail(h, FIB_ERR_STATE, "fib_md_fire: call fib_md_begin first");
+  if (run_in_flight(h, "fib_md_fire")) return FIB_ERR_STATE;
   if (count == 0) return FIB_OK;
   const int n = (int)h->inst.size();
   std::vector<int> fired(n, 0);
@@ -1231,58 +1264,59 @@ static int32_t md_steps_done(fib_handle *h, double &steps) {
   return FIB_OK;
 }
 
-extern "C" int32_t fib_md_run(fib_handle *h, int32_t nsteps) {
-  if (!h || nsteps < 0) return FIB_ERR_ARG;
-  if (!h->md_on) return fail(h, FIB_ERR_STATE, "fib_md_run: call fib_md_begin first");
-  if (nsteps == 0) return FIB_OK;
-  // every instance steps together: instance 0's count is the batch's
-  double s0 = 0;
-  int32_t r = md_steps_done(h, s0);
-  if (r != FIB_OK) return r;
-  const double target = s0 + nsteps;
-  for (int rounds = 0;; ++rounds) {
-    double now = 0;
-    r = md_steps_done(h, now);
-    if (r != FIB_OK) return r;
-    const int left = (int)(target - now + 0.5);
-    if (left <= 0) break;
-    if (h->md.use_graph) {
-      if (!h->graph_valid) {
-        if (h->graph_exec) { (void)hipGraphExecDestroy(h->graph_exec); h->graph_exec = nullptr; }
-        if (h->graph) { (void)hipGraphDestroy(h->graph); h->graph = nullptr; }
-        const KTimer *kt = h->d.kt;
-        h->d.kt = nullptr;  // no timing events inside the graph
-        HIPCHK(h, hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
-        enqueue_md_step(h);
-        hipError_t e = hipStreamEndCapture(h->stream, &h->graph);
-        h->d.kt = kt;
-        HIPCHK(h, e);
-        HIPCHK(h, hipGraphInstantiate(&h->graph_exec, h->graph, nullptr, nullptr, 0));
-        h->graph_valid = true;
-      }
-      for (int k = 0; k < left; ++k) {
-        const int t = h->tim.begin(KT_STEP);
-        HIPCHK(h, hipGraphLaunch(h->graph_exec, h->stream));
-        h->tim.end(t, KT_STEP);
-      }
-    } else {
-      for (int k = 0; k < left; ++k) {
-        const int t = h->tim.begin(KT_STEP);
-        enqueue_md_step(h);
-        h->tim.end(t, KT_STEP);
-      }
+// Enqueue `left` MD steps on the handle's stream (the captured graph, made
+// when missing, or ordinary launches), with the step timer around each.
+static int32_t md_enqueue_steps(fib_handle *h, int left) {
+  if (h->md.use_graph) {
+    if (!h->graph_valid) {
+      if (h->graph_exec) { (void)hipGraphExecDestroy(h->graph_exec); h->graph_exec = nullptr; }
+      if (h->graph) { (void)hipGraphDestroy(h->graph); h->graph = nullptr; }
+      const KTimer *kt = h->d.kt;
+      h->d.kt = nullptr;  // no timing events inside the graph
+      HIPCHK(h, hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
+      enqueue_md_step(h);
+      hipError_t e = hipStreamEndCapture(h->stream, &h->graph);
+      h->d.kt = kt;
+      HIPCHK(h, e);
+      HIPCHK(h, hipGraphInstantiate(&h->graph_exec, h->graph, nullptr, nullptr, 0));
+      h->graph_valid = true;
     }
-    HIPCHK(h, hipGetLastError());
+    for (int k = 0; k < left; ++k) {
+      const int t = h->tim.begin(KT_STEP);
+      HIPCHK(h, hipGraphLaunch(h->graph_exec, h->stream));
+      h->tim.end(t, KT_STEP);
+    }
+  } else {
+    for (int k = 0; k < left; ++k) {
+      const int t = h->tim.begin(KT_STEP);
+      enqueue_md_step(h);
+      h->tim.end(t, KT_STEP);
+    }
+  }
+  HIPCHK(h, hipGetLastError());
+  return FIB_OK;
+}
+
+// Wait for the enqueued steps; while a gated build of some step overflowed
+// (the steps after it did nothing), grow the lists, finish that step and run
+// the rest; then the per-instance error flags.
+static int32_t md_finish(fib_handle *h) {
+  for (int rounds = 0;; ++rounds) {
     HIPCHK(h, hipStreamSynchronize(h->stream));
     bool halted = false;
-    r = md_halted(h, halted);
+    int32_t r = md_halted(h, halted);
     if (r != FIB_OK) return r;
     if (!halted) break;
-    // a gated build of some step overflowed: the steps after it did nothing;
-    // grow the lists, finish that step and run the rest
     r = md_recover(h, true);
     if (r != FIB_OK) return r;
     if (rounds > 64) return md_abort(h, "fib_md_run: lists keep overflowing");
+    double now = 0;
+    r = md_steps_done(h, now);
+    if (r != FIB_OK) return r;
+    const int left = (int)(h->run_target - now + 0.5);
+    if (left <= 0) break;
+    r = md_enqueue_steps(h, left);
+    if (r != FIB_OK) return r;
   }
   h->have_force = false;
   std::vector<DCtl> c(h->inst.size());
@@ -1306,9 +1340,93 @@ extern "C" int32_t fib_md_run(fib_handle *h, int32_t nsteps) {
   return FIB_OK;
 }
 
+extern "C" int32_t fib_md_run_begin(fib_handle *h, int32_t nsteps) {
+  if (!h || nsteps < 0) return FIB_ERR_ARG;
+  if (!h->md_on) return fail(h, FIB_ERR_STATE, "fib_md_run: call fib_md_begin first");
+  if (run_in_flight(h, "fib_md_run")) return FIB_ERR_STATE;
+  if (nsteps == 0) return FIB_OK;
+  // every instance steps together: instance 0's count is the batch's
+  HIPCHK(h, hipStreamSynchronize(h->stream));
+  double s0 = 0;
+  int32_t r = md_steps_done(h, s0);
+  if (r != FIB_OK) return r;
+  h->run_target = s0 + nsteps;
+  r = md_enqueue_steps(h, nsteps);
+  if (r != FIB_OK) {
+    (void)hipStreamSynchronize(h->stream);
+    return r;
+  }
+  h->run_pending = true;
+  return FIB_OK;
+}
+
+extern "C" int32_t fib_md_run_end(fib_handle *h) {
+  if (!h) return FIB_ERR_ARG;
+  if (!h->run_pending) return FIB_OK;
+  h->run_pending = false;
+  return md_finish(h);
+}
+
+extern "C" int32_t fib_md_run(fib_handle *h, int32_t nsteps) {
+  int32_t r = fib_md_run_begin(h, nsteps);
+  if (r != FIB_OK) return r;
+  return fib_md_run_end(h);
+}
+
+extern "C" int32_t fib_snapshot_begin(fib_handle *h, int32_t inst, int32_t slot) {
+  if (bad_inst(h, inst) || slot < 0 || slot >= FIB_SNAPSHOT_SLOTS)
+    return fail(h, FIB_ERR_ARG, "fib_snapshot_begin: bad instance or slot");
+  if (!h->have_state) return fail(h, FIB_ERR_STATE, "fib_snapshot_begin: no state");
+  if (run_in_flight(h, "fib_snapshot_begin")) return FIB_ERR_STATE;
+  auto &sn = h->snap[slot];
+  if (!h->copy_stream) HIPCHK(h, hipStreamCreateWithFlags(&h->copy_stream, hipStreamNonBlocking));
+  if (!sn.ready) {
+    HIPCHK(h, hipEventCreateWithFlags(&sn.ready, hipEventDisableTiming));
+    HIPCHK(h, hipEventCreateWithFlags(&sn.copied, hipEventDisableTiming));
+  }
+  // the slot's previous copy must have left the staging buffer
+  if (sn.pending) HIPCHK(h, hipEventSynchronize(sn.copied));
+  sn.pending = false;
+  const DInst &in = h->inst[inst];
+  const size_t n = (size_t)in.natm, need = 6 * n;
+  if (need > sn.cap) {
+    if (sn.dev) { (void)hipFree(sn.dev); sn.dev = nullptr; }
+    if (sn.host) { (void)hipHostFree(sn.host); sn.host = nullptr; }
+    sn.cap = 0;
+    HIPCHK(h, hipMalloc((void **)&sn.dev, need * sizeof(double)));
+    HIPCHK(h, hipHostMalloc((void **)&sn.host, need * sizeof(double), hipHostMallocDefault));
+    sn.cap = need;
+  }
+  DevBufs &d = h->d;
+  fibk::launch_export_x(d, inst, in, sn.dev, h->stream);
+  fibk::launch_export_vec(d, inst, in, d.V[d.cur], sn.dev + 3 * n, h->stream);
+  HIPCHK(h, hipGetLastError());
+  HIPCHK(h, hipEventRecord(sn.ready, h->stream));
+  HIPCHK(h, hipStreamWaitEvent(h->copy_stream, sn.ready, 0));
+  HIPCHK(h, hipMemcpyAsync(sn.host, sn.dev, need * sizeof(double), hipMemcpyDeviceToHost,
+                           h->copy_stream));
+  HIPCHK(h, hipEventRecord(sn.copied, h->copy_stream));
+  sn.n = n;
+  sn.pending = true;
+  return FIB_OK;
+}
+
+extern "C" int32_t fib_snapshot_wait(fib_handle *h, int32_t slot, double *X, double *V) {
+  if (!h || slot < 0 || slot >= FIB_SNAPSHOT_SLOTS)
+    return fail(h, FIB_ERR_ARG, "fib_snapshot_wait: bad slot");
+  auto &sn = h->snap[slot];
+  if (!sn.pending) return fail(h, FIB_ERR_STATE, "fib_snapshot_wait: no snapshot in this slot");
+  HIPCHK(h, hipEventSynchronize(sn.copied));
+  sn.pending = false;
+  if (X) std::memcpy(X, sn.host, 3 * sn.n * sizeof(double));
+  if (V) std::memcpy(V, sn.host + 3 * sn.n, 3 * sn.n * sizeof(double));
+  return FIB_OK;
+}
+
 extern "C" int32_t fib_md_use_graph(fib_handle *h, int32_t on) {
   if (!h) return FIB_ERR_ARG;
   if (!h->md_on) return fail(h, FIB_ERR_STATE, "fib_md_use_graph: call fib_md_begin first");
+  if (run_in_flight(h, "fib_md_use_graph")) return FIB_ERR_STATE;
   h->md.use_graph = on ? 1 : 0;
   return FIB_OK;
 }
@@ -1336,6 +1454,7 @@ extern "C" int32_t fib_get_list_width(const fib_handle *h, int32_t *wsw, int32_t
 extern "C" int32_t fib_md_get_report(fib_handle *h, int32_t inst, fib_md_report *r) {
   if (!h || !r) return FIB_ERR_ARG;
   if (!h->md_on) return fail(h, FIB_ERR_STATE, "fib_md_get_report: call fib_md_begin first");
+  if (run_in_flight(h, "fib_md_get_report")) return FIB_ERR_STATE;
   if (bad_inst(h, inst)) return fail(h, FIB_ERR_ARG, "fib_md_get_report: bad instance");
   fibk::launch_md_report(h->d, h->md_sa, h->stream);
   DCtl c;
@@ -1361,6 +1480,7 @@ extern "C" int32_t fib_md_get_report(fib_handle *h, int32_t inst, fib_md_report 
 extern "C" int32_t fib_md_energy(fib_handle *h, double *epot) {
   if (!h || !epot) return FIB_ERR_ARG;
   if (!h->md_on) return fail(h, FIB_ERR_STATE, "fib_md_energy: call fib_md_begin first");
+  if (run_in_flight(h, "fib_md_energy")) return FIB_ERR_STATE;
   // an evaluation with energies into the output buffer: the step's own
   // forces (F of the current state) stay as they are
   int32_t r = run_force(h, FIB_EVAL_FORCE | FIB_EVAL_ENERGY, 0, h->md_sa, h->d.Fout);

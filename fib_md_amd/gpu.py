@@ -266,6 +266,26 @@ class FibGPU:
     def md_run(self, nsteps):
         self._chk("fib_md_run", self.L.fib_md_run(self.h, int(nsteps)))
 
+    def md_run_begin(self, nsteps):
+        """Enqueue nsteps MD steps and return at once (md_run_end waits)."""
+        self._chk("fib_md_run_begin", self.L.fib_md_run_begin(self.h, int(nsteps)))
+
+    def md_run_end(self):
+        self._chk("fib_md_run_end", self.L.fib_md_run_end(self.h))
+
+    def snapshot_begin(self, inst=0, slot=0):
+        """Order an asynchronous copy of instance inst's X and V into `slot`."""
+        self._chk("fib_snapshot_begin", self.L.fib_snapshot_begin(self.h, int(inst), int(slot)))
+        self._snap_n = getattr(self, "_snap_n", {})
+        self._snap_n[slot] = self.natm(inst)
+
+    def snapshot_wait(self, slot=0):
+        """X, V of the snapshot in `slot`, once its copy has arrived."""
+        n = getattr(self, "_snap_n", {}).get(slot, 0)
+        X = np.zeros((n, 3), order="F"); V = np.zeros((n, 3), order="F")
+        self._chk("fib_snapshot_wait", self.L.fib_snapshot_wait(self.h, int(slot), _p(X), _p(V)))
+        return X, V
+
     def md_report(self, inst=0):
         r = L.fib_md_report()
         self._chk("fib_md_get_report", self.L.fib_md_get_report(self.h, int(inst), C.byref(r)))

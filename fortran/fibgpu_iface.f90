@@ -255,6 +255,31 @@ module fibgpu
        integer(c_int32_t), value :: nsteps
      end function fib_md_run
 
+     ! fib_md_run in two halves: the host works while the steps run
+     integer(c_int32_t) function fib_md_run_begin(h, nsteps) bind(C, name="fib_md_run_begin")
+       import :: c_int32_t, c_ptr
+       type(c_ptr), value :: h
+       integer(c_int32_t), value :: nsteps
+     end function fib_md_run_begin
+
+     integer(c_int32_t) function fib_md_run_end(h) bind(C, name="fib_md_run_end")
+       import :: c_int32_t, c_ptr
+       type(c_ptr), value :: h
+     end function fib_md_run_end
+
+     ! asynchronous snapshot of X, V (original order) into a slot (0 or 1)
+     integer(c_int32_t) function fib_snapshot_begin(h, inst, slot) bind(C, name="fib_snapshot_begin")
+       import :: c_int32_t, c_ptr
+       type(c_ptr), value :: h
+       integer(c_int32_t), value :: inst, slot
+     end function fib_snapshot_begin
+
+     integer(c_int32_t) function fib_snapshot_wait(h, slot, X, V) bind(C, name="fib_snapshot_wait")
+       import :: c_int32_t, c_ptr
+       type(c_ptr), value :: h, X, V
+       integer(c_int32_t), value :: slot
+     end function fib_snapshot_wait
+
      integer(c_int32_t) function fib_md_get_report(h, inst, r) bind(C, name="fib_md_get_report")
        import :: c_int32_t, c_ptr, fib_md_report
        type(c_ptr), value :: h

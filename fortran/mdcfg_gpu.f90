@@ -17,18 +17,30 @@
 ! sputter yield, the energy ledger's drift since the start (kinetic +
 ! potential energy + what the shells took + what removed atoms carried out -
 ! what the ions brought in, in eV: mean and largest magnitude), the mean
-! energy fired so far, the simulated time and the timestep of replica 1.
-! With --frames, replica 1 is written to data/mdrun2.xyz and data/3dmol2.xyz
-! in the reference's format, at the start and after each impact.
+! energy fired so far, the simulated time and the timestep of replica 1, and
+! the wall time of the impact (firing and steps).
+! With --frames, the first simulated replica is written to data/mdrun2.xyz
+! and data/3dmol2.xyz in the reference's format, at the start and after each
+! impact.  The frame of an impact is an asynchronous snapshot
+! (fib_snapshot_begin), written while the next impact's steps run
+! (fib_md_run_begin / fib_md_run_end); --sync-frames fetches and writes each
+! frame before going on instead (same files).
 !
 ! All replicas fire their next ion in one call (fib_md_fire_many: one list
 ! rebuild of the fired replicas).  --listfac sets the list radius factor
 ! (cell size and skin; the autotuner's choice for the dose batch is 1.15).
 !
+! --first F --count C simulate only replicas F .. F+C-1 of the R: every
+! replica's random numbers are still drawn in the same order, and a
+! replica's trajectory does not depend on the others in the batch, so runs
+! over disjoint groups give exactly the replicas of one run over all R.
+! A second line per impact lists each simulated replica's cumulative yield
+! and ledger drift.
+!
 ! Usage: mdcfg_gpu [--cells N] [--nz NZ] [--replicas R] [--impacts M]
 !                  [--steps S] [--energy EV] [--polar DEG] [--azimuth DEG]
 !                  [--temp K] [--shell A] [--seed N] [--listfac F] [--mixed]
-!                  [--frames]
+!                  [--first F] [--count C] [--frames] [--sync-frames]
 !------------------------------------------------------------------------------
 program mdcfg_gpu
   use iso_c_binding
@@ -40,11 +52,13 @@ program mdcfg_gpu
   real(dp), parameter :: a_cfg = 5.431E-10_dp, vacuum = 40.0E-10_dp, park = 8.0E-10_dp
   real(dp), parameter :: ga_speed = 1663.6334_dp   ! m/s per sqrt(eV) (knock.f90:29)
 
-  integer :: n, nz, nrep, nimp, nsteps, k, nargs, r, m, ios
+  integer :: n, nz, nrep, nimp, nsteps, k, nargs, r, m, ios, first, cnt, q
   integer(c_int32_t) :: prec, rc
   real(dp) :: energy, polar, azim, tempk, shell, surf, th, ph, spd, listfac
   real(dp) :: x(3), v(3), ymean, dmean, dmax, fmean, temp
-  logical :: frames
+  logical :: frames, sync_frames, pend
+  integer(c_int64_t) :: plt
+  real(dp) :: ptime
   character(len=64) :: arg
   type(siga_t) :: s
   type(state_t) :: st
@@ -58,13 +72,17 @@ program mdcfg_gpu
   real(dp), allocatable, target :: surf_z(:), bot_z(:), epot(:), led0(:), led(:)
   real(dp), allocatable :: xf(:,:), vf(:,:)
   integer(c_int32_t), allocatable :: finst(:), fion(:)
-  integer(c_int64_t) :: lt
+  integer(c_int64_t) :: lt, c0, c1, crate
+  integer, allocatable :: yrep(:)
 
   n = 4; nz = 4; nrep = 2; nimp = 2; nsteps = 1000
+  first = 1; cnt = 0
   energy = 2000.0_dp; polar = 0.0_dp; azim = 0.0_dp; tempk = 300.0_dp; shell = 0.0_dp
   listfac = 0.0_dp
   prec = FIB_PREC_FP64
   frames = .false.
+  sync_frames = .false.
+  pend = .false.
   ranseed = -4242
   nargs = command_argument_count()
   k = 1
@@ -75,7 +93,11 @@ program mdcfg_gpu
         prec = FIB_PREC_MIXED
      case ('--frames')
         frames = .true.
-     case ('--cells', '--nz', '--replicas', '--impacts', '--steps', '--seed')
+     case ('--sync-frames')
+        frames = .true.
+        sync_frames = .true.
+     case ('--cells', '--nz', '--replicas', '--impacts', '--steps', '--seed', '--first', &
+          '--count')
         k = k + 1
         call int_arg(trim(arg))
      case ('--energy', '--polar', '--azimuth', '--temp', '--shell', '--listfac')
@@ -91,11 +113,16 @@ program mdcfg_gpu
      write(0, *) 'mdcfg_gpu: sizes must be positive'
      stop 2
   end if
+  if (cnt == 0) cnt = nrep - first + 1
+  if (first < 1 .or. cnt < 1 .or. first + cnt - 1 > nrep) then
+     write(0, *) 'mdcfg_gpu: --first/--count outside the replicas'
+     stop 2
+  end if
 
   ! ---- one slab (every replica starts from its own velocities) -----------
   call make_slab()
-  allocate(desc(nrep), surf_z(nrep), bot_z(nrep), epot(nrep), led0(nrep), led(nrep))
-  do r = 1, nrep
+  allocate(desc(cnt), surf_z(cnt), bot_z(cnt), epot(cnt), led0(cnt), led(cnt), yrep(cnt))
+  do r = 1, cnt
      desc(r)%natm = st%natm
      desc(r)%nsg = st%nsg
      desc(r)%nlj = st%nlj
@@ -106,7 +133,7 @@ program mdcfg_gpu
   bot_z = 0.0_dp
   call fib_default_params(prm)
   if (listfac > 0.0_dp) prm%listfac = listfac
-  rc = fib_create(prm, int(nrep, c_int32_t), desc, prec, h)
+  rc = fib_create(prm, int(cnt, c_int32_t), desc, prec, h)
   if (rc /= FIB_OK) then
      write(0, *) 'mdcfg_gpu: fib_create failed, status ', rc
      stop 1
@@ -114,9 +141,10 @@ program mdcfg_gpu
   allocate(Xo(st%natm,3), Vo(st%natm,3))
   do r = 1, nrep
      call thermal_velocities()
-     if (r == 1) st%V = V0
-     call check(fib_set_state(h, int(r - 1, c_int32_t), c_loc(X0), c_loc(V0), c_loc(at), c_loc(xs), &
-          c_loc(X0)), 'fib_set_state')
+     if (r < first .or. r >= first + cnt) cycle
+     if (r == first) st%V = V0
+     call check(fib_set_state(h, int(r - first, c_int32_t), c_loc(X0), c_loc(V0), c_loc(at), &
+          c_loc(xs), c_loc(X0)), 'fib_set_state')
   end do
   mp%dt_max = 1.0E-15_dp
   mp%dx_max = 0.05_dp * ang
@@ -129,8 +157,8 @@ program mdcfg_gpu
   mp%reserved = 0
   call check(fib_md_begin(h, mp, surf_z, bot_z), 'fib_md_begin')
   call ledger(led0)
-  print '(A,I3,A,I9,A,I9)', ' replicas', nrep, '  atoms per replica', st%natm, &
-       '  steps per impact', nsteps
+  print '(A,I3,A,I3,A,I3,A,I9,A,I9)', ' replicas', nrep, '  simulated', first, ' to', &
+       first + cnt - 1, '  atoms per replica', st%natm, '  steps per impact', nsteps
   flush(6)
   if (frames) then
      call open_outputs(1, ios)
@@ -147,41 +175,63 @@ program mdcfg_gpu
   ph = azim * pi_ref() / 180.0_dp
   spd = ga_speed * sqrt(energy)
   v = [spd * sin(th) * cos(ph), spd * sin(th) * sin(ph), -spd * cos(th)]
-  allocate(xf(3,nrep), vf(3,nrep), finst(nrep), fion(nrep))
+  allocate(xf(3,cnt), vf(3,cnt), finst(cnt), fion(cnt))
   do m = 1, nimp
      do r = 1, nrep
-        xf(:,r) = [ran1() * st%lb(1), ran1() * st%lb(2), surf + 5.0_dp * ang]
-        vf(:,r) = v
-        finst(r) = int(r - 1, c_int32_t)
-        fion(r) = int(m, c_int32_t)
+        x = [ran1() * st%lb(1), ran1() * st%lb(2), surf + 5.0_dp * ang]
+        if (r < first .or. r >= first + cnt) cycle
+        q = r - first + 1
+        xf(:,q) = x
+        vf(:,q) = v
+        finst(q) = int(q - 1, c_int32_t)
+        fion(q) = int(m, c_int32_t)
      end do
-     call check(fib_md_fire_many(h, int(nrep, c_int32_t), finst, fion, xf, vf), 'fib_md_fire_many')
-     call check(fib_md_run(h, int(nsteps, c_int32_t)), 'fib_md_run')
+     call system_clock(c0, crate)
+     call check(fib_md_fire_many(h, int(cnt, c_int32_t), finst, fion, xf, vf), 'fib_md_fire_many')
+     call check(fib_md_run_begin(h, int(nsteps, c_int32_t)), 'fib_md_run_begin')
+     if (pend) then  ! the previous impact's frame, written while this one runs
+        call check(fib_snapshot_wait(h, 0_c_int32_t, c_loc(Xo), c_loc(Vo)), 'fib_snapshot_wait')
+        call write_frame(plt, ptime)
+        pend = .false.
+     end if
+     call check(fib_md_run_end(h), 'fib_md_run')
+     call system_clock(c1)
      call ledger(led)
      ymean = 0; fmean = 0
-     do r = 1, nrep
+     do r = 1, cnt
         call check(fib_md_get_report(h, int(r - 1, c_int32_t), rep), 'fib_md_get_report')
         ymean = ymean + rep%removed
         fmean = fmean + rep%e_fired
+        yrep(r) = int(rep%removed)
      end do
-     ymean = ymean / nrep
-     fmean = fmean / nrep / ev
-     dmean = sum(led - led0) / nrep / ev
+     ymean = ymean / cnt
+     fmean = fmean / cnt / ev
+     dmean = sum(led - led0) / cnt / ev
      dmax = maxval(abs(led - led0)) / ev
      call check(fib_md_get_report(h, 0_c_int32_t, rep), 'fib_md_get_report')
-     print '(A,I5,A,F9.3,A,ES12.4,A,ES12.4,A,F12.3,A,ES12.4,A,ES12.4)', ' impact', m, &
+     print '(A,I5,A,F9.3,A,ES12.4,A,ES12.4,A,F12.3,A,ES12.4,A,ES12.4,A,F9.3)', ' impact', m, &
           '  yield', ymean, '  drift_eV', dmean, '  drift_max_eV', dmax, '  fired_eV', fmean, &
-          '  time_s', rep%time, '  dt_s', rep%dt
-     flush(6)  ! one line per impact, as it happens
+          '  time_s', rep%time, '  dt_s', rep%dt, '  wall_s', real(c1 - c0, dp) / real(crate, dp)
+     print '(A,I5,*(1X,I0,1X,ES11.3))', ' replicas', m, (yrep(r), (led(r) - led0(r)) / ev, r = 1, cnt)
+     flush(6)  ! two lines per impact, as it happens
      if (frames) then
-        call check(fib_get_state(h, 0_c_int32_t, c_loc(Xo), c_loc(Vo), c_null_ptr), 'fib_get_state')
-        st%X = Xo
-        st%V = Vo
-        temp = host_temperature(st)
         lt = int(m, c_int64_t) * nsteps
-        call writeatoms(s, st, lt, rep%time, temp)
+        if (sync_frames) then
+           call check(fib_get_state(h, 0_c_int32_t, c_loc(Xo), c_loc(Vo), c_null_ptr), &
+                'fib_get_state')
+           call write_frame(lt, rep%time)
+        else
+           call check(fib_snapshot_begin(h, 0_c_int32_t, 0_c_int32_t), 'fib_snapshot_begin')
+           pend = .true.
+           plt = lt
+           ptime = rep%time
+        end if
      end if
   end do
+  if (pend) then
+     call check(fib_snapshot_wait(h, 0_c_int32_t, c_loc(Xo), c_loc(Vo)), 'fib_snapshot_wait')
+     call write_frame(plt, ptime)
+  end if
   if (frames) call close_outputs(1)
   rc = fib_destroy(h)
   print *, 'DONE'
@@ -210,6 +260,8 @@ contains
     case ('--impacts');  nimp = val
     case ('--steps');    nsteps = val
     case ('--seed');     ranseed = -abs(val)
+    case ('--first');    first = val
+    case ('--count');    cnt = val
     end select
   end subroutine int_arg
 
@@ -300,14 +352,23 @@ contains
     end do
   end subroutine thermal_velocities
 
+  ! frame of the first simulated replica from Xo, Vo (reference format)
+  subroutine write_frame(step, t)
+    integer(c_int64_t), intent(in) :: step
+    real(dp), intent(in) :: t
+    st%X = Xo
+    st%V = Vo
+    call writeatoms(s, st, step, t, host_temperature(st))
+  end subroutine write_frame
+
   ! kinetic + potential + e_bath + e_removed - e_fired per replica (J)
   subroutine ledger(out)
     real(dp), intent(out) :: out(:)
-    integer :: q
+    integer :: i
     call check(fib_md_energy(h, epot), 'fib_md_energy')
-    do q = 1, nrep
-       call check(fib_md_get_report(h, int(q - 1, c_int32_t), rep), 'fib_md_get_report')
-       out(q) = rep%ke + epot(q) + rep%e_bath + rep%e_removed - rep%e_fired
+    do i = 1, cnt
+       call check(fib_md_get_report(h, int(i - 1, c_int32_t), rep), 'fib_md_get_report')
+       out(i) = rep%ke + epot(i) + rep%e_bath + rep%e_removed - rep%e_fired
     end do
   end subroutine ledger
 

@@ -5,7 +5,10 @@
 !
 ! Run it in a directory holding siga.in and a data/ folder, like the
 ! reference.  It writes si.dat, fwhm.dat, data/mdrun2.xyz, data/3dmol2.xyz
-! and the reference's progress lines.
+! and the reference's progress lines.  A frame due during an impact is an
+! asynchronous snapshot (fib_snapshot_begin): its copy to the host runs
+! beside the next steps, and it is written when the steps stop next, before
+! anything else is written to the frame files.
 !
 ! Usage: mdrun_gpu [--seed N] [--mixed]
 !   --seed N  ranseed (the reference draws -(seconds*milliseconds) from the
@@ -35,10 +38,13 @@ program mdrun_gpu
   integer(c_int32_t), allocatable, target :: at(:), xs(:)
   real(dp), allocatable, target :: Xi(:,:)
   character(len=64) :: arg
-  logical :: have_seed
+  logical :: have_seed, pend
+  integer(c_int64_t) :: plt
+  real(dp) :: ptime, ptemp
 
   prec = FIB_PREC_FP64
   have_seed = .false.
+  pend = .false.
   nargs = command_argument_count()
   k = 1
   do while (k <= nargs)
@@ -167,6 +173,7 @@ program mdrun_gpu
      ctl%stop_on_output = 1
      do
         call check(fib_step(h, ctl, rep), 'fib_step')
+        call flush_frame()
         lt = rep%lt
         time = rep%time
         temp = rep%temp
@@ -180,11 +187,15 @@ program mdrun_gpu
            end if
         end if
         if (rep%atoms_due /= 0 .and. s%atm_out_i > 0) then
-           call fetch()
-           call writeatoms(s, st, lt, time, temp)
+           call check(fib_snapshot_begin(h, 0_c_int32_t, 0_c_int32_t), 'fib_snapshot_begin')
+           pend = .true.
+           plt = lt
+           ptime = time
+           ptemp = temp
         end if
         if (rep%ended /= 0) exit
      end do
+     call flush_frame()
      call fetch()
      call writeatoms(s, st, lt, time, temp)
   end do
@@ -202,6 +213,16 @@ contains
        stop 1
     end if
   end subroutine check
+
+  ! the frame whose snapshot is pending, once its copy has arrived
+  subroutine flush_frame()
+    if (.not. pend) return
+    call check(fib_snapshot_wait(h, 0_c_int32_t, c_loc(X), c_loc(V)), 'fib_snapshot_wait')
+    st%X = X
+    st%V = V
+    call writeatoms(s, st, plt, ptime, ptemp)
+    pend = .false.
+  end subroutine flush_frame
 
   ! current X, V from the device into the host state (original atom order)
   subroutine fetch()

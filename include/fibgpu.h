@@ -268,6 +268,27 @@ int32_t fib_md_fire_many(fib_handle *h, int32_t count, const int32_t *inst,
  * step returns FIB_ERR_OVERFLOW: the step is complete, but the ledger
  * misses those atoms' interaction energy. */
 int32_t fib_md_run(fib_handle *h, int32_t nsteps);
+/* fib_md_run in two halves, so that the host can work (write a frame) while
+ * the device steps: _begin enqueues the steps and returns at once, _end
+ * waits for them, recovers from a list overflow as fib_md_run does, and
+ * returns its status.  Between the two, only fib_snapshot_wait may be
+ * called on the handle (other calls return FIB_ERR_STATE).  No reference
+ * counterpart (onestep runs on the host, timeint.f90:158-242). */
+int32_t fib_md_run_begin(fib_handle *h, int32_t nsteps);
+int32_t fib_md_run_end(fib_handle *h);
+
+/* Asynchronous snapshots for the frame writers (the reference writes X and
+ * V of every atom from the host, out.f90:63-100).  _begin orders, on the
+ * handle's stream, a copy of instance inst's X and V (original atom order)
+ * into a device staging buffer of `slot`, then a device-to-host copy of it
+ * into pinned memory on a second stream, and returns at once: later steps
+ * run while the copy is in flight.  _wait blocks until the slot's copy has
+ * arrived and copies X, V ((natm,3) each, either may be null) out of it.
+ * A slot holds one snapshot; beginning a slot whose snapshot was not
+ * waited for first waits for its copy.  Not while a run is in flight. */
+#define FIB_SNAPSHOT_SLOTS 2
+int32_t fib_snapshot_begin(fib_handle *h, int32_t inst, int32_t slot);
+int32_t fib_snapshot_wait(fib_handle *h, int32_t slot, double *X, double *V);
 
 typedef struct fib_md_report {
   double time;            /* s, simulated time of the instance                  */

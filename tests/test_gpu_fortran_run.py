@@ -234,3 +234,35 @@ def test_fortran_config_driver(tmp_path, prec):
         labels = [row[36:44].strip() for row in fr]
         assert labels.count("Si") == natm - 2 and labels.count("Ga") == 2
     assert int(open(tmp_path / "data" / "3dmol2.xyz").read().split("\n")[0]) == natm
+
+
+def test_fortran_config_driver_groups_and_async_frames(tmp_path):
+    """mdcfg_gpu over three replicas: the frames written from asynchronous
+    snapshots (each written while the next impact runs) are byte for byte
+    those of synchronous fetches (--sync-frames), and a run over replicas
+    2..3 only (--first 2 --count 2) gives those replicas' yields and ledger
+    drifts exactly as the run over all three."""
+    assert os.path.exists(CFG), "build fortran/mdcfg_gpu first"
+    base = ["--cells", "4", "--nz", "4", "--replicas", "3", "--impacts", "2", "--steps", "300",
+            "--energy", "1000", "--polar", "7", "--shell", "3", "--seed", "12"]
+    out = {}
+    for name, extra in (("async", ["--frames"]), ("sync", ["--sync-frames"]),
+                        ("group", ["--first", "2", "--count", "2"])):
+        d = tmp_path / name
+        os.makedirs(d / "data")
+        print("mdcfg_gpu " + " ".join(base + extra), flush=True)
+        r = subprocess.run([CFG, *base, *extra], cwd=d, capture_output=True, text=True,
+                           timeout=300)
+        assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
+        per = [ln.split()[2:] for ln in r.stdout.splitlines()
+               if ln.split()[:1] == ["replicas"] and ln.split()[2] != "simulated"]
+        out[name] = (d, per)
+    for f in ("mdrun2.xyz", "3dmol2.xyz"):
+        a = open(out["async"][0] / "data" / f, "rb").read()
+        b = open(out["sync"][0] / "data" / f, "rb").read()
+        assert len(a) > 0 and a == b, f
+    full, grp = out["async"][1], out["group"][1]
+    assert len(full) == len(grp) == 2
+    for a, b in zip(full, grp):
+        assert len(a) == 6 and len(b) == 4
+        assert a[2:] == b, (a, b)
