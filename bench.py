@@ -322,7 +322,7 @@ def run_gpu(X, at, L, precision, steps, warmup):
 
 
 STEPS_PER_IMPACT = 2000  # the single-impact configuration's step count
-STEP_KERNELS = ("k_pos", "k_disp2", "k_bin", "sort", "k_nbuild", "list_aux", "k_legs", "k_tri",
+STEP_KERNELS = ("k_pos", "k_bin", "sort", "k_nbuild", "list_aux", "k_legs", "k_tri",
                 "k_extra", "k_force3", "reduce")
 
 
@@ -497,7 +497,19 @@ def tune_dose_listfac(precision, steps=40, warmup=10):
     return best, {f"{a:.2f}": b for a, b in table}
 
 
-def dose_rate(precision, steps, warmup, kernel_steps=20, listfac=None):
+def tune_dose_tiles(precision, listfac, steps=30, warmup=5):
+    """The autotuner's launch tiles on the headline batch at its list factor,
+    after the settling steps (untimed set-up of the headline run)."""
+    from fib_md_amd import autotune
+    g, reps, shots = _dose_handle(precision, listfac=listfac)
+    _fire_all(g, shots, 0)
+    g.md_run(DOSE_SETTLE)
+    best, table = autotune.tune_tiles(g, steps=steps, warmup=warmup)
+    g.close()
+    return best, {"/".join(str(v) for v in t): ms for t, ms in table}
+
+
+def dose_rate(precision, steps, warmup, kernel_steps=20, listfac=None, tiles=None):
     """The headline: the cumulative-dose batch during its first impact,
     DOSE_SETTLE + `warmup` untimed steps after the fire, then `steps` timed
     steps (wall clock around them, and HIP events around each step's graph
@@ -505,6 +517,8 @@ def dose_rate(precision, steps, warmup, kernel_steps=20, listfac=None):
     `kernel_steps` further steps."""
     import torch
     g, reps, shots = _dose_handle(precision, listfac=listfac)
+    if tiles is not None:
+        g.set_tiles(*tiles)
     _fire_all(g, shots, 0)
     g.md_run(DOSE_SETTLE + warmup)
     b0 = [g.md_report(k)["builds"] for k in range(len(reps))]
@@ -522,6 +536,7 @@ def dose_rate(precision, steps, warmup, kernel_steps=20, listfac=None):
     n = sum(len(s["X"]) for s in reps)
     return dict(atoms=n, replicas=len(reps), steps=steps, warmup=warmup,
                 listfac=listfac if listfac is not None else 1.1,
+                tiles=list(tiles) if tiles is not None else [256, 4, 8],
                 atom_steps_per_s=n * steps / t, ms_per_step=t * 1e3 / steps,
                 step_ms_hip_events=ev["avg_ms"], step_events=ev["calls"],
                 builds_per_replica_timed=float(np.mean([r["builds"] - b for r, b in zip(rr, b0)])),
@@ -976,6 +991,8 @@ def main():
     ap.add_argument("--summarize-stats", nargs="+", metavar="DIR OUT CMD [LAST_STEPS]")
     ap.add_argument("--listfac", type=float, default=None,
                     help="list radius factor of the dose profile (default: the reference's)")
+    ap.add_argument("--tiles", type=int, nargs=3, default=None, metavar=("TRI_NS", "TRI_BATCH", "NB_CELLS"),
+                    help="launch tiles of the dose profile (default: the library's)")
     ap.add_argument("--cpu-worker", nargs=2, metavar=("NCELL", "REPS"), help=argparse.SUPPRESS)
     ap.add_argument("--cpu-md-worker", nargs=4, metavar=("NCELL", "STEPS", "SEED", "SETTLE"),
                     help=argparse.SUPPRESS)
@@ -1026,6 +1043,8 @@ def main():
         # the settling steps, then PROFILE_STEPS steps (for rocprofv3; the
         # summaries take the last PROFILE_STEPS steps)
         g, reps, shots = _dose_handle(args.profile[5:], listfac=args.listfac)
+        if args.tiles:
+            g.set_tiles(*args.tiles)
         _fire_all(g, shots, 0)
         g.md_run(DOSE_SETTLE)
         g.md_run(PROFILE_STEPS)
@@ -1053,10 +1072,14 @@ def main():
     head, tuned, default_lf = {}, {}, {}
     for prec in ("mixed", "fp64"):
         lf, table = (None, {}) if args.no_tune else tune_dose_listfac(prec)
-        tuned[prec] = {"listfac": lf if lf is not None else 1.1, "ms_per_step": table}
+        tiles, ttable = (None, {}) if args.no_tune else tune_dose_tiles(prec, lf)
+        tuned[prec] = {"listfac": lf if lf is not None else 1.1, "ms_per_step": table,
+                       "tiles": list(tiles) if tiles is not None else [256, 4, 8],
+                       "tiles_ms_per_step": ttable}
         log(f"dose batch {prec}: list factor {tuned[prec]['listfac']:.2f} (autotuner: {table}); "
+            f"tiles {tuned[prec]['tiles']} ({ttable}); "
             f"{args.warmup} warmup + {args.steps} timed steps ...")
-        head[prec] = dose_rate(prec, args.steps, args.warmup, listfac=lf)
+        head[prec] = dose_rate(prec, args.steps, args.warmup, listfac=lf, tiles=tiles)
         if lf is not None and abs(lf - 1.1) > 1e-9:
             d = dose_rate(prec, args.steps, args.warmup, kernel_steps=0)
             default_lf[prec] = {"atom_steps_per_s": d["atom_steps_per_s"],
@@ -1106,13 +1129,16 @@ def main():
                                f"of every replica, 3 A Berendsen shells at 300 K; "
                                f"{DOSE_SETTLE} steps after the fire (thermalized lattice), "
                                f"{args.warmup} warmup + {args.steps} timed MD steps; list "
-                               f"factor {m['listfac']:.2f} from the autotuner",
+                               f"factor {m['listfac']:.2f} and tiles {m['tiles']} from the "
+                               f"autotuner",
                    "atoms": n, "replicas": DOSE_REPLICAS, "parallelism": "single GPU"},
         "fp64": {"value": head["fp64"]["atom_steps_per_s"],
                  "ms_per_step": head["fp64"]["ms_per_step"],
                  "step_ms_hip_events": head["fp64"]["step_ms_hip_events"],
-                 "listfac": head["fp64"]["listfac"]},
-        "autotuner": {"knob": "list radius factor (cell size and skin)", **tuned},
+                 "listfac": head["fp64"]["listfac"], "tiles": head["fp64"]["tiles"]},
+        "autotuner": {"knobs": "list radius factor (cell size and skin), then launch tiles "
+                               "(vertex-pass stage legs / loads in flight / list-build cells "
+                               "per wavefront)", **tuned},
         "reference_listfac_1.10": default_lf,
         "mixed_over_fp64": m["atom_steps_per_s"] / head["fp64"]["atom_steps_per_s"],
         "step_ms_hip_events": m["step_ms_hip_events"],

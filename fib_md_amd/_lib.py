@@ -124,6 +124,8 @@ SIGNATURES = {
     "fib_get_list_width": ([vp, vp, vp], i32),
     "fib_md_run": ([vp, i32], i32),
     "fib_md_run_begin": ([vp, i32], i32),
+    "fib_set_tiles": ([vp, i32, i32, i32], i32),
+    "fib_get_tiles": ([vp, vp], i32),
     "fib_md_run_end": ([vp], i32),
     "fib_snapshot_begin": ([vp, i32, i32], i32),
     "fib_snapshot_wait": ([vp, i32, vp, vp], i32),

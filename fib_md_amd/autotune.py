@@ -1,4 +1,4 @@
-"""Autotuner of the MD-step mode's list parameters.
+"""Autotuner of the MD-step mode's list parameters and launch tiles.
 
 The list radius factor `listfac` (rc = listfac * a * sigma) sets both the
 cell size of the list build and the skin rc - 0.999 a sigma that decides
@@ -15,6 +15,11 @@ the factor.
 
 The reference-mode path (fib_step) keeps the reference's factor 1.1: its
 list sets are part of the parity contract.
+
+The launch tiles (fib_set_tiles: legs per stage run of the vertex pass and
+neighbour loads in flight per lane, cells per wavefront of the list build)
+change no result, bit for bit, only the time; tune_tiles measures them on a
+live handle, one after the other.
 """
 from __future__ import annotations
 
@@ -23,6 +28,10 @@ import time
 from .gpu import FibGPU, GPUError, default_params
 
 CANDIDATES = (1.10, 1.15, 1.20, 1.25, 1.30)
+# (legs per stage run, loads in flight per lane) of the vertex pass; cells
+# per wavefront of the list build
+TRI_TILES = ((256, 4), (256, 2), (128, 4), (128, 2))
+NB_CELLS = (8, 4, 16)
 
 
 def md_handle(instances, precision, params=None, use_graph=True):
@@ -63,4 +72,30 @@ def tune_listfac(instances, precision="mixed", candidates=CANDIDATES, steps=60, 
     if not ok:
         raise RuntimeError(f"no listfac candidate ran: {table}")
     best = min(ok, key=lambda r: r[1])[0]
+    return best, table
+
+
+def _ms_per_step(g, steps, warmup):
+    g.md_run(warmup)
+    t0 = time.perf_counter()
+    g.md_run(steps)  # returns after the stream has drained
+    return (time.perf_counter() - t0) * 1e3 / steps
+
+
+def tune_tiles(g, steps=30, warmup=5, tri_tiles=TRI_TILES, nb_cells=NB_CELLS):
+    """Launch tiles of a live handle in MD-step mode (ions fired, mid-run):
+    first the vertex pass's (stage size, loads in flight), then the list
+    build's cells per wavefront with the best of those, each timed over
+    `steps` MD steps after `warmup`.  Leaves the handle in the best tiles and
+    returns (best (tri_ns, tri_batch, nb_cells), [(tiles, ms per step)])."""
+    table = []
+    for ns, tb in tri_tiles:
+        g.set_tiles(ns, tb, nb_cells[0])
+        table.append(((ns, tb, nb_cells[0]), _ms_per_step(g, steps, warmup)))
+    ns, tb, _ = min(table, key=lambda r: r[1])[0]
+    for nc in nb_cells[1:]:
+        g.set_tiles(ns, tb, nc)
+        table.append(((ns, tb, nc), _ms_per_step(g, steps, warmup)))
+    best = min(table, key=lambda r: r[1])[0]
+    g.set_tiles(*best)
     return best, table

@@ -41,7 +41,6 @@ enum KKey {
   KT_POS,       // k_pos
   KT_REDUCE,    // k_finalize, k_sums
   KT_IO,        // import / export / set-atom / cell-id kernels
-  KT_DISP2,     // k_disp2
   KT_STEP,      // one whole MD step (a graph launch or the step's launches)
   KT_N
 };
@@ -63,6 +62,10 @@ struct DevBufs {
   int total_cells;           // total real cells
   int Wsw, Wion;             // ELL widths
   int cur;                   // current buffer of the double-buffered arrays
+  // launch tiles (0: the defaults): legs per k_tri stage run and legs per
+  // lane in flight; cells per k_nbuild wavefront.  Results do not depend on
+  // them, only the time.
+  int tri_ns = 0, tri_tb = 0, nb_cells = 0;
   DInst *I;                  // [ninst]
   DCtl *ctl;                 // [ninst]
   double4 *X[2];

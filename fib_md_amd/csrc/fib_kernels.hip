@@ -172,33 +172,73 @@ __device__ __forceinline__ void kadd(float &s, float &c, float v) {
 }
 
 // ---------------------------------------------------------------------------
-// Block reduction of NV doubles (sum or max per slot) into per-block
-// partials, layout partials[k * pstride + block].  k_finalize combines them
-// per instance in a fixed order, so results are deterministic.
+// Reductions of NV doubles per slot.  Slot operations: RED_SUM, RED_MAX, and
+// the exact top two of a set of values over three slots (RED_TOP2 at slot k:
+// the largest; k + 1 RED_TOP2_CNT: how many values equal it; k + 2
+// RED_TOP2_SND: the largest value below it, -1e300 when there is none).
+// Every operation is associative and commutative and the combining order is
+// fixed, so results are deterministic.
+enum { RED_SUM = 0, RED_MAX = 1, RED_TOP2 = 2, RED_TOP2_CNT = 3, RED_TOP2_SND = 4 };
+
+__device__ __forceinline__ double red_identity(int op) {
+  return (op == RED_SUM || op == RED_TOP2_CNT) ? 0.0 : -1.0e300;
+}
+
+__device__ __forceinline__ void top2_merge(double &m, double &c, double &s, double m2, double c2,
+                                           double s2) {
+  const double M = fmax(m, m2);
+  const double cn = (m == M ? c : 0.0) + (m2 == M ? c2 : 0.0);
+  const double sn = fmax(m < M ? m : s, m2 < M ? m2 : s2);
+  m = M;
+  c = cn;
+  s = sn;
+}
+
+// x <- x (op) y, slot by slot (a top-two triple at once).
+template <int NV, typename OP>
+__device__ __forceinline__ void red_merge(double (&x)[NV], const double (&y)[NV], const OP &op) {
+#pragma unroll
+  for (int k = 0; k < NV; ++k) {
+    if (op[k] == RED_TOP2) {
+      if (k + 2 < NV) top2_merge(x[k], x[k + 1], x[k + 2], y[k], y[k + 1], y[k + 2]);
+    } else if (op[k] == RED_SUM) {
+      x[k] += y[k];
+    } else if (op[k] == RED_MAX) {
+      x[k] = fmax(x[k], y[k]);
+    }
+  }
+}
+
+// Block reduction into per-block partials, layout partials[k * pstride +
+// block]; k_finalize combines them per instance.
 template <int NV, int BS>
 __device__ void block_partials(double (&v)[NV], const int (&op)[NV], int lblock, int pstride,
                                double *__restrict__ partials) {
   __shared__ double red[BS / FIB_WAVE][NV];
   const int lane = threadIdx.x % FIB_WAVE, wid = threadIdx.x / FIB_WAVE;
+  for (int o = FIB_WAVE / 2; o > 0; o >>= 1) {
+    double y[NV];
 #pragma unroll
-  for (int k = 0; k < NV; ++k) {
-    double x = v[k];
-    for (int o = FIB_WAVE / 2; o > 0; o >>= 1) {
-      double y = __shfl_xor(x, o);
-      x = op[k] ? fmax(x, y) : x + y;
-    }
-    v[k] = x;
+    for (int k = 0; k < NV; ++k) y[k] = __shfl_xor(v[k], o);
+    red_merge<NV>(v, y, op);
   }
   if (lane == 0) {
 #pragma unroll
     for (int k = 0; k < NV; ++k) red[wid][k] = v[k];
   }
   __syncthreads();
-  if (threadIdx.x < NV) {
-    const int k = threadIdx.x;
-    double x = red[0][k];
-    for (int w = 1; w < BS / FIB_WAVE; ++w) x = op[k] ? fmax(x, red[w][k]) : x + red[w][k];
-    partials[(size_t)k * pstride + lblock] = x;
+  if (threadIdx.x == 0) {
+    double x[NV];
+#pragma unroll
+    for (int k = 0; k < NV; ++k) x[k] = red[0][k];
+    for (int w = 1; w < BS / FIB_WAVE; ++w) {
+      double y[NV];
+#pragma unroll
+      for (int k = 0; k < NV; ++k) y[k] = red[w][k];
+      red_merge<NV>(x, y, op);
+    }
+#pragma unroll
+    for (int k = 0; k < NV; ++k) partials[(size_t)k * pstride + lblock] = x[k];
   }
 }
 
@@ -213,13 +253,15 @@ template <int NV>
 __device__ __forceinline__ void fin_block_tree(double (&x)[NV], const RedSpec &rs,
                                                double (*red)[NV]) {
   const int lane = threadIdx.x % FIB_WAVE, wid = threadIdx.x / FIB_WAVE;
+  for (int o = FIB_WAVE / 2; o > 0; o >>= 1) {
+    double y[NV];
 #pragma unroll
-  for (int k = 0; k < NV; ++k) {
-    for (int o = FIB_WAVE / 2; o > 0; o >>= 1) {
-      const double y = __shfl_xor(x[k], o);
-      x[k] = rs.op[k] ? fmax(x[k], y) : x[k] + y;
-    }
-    if (lane == 0) red[wid][k] = x[k];
+    for (int k = 0; k < NV; ++k) y[k] = __shfl_xor(x[k], o);
+    red_merge<NV>(x, y, rs.op);
+  }
+  if (lane == 0) {
+#pragma unroll
+    for (int k = 0; k < NV; ++k) red[wid][k] = x[k];
   }
   __syncthreads();
 }
@@ -238,29 +280,39 @@ __global__ __launch_bounds__(FIN_T) void k_finalize(const DInst *__restrict__ I,
   const int blk0 = I[inst].atom_off / rs.bs, nblk = I[inst].natm_pad / rs.bs;
   double x[NV];
 #pragma unroll
-  for (int k = 0; k < NV; ++k) x[k] = rs.op[k] ? -1.0e300 : 0.0;
+  for (int k = 0; k < NV; ++k) x[k] = red_identity(rs.op[k]);
   const int b_lo = nch == 1 ? 0 : ch * FIN_T;
   const int b_hi = nch == 1 ? nblk : min(nblk, b_lo + FIN_T);
   for (int b = b_lo + (int)threadIdx.x; b < b_hi; b += FIN_T) {
+    double y[NV];
 #pragma unroll
-    for (int k = 0; k < NV; ++k) {
-      double y = partials[(size_t)k * pstride + blk0 + b];
-      for (int r = 1; r < rs.nreg; ++r) {
-        const double y2 = partials[(size_t)(r * FIB_NRED + k) * pstride + blk0 + b];
-        y = rs.op[k] ? fmax(y, y2) : y + y2;
-      }
-      x[k] = rs.op[k] ? fmax(x[k], y) : x[k] + y;
+    for (int k = 0; k < NV; ++k) y[k] = partials[(size_t)k * pstride + blk0 + b];
+    for (int r = 1; r < rs.nreg; ++r) {
+      double y2[NV];
+#pragma unroll
+      for (int k = 0; k < NV; ++k) y2[k] = partials[(size_t)(r * FIB_NRED + k) * pstride + blk0 + b];
+      red_merge<NV>(y, y2, rs.op);
     }
+    red_merge<NV>(x, y, rs.op);
   }
   fin_block_tree<NV>(x, rs, red);
-  if (threadIdx.x < NV) {
-    const int k = threadIdx.x;
-    double t = red[0][k];
-    for (int w = 1; w < FIN_T / FIB_WAVE; ++w) t = rs.op[k] ? fmax(t, red[w][k]) : t + red[w][k];
-    if (nch == 1) {
-      if (rs.dst[k] >= 0) reinterpret_cast<double *>(&ctl[inst])[rs.dst[k]] = t;
-    } else {
-      tmp[((size_t)inst * nch + ch) * NV + k] = t;
+  if (threadIdx.x == 0) {
+    double t[NV];
+#pragma unroll
+    for (int k = 0; k < NV; ++k) t[k] = red[0][k];
+    for (int w = 1; w < FIN_T / FIB_WAVE; ++w) {
+      double y[NV];
+#pragma unroll
+      for (int k = 0; k < NV; ++k) y[k] = red[w][k];
+      red_merge<NV>(t, y, rs.op);
+    }
+#pragma unroll
+    for (int k = 0; k < NV; ++k) {
+      if (nch == 1) {
+        if (rs.dst[k] >= 0) reinterpret_cast<double *>(&ctl[inst])[rs.dst[k]] = t[k];
+      } else {
+        tmp[((size_t)inst * nch + ch) * NV + k] = t[k];
+      }
     }
   }
 }
@@ -270,13 +322,22 @@ __global__ void k_finalize2(const DInst *__restrict__ I, int nch, RedSpec rs,
                             const double *__restrict__ tmp, const int *__restrict__ halt) {
   GATE_OR_RETURN(gate);
   HALT_OR_RETURN(halt);
-  const int inst = blockIdx.x, k = threadIdx.x;
-  if (k >= NV) return;
+  const int inst = blockIdx.x;
+  if (threadIdx.x != 0) return;
   const int nci = (I[inst].natm_pad / rs.bs + FIN_T - 1) / FIN_T;
-  const double *t0 = tmp + (size_t)inst * nch * NV + k;
-  double t = rs.op[k] ? -1.0e300 : 0.0;
-  for (int c = 0; c < nci; ++c) t = rs.op[k] ? fmax(t, t0[(size_t)c * NV]) : t + t0[(size_t)c * NV];
-  if (rs.dst[k] >= 0) reinterpret_cast<double *>(&ctl[inst])[rs.dst[k]] = t;
+  const double *t0 = tmp + (size_t)inst * nch * NV;
+  double t[NV];
+#pragma unroll
+  for (int k = 0; k < NV; ++k) t[k] = red_identity(rs.op[k]);
+  for (int c = 0; c < nci; ++c) {
+    double y[NV];
+#pragma unroll
+    for (int k = 0; k < NV; ++k) y[k] = t0[(size_t)c * NV + k];
+    red_merge<NV>(t, y, rs.op);
+  }
+#pragma unroll
+  for (int k = 0; k < NV; ++k)
+    if (rs.dst[k] >= 0) reinterpret_cast<double *>(&ctl[inst])[rs.dst[k]] = t[k];
 }
 
 // ---------------------------------------------------------------------------
@@ -761,7 +822,7 @@ __device__ __forceinline__ int locate_cell(const DInst *__restrict__ I, int nins
   return lo;
 }
 
-// Each wavefront builds NB_CELLS consecutive cells; a workgroup holds
+// Each wavefront builds NBC consecutive cells (NB_CELLS by default); a workgroup holds
 // NB_WAVES independent wavefronts (their own LDS, wave-level sync only), so
 // that the workgroup-per-CU limit does not cap the resident waves.  The cells
 // to build are all real cells, or under a gate the cells of the gated
@@ -774,6 +835,7 @@ __device__ __forceinline__ int locate_cell(const DInst *__restrict__ I, int nins
 // asks for seven waves per SIMD; allocation still ends at six (73 VGPRs),
 // but the schedule built for seven runs 3-4 % faster than the one for six
 // (same-box timing of both builds).
+template <int NBC>
 __global__ __launch_bounds__(NB_WAVES * FIB_WAVE) __attribute__((amdgpu_waves_per_eu(7)))
 void k_nbuild(
     const double4 *__restrict__ X, const DInst *__restrict__ I, int ninst, int total_cells,
@@ -790,11 +852,11 @@ void k_nbuild(
   const int w = __builtin_amdgcn_readfirstlane(threadIdx.x / FIB_WAVE);  // wave-uniform
   const int *cplan = gate ? gate + 1 + ninst : nullptr;
   const int ncell = gate ? cplan[ninst] : total_cells;
-  constexpr int CPB = NB_WAVES * NB_CELLS;  // cells per workgroup
+  constexpr int CPB = NB_WAVES * NBC;  // cells per workgroup
   const int nbusy = (ncell + CPB - 1) / CPB;
   if ((int)blockIdx.x >= nbusy) return;
-  const int c0 = xcd_remap(blockIdx.x, nbusy) * CPB + w * NB_CELLS;
-  const int c1 = min(ncell, c0 + NB_CELLS);
+  const int c0 = xcd_remap(blockIdx.x, nbusy) * CPB + w * NBC;
+  const int c1 = min(ncell, c0 + NBC);
   const int lane = threadIdx.x % FIB_WAVE;
   const int sd0 = lane % 3 - 1, sd1 = (lane / 3) % 3 - 1, sd2 = lane / 9 - 1;
   int cend = c0, k = 0, cx = 0, cy = 0, cz = 0;
@@ -1247,7 +1309,7 @@ template <bool MIXED>
 struct TriStage {
   static constexpr int N = MIXED ? TRI_NS_MIXED : TRI_NS_FP64;  // legs per run
 };
-template <bool MIXED, bool EN, bool VIR>
+template <bool MIXED, bool EN, bool VIR, int NS, int TB>
 __global__ __launch_bounds__(FIB_BLOCK) void k_tri(
     const double4 *__restrict__ X, const DInst *__restrict__ I, const FConst fc, int NA,
     int Wsw, const int *__restrict__ cnt_leg, const int *__restrict__ nbr_sw,
@@ -1258,7 +1320,6 @@ __global__ __launch_bounds__(FIB_BLOCK) void k_tri(
   HALT_OR_RETURN(halt);
   typedef typename Prec<MIXED>::T T;
   typedef typename Prec<MIXED>::T4 T4;
-  constexpr int NS = TriStage<MIXED>::N;
   constexpr int NW = FIB_BLOCK / FIB_WAVE;
   // sA: e0, e1, e2 and E, negated when the leg's atom is Ge (E > 0);
   // sB: g, 1/r, neighbour bits, list slot bits
@@ -1314,13 +1375,13 @@ __global__ __launch_bounds__(FIB_BLOCK) void k_tri(
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     // stage: each lane computes its atom's legs (the pairs of its leg mask,
-    // in list order) from the positions, TRI_BATCH neighbours in flight
+    // in list order) from the positions, TB neighbours in flight
     {
       uint64_t m = amc;
-      for (int a0 = 0; __ballot(mine && a0 < nl) != 0ull; a0 += TRI_BATCH) {
-        int kq[TRI_BATCH], jq[TRI_BATCH], rq[TRI_BATCH];
+      for (int a0 = 0; __ballot(mine && a0 < nl) != 0ull; a0 += TB) {
+        int kq[TB], jq[TB], rq[TB];
 #pragma unroll
-        for (int u = 0; u < TRI_BATCH; ++u) {
+        for (int u = 0; u < TB; ++u) {
           const bool ok = mine && a0 + u < nl;
           kq[u] = ok ? __ffsll((unsigned long long)m) - 1 : 0;
           if (ok) m &= m - 1;
@@ -1332,17 +1393,17 @@ __global__ __launch_bounds__(FIB_BLOCK) void k_tri(
         // atom in j's row follows from the stencil offsets (see k_nbuild), n
         // from j's leg mask (legs are symmetric); 255 offsets (r >= Wsw)
         // mark a j without a row.
-        double4 xq[TRI_BATCH];
-        uint64_t aq[TRI_BATCH];
-        int oq[TRI_BATCH];
+        double4 xq[TB];
+        uint64_t aq[TB];
+        int oq[TB];
 #pragma unroll
-        for (int u = 0; u < TRI_BATCH; ++u) {
+        for (int u = 0; u < TB; ++u) {
           xq[u] = X[jq[u]];
           aq[u] = amask[jq[u]];
           oq[u] = off_sw[(size_t)jq[u] * OFF_W + 26 - (rq[u] >> 8)];
         }
 #pragma unroll
-        for (int u = 0; u < TRI_BATCH; ++u) {
+        for (int u = 0; u < TB; ++u) {
           if (mine && a0 + u < nl) {
             Leg<T> L;
             make_leg<MIXED>(xc, xq[u], in, fc, tci, type_of(meta_of(xq[u])), (uint32_t)jq[u],
@@ -1641,8 +1702,8 @@ __global__ void k_mark_build(int NA, int ninst, int Wsw, int Wion, const double4
 // dt = min(dt_max, dx_max / |v|max), all step control on the device, so one
 // step is a fixed sequence of launches that can be captured in a graph:
 //   k_md_pos     Berendsen shells, half kick, drift, pinned atoms, removal
-//                above the surface, displacement since the list build;
-//                k_disp2      the second largest displacement
+//                above the surface, displacement since the list build and
+//                the exact two largest per instance;
 //                k_md_decide sets the rebuild gate (two largest displacements
 //                add up to more than the skin)
 //   gated build  bin / sort in place / k_nbuild (skipped on the
@@ -1674,8 +1735,10 @@ __global__ __launch_bounds__(FIB_BLOCK) void k_md_pos(double4 *__restrict__ X,
   const DInst &in = I[kin];
   const bool live = !(mi & (META_PAD | META_DEAD));
   const double dt = ctl[kin].cfg_dt, lam = ctl[kin].cfg_lambda;
-  // max displacement, energy taken by the shells, energy of removed atoms
-  double red[3] = {0.0, 0.0, 0.0};
+  // the two largest displacements since the build (largest, how many share
+  // it, largest below it), energy taken by the shells, energy of removed
+  // atoms; an atom out of the simulation counts as displaced by 0
+  double red[5] = {0.0, 1.0, -1.0e300, 0.0, 0.0};
   if (live) {
     const int t = type_of(mi);
     const double m = sa.mass[t - 1];
@@ -1683,7 +1746,7 @@ __global__ __launch_bounds__(FIB_BLOCK) void k_md_pos(double4 *__restrict__ X,
     for (int c = 0; c < 3; ++c) v[c] = V[(size_t)c * NA + i];
     if (sa.shell_w > 0.0 && t < 3 && !(mi & META_FROZEN) &&
         in_shell(xx[0], xx[1], xx[2], in, sa.shell_w)) {
-      red[1] = 0.5 * m * (v[0] * v[0] + v[1] * v[1] + v[2] * v[2]) * (1.0 - lam * lam);
+      red[3] = 0.5 * m * (v[0] * v[0] + v[1] * v[1] + v[2] * v[2]) * (1.0 - lam * lam);
       for (int c = 0; c < 3; ++c) v[c] *= lam;
     }
     for (int c = 0; c < 3; ++c) {
@@ -1703,7 +1766,7 @@ __global__ __launch_bounds__(FIB_BLOCK) void k_md_pos(double4 *__restrict__ X,
       // that stay (an atom of a sputtered cluster is still bound) is booked
       // by k_md_decide from the removal list
       mi |= META_DEAD | META_LEAVING;
-      red[2] = 0.5 * m * (v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+      red[4] = 0.5 * m * (v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
       for (int c = 0; c < 3; ++c) v[c] = 0.0;
       x.w = __longlong_as_double((long long)((wbits(x) & ~0xffffffffull) | mi));
       const int slot = atomicAdd(&leave[0], 1);
@@ -1715,34 +1778,13 @@ __global__ __launch_bounds__(FIB_BLOCK) void k_md_pos(double4 *__restrict__ X,
     double e0, e1, e2;
     min_image_fast(x, Xb[i], in, e0, e1, e2);
     const float dd = (float)sqrt(e0 * e0 + e1 * e1 + e2 * e2);
-    // the reduction takes the stored value, so that k_disp2 can find the
-    // atoms at the maximum by comparison
     disp[i] = dd;
     red[0] = (double)dd;
   } else {
     disp[i] = 0.0f;
   }
-  const int op[3] = {1, 0, 0};
-  block_partials<3, FIB_BLOCK>(red, op, lb, pstride, partials);
-}
-
-// Second largest displacement since the build per instance (MD-step mode):
-// the largest below the maximum, and how many atoms sit at the maximum.
-__global__ __launch_bounds__(FIB_BLOCK) void k_disp2(const double4 *__restrict__ X, int NA,
-                                                     const float *__restrict__ disp,
-                                                     const DCtl *__restrict__ ctl, int pstride,
-                                                     double *__restrict__ partials,
-                                                     const int *__restrict__ halt) {
-  HALT_OR_RETURN(halt);
-  const int lb = xcd_remap(blockIdx.x, gridDim.x);
-  const int i = lb * FIB_BLOCK + threadIdx.x;
-  // every block lies inside one instance (its first atom names it)
-  const int kin = __builtin_amdgcn_readfirstlane(inst_of(meta_of(X[lb * FIB_BLOCK])));
-  const double m1 = ctl[kin].maxdisp;
-  const double d = (double)disp[i];
-  double v[2] = {d < m1 ? d : 0.0, d == m1 ? 1.0 : 0.0};
-  const int op[2] = {1, 0};
-  block_partials<2, FIB_BLOCK>(v, op, lb, pstride, partials);
+  const int op[5] = {RED_TOP2, RED_TOP2_CNT, RED_TOP2_SND, RED_SUM, RED_SUM};
+  block_partials<5, FIB_BLOCK>(red, op, lb, pstride, partials);
 }
 
 // Rebuild gates of the step: each instance whose two largest displacements
@@ -1937,7 +1979,7 @@ __global__ __launch_bounds__(DECIDE_T) void k_md_decide(
     int g = 0, w = 0;
     if (k < ninst) {
       const DCtl &c = ctl[k];
-      const double m2 = c.nmaxdisp >= 2.0 ? c.maxdisp : c.maxdisp2;
+      const double m2 = c.nmaxdisp >= 2.0 ? c.maxdisp : fmax(c.maxdisp2, 0.0);
       g = (c.maxdisp + m2 > skin) ? 1 : 0;
       ctl[k].maxdisp2 = m2;  // the effective second maximum, for k_extra
       gate[1 + k] = g;
@@ -2564,10 +2606,17 @@ void launch_nbuild(const DevBufs &d, const FConst &fc, hipStream_t s) {
   }
   if (d.total_cells > 0) {
     KScope ks_(d, KT_NBUILD);
-    hipLaunchKernelGGL(k_nbuild, dim3(nblk(d.total_cells, NB_CELLS * NB_WAVES)),
-                       dim3(NB_WAVES * FIB_WAVE), 0, s, d.X[d.cur], d.I, d.ninst,
-                       d.total_cells, d.cstart, d.bcnt, fc.rv2, fc.rv2lj, d.Wsw, d.Wion,
-                       d.cnt_sw, d.nbr_sw, d.cnt_ion, d.nbr_ion, d.rk_sw, d.off_sw, d.gate);
+    const int nbc = d.nb_cells ? d.nb_cells : NB_CELLS;
+#define NB_LAUNCH(C)                                                                          \
+  hipLaunchKernelGGL(k_nbuild<C>, dim3(nblk(d.total_cells, C * NB_WAVES)),                    \
+                     dim3(NB_WAVES * FIB_WAVE), 0, s, d.X[d.cur], d.I, d.ninst, d.total_cells, \
+                     d.cstart, d.bcnt, fc.rv2, fc.rv2lj, d.Wsw, d.Wion, d.cnt_sw, d.nbr_sw,   \
+                     d.cnt_ion, d.nbr_ion, d.rk_sw, d.off_sw, d.gate)
+    if (nbc == NB_CELLS) NB_LAUNCH(NB_CELLS);
+    else if (nbc == 4) NB_LAUNCH(4);
+    else if (nbc == 16) NB_LAUNCH(16);
+    else NB_LAUNCH(8);
+#undef NB_LAUNCH
   }
   {
     KScope ks_(d, KT_LISTAUX);
@@ -2599,10 +2648,25 @@ static void force_ab(const DevBufs &d, const FConst &fc, hipStream_t s, bool ext
   }
   {
     KScope ks_(d, KT_TRI);
-    hipLaunchKernelGGL((k_tri<MIXED, EN, VIR>), dim3(d.NA / FIB_BLOCK), dim3(FIB_BLOCK), 0, s,
-                       d.X[d.cur], d.I, fc, d.NA, d.Wsw, d.cnt_leg, d.nbr_sw, d.rk_sw,
-                       d.off_sw, d.amask, (T4 *)d.G, d.RA, d.pstride,
-                       d.partials + (size_t)FIB_NRED * d.pstride, d.halt);
+    // the MD step's variant (no energy, no virial) in every tile; the others
+    // in the default tile
+    const int ns = (EN || VIR || !d.tri_ns) ? TriStage<MIXED>::N : d.tri_ns;
+    const int tb = (EN || VIR || !d.tri_tb) ? TRI_BATCH : d.tri_tb;
+#define TRI_LAUNCH(NS_, TB_)                                                                  \
+  hipLaunchKernelGGL((k_tri<MIXED, EN, VIR, NS_, TB_>), dim3(d.NA / FIB_BLOCK), dim3(FIB_BLOCK), \
+                     0, s, d.X[d.cur], d.I, fc, d.NA, d.Wsw, d.cnt_leg, d.nbr_sw, d.rk_sw,   \
+                     d.off_sw, d.amask, (T4 *)d.G, d.RA, d.pstride,                          \
+                     d.partials + (size_t)FIB_NRED * d.pstride, d.halt)
+    if constexpr (EN || VIR) {
+      TRI_LAUNCH(TriStage<MIXED>::N, TRI_BATCH);
+    } else {
+      if (ns == TriStage<MIXED>::N && tb == TRI_BATCH) TRI_LAUNCH(TriStage<MIXED>::N, TRI_BATCH);
+      else if (ns == 128 && tb == 2) TRI_LAUNCH(128, 2);
+      else if (ns == 128 && tb == 4) TRI_LAUNCH(128, 4);
+      else if (ns == 256 && tb == 2) TRI_LAUNCH(256, 2);
+      else TRI_LAUNCH(256, 4);
+    }
+#undef TRI_LAUNCH
   }
   // k_extra is launched whenever its partials are reduced (energy, virial)
   // or stale-list terms can exist; the MD step's gate rules the latter out
@@ -2691,17 +2755,10 @@ void launch_md_pos(const DevBufs &d, const FConst &fc, const StepArgs &sa, int *
                        d.NA, d.V[d.cur], d.F[d.cur], d.Xi[d.cur], sa, d.pstride, d.partials,
                        d.ctl, d.Xb, d.disp, leave, d.halt);
   }
-  const int op[3] = {1, 0, 0};
-  const int dst[3] = {DCTL_IDX(maxdisp), DCTL_IDX(e_bath_step), DCTL_IDX(e_out_step)};
-  finalize(d, 3, FIB_BLOCK, op, dst, s);
-  {
-    KScope ks_(d, KT_DISP2);
-    hipLaunchKernelGGL(k_disp2, dim3(d.NA / FIB_BLOCK), dim3(FIB_BLOCK), 0, s, d.X[d.cur], d.NA,
-                       d.disp, d.ctl, d.pstride, d.partials, d.halt);
-  }
-  const int op2[2] = {1, 0};
-  const int dst2[2] = {DCTL_IDX(maxdisp2), DCTL_IDX(nmaxdisp)};
-  finalize(d, 2, FIB_BLOCK, op2, dst2, s);
+  const int op[5] = {RED_TOP2, RED_TOP2_CNT, RED_TOP2_SND, RED_SUM, RED_SUM};
+  const int dst[5] = {DCTL_IDX(maxdisp), DCTL_IDX(nmaxdisp), DCTL_IDX(maxdisp2),
+                      DCTL_IDX(e_bath_step), DCTL_IDX(e_out_step)};
+  finalize(d, 5, FIB_BLOCK, op, dst, s);
   hipLaunchKernelGGL(k_md_decide, dim3(1), dim3(DECIDE_T), 0, s, d.ninst, d.I, d.ctl, sa.skin,
                      gate, leave, d.X[d.cur], fc, d.Wsw, d.Wion, d.cnt_sw, d.nbr_sw, d.cnt_ion,
                      d.nbr_ion, d.halt);

@@ -78,7 +78,7 @@ struct Timer {
 
 static const char *const kTimerNames[KT_N] = {
     "k_bin", "sort", "k_nbuild", "list_aux", "k_legs",
-    "k_tri", "k_force3", "k_extra", "k_pos", "reduce", "io", "k_disp2", "md_step"};
+    "k_tri", "k_force3", "k_extra", "k_pos", "reduce", "io", "md_step"};
 static_assert(KT_N <= FIB_NTIMERS, "timer table");
 
 struct RefCtl {
@@ -1441,6 +1441,30 @@ extern "C" int32_t fib_set_list_width(fib_handle *h, int32_t wsw, int32_t wion) 
   h->have_lists = false;
   h->have_force = false;
   h->graph_valid = false;
+  return FIB_OK;
+}
+
+extern "C" int32_t fib_set_tiles(fib_handle *h, int32_t tri_ns, int32_t tri_batch,
+                                 int32_t nb_cells) {
+  if (!h) return FIB_ERR_ARG;
+  if (run_in_flight(h, "fib_set_tiles")) return FIB_ERR_STATE;
+  if ((tri_ns != 0 && tri_ns != 128 && tri_ns != 256) ||
+      (tri_batch != 0 && tri_batch != 2 && tri_batch != 4) ||
+      (nb_cells != 0 && nb_cells != 4 && nb_cells != 8 && nb_cells != 16))
+    return fail(h, FIB_ERR_ARG, "fib_set_tiles: tri_ns 128|256, tri_batch 2|4, nb_cells 4|8|16 "
+                                "(0: default)");
+  h->d.tri_ns = tri_ns;
+  h->d.tri_tb = tri_batch;
+  h->d.nb_cells = nb_cells;
+  h->graph_valid = false;  // the captured step holds the old launches
+  return FIB_OK;
+}
+
+extern "C" int32_t fib_get_tiles(const fib_handle *h, int32_t tiles[3]) {
+  if (!h || !tiles) return FIB_ERR_ARG;
+  tiles[0] = h->d.tri_ns ? h->d.tri_ns : 256;
+  tiles[1] = h->d.tri_tb ? h->d.tri_tb : 4;
+  tiles[2] = h->d.nb_cells ? h->d.nb_cells : 8;
   return FIB_OK;
 }
 

@@ -258,6 +258,16 @@ class FibGPU:
     def set_list_width(self, wsw, wion):
         self._chk("fib_set_list_width", self.L.fib_set_list_width(self.h, int(wsw), int(wion)))
 
+    def set_tiles(self, tri_ns=0, tri_batch=0, nb_cells=0):
+        """Launch tiles (0: default); results do not depend on them."""
+        self._chk("fib_set_tiles", self.L.fib_set_tiles(self.h, int(tri_ns), int(tri_batch),
+                                                        int(nb_cells)))
+
+    def tiles(self):
+        t = np.zeros(3, np.int32)
+        self._chk("fib_get_tiles", self.L.fib_get_tiles(self.h, _p(t)))
+        return tuple(int(v) for v in t)
+
     def list_width(self):
         a, b = C.c_int32(), C.c_int32()
         self._chk("fib_get_list_width", self.L.fib_get_list_width(self.h, C.byref(a), C.byref(b)))

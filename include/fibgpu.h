@@ -323,6 +323,15 @@ int32_t fib_md_energy(fib_handle *h, double *epot);
 int32_t fib_set_list_width(fib_handle *h, int32_t wsw, int32_t wion);
 int32_t fib_get_list_width(const fib_handle *h, int32_t *wsw, int32_t *wion);
 
+/* Launch tiles, for autotuning: tri_ns legs per stage run of the vertex pass
+ * (128 or 256), tri_batch legs per lane whose neighbour loads are in flight
+ * together (2 or 4), nb_cells cells per wavefront of the list build (4, 8 or
+ * 16); 0 keeps a tile at its default (256, 4, 8).  Results are the same bit
+ * for bit in every tile, only the time changes.  The reference's only
+ * tuning knob of this kind is its cell count, rc and Nc (si.f90:55-56). */
+int32_t fib_set_tiles(fib_handle *h, int32_t tri_ns, int32_t tri_batch, int32_t nb_cells);
+int32_t fib_get_tiles(const fib_handle *h, int32_t tiles[3]);
+
 /* Device-side kernel timers (HIP events on the handle's stream around each
  * launch), accumulated since fib_timing_enable(h, mask); for benchmarks.
  * Bit k of mask turns on timer k, which covers the kernels fib_timer_name(k)

@@ -44,3 +44,39 @@ def test_tune_listfac_small_batch():
     print(f"\n{table} -> {best}", flush=True)
     assert best in (1.10, 1.20)
     assert len(table) == 2 and all(ms is not None and ms > 0 for _, ms, _ in table)
+
+
+def test_tiles_leave_trajectory_unchanged():
+    """Launch tiles change no result: an MD run of a cascade in non-default
+    tiles (vertex-pass stage of 128 legs, 2 loads in flight; 16 cells per
+    list-build wavefront) equals the default tiles' run bit for bit, in both
+    precisions; the tuner returns one of its candidates and leaves the handle
+    in it."""
+    from fib_md_amd import FibGPU, autotune
+    s = configs.si_slab(5, 61, nz=4)
+    x, v = configs.ion_launch(s["L"], s["surface_z"], 2000.0, 7.0, 30.0,
+                              np.random.default_rng(61))
+    for prec in ("fp64", "mixed"):
+        out = []
+        for tiles in ((0, 0, 0), (128, 2, 16), (256, 2, 4)):
+            g = FibGPU([(len(s["X"]), s["nsg"], s["nlj"], s["L"])], precision=prec)
+            g.set_tiles(*tiles)
+            g.set_state(s["X"], V=s["V"], atype=s["atype"], xsides=s["xsides"], Xi=s["X"])
+            g.md_begin([s["surface_z"]], [0.0])
+            g.md_fire(0, 1, x, v)
+            g.md_run(150)
+            out.append((*g.state()[:2], g.md_report(0)))
+            if tiles == (256, 2, 4):
+                best, table = autotune.tune_tiles(g, steps=4, warmup=1)
+                assert best in [t for t, _ in table] and g.tiles() == best
+                assert len(table) == 6
+            g.close()
+        for X, V, r in out[1:]:
+            np.testing.assert_array_equal(X, out[0][0])
+            np.testing.assert_array_equal(V, out[0][1])
+            assert r == out[0][2]
+    from fib_md_amd.gpu import GPUError
+    g = FibGPU([(len(s["X"]), s["nsg"], s["nlj"], s["L"])], precision="mixed")
+    with pytest.raises(GPUError, match="tri_ns"):
+        g.set_tiles(64, 4, 8)
+    g.close()
