@@ -1046,7 +1046,11 @@ def main():
         if args.tiles:
             g.set_tiles(*args.tiles)
         _fire_all(g, shots, 0)
-        g.md_run(DOSE_SETTLE)
+        # in chunks, with a line each: under counter collection every
+        # dispatch is serialized and the settling steps take minutes
+        for k in range(0, DOSE_SETTLE, 25):
+            g.md_run(min(25, DOSE_SETTLE - k))
+            log(f"  settling: {k + 25} steps")
         g.md_run(PROFILE_STEPS)
         g.close()
         log(f"dose profile {args.profile}: {DOSE_SETTLE} + {PROFILE_STEPS} steps")
