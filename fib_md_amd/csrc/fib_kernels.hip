@@ -277,6 +277,7 @@ __global__ __launch_bounds__(FIN_T) void k_finalize(const DInst *__restrict__ I,
   HALT_OR_RETURN(halt);
   __shared__ double red[FIN_T / FIB_WAVE][NV];
   const int inst = blockIdx.x, ch = blockIdx.y, nch = gridDim.y;
+  if (gated_out(gate, inst)) return;  // its producer skipped its blocks
   const int blk0 = I[inst].atom_off / rs.bs, nblk = I[inst].natm_pad / rs.bs;
   double x[NV];
 #pragma unroll
@@ -323,7 +324,7 @@ __global__ void k_finalize2(const DInst *__restrict__ I, int nch, RedSpec rs,
   GATE_OR_RETURN(gate);
   HALT_OR_RETURN(halt);
   const int inst = blockIdx.x;
-  if (threadIdx.x != 0) return;
+  if (threadIdx.x != 0 || gated_out(gate, inst)) return;
   const int nci = (I[inst].natm_pad / rs.bs + FIN_T - 1) / FIN_T;
   const double *t0 = tmp + (size_t)inst * nch * NV;
   double t[NV];
@@ -901,12 +902,16 @@ __global__ void k_clear_counts(int NA, int *__restrict__ cnt_sw, int *__restrict
 }
 
 // Longest SW / ion lists of the build (overflow check).
+// Under a gate only the rebuilt instances' blocks take part (their finalize
+// skips the others, whose maxima stay those of their own last build).
 __global__ __launch_bounds__(FIB_BLOCK) void k_maxcnt(const int *__restrict__ cnt_sw,
                                                       const int *__restrict__ cnt_ion,
                                                       int pstride, double *__restrict__ partials,
+                                                      const int *__restrict__ binst,
                                                       const int *__restrict__ gate) {
   GATE_OR_RETURN(gate);
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (block_gated_out(gate, binst, i)) return;
   double v[2] = {(double)cnt_sw[i], (double)cnt_ion[i]};
   const int op[2] = {1, 1};
   block_partials<2, FIB_BLOCK>(v, op, blockIdx.x, pstride, partials);
@@ -2621,7 +2626,7 @@ void launch_nbuild(const DevBufs &d, const FConst &fc, hipStream_t s) {
   {
     KScope ks_(d, KT_LISTAUX);
     hipLaunchKernelGGL(k_maxcnt, dim3(d.NA / FIB_BLOCK), dim3(FIB_BLOCK), 0, s, d.cnt_sw,
-                       d.cnt_ion, d.pstride, d.partials, d.gate);
+                       d.cnt_ion, d.pstride, d.partials, d.binst, d.gate);
   }
   const int op[2] = {1, 1};
   const int dst[2] = {DCTL_IDX(maxcnt_sw), DCTL_IDX(maxcnt_ion)};
