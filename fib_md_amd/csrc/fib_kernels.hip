@@ -599,6 +599,11 @@ __global__ void k_copyback(int NA, const double4 *__restrict__ X1, double4 *__re
 #define NB_WAVES 4  // independent wavefronts per k_nbuild workgroup
 #endif
 #define OFF_W 32    // bytes per atom of the stencil-offset table (27 used)
+// A row entry of rk_sw holds (stencil cell << 8 | rank) as the build wrote
+// it, or, once a force pass has worked it out, RK_SLOT | the slot of the
+// row's atom in the neighbour's row (values >= the list width, at most 127,
+// mark a neighbour without a row or with the atom past its end).
+#define RK_SLOT 0x8000
 #define NB_JBITS 27  // atom index bits beside the stencil cell in a staged w (fib_create keeps NA below)
 
 // Stencil bookkeeping for the force gather.  The list row of an atom is the
@@ -1318,7 +1323,7 @@ template <bool MIXED, bool EN, bool VIR, int NS, int TB>
 __global__ __launch_bounds__(FIB_BLOCK) void k_tri(
     const double4 *__restrict__ X, const DInst *__restrict__ I, const FConst fc, int NA,
     int Wsw, const int *__restrict__ cnt_leg, const int *__restrict__ nbr_sw,
-    const uint16_t *__restrict__ rk_sw, const uint8_t *__restrict__ off_sw,
+    uint16_t *__restrict__ rk_sw, const uint8_t *__restrict__ off_sw,
     const uint64_t *__restrict__ amask, typename Prec<MIXED>::T4 *__restrict__ G,
     uint32_t *__restrict__ RA, int pstride, double *__restrict__ partials,
     const int *__restrict__ halt) {
@@ -1397,7 +1402,10 @@ __global__ __launch_bounds__(FIB_BLOCK) void k_tri(
         // G[n * NA + j], n = j's leg index of the atom.  The slot r of the
         // atom in j's row follows from the stencil offsets (see k_nbuild), n
         // from j's leg mask (legs are symmetric); 255 offsets (r >= Wsw)
-        // mark a j without a row.
+        // mark a j without a row.  The first time a pair is a leg after a
+        // build, r is worked out from j's offsets and kept in the row entry
+        // (RK_SLOT set); later steps read it there, without the gather of
+        // j's offsets.
         double4 xq[TB];
         uint64_t aq[TB];
         int oq[TB];
@@ -1405,7 +1413,7 @@ __global__ __launch_bounds__(FIB_BLOCK) void k_tri(
         for (int u = 0; u < TB; ++u) {
           xq[u] = X[jq[u]];
           aq[u] = amask[jq[u]];
-          oq[u] = off_sw[(size_t)jq[u] * OFF_W + 26 - (rq[u] >> 8)];
+          oq[u] = (rq[u] & RK_SLOT) ? 0 : off_sw[(size_t)jq[u] * OFF_W + 26 - (rq[u] >> 8)];
         }
 #pragma unroll
         for (int u = 0; u < TB; ++u) {
@@ -1420,7 +1428,8 @@ __global__ __launch_bounds__(FIB_BLOCK) void k_tri(
             sA[wid][q] = x;
             sB[wid][q] = y;
             owner[wid][q] = (uint8_t)lane;
-            const int r = oq[u] + (rq[u] & 0xff);
+            const int r = (rq[u] & RK_SLOT) ? (rq[u] & ~RK_SLOT) : min(oq[u] + (rq[u] & 0xff), 127);
+            if (!(rq[u] & RK_SLOT)) rk_sw[rowc + kq[u]] = (uint16_t)(RK_SLOT | r);
             sR[wid][q] = r < Wsw ? (uint32_t)__popcll(aq[u] & ((1ull << r) - 1ull)) *
                                            (uint32_t)NA + (uint32_t)jq[u]
                                  : ~0u;
@@ -2169,7 +2178,8 @@ __global__ __launch_bounds__(FIB_BLOCK) void k_force3(
         const uint64_t mj = exmask[j];
         if (!mj) continue;
         const int sr = rk_sw[row + k];
-        const int r = off_sw[(size_t)j * OFF_W + 26 - (sr >> 8)] + (sr & 0xff);
+        const int r = (sr & RK_SLOT) ? (sr & ~RK_SLOT)
+                                     : off_sw[(size_t)j * OFF_W + 26 - (sr >> 8)] + (sr & 0xff);
         if (r < Wsw && ((mj >> r) & 1ull)) {
           const T4 g = EX[(size_t)r * NA + j];
           g0 += (double)g.x; g1 += (double)g.y; g2 += (double)g.z;
