@@ -1365,7 +1365,6 @@ __global__ __launch_bounds__(FIB_BLOCK) void k_tri(
   const double4 xc = X[ic];
   const int tci = type_of(meta_of(xc));
   const uint64_t amc = amask[ic];
-  const size_t rowc = (size_t)ic * Wsw;
   const DInst &in = I[__builtin_amdgcn_readfirstlane(inst_of(meta_of(xc)))];
   sge[wid][lane] = (tci == 2) ? 1 : 0;
 
@@ -1384,23 +1383,35 @@ __global__ __launch_bounds__(FIB_BLOCK) void k_tri(
     const bool mine = lane >= o0 && lane < o1;
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-    // stage: each lane computes its atom's legs (the pairs of its leg mask,
-    // in list order) from the positions, TB neighbours in flight
+    // owner of every leg of the run (LDS only), so that the stage below can
+    // give each lane legs rather than atoms
+    if (mine)
+      for (int a = 0; a < nl; ++a) owner[wid][inc - nl + a - s0] = (uint8_t)lane;
+    wave_lds_sync();
+    // stage: one lane per leg, in list order within each atom, computed from
+    // the positions; TB legs per lane with their loads in flight together,
+    // every lane busy whatever the atoms' leg counts
     {
-      uint64_t m = amc;
-      for (int a0 = 0; __ballot(mine && a0 < nl) != 0ull; a0 += TB) {
-        int kq[TB], jq[TB], rq[TB];
+      const int nq = s1 - s0;
+      const int amlo = (int)(uint32_t)amc, amhi = (int)(uint32_t)(amc >> 32);
+      for (int q0 = 0; q0 < nq; q0 += FIB_WAVE * TB) {
+        int qv[TB], ov[TB], kq[TB], jq[TB], rq[TB];
 #pragma unroll
         for (int u = 0; u < TB; ++u) {
-          const bool ok = mine && a0 + u < nl;
-          kq[u] = ok ? __ffsll((unsigned long long)m) - 1 : 0;
-          if (ok) m &= m - 1;
-          jq[u] = ok ? nbr_sw[rowc + kq[u]] : ic;
-          rq[u] = ok ? rk_sw[rowc + kq[u]] : 0;
+          qv[u] = q0 + u * FIB_WAVE + lane;
+          const bool ok = qv[u] < nq;
+          ov[u] = ok ? owner[wid][min(qv[u], NS - 1)] : 0;
+          // the owner's leg mask, from the owner's lane (every lane takes part)
+          const uint64_t am = ((uint64_t)(uint32_t)__shfl(amhi, ov[u]) << 32) |
+                              (uint64_t)(uint32_t)__shfl(amlo, ov[u]);
+          kq[u] = ok ? nth_bit(am, qv[u] + s0 - soff[wid][ov[u]]) : 0;
+          const size_t rw = (size_t)(i0 + ov[u]) * Wsw;
+          jq[u] = ok ? nbr_sw[rw + kq[u]] : i0 + ov[u];
+          rq[u] = ok ? rk_sw[rw + kq[u]] : 0xffff;
         }
-        // Where the partner j keeps the force of its own terms on this atom:
-        // G[n * NA + j], n = j's leg index of the atom.  The slot r of the
-        // atom in j's row follows from the stencil offsets (see k_nbuild), n
+        // Where the partner j keeps the force of its own terms on the owner:
+        // G[n * NA + j], n = j's leg index of the owner.  The slot r of the
+        // owner in j's row follows from the stencil offsets (see k_nbuild), n
         // from j's leg mask (legs are symmetric); 255 offsets (r >= Wsw)
         // mark a j without a row.  The first time a pair is a leg after a
         // build, r is worked out from j's offsets and kept in the row entry
@@ -1417,19 +1428,26 @@ __global__ __launch_bounds__(FIB_BLOCK) void k_tri(
         }
 #pragma unroll
         for (int u = 0; u < TB; ++u) {
-          if (mine && a0 + u < nl) {
+          // the owner's position and species, from the owner's lane
+          double4 xo;
+          xo.x = __shfl(xc.x, ov[u]);
+          xo.y = __shfl(xc.y, ov[u]);
+          xo.z = __shfl(xc.z, ov[u]);
+          xo.w = 0.0;
+          const int to = __shfl(tci, ov[u]);
+          if (qv[u] < nq) {
             Leg<T> L;
-            make_leg<MIXED>(xc, xq[u], in, fc, tci, type_of(meta_of(xq[u])), (uint32_t)jq[u],
+            make_leg<MIXED>(xo, xq[u], in, fc, to, type_of(meta_of(xq[u])), (uint32_t)jq[u],
                             (uint32_t)kq[u], L);
             T4 x, y;
             x.x = L.e0; x.y = L.e1; x.z = L.e2; x.w = ks_ge(L.ks) ? -L.E : L.E;
             y.x = L.g; y.y = L.rinv; y.z = ks_pack(T(0), L.ks); y.w = ks_pack(T(0), L.slot);
-            const int q = inc - nl + a0 + u - s0;
+            const int q = qv[u];
             sA[wid][q] = x;
             sB[wid][q] = y;
-            owner[wid][q] = (uint8_t)lane;
             const int r = (rq[u] & RK_SLOT) ? (rq[u] & ~RK_SLOT) : min(oq[u] + (rq[u] & 0xff), 127);
-            if (!(rq[u] & RK_SLOT)) rk_sw[rowc + kq[u]] = (uint16_t)(RK_SLOT | r);
+            if (!(rq[u] & RK_SLOT))
+              rk_sw[(size_t)(i0 + ov[u]) * Wsw + kq[u]] = (uint16_t)(RK_SLOT | r);
             sR[wid][q] = r < Wsw ? (uint32_t)__popcll(aq[u] & ((1ull << r) - 1ull)) *
                                            (uint32_t)NA + (uint32_t)jq[u]
                                  : ~0u;
