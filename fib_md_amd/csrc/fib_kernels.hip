@@ -560,7 +560,7 @@ __global__ void k_reorder(int NA, const int *__restrict__ perm, const DInst *__r
   for (int c = 0; c < 3; ++c) {
     size_t a = (size_t)c * NA;
     V1[a + p] = V0[a + s];
-    F1[a + p] = F0[a + s];
+    if (F0) F1[a + p] = F0[a + s];
     Xi1[a + p] = Xi0[a + s];
   }
   const DInst &in = I[inst_of(mp)];
@@ -582,7 +582,7 @@ __global__ void k_copyback(int NA, const double4 *__restrict__ X1, double4 *__re
     for (int c = 0; c < 3; ++c) {
       const size_t a = (size_t)c * NA + p;
       V0[a] = V1[a];
-      F0[a] = F1[a];
+      if (F0) F0[a] = F1[a];
       Xi0[a] = Xi1[a];
     }
   }
@@ -2628,17 +2628,21 @@ void launch_sort(const DevBufs &d, hipStream_t s, bool inplace) {
                        d.bcnt, d.nbucket, d.perm, d.gate);
   }
   const int c = d.cur, n = 1 - d.cur;
+  // the in-place sort of MD-step mode leaves the forces behind: every caller
+  // (an MD step's build, the fire-time build) computes them anew before
+  // anything reads them
+  double *const F0 = inplace ? nullptr : d.F[c];
   {
     KScope ks_(d, KT_SORT);
     hipLaunchKernelGGL(k_reorder, dim3(gcap(d.NA)), dim3(FIB_BLOCK), 0, s, d.NA, d.perm,
-                       d.I, d.X[c], d.X[n], d.V[c], d.V[n], d.F[c], d.F[n], d.Xi[c], d.Xi[n],
+                       d.I, d.X[c], d.X[n], d.V[c], d.V[n], F0, d.F[n], d.Xi[c], d.Xi[n],
                        d.inv, d.binst, d.gate);
   }
   if (inplace) {
     KScope ks_(d, KT_SORT);
     hipLaunchKernelGGL(k_copyback, dim3(gcap(d.NA)), dim3(FIB_BLOCK), 0, s, d.NA, d.X[n],
-                       d.X[c], d.V[n], d.V[c], d.F[n], d.F[c], d.Xi[n], d.Xi[c], d.binst,
-                       d.gate);
+                       d.X[c], d.V[n], d.V[c], (const double *)nullptr, (double *)nullptr,
+                       d.Xi[n], d.Xi[c], d.binst, d.gate);
   }
 }
 
