@@ -571,6 +571,7 @@ static bool run_in_flight(fib_handle *h, const char *who) {
 
 extern "C" int32_t fib_set_state(fib_handle *h, int32_t inst, const double *X, const double *V,
                                  const int32_t *atype, const int32_t *xsides, const double *Xi) {
+  if (run_in_flight(h, "fib_set_state")) return FIB_ERR_STATE;
   if (bad_inst(h, inst) || !X || !atype) return fail(h, FIB_ERR_ARG, "fib_set_state: bad argument");
   const DInst &in = h->inst[inst];
   const size_t n = (size_t)in.natm;
@@ -638,6 +639,7 @@ extern "C" int32_t fib_get_state(fib_handle *h, int32_t inst, double *X, double 
 }
 
 extern "C" int32_t fib_set_impact(fib_handle *h, int32_t inst, int32_t impact) {
+  if (run_in_flight(h, "fib_set_impact")) return FIB_ERR_STATE;
   if (bad_inst(h, inst) || impact < 0) return fail(h, FIB_ERR_ARG, "fib_set_impact: bad argument");
   h->inst[inst].impact = impact;
   HIPCHK(h, hipMemcpyAsync(&h->d.I[inst], &h->inst[inst], sizeof(DInst), hipMemcpyHostToDevice,
@@ -647,6 +649,7 @@ extern "C" int32_t fib_set_impact(fib_handle *h, int32_t inst, int32_t impact) {
 }
 
 extern "C" int32_t fib_bin_cells(fib_handle *h, int32_t inst, int32_t *icell) {
+  if (run_in_flight(h, "fib_bin_cells")) return FIB_ERR_STATE;
   if (bad_inst(h, inst) || !icell) return fail(h, FIB_ERR_ARG, "fib_bin_cells: bad argument");
   if (!h->have_state) return fail(h, FIB_ERR_STATE, "fib_bin_cells: no state");
   const size_t n = (size_t)h->inst[inst].natm;
@@ -706,6 +709,7 @@ static int32_t build_lists(fib_handle *h) {
 }
 
 extern "C" int32_t fib_build_nlist(fib_handle *h) {
+  if (run_in_flight(h, "fib_build_nlist")) return FIB_ERR_STATE;
   if (!h) return FIB_ERR_ARG;
   if (!h->have_state) return fail(h, FIB_ERR_STATE, "fib_build_nlist: no state");
   return build_lists(h);
@@ -714,6 +718,7 @@ extern "C" int32_t fib_build_nlist(fib_handle *h) {
 extern "C" int32_t fib_get_nlist(fib_handle *h, int32_t inst, int32_t *nss, int32_t *nlist2,
                                  int32_t *len2, int32_t *nsslj, int32_t *nlistlj,
                                  int32_t *lenlj) {
+  if (run_in_flight(h, "fib_get_nlist")) return FIB_ERR_STATE;
   if (bad_inst(h, inst)) return fail(h, FIB_ERR_ARG, "fib_get_nlist: bad instance");
   if (!h->have_lists) return fail(h, FIB_ERR_STATE, "fib_get_nlist: no lists");
   const DInst &in = h->inst[inst];
@@ -781,6 +786,7 @@ static int32_t run_force(fib_handle *h, int flags, int mode, const StepArgs &sa,
 }
 
 extern "C" int32_t fib_compute(fib_handle *h, int32_t flags) {
+  if (run_in_flight(h, "fib_compute")) return FIB_ERR_STATE;
   if (!h) return FIB_ERR_ARG;
   if (!h->have_lists) return fail(h, FIB_ERR_STATE, "fib_compute: build the lists first");
   int32_t r = run_force(h, flags, 0, h->base);
@@ -793,6 +799,7 @@ extern "C" int32_t fib_compute(fib_handle *h, int32_t flags) {
 
 extern "C" int32_t fib_get_forces(fib_handle *h, int32_t inst, double *F, double *u, double *utot,
                                   double virial[9]) {
+  if (run_in_flight(h, "fib_get_forces")) return FIB_ERR_STATE;
   if (bad_inst(h, inst)) return fail(h, FIB_ERR_ARG, "fib_get_forces: bad instance");
   if (!h->have_force) return fail(h, FIB_ERR_STATE, "fib_get_forces: no force evaluation");
   const DInst &in = h->inst[inst];
@@ -818,6 +825,7 @@ extern "C" int32_t fib_get_forces(fib_handle *h, int32_t inst, double *F, double
 
 extern "C" int32_t fib_force(fib_handle *h, int32_t inst, int32_t impact, double *F, double *u,
                              double *utot, double virial[9]) {
+  if (run_in_flight(h, "fib_force")) return FIB_ERR_STATE;
   int32_t r = fib_set_impact(h, inst, impact);
   if (r != FIB_OK) return r;
   int flags = FIB_EVAL_FORCE;
@@ -861,6 +869,7 @@ static StepArgs step_args(fib_handle *h, int mode) {
 }
 
 extern "C" int32_t fib_set_run(fib_handle *h, int32_t inst, const fib_run_params *rp) {
+  if (run_in_flight(h, "fib_set_run")) return FIB_ERR_STATE;
   if (bad_inst(h, inst) || inst != 0 || !rp) return fail(h, FIB_ERR_ARG, "fib_set_run: bad argument");
   if (rp->ntlist_i < 1 || rp->ntlist_r < 1 || rp->atlist_i < 1 || rp->atlist_r < 1 ||
       rp->dslist_i < 1 || rp->dslist_r < 1)
@@ -882,6 +891,7 @@ static double temp_of(const fib_handle *h, const DCtl &c) {
 }
 
 extern "C" int32_t fib_temperature(fib_handle *h, int32_t inst, double *temp) {
+  if (run_in_flight(h, "fib_temperature")) return FIB_ERR_STATE;
   if (bad_inst(h, inst) || !temp) return fail(h, FIB_ERR_ARG, "fib_temperature: bad argument");
   if (!h->have_state) return fail(h, FIB_ERR_STATE, "fib_temperature: no state");
   StepArgs sa = h->ref.have_run ? step_args(h, 1) : h->base;
@@ -894,6 +904,7 @@ extern "C" int32_t fib_temperature(fib_handle *h, int32_t inst, double *temp) {
 }
 
 extern "C" int32_t fib_tint_init(fib_handle *h, int64_t lt0, double time0) {
+  if (run_in_flight(h, "fib_tint_init")) return FIB_ERR_STATE;
   if (!h) return FIB_ERR_ARG;
   if (!h->ref.have_run) return fail(h, FIB_ERR_STATE, "fib_tint_init: call fib_set_run first");
   if (!h->have_lists) return fail(h, FIB_ERR_STATE, "fib_tint_init: build the lists first");
@@ -914,6 +925,7 @@ extern "C" int32_t fib_tint_init(fib_handle *h, int64_t lt0, double time0) {
 }
 
 extern "C" int32_t fib_ti_begin(fib_handle *h, int32_t whichctrl, int32_t impact) {
+  if (run_in_flight(h, "fib_ti_begin")) return FIB_ERR_STATE;
   if (!h || whichctrl < 0 || whichctrl > 2) return FIB_ERR_ARG;
   if (!h->ref.have_run) return fail(h, FIB_ERR_STATE, "fib_ti_begin: call fib_set_run first");
   RefCtl &R = h->ref;
@@ -939,6 +951,7 @@ extern "C" int32_t fib_ti_begin(fib_handle *h, int32_t whichctrl, int32_t impact
 
 extern "C" int32_t fib_knock(fib_handle *h, int32_t knc, const double xion[3],
                              const double vion[3]) {
+  if (run_in_flight(h, "fib_knock")) return FIB_ERR_STATE;
   if (!h || !xion || !vion) return FIB_ERR_ARG;
   const DInst &in = h->inst[0];
   if (knc < 1 || knc > in.nlj) return fail(h, FIB_ERR_ARG, "fib_knock: ion number out of range");
@@ -951,6 +964,7 @@ extern "C" int32_t fib_knock(fib_handle *h, int32_t knc, const double xion[3],
 }
 
 extern "C" int32_t fib_step(fib_handle *h, const fib_step_ctl *ctl, fib_step_report *rep) {
+  if (run_in_flight(h, "fib_step")) return FIB_ERR_STATE;
   if (!h || !ctl || !rep) return FIB_ERR_ARG;
   if (!h->ref.have_run || !h->ref.begun) return fail(h, FIB_ERR_STATE, "fib_step: call fib_ti_begin first");
   if (!h->have_lists) return fail(h, FIB_ERR_STATE, "fib_step: build the lists first");
@@ -1432,6 +1446,7 @@ extern "C" int32_t fib_md_use_graph(fib_handle *h, int32_t on) {
 }
 
 extern "C" int32_t fib_set_list_width(fib_handle *h, int32_t wsw, int32_t wion) {
+  if (run_in_flight(h, "fib_set_list_width")) return FIB_ERR_STATE;
   if (!h) return FIB_ERR_ARG;
   if (wsw < 4 || wsw > 64 || wsw % 4 || wion < 4 || wion % 4)
     return fail(h, FIB_ERR_ARG, "fib_set_list_width: widths must be multiples of 4, SW at most 64");
@@ -1521,6 +1536,7 @@ extern "C" int32_t fib_md_energy(fib_handle *h, double *epot) {
 }
 
 extern "C" int32_t fib_timing_enable(fib_handle *h, int32_t on) {
+  if (run_in_flight(h, "fib_timing_enable")) return FIB_ERR_STATE;
   if (!h) return FIB_ERR_ARG;
   h->tim.reset();
   h->tim.on = on != 0;
@@ -1529,6 +1545,7 @@ extern "C" int32_t fib_timing_enable(fib_handle *h, int32_t on) {
 }
 
 extern "C" int32_t fib_get_timing(fib_handle *h, fib_timing *t) {
+  if (run_in_flight(h, "fib_get_timing")) return FIB_ERR_STATE;
   if (!h || !t) return FIB_ERR_ARG;
   h->tim.flush();
   std::memset(t, 0, sizeof(*t));
@@ -1544,6 +1561,7 @@ extern "C" const char *fib_timer_name(int32_t k) {
 }
 
 extern "C" int32_t fib_synchronize(fib_handle *h) {
+  if (run_in_flight(h, "fib_synchronize")) return FIB_ERR_STATE;
   if (!h) return FIB_ERR_ARG;
   HIPCHK(h, hipStreamSynchronize(h->stream));
   HIPCHK(h, hipGetLastError());

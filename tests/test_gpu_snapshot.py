@@ -99,7 +99,9 @@ def test_calls_refused_while_steps_in_flight():
     g.md_run_begin(50)
     for call in (lambda: g.state(), lambda: g.md_report(0), lambda: g.md_energy(),
                  lambda: g.md_fire(0, 1, x, v), lambda: g.md_run(1),
-                 lambda: g.snapshot_begin(0, 0)):
+                 lambda: g.snapshot_begin(0, 0), lambda: g.compute(),
+                 lambda: g.build_nlist(), lambda: g.set_tiles(128, 2, 4),
+                 lambda: g.set_state(s["X"], V=s["V"], atype=s["atype"])):
         with pytest.raises(GPUError, match="in flight"):
             call()
     g.md_run_end()
