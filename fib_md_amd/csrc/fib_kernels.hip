@@ -665,13 +665,20 @@ __device__ __forceinline__ void nbuild_cell(
     int y = __shfl_up(sc, o);
     if (lane >= o) sc += y;
   }
-  const int total = __shfl(sc, 26);
+  const int total = __builtin_amdgcn_readfirstlane(__shfl(sc, 26));
   // stencil cell s covers candidates [my_lo, my_lo + rc)
   const int my_lo = sc - rc;
+  // the candidates of a 64-batch starting at b that lie before stencil cell
+  // s, as a lane mask (lane s); the one-chunk cell has batches 0 and 64 only
+  auto before_mask = [&](int b) -> uint64_t {
+    const int cut = min(max(my_lo - b, 0), 64);
+    return cut >= 64 ? ~0ull : ((1ull << cut) - 1ull);
+  };
   const bool masks = total <= NB_ACC && ni <= 32;
   if (masks)
     for (int g = lane; g < total; g += FIB_WAVE) s_acc[g] = 0;
   const bool one_chunk = total <= NB_CHUNK;
+  const uint64_t lm0 = before_mask(0), lm1 = before_mask(FIB_WAVE);
   if (one_chunk) {
     // each stencil lane writes its own run of candidates
     for (int q = 0; __ballot(q < rc) != 0ull; ++q)
@@ -688,7 +695,7 @@ __device__ __forceinline__ void nbuild_cell(
   const double4 *Xc = X;
   // the cell's own atoms are stencil range 13: read them from the staged
   // chunk when it holds them (always, unless the halo is very dense)
-  const int own0 = __shfl(my_lo, 13);
+  const int own0 = __builtin_amdgcn_readfirstlane(__shfl(my_lo, 13));
   const bool own_staged = one_chunk;
   const double qb0 = 0.25 * in.lb[0], qb1 = 0.25 * in.lb[1], qb2 = 0.25 * in.lb[2];
   for (int ii = 0; ii < ni; ++ii) {
@@ -790,8 +797,8 @@ __device__ __forceinline__ void nbuild_cell(
         // stencil cell s starts at candidate my_lo: count the accepted ones
         // of this batch of 64 candidates that lie before it
         {
-          const int cut = min(max(my_lo - (base + q0), 0), 64);
-          const uint64_t lm = cut >= 64 ? ~0ull : ((1ull << cut) - 1ull);
+          const int b = base + q0;
+          const uint64_t lm = b == 0 ? lm0 : (b == FIB_WAVE ? lm1 : before_mask(b));
           before += __popcll(msw & lm);
         }
         nsw += __popcll(msw);
