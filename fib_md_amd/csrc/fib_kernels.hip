@@ -162,6 +162,56 @@ __device__ __forceinline__ int find_inst_by_bucket(const DInst *I, int ninst, in
   return lo;
 }
 
+// Streaming accesses: arrays that a force pass reads or writes once per step
+// (its own rows, counts and per-atom results), loaded and stored
+// non-temporally so that they do not push the gathered neighbour positions
+// out of L2.  FIB_NT_LD (loads) and FIB_NT_ST (stores) hold one bit per
+// pass: 1 k_legs, 2 k_tri, 4 k_force3.  k_legs reads each list row as four
+// or five 16-byte loads in turn, and non-temporal loads fetch the row again
+// for each of them (k_legs 0.70 -> 1.04 ms on the 8.2 M-atom dose batch), so
+// its loads stay cached; so do k_force3's reads of its own G entries, which
+// the partners' lookups read again.  Streaming the rest makes the dose
+// batch's step 1.5 % faster in the mixed mode and 1 % in FP64.
+#ifndef FIB_NT_LD
+#define FIB_NT_LD 6
+#endif
+#ifndef FIB_NT_ST
+#define FIB_NT_ST 7
+#endif
+template <class V>
+struct NtVec { typedef V type; };
+template <>
+struct NtVec<int4> { typedef int __attribute__((ext_vector_type(4))) type; };
+template <>
+struct NtVec<float4> { typedef float __attribute__((ext_vector_type(4))) type; };
+template <>
+struct NtVec<double4> { typedef double __attribute__((ext_vector_type(4))) type; };
+template <bool NT, class V>
+__device__ __forceinline__ V ld_nt(const V *p) {
+  if constexpr (NT) {
+    typedef typename NtVec<V>::type N;
+    static_assert(sizeof(N) == sizeof(V), "vector size");
+    const N t = __builtin_nontemporal_load(reinterpret_cast<const N *>(p));
+    V v;
+    __builtin_memcpy(&v, &t, sizeof(V));
+    return v;
+  } else {
+    return *p;
+  }
+}
+template <bool NT, class V>
+__device__ __forceinline__ void st_nt(V *p, const V &v) {
+  if constexpr (NT) {
+    typedef typename NtVec<V>::type N;
+    static_assert(sizeof(N) == sizeof(V), "vector size");
+    N t;
+    __builtin_memcpy(&t, &v, sizeof(V));
+    __builtin_nontemporal_store(t, reinterpret_cast<N *>(p));
+  } else {
+    *p = v;
+  }
+}
+
 // Compensated FP32 accumulation (TwoSum): s + c carries the exact sum of the
 // added terms up to O(u^2) relative error.
 __device__ __forceinline__ void kadd(float &s, float &c, float v) {
@@ -1100,6 +1150,7 @@ __global__ __launch_bounds__(FIB_BLOCK) void k_legs(
     double *__restrict__ partials, const int *__restrict__ halt) {
   HALT_OR_RETURN(halt);
   typedef typename Prec<MIXED>::T T;
+  constexpr bool NT = (FIB_NT_LD & 1) != 0, NTS = (FIB_NT_ST & 1) != 0;
   const int lb = xcd_remap(blockIdx.x, gridDim.x);
   const int i = lb * FIB_BLOCK + threadIdx.x;
   const double4 xi = X[i];
@@ -1213,11 +1264,11 @@ __global__ __launch_bounds__(FIB_BLOCK) void k_legs(
   };
 
   if (sw_atom) {
-    const int cs = min(cnt_sw[i], Wsw);
+    const int cs = min(ld_nt<NT>(cnt_sw + i), Wsw);
     const int4 *r4 = reinterpret_cast<const int4 *>(nbr_sw + row);
     // four slots at a time: their indices, then their positions; the next
     // four indices are loaded while the positions arrive
-    int4 rn = (0 < cs) ? r4[0] : make_int4(i, i, i, i);
+    int4 rn = (0 < cs) ? ld_nt<NT>(r4) : make_int4(i, i, i, i);
     for (int k0 = 0; k0 < cs; k0 += 4) {
       const int4 q = rn;
       const int jq[4] = {q.x, (k0 + 1 < cs) ? q.y : i, (k0 + 2 < cs) ? q.z : i,
@@ -1225,14 +1276,14 @@ __global__ __launch_bounds__(FIB_BLOCK) void k_legs(
       double4 xq[4];
 #pragma unroll
       for (int v = 0; v < 4; ++v) xq[v] = X[jq[v]];
-      if (k0 + 4 < cs) rn = r4[(k0 >> 2) + 1];
+      if (k0 + 4 < cs) rn = ld_nt<NT>(r4 + (k0 >> 2) + 1);
 #pragma unroll
       for (int v = 0; v < 4; ++v)
         if (k0 + v < cs) pair(k0 + v, jq[v], xq[v]);
     }
   }
-  cnt_leg[i] = na;
-  amask[i] = am;
+  st_nt<NTS>(cnt_leg + i, na);
+  st_nt<NTS>(amask + i, am);
 
   // ---- Moliere pairs (si.f90:426-446): both atoms must be fired ----------
   if (live) {
@@ -1291,9 +1342,9 @@ __global__ __launch_bounds__(FIB_BLOCK) void k_legs(
 
   double fx, fy, fz;
   f.get(fscale, fx, fy, fz);
-  Fpart[i] = fx;
-  Fpart[(size_t)NA + i] = fy;
-  Fpart[2 * (size_t)NA + i] = fz;
+  st_nt<NTS>(Fpart + i, fx);
+  st_nt<NTS>(Fpart + (size_t)NA + i, fy);
+  st_nt<NTS>(Fpart + 2 * (size_t)NA + i, fz);
   if (EN) upart[i] = ui;
   if (EN || VIR) {
     double v[10];
@@ -1338,6 +1389,7 @@ __global__ __launch_bounds__(FIB_BLOCK) void k_tri(
   typedef typename Prec<MIXED>::T T;
   typedef typename Prec<MIXED>::T4 T4;
   constexpr int NW = FIB_BLOCK / FIB_WAVE;
+  constexpr bool NT = (FIB_NT_LD & 2) != 0, NTS = (FIB_NT_ST & 2) != 0;
   // sA: e0, e1, e2 and E, negated when the leg's atom is Ge (E > 0);
   // sB: g, 1/r, neighbour bits, list slot bits
   __shared__ T4 sA[NW][NS];
@@ -1359,7 +1411,7 @@ __global__ __launch_bounds__(FIB_BLOCK) void k_tri(
                    (T)(fc.leps[0][1][1] * lu), (T)(fc.leps[1][1][1] * lu)};
 
   // legs of the wave's atoms: exclusive prefix sum in LDS
-  const int nl = cnt_leg[i0 + lane];
+  const int nl = ld_nt<NT>(cnt_leg + i0 + lane);
   int inc = nl;
   for (int o = 1; o < FIB_WAVE; o <<= 1) {
     const int y = __shfl_up(inc, o);
@@ -1413,8 +1465,8 @@ __global__ __launch_bounds__(FIB_BLOCK) void k_tri(
                               (uint64_t)(uint32_t)__shfl(amlo, ov[u]);
           kq[u] = ok ? nth_bit(am, qv[u] + s0 - soff[wid][ov[u]]) : 0;
           const size_t rw = (size_t)(i0 + ov[u]) * Wsw;
-          jq[u] = ok ? nbr_sw[rw + kq[u]] : i0 + ov[u];
-          rq[u] = ok ? rk_sw[rw + kq[u]] : 0xffff;
+          jq[u] = ok ? ld_nt<NT>(nbr_sw + rw + kq[u]) : i0 + ov[u];
+          rq[u] = ok ? (int)ld_nt<NT>(rk_sw + rw + kq[u]) : 0xffff;
         }
         // Where the partner j keeps the force of its own terms on the owner:
         // G[n * NA + j], n = j's leg index of the owner.  The slot r of the
@@ -1511,8 +1563,8 @@ __global__ __launch_bounds__(FIB_BLOCK) void k_tri(
       if (mine && a < nl) {
         const size_t at = (size_t)a * NA + i0 + lane;
         const int q = inc - nl + a - s0;
-        G[at] = sB[wid][q];
-        RA[at] = sR[wid][q];
+        st_nt<NTS>(G + at, sB[wid][q]);
+        st_nt<NTS>(RA + at, sR[wid][q]);
       }
     }
     o0 = o1;
@@ -2159,6 +2211,7 @@ __global__ __launch_bounds__(FIB_BLOCK) void k_force3(
     const int *__restrict__ halt) {
   HALT_OR_RETURN(halt);
   typedef typename Prec<MIXED>::T4 T4;
+  constexpr bool NT = (FIB_NT_LD & 4) != 0, NTS = (FIB_NT_ST & 4) != 0;
   const int lb = xcd_remap(blockIdx.x, gridDim.x);
   const int i = lb * FIB_BLOCK + threadIdx.x;
   const double4 xi = X[i];
@@ -2168,12 +2221,14 @@ __global__ __launch_bounds__(FIB_BLOCK) void k_force3(
   const bool live = !(mi & (META_PAD | META_DEAD));
   double f0 = 0, f1 = 0, f2 = 0, u = 0;
   if (live) {
-    f0 = Fpart[i]; f1 = Fpart[(size_t)NA + i]; f2 = Fpart[2 * (size_t)NA + i];
+    f0 = ld_nt<NT>(Fpart + i);
+    f1 = ld_nt<NT>(Fpart + (size_t)NA + i);
+    f2 = ld_nt<NT>(Fpart + 2 * (size_t)NA + i);
     if (EN) u = upart[i];
     const size_t row = (size_t)i * Wsw;
     double g0 = 0, g1 = 0, g2 = 0, gu = 0;
     // reaction on the centre: minus the forces its own terms put on its legs
-    const int na = cnt_leg[i];
+    const int na = ld_nt<NT>(cnt_leg + i);
     for (int n = 0; n < na; ++n) {
       const T4 g = G[(size_t)n * NA + i];
       g0 -= (double)g.x; g1 -= (double)g.y; g2 -= (double)g.z;
@@ -2187,7 +2242,7 @@ __global__ __launch_bounds__(FIB_BLOCK) void k_force3(
       T4 g4[F3_BATCH];
 #pragma unroll
       for (int u = 0; u < F3_BATCH; ++u)
-        ra[u] = (n0 + u < na) ? RA[(size_t)(n0 + u) * NA + i] : ~0u;
+        ra[u] = (n0 + u < na) ? ld_nt<NT>(RA + (size_t)(n0 + u) * NA + i) : ~0u;
 #pragma unroll
       for (int u = 0; u < F3_BATCH; ++u) {
         if (ra[u] != ~0u) {
@@ -2246,9 +2301,9 @@ __global__ __launch_bounds__(FIB_BLOCK) void k_force3(
       double vv[3];
       for (int c = 0; c < 3; ++c) {
         const size_t a = (size_t)c * NA + i;
-        vv[c] = pinned ? 0.0 : V[a] + 0.5 * fn[c] * dt / m;
-        V[a] = vv[c];
-        Fout[a] = fn[c];
+        vv[c] = pinned ? 0.0 : ld_nt<NT>(V + a) + 0.5 * fn[c] * dt / m;
+        st_nt<NTS>(V + a, vv[c]);
+        st_nt<NTS>(Fout + a, fn[c]);
       }
       const double v2 = vv[0] * vv[0] + vv[1] * vv[1] + vv[2] * vv[2];
       v[0] = sqrt(v2);
