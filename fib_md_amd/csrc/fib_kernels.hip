@@ -166,17 +166,20 @@ __device__ __forceinline__ int find_inst_by_bucket(const DInst *I, int ninst, in
 // (its own rows, counts and per-atom results), loaded and stored
 // non-temporally so that they do not push the gathered neighbour positions
 // out of L2.  FIB_NT_LD (loads) and FIB_NT_ST (stores) hold one bit per
-// pass: 1 k_legs, 2 k_tri, 4 k_force3.  k_legs reads each list row as four
-// or five 16-byte loads in turn, and non-temporal loads fetch the row again
-// for each of them (k_legs 0.70 -> 1.04 ms on the 8.2 M-atom dose batch), so
-// its loads stay cached; so do k_force3's reads of its own G entries, which
-// the partners' lookups read again.  Streaming the rest makes the dose
-// batch's step 1.5 % faster in the mixed mode and 1 % in FP64.
+// pass: 1 k_legs, 2 k_tri, 4 k_force3, 8 k_md_pos (velocities, forces,
+// build positions, displacements) and 16 (stores only) k_nbuild's rows.
+// k_legs reads each list row as four or five 16-byte loads in turn, and
+// non-temporal loads fetch the row again for each of them (k_legs 0.70 ->
+// 1.04 ms on the 8.2 M-atom dose batch), so its loads stay cached; so do
+// k_force3's reads of its own G entries, which the partners' lookups read
+// again.  On that batch, streaming the force passes' other accesses makes
+// the step 1.5 % faster (mixed), k_md_pos's a further 1.5 % (k_md_pos
+// itself 13 %), and k_nbuild's row stores make the next k_md_pos faster too.
 #ifndef FIB_NT_LD
-#define FIB_NT_LD 6
+#define FIB_NT_LD 14
 #endif
 #ifndef FIB_NT_ST
-#define FIB_NT_ST 7
+#define FIB_NT_ST 31
 #endif
 template <class V>
 struct NtVec { typedef V type; };
@@ -690,6 +693,7 @@ __device__ __forceinline__ void nbuild_cell(
     uint8_t *__restrict__ off_sw, double4 *s_x, int *s_j, uint8_t *s_st, int *s_rs, int *s_off,
     uint32_t *s_acc) {
   const int lane = threadIdx.x % FIB_WAVE;
+  constexpr bool NBNT = (FIB_NT_ST & 16) != 0;
   // the previous cell of this wave is done with the shared arrays
   wave_lds_sync();
   const int bi = in.bucket_off + (cz * in.next[1] + cy) * in.next[0] + cx;
@@ -832,8 +836,8 @@ __device__ __forceinline__ void nbuild_cell(
                                     (uint32_t)(msw >> 32),
                                     __builtin_amdgcn_mbcnt_lo((uint32_t)msw, 0u));
           if (pos < Wsw) {
-            rowj[pos] = j;
-            rowk[pos] = (uint16_t)((st << 8) | (rank & 0xff));
+            st_nt<NBNT>(rowj + pos, j);
+            st_nt<NBNT>(rowk + pos, (uint16_t)((st << 8) | (rank & 0xff)));
           }
         }
         if (mio) {
@@ -841,7 +845,7 @@ __device__ __forceinline__ void nbuild_cell(
             const int pos = nion + (int)__builtin_amdgcn_mbcnt_hi(
                                        (uint32_t)(mio >> 32),
                                        __builtin_amdgcn_mbcnt_lo((uint32_t)mio, 0u));
-            if (pos < Wion) rowi[pos] = j;
+            if (pos < Wion) st_nt<NBNT>(rowi + pos, j);
           }
         }
         // stencil cell s starts at candidate my_lo: count the accepted ones
@@ -855,10 +859,10 @@ __device__ __forceinline__ void nbuild_cell(
         nion += __popcll(mio);
       }
     }
-    if (lane < 27) off_sw[(size_t)i * OFF_W + lane] = (uint8_t)min(before, 254);
+    if (lane < 27) st_nt<NBNT>(off_sw + (size_t)i * OFF_W + lane, (uint8_t)min(before, 254));
     if (lane == 0) {
-      cnt_sw[i] = nsw;
-      cnt_ion[i] = nion;
+      st_nt<NBNT>(cnt_sw + i, nsw);
+      st_nt<NBNT>(cnt_ion + i, nion);
     }
   }
 }
@@ -1117,6 +1121,16 @@ __device__ __forceinline__ void make_leg(const double4 &xc, const double4 &xn, c
   }
 }
 
+// k_legs: list slots whose positions are loaded together (4 or 8).  Eight
+// make the mixed pass 3.5 % faster on the 8.2 M-atom dose batch (110
+// VGPRs, four waves per SIMD); in FP64 (130 VGPRs) they gain nothing.
+#ifndef LEGS_BATCH_MIXED
+#define LEGS_BATCH_MIXED 8
+#endif
+#ifndef LEGS_BATCH_FP64
+#define LEGS_BATCH_FP64 4
+#endif
+
 // Register accumulator of one force vector: FP64, or FP32 with a TwoSum
 // compensation term (mixed).
 template <bool MIXED>
@@ -1266,19 +1280,33 @@ __global__ __launch_bounds__(FIB_BLOCK) void k_legs(
   if (sw_atom) {
     const int cs = min(ld_nt<NT>(cnt_sw + i), Wsw);
     const int4 *r4 = reinterpret_cast<const int4 *>(nbr_sw + row);
-    // four slots at a time: their indices, then their positions; the next
-    // four indices are loaded while the positions arrive
-    int4 rn = (0 < cs) ? ld_nt<NT>(r4) : make_int4(i, i, i, i);
-    for (int k0 = 0; k0 < cs; k0 += 4) {
-      const int4 q = rn;
-      const int jq[4] = {q.x, (k0 + 1 < cs) ? q.y : i, (k0 + 2 < cs) ? q.z : i,
-                         (k0 + 3 < cs) ? q.w : i};
-      double4 xq[4];
+    // LEGS_BATCH slots at a time: their indices, then their positions; the
+    // next indices are loaded while the positions arrive (every index read
+    // lies below cs <= Wsw, inside the row)
+    constexpr int LEGS_BATCH = MIXED ? LEGS_BATCH_MIXED : LEGS_BATCH_FP64;
+    static_assert(LEGS_BATCH == 4 || LEGS_BATCH == 8, "LEGS_BATCH");
+    constexpr int NQ = LEGS_BATCH / 4;
+    int4 rn[NQ];
 #pragma unroll
-      for (int v = 0; v < 4; ++v) xq[v] = X[jq[v]];
-      if (k0 + 4 < cs) rn = ld_nt<NT>(r4 + (k0 >> 2) + 1);
+    for (int u = 0; u < NQ; ++u) rn[u] = (4 * u < cs) ? ld_nt<NT>(r4 + u) : make_int4(i, i, i, i);
+    for (int k0 = 0; k0 < cs; k0 += LEGS_BATCH) {
+      int jq[LEGS_BATCH];
 #pragma unroll
-      for (int v = 0; v < 4; ++v)
+      for (int u = 0; u < NQ; ++u) {
+        const int b = k0 + 4 * u;
+        jq[4 * u] = (b < cs) ? rn[u].x : i;
+        jq[4 * u + 1] = (b + 1 < cs) ? rn[u].y : i;
+        jq[4 * u + 2] = (b + 2 < cs) ? rn[u].z : i;
+        jq[4 * u + 3] = (b + 3 < cs) ? rn[u].w : i;
+      }
+      double4 xq[LEGS_BATCH];
+#pragma unroll
+      for (int v = 0; v < LEGS_BATCH; ++v) xq[v] = X[jq[v]];
+#pragma unroll
+      for (int u = 0; u < NQ; ++u)
+        if (k0 + LEGS_BATCH + 4 * u < cs) rn[u] = ld_nt<NT>(r4 + ((k0 + LEGS_BATCH) >> 2) + u);
+#pragma unroll
+      for (int v = 0; v < LEGS_BATCH; ++v)
         if (k0 + v < cs) pair(k0 + v, jq[v], xq[v]);
     }
   }
@@ -1818,6 +1846,7 @@ __global__ __launch_bounds__(FIB_BLOCK) void k_md_pos(double4 *__restrict__ X,
                                                       int *__restrict__ leave,
                                                       const int *__restrict__ halt) {
   HALT_OR_RETURN(halt);
+  constexpr bool NT = (FIB_NT_LD & 8) != 0, NTS = (FIB_NT_ST & 8) != 0;
   const int lb = xcd_remap(blockIdx.x, gridDim.x);
   const int i = lb * FIB_BLOCK + threadIdx.x;
   double4 x = X[i];
@@ -1834,14 +1863,14 @@ __global__ __launch_bounds__(FIB_BLOCK) void k_md_pos(double4 *__restrict__ X,
     const int t = type_of(mi);
     const double m = sa.mass[t - 1];
     double v[3], xx[3] = {x.x, x.y, x.z};
-    for (int c = 0; c < 3; ++c) v[c] = V[(size_t)c * NA + i];
+    for (int c = 0; c < 3; ++c) v[c] = ld_nt<NT>(V + (size_t)c * NA + i);
     if (sa.shell_w > 0.0 && t < 3 && !(mi & META_FROZEN) &&
         in_shell(xx[0], xx[1], xx[2], in, sa.shell_w)) {
       red[3] = 0.5 * m * (v[0] * v[0] + v[1] * v[1] + v[2] * v[2]) * (1.0 - lam * lam);
       for (int c = 0; c < 3; ++c) v[c] *= lam;
     }
     for (int c = 0; c < 3; ++c) {
-      v[c] += 0.5 * F[(size_t)c * NA + i] * dt / m;
+      v[c] += 0.5 * ld_nt<NT>(F + (size_t)c * NA + i) * dt / m;
       xx[c] += v[c] * dt;
     }
     if (mi & META_FROZEN) {
@@ -1865,14 +1894,14 @@ __global__ __launch_bounds__(FIB_BLOCK) void k_md_pos(double4 *__restrict__ X,
     }
     x.x = xx[0]; x.y = xx[1]; x.z = xx[2];
     X[i] = x;
-    for (int c = 0; c < 3; ++c) V[(size_t)c * NA + i] = v[c];
+    for (int c = 0; c < 3; ++c) st_nt<NTS>(V + (size_t)c * NA + i, v[c]);
     double e0, e1, e2;
-    min_image_fast(x, Xb[i], in, e0, e1, e2);
+    min_image_fast(x, ld_nt<NT>(Xb + i), in, e0, e1, e2);
     const float dd = (float)sqrt(e0 * e0 + e1 * e1 + e2 * e2);
-    disp[i] = dd;
+    st_nt<NTS>(disp + i, dd);
     red[0] = (double)dd;
   } else {
-    disp[i] = 0.0f;
+    st_nt<NTS>(disp + i, 0.0f);
   }
   const int op[5] = {RED_TOP2, RED_TOP2_CNT, RED_TOP2_SND, RED_SUM, RED_SUM};
   block_partials<5, FIB_BLOCK>(red, op, lb, pstride, partials);
