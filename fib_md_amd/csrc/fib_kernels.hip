@@ -1108,16 +1108,17 @@ __device__ __forceinline__ void make_leg(const double4 &xc, const double4 &xn, c
     L.E = exp(fc.gamma * inv_ra);
     L.g = -fc.gamma * inv_ra * inv_ra * isg;
   } else {
-    const float r0 = (float)(d0 * isg), r1 = (float)(d1 * isg), r2 = (float)(d2 * isg);
-    const float rr = r0 * r0 + r1 * r1 + r2 * r2;
-    const float rinv = __builtin_amdgcn_rsqf(rr);
-    const float rs = rr * rinv;
-    const float inv_ra = __builtin_amdgcn_rcpf(rs - (float)fc.al);
-    const float rho = fc.rho[tc == 2 ? 1 : 0][tn == 2 ? 1 : 0];
-    L.e0 = r0 * rinv; L.e1 = r1 * rinv; L.e2 = r2 * rinv;
-    L.rinv = rinv * rho;
-    L.E = __expf((float)fc.gamma * inv_ra);
-    L.g = -(float)fc.gamma * inv_ra * inv_ra * rho;
+    // every leg value from FP64 geometry, rounded to FP32 once: the FP32
+    // vertex terms cannot recover accuracy their inputs do not carry
+    const double s0 = d0 * isg, s1 = d1 * isg, s2 = d2 * isg;
+    const double rs = sqrt(s0 * s0 + s1 * s1 + s2 * s2);
+    const double rinv = 1.0 / rs;
+    const double inv_ra = 1.0 / (rs - fc.al);
+    const double rho = fc.rho[tc == 2 ? 1 : 0][tn == 2 ? 1 : 0];
+    L.e0 = (float)(s0 * rinv); L.e1 = (float)(s1 * rinv); L.e2 = (float)(s2 * rinv);
+    L.rinv = (float)(rinv * rho);
+    L.E = expf((float)(fc.gamma * inv_ra));
+    L.g = (float)(-fc.gamma * inv_ra * inv_ra * rho);
   }
 }
 
@@ -1241,21 +1242,25 @@ __global__ __launch_bounds__(FIB_BLOCK) void k_legs(
         ++na;
       }
     } else {
-      // FP64 difference, everything after in FP32 (pair units: sigma_ij)
-      const float r0 = (float)(d0 * isg), r1 = (float)(d1 * isg), r2 = (float)(d2 * isg);
-      const float rr = r0 * r0 + r1 * r1 + r2 * r2;
-      if (!(rr < (float)fc.cut_sw2)) return;
-      const float rinv = __builtin_amdgcn_rsqf(rr);
-      const float rs = rr * rinv;
-      if (!(rs < (float)fc.cut_sw)) return;
-      const float inv_ra = __builtin_amdgcn_rcpf(rs - (float)fc.al);
-      const float ex = __expf(inv_ra);
-      const float B = (float)fc.B, A = (float)fc.A;
-      const float ir2 = rinv * rinv;
-      const float ir4 = ir2 * ir2;
+      // Pair geometry in FP64 (pair units: sigma_ij), the pair force factor
+      // rounded to FP32 once.  Near the bond length the two parts of dphi/dr
+      // cancel to a small remainder, and formed in FP32 their rounding was
+      // the largest error of the mixed forces; the exponential and the
+      // force vector stay FP32.
+      const double s0 = d0 * isg, s1 = d1 * isg, s2 = d2 * isg;
+      const double rr = s0 * s0 + s1 * s1 + s2 * s2;
+      if (!(rr < fc.cut_sw2)) return;
+      const double rs = sqrt(rr);
+      if (!(rs < fc.cut_sw)) return;
+      const double rinv = 1.0 / rs;
+      const double inv_ra = 1.0 / (rs - fc.al);
+      const double ex = (double)expf((float)inv_ra);
+      const double ir2 = rinv * rinv;
+      const double ir4 = ir2 * ir2;
       // phi2(r) = A (B r^-4 - 1) e^{1/(r-a)};  force on i = -dphi/dr e
-      const float dphi = A * ex * (-4.0f * B * ir4 * rinv - (B * ir4 - 1.0f) * inv_ra * inv_ra);
-      const float fs = -dphi * rinv * (jA ? fsA : fsB);
+      const double br = -4.0 * fc.B * ir4 * rinv - (fc.B * ir4 - 1.0) * inv_ra * inv_ra;
+      const float fs = (float)(-fc.A * ex * br * rinv * (double)(jA ? fsA : fsB));
+      const float r0 = (float)s0, r1 = (float)s1, r2 = (float)s2;
       const float f0 = fs * r0, f1 = fs * r1, f2 = fs * r2;
       f.add(f0, f1, f2);
       const float rho = jA ? rhoA : rhoB;  // 1 for Si-Si
@@ -1264,13 +1269,13 @@ __global__ __launch_bounds__(FIB_BLOCK) void k_legs(
         addW(r0 * q, r1 * q, r2 * q, 0.5f * f0, 0.5f * f1, 0.5f * f2);
       }
       if (EN) {
-        const double ul = 0.5 * (jA ? epsA : epsB) * (double)(A * (B * ir4 - 1.0f) * ex);
+        const double ul = 0.5 * (jA ? epsA : epsB) * (fc.A * (fc.B * ir4 - 1.0) * ex);
         ui += ul;
         etot += ul;
       }
       // legs with E = exp(gamma/(r-a)) < e^-30 are left out of the
       // three-body sum: each of their terms is below 1e-9 eps/sigma
-      if (rs < (float)fc.mixed_xleg) {
+      if (rs < fc.mixed_xleg) {
         am |= 1ull << k;
         ++na;
       }

@@ -108,6 +108,8 @@ def test_microbench_cells_lists_forces(microbench):
     print(f"mixed vs fp64: rms {rms:.2e}, norm {norm:.2e}, above 1e-5: {frac:.2e} of atoms, "
           f"max {e.max():.2e}", flush=True)
     assert rms < 1e-5 and norm < 1e-5 and frac < 1e-4
+    # every atom within 1e-5 of the larger of its force and 1e-3 F_rms
+    assert e.max() <= 1e-5
     assert abs(res["mixed"]["utot"] - res["fp64"]["utot"]) <= 1e-6 * abs(res["fp64"]["utot"])
 
 
