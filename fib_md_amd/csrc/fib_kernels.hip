@@ -1036,7 +1036,43 @@ struct Leg {
   T g;           // d ln E / dr (same length unit as rinv)
   uint32_t ks;   // neighbour index | (species == Ge) << 31
   uint32_t slot; // list slot of the neighbour in the centre's row
+  uint32_t elo;  // mixed: low parts of e0, e1, e2 (see pack_lo); FP64: 0
 };
+
+// Low part of an FP32 unit-vector component h of the FP64 value x, in
+// units of ulp(h) / 1024 as a signed 10-bit field c of a word (|x - h| <=
+// ulp(h) / 2, so the field carries x - h to 2^-34 of |h|).  The three
+// fields ride in one word of the vertex pass's stage; with them the pass
+// forms cos + 1/3 from (nearly) the FP64 unit vectors, where it cancels
+// near the tetrahedral angle.
+__device__ __forceinline__ float ulp_of(float h) {
+  const uint32_t ex = __float_as_uint(h) & 0x7f800000u;
+  return ex > (23u << 23) ? __uint_as_float(ex - (23u << 23)) : 0.0f;
+}
+__device__ __forceinline__ uint32_t pack_lo(double x, float h, int c) {
+  const float u = ulp_of(h);
+  if (!(u > 0.0f)) return 0u;
+  const int q = min(max((int)rint((x - (double)h) * (1024.0 / (double)u)), -511), 511);
+  return ((uint32_t)q & 0x3ffu) << (10 * c);
+}
+__device__ __forceinline__ int lo_field(uint32_t w, int c) {
+  return (int)(w << (22 - 10 * c)) >> 22;
+}
+__device__ __forceinline__ float lo_part(uint32_t w, float h, int c) {
+  return (float)lo_field(w, c) * (1.0f / 1024.0f) * ulp_of(h);
+}
+// the low parts of the reversed vector
+__device__ __forceinline__ uint32_t neg_lo(uint32_t w) {
+  uint32_t r = 0;
+#pragma unroll
+  for (int c = 0; c < 3; ++c) r |= ((uint32_t)(-lo_field(w, c)) & 0x3ffu) << (10 * c);
+  return r;
+}
+// a + b - s exactly, for s = fl(a + b) (TwoSum)
+__device__ __forceinline__ float two_sum_err(float a, float b, float s) {
+  const float bb = s - a;
+  return (a - (s - bb)) + (b - bb);
+}
 
 __device__ __forceinline__ double ks_pack(double, uint32_t ks) {
   return __longlong_as_double((long long)ks);
@@ -1060,6 +1096,41 @@ __device__ __forceinline__ void vterm(T leps, const Leg<T> &a, const Leg<T> &b, 
   hh = Eab * D * D;
   const T c1 = hh * a.g;
   const T c2 = T(2) * Eab * D * a.rinv;
+  Fa[0] = c1 * a.e0 + c2 * (b.e0 - cs * a.e0);
+  Fa[1] = c1 * a.e1 + c2 * (b.e1 - cs * a.e1);
+  Fa[2] = c1 * a.e2 + c2 * (b.e2 - cs * a.e2);
+}
+// Mixed: cos and D = cos + 1/3 from the FP32 unit vectors and their low
+// parts, with the products' rounding errors (FMA) and the sums' (TwoSum)
+// carried along, each rounded to FP32 once; the rest of the term in FP32.
+// (The products and sums of cos_d stay unfused: an FMA formed from a
+// product and the following sum would break the TwoSum error terms.)
+__device__ __forceinline__ void cos_d(const Leg<float> &a, const Leg<float> &b, float &cs,
+                                      float &D) {
+#pragma clang fp contract(off)
+  const float p0 = a.e0 * b.e0, p1 = a.e1 * b.e1, p2 = a.e2 * b.e2;
+  const float rp = (__builtin_fmaf(a.e0, b.e0, -p0) + __builtin_fmaf(a.e1, b.e1, -p1)) +
+                   __builtin_fmaf(a.e2, b.e2, -p2);
+  const float x = ((a.e0 * lo_part(b.elo, b.e0, 0) + lo_part(a.elo, a.e0, 0) * b.e0) +
+                   (a.e1 * lo_part(b.elo, b.e1, 1) + lo_part(a.elo, a.e1, 1) * b.e1)) +
+                  (a.e2 * lo_part(b.elo, b.e2, 2) + lo_part(a.elo, a.e2, 2) * b.e2);
+  const float s1 = p0 + p1, s2 = s1 + p2;
+  const float lo = (two_sum_err(p0, p1, s1) + two_sum_err(s1, p2, s2)) + (rp + x);
+  cs = s2 + lo;
+  // 1/3 = THIRD_HI + THIRD_LO
+  const float THIRD_HI = 0.333333343267440796f, THIRD_LO = -9.93410748e-09f;
+  const float s3 = s2 + THIRD_HI;
+  D = s3 + (two_sum_err(s2, THIRD_HI, s3) + (lo + THIRD_LO));
+}
+template <>
+__device__ __forceinline__ void vterm<float>(float leps, const Leg<float> &a, const Leg<float> &b,
+                                             float *Fa, float &hh) {
+  float cs, D;
+  cos_d(a, b, cs, D);
+  const float Eab = leps * a.E * b.E;
+  hh = Eab * D * D;
+  const float c1 = hh * a.g;
+  const float c2 = 2.0f * Eab * D * a.rinv;
   Fa[0] = c1 * a.e0 + c2 * (b.e0 - cs * a.e0);
   Fa[1] = c1 * a.e1 + c2 * (b.e1 - cs * a.e1);
   Fa[2] = c1 * a.e2 + c2 * (b.e2 - cs * a.e2);
@@ -1097,6 +1168,7 @@ __device__ __forceinline__ void make_leg(const double4 &xc, const double4 &xn, c
   const double isg = fc.isig[tc == 2 ? 1 : 0][tn == 2 ? 1 : 0];
   L.ks = jn | (tn == 2 ? 0x80000000u : 0u);
   L.slot = slot;
+  L.elo = 0u;
   if (!MIXED) {
     const double rr = d0 * d0 + d1 * d1 + d2 * d2;
     const double r = sqrt(rr);
@@ -1115,7 +1187,9 @@ __device__ __forceinline__ void make_leg(const double4 &xc, const double4 &xn, c
     const double rinv = 1.0 / rs;
     const double inv_ra = 1.0 / (rs - fc.al);
     const double rho = fc.rho[tc == 2 ? 1 : 0][tn == 2 ? 1 : 0];
-    L.e0 = (float)(s0 * rinv); L.e1 = (float)(s1 * rinv); L.e2 = (float)(s2 * rinv);
+    const double x0 = s0 * rinv, x1 = s1 * rinv, x2 = s2 * rinv;
+    L.e0 = (float)x0; L.e1 = (float)x1; L.e2 = (float)x2;
+    L.elo = pack_lo(x0, L.e0, 0) | pack_lo(x1, L.e1, 1) | pack_lo(x2, L.e2, 2);
     L.rinv = (float)(rinv * rho);
     L.E = expf((float)(fc.gamma * inv_ra));
     L.g = (float)(-fc.gamma * inv_ra * inv_ra * rho);
@@ -1533,7 +1607,7 @@ __global__ __launch_bounds__(FIB_BLOCK) void k_tri(
                             (uint32_t)kq[u], L);
             T4 x, y;
             x.x = L.e0; x.y = L.e1; x.z = L.e2; x.w = ks_ge(L.ks) ? -L.E : L.E;
-            y.x = L.g; y.y = L.rinv; y.z = ks_pack(T(0), L.ks); y.w = ks_pack(T(0), L.slot);
+            y.x = L.g; y.y = L.rinv; y.z = ks_pack(T(0), L.ks); y.w = ks_pack(T(0), L.elo);
             const int q = qv[u];
             sA[wid][q] = x;
             sB[wid][q] = y;
@@ -1550,7 +1624,13 @@ __global__ __launch_bounds__(FIB_BLOCK) void k_tri(
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-    for (int g = s0 + lane; g < s1; g += FIB_WAVE) {
+    // mixed: each lane's legs' energy shares wait in registers until every
+    // lane is done reading the stage's fourth words (the low parts of the
+    // unit vectors)
+    T gus[MIXED ? NS / FIB_WAVE : 1];
+#pragma unroll
+    for (int k = 0; k < (MIXED ? NS / FIB_WAVE : 1); ++k) gus[k] = 0;
+    for (int g = s0 + lane, it = 0; g < s1; g += FIB_WAVE, ++it) {
       const int q = g - s0;
       const int o = owner[wid][q];
       const int c0 = soff[wid][o] - s0;  // the centre's first leg in the stage
@@ -1560,7 +1640,7 @@ __global__ __launch_bounds__(FIB_BLOCK) void k_tri(
       {
         const T4 x = sA[wid][q], y = sB[wid][q];
         La.e0 = x.x; La.e1 = x.y; La.e2 = x.z; La.E = fabs(x.w);
-        La.g = y.x; La.rinv = y.y; La.ks = ks_unpack(y.z); La.slot = ks_unpack(y.w);
+        La.g = y.x; La.rinv = y.y; La.ks = ks_unpack(y.z); La.elo = ks_unpack(y.w);
       }
       const int ga = sge[wid][o] + ks_ge(La.ks);
       T Ga0 = 0, Ga1 = 0, Ga2 = 0, Gu = 0;
@@ -1569,16 +1649,26 @@ __global__ __launch_bounds__(FIB_BLOCK) void k_tri(
         const T4 xb = sA[wid][c0 + b];
         Leg<T> Lb;
         Lb.e0 = xb.x; Lb.e1 = xb.y; Lb.e2 = xb.z; Lb.E = fabs(xb.w);
+        Lb.elo = MIXED ? ks_unpack(sB[wid][c0 + b].w) : 0u;
         T Fa[3], hh;
         vterm<T>(pick4(lp, ga + (xb.w < T(0) ? 1 : 0)), La, Lb, Fa, hh);
         Ga0 += Fa[0]; Ga1 += Fa[1]; Ga2 += Fa[2];
         Gu += hh;
       }
-      T4 gv;
-      gv.x = Ga0; gv.y = Ga1; gv.z = Ga2; gv.w = Gu * T(1.0 / 3.0);
-      // results go back through the stage (this lane alone reads sB[q]) and
-      // leave plane by plane below, with sR, so the stores coalesce
-      sB[wid][q] = gv;
+      // results go back through the stage and leave plane by plane below,
+      // with sR, so the stores coalesce: FP64, all four words at once (this
+      // lane alone reads sB[q]); mixed, the forces into the first three
+      // words now and the energy share after the loop
+      if constexpr (MIXED) {
+        sB[wid][q].x = Ga0; sB[wid][q].y = Ga1; sB[wid][q].z = Ga2;
+#pragma unroll
+        for (int k = 0; k < NS / FIB_WAVE; ++k)
+          if (k == it) gus[k] = Gu * T(1.0 / 3.0);
+      } else {
+        T4 gv;
+        gv.x = Ga0; gv.y = Ga1; gv.z = Ga2; gv.w = Gu * T(1.0 / 3.0);
+        sB[wid][q] = gv;
+      }
       uc += Gu;
       if (VIR) {
         // r_a = x_a - x_i = -e_a / rinv_a
@@ -1587,6 +1677,15 @@ __global__ __launch_bounds__(FIB_BLOCK) void k_tri(
         W[0] += d0 * Ga0; W[1] += d0 * Ga1; W[2] += d0 * Ga2;
         W[3] += d1 * Ga0; W[4] += d1 * Ga1; W[5] += d1 * Ga2;
         W[6] += d2 * Ga0; W[7] += d2 * Ga1; W[8] += d2 * Ga2;
+      }
+    }
+    if constexpr (MIXED) {
+      // every lane is done with the fourth words: the energy shares go there
+      wave_lds_sync();
+#pragma unroll
+      for (int k = 0; k < NS / FIB_WAVE; ++k) {
+        const int g = s0 + lane + k * FIB_WAVE;
+        if (g < s1) sB[wid][g - s0].w = gus[k];
       }
     }
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -1697,6 +1796,7 @@ __global__ __launch_bounds__(FIB_BLOCK) void k_extra(
         min_image_fast(xc, xa, in, d0, d1, d2);
         const double isg = fc.isig[(tc == 2) ? 1 : 0][(ta == 2) ? 1 : 0];
         Leg<T> Lca;
+        Lca.elo = 0u;
         if (!MIXED) {
           const double r = sqrt(d0 * d0 + d1 * d1 + d2 * d2);
           const double rs = r * isg;
@@ -1710,25 +1810,29 @@ __global__ __launch_bounds__(FIB_BLOCK) void k_extra(
           Lca.E = exp(fc.gamma * inv_ra);
           Lca.g = -fc.gamma * inv_ra * inv_ra * isg;
         } else {
-          const float r0 = (float)(d0 * isg), r1 = (float)(d1 * isg), r2 = (float)(d2 * isg);
-          const float rr = r0 * r0 + r1 * r1 + r2 * r2;
-          if (!(rr < (float)fc.cut_sw2)) continue;
-          const float rinv = __builtin_amdgcn_rsqf(rr);
-          const float rs = rr * rinv;
-          if (!(rs < (float)fc.mixed_xleg)) continue;
+          // the leg values from FP64 geometry, as make_leg forms them
+          const double s0 = d0 * isg, s1 = d1 * isg, s2 = d2 * isg;
+          const double rr = s0 * s0 + s1 * s1 + s2 * s2;
+          if (!(rr < fc.cut_sw2)) continue;
+          const double rs = sqrt(rr);
+          if (!(rs < fc.mixed_xleg)) continue;
           bool listed = false;
           for (int q = 0; q < cc; ++q) listed |= (rowc[q] == a);
           if (listed) continue;
-          const float rho = fc.rho[(tc == 2) ? 1 : 0][(ta == 2) ? 1 : 0];
-          const float inv_ra = __builtin_amdgcn_rcpf(rs - (float)fc.al);
-          Lca.e0 = r0 * rinv; Lca.e1 = r1 * rinv; Lca.e2 = r2 * rinv;
-          Lca.rinv = rinv * rho;
-          Lca.E = __expf((float)fc.gamma * inv_ra);
-          Lca.g = -(float)fc.gamma * inv_ra * inv_ra * rho;
+          const double rho = fc.rho[(tc == 2) ? 1 : 0][(ta == 2) ? 1 : 0];
+          const double rinv = 1.0 / rs;
+          const double inv_ra = 1.0 / (rs - fc.al);
+          const double x0 = s0 * rinv, x1 = s1 * rinv, x2 = s2 * rinv;
+          Lca.e0 = (float)x0; Lca.e1 = (float)x1; Lca.e2 = (float)x2;
+          Lca.elo = pack_lo(x0, Lca.e0, 0) | pack_lo(x1, Lca.e1, 1) | pack_lo(x2, Lca.e2, 2);
+          Lca.rinv = (float)(rinv * rho);
+          Lca.E = expf((float)(fc.gamma * inv_ra));
+          Lca.g = (float)(-fc.gamma * inv_ra * inv_ra * rho);
         }
         // leg c -> b is b's leg to c, reversed
         Leg<T> Lcb = Lbc;
         Lcb.e0 = -Lbc.e0; Lcb.e1 = -Lbc.e1; Lcb.e2 = -Lbc.e2;
+        Lcb.elo = neg_lo(Lbc.elo);
         const int nge = (tc == 2) + (ta == 2) + (tb == 2);
         T Fa[3], Fb[3], hh;
         // vterm gives the force on its first leg: once per end
