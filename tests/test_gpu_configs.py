@@ -13,11 +13,8 @@ small slabs in test_gpu_md_oracle.py).
   the smallest dt at most 0.05 A over the ion's launch speed (a Si recoil
   can outrun the ion by up to 2 m_Ga / (m_Ga + m_Si) = 1.43x); mixed forces
   against FP64 on the FP64 run's configuration: the norm-wise relative
-  error below 1e-5, every atom's error below 1e-5 of the larger of its own
-  force and F_rms, and the median per-atom relative error below 1e-5.  (The FP32 terms leave an absolute
-  error of ~1e-6 eps/sigma per atom, so atoms whose net force nearly
-  cancels show larger relative errors; that floor is measured here, not
-  hidden by a tolerance on the maximum.)
+  error below 1e-5, and every atom's error below 1e-5 of the larger of its
+  own force and 1e-3 F_rms.
 - Cumulative dose (16 replicas of 40^3 cells, 8,192,000 Si + 32 Ga), two
   sequential 2 keV impacts per replica (all replicas fired at once), shells
   on: per replica the removed count equals the atoms found removed in the
@@ -128,8 +125,8 @@ def test_big_impact_first_300_steps():
     norm = float(np.linalg.norm(F["mixed"] - F["fp64"]) / np.linalg.norm(F["fp64"]))
     print(f"force error mixed vs fp64: rms {rms:.2e}, norm {norm:.2e}, max {e.max():.2e}, "
           f"median {np.median(e):.2e}", flush=True)
-    assert norm < 1e-5 and np.median(e) < 1e-5
-    assert np.all(d <= 1e-5 * np.maximum(mag, frms))
+    assert norm < 1e-5
+    assert e.max() <= 1e-5
 
 
 def test_dose_two_impacts():
