@@ -7,7 +7,8 @@ small slabs in test_gpu_md_oracle.py).
   1 fs nor lets the fastest atom move more than 0.05 A; the clock is the sum
   of the steps; every removed atom sits more than 10 A above the surface,
   at rest, and no live atom does; the energy ledger stays within 1e-3 of
-  the ion's energy.
+  the ion's energy; on the FP64 run's last configuration, every atom's
+  mixed force within 1e-5 of FP64 (floor 1e-3 F_rms).
 - Single large impact (60x60x80 cells, 2,304,000 Si, 30 keV at 7 deg), the
   first 300 steps in both precisions: ledger within 1e-3 of the ion energy,
   the smallest dt at most 0.05 A over the ion's launch speed (a Si recoil
@@ -81,6 +82,7 @@ def test_single_impact_2000_steps(precision):
             assert r1["time"] - r0["time"] == pytest.approx(r0["dt"], rel=1e-12)
     r = _removal_consistent(g, s)
     L1 = g.md_ledger()[0]
+    X1 = g.state()[0]
     g.close()
     assert r["steps"] == 2000
     assert 0 < r["dt_min"] <= 1e-15 and r["time"] >= 2000 * r["dt_min"] * (1 - 1e-12)
@@ -89,6 +91,32 @@ def test_single_impact_2000_steps(precision):
           f"smallest dt {r['dt_min'] * 1e15:.4f} fs, largest step {worst_dx / A:.4f} A",
           flush=True)
     assert abs(drift) < 1e-3 * 500.0
+    if precision == "fp64":
+        _mixed_force_error_within(s, X1, impact=1)
+
+
+def _mixed_force_error_within(s, X, impact):
+    """Forces of one configuration in both precisions: the norm-wise error
+    below 1e-5 and every atom's error below 1e-5 of the larger of its own
+    force and 1e-3 F_rms."""
+    from fib_md_amd import FibGPU
+    F = {}
+    for prec in ("fp64", "mixed"):
+        g = FibGPU([(len(s["X"]), s["nsg"], s["nlj"], s["L"])], precision=prec)
+        g.set_state(X, atype=s["atype"])
+        g.build_nlist()
+        F[prec] = g.force(impact=impact)["F"]
+        g.close()
+    d = np.sqrt(((F["mixed"] - F["fp64"]) ** 2).sum(1))
+    mag = np.sqrt((F["fp64"] ** 2).sum(1))
+    frms = np.sqrt((mag ** 2).mean())
+    e = d / np.maximum(mag, 1e-3 * frms)
+    rms = float(np.sqrt((e ** 2).mean()))
+    norm = float(np.linalg.norm(F["mixed"] - F["fp64"]) / np.linalg.norm(F["fp64"]))
+    print(f"force error mixed vs fp64: rms {rms:.2e}, norm {norm:.2e}, max {e.max():.2e}, "
+          f"median {np.median(e):.2e}", flush=True)
+    assert norm < 1e-5
+    assert e.max() <= 1e-5
 
 
 def test_big_impact_first_300_steps():
@@ -109,24 +137,7 @@ def test_big_impact_first_300_steps():
         assert 0.05 * A / (1.43 * v0) <= r["dt_min"] <= 0.05 * A / v0 * (1 + 1e-9)
         assert abs(res[prec]["drift"]) < 1e-3 * 30000.0
     # forces of the FP64 run's configuration in both precisions
-    from fib_md_amd import FibGPU
-    F = {}
-    for prec in ("fp64", "mixed"):
-        g = FibGPU([(len(s["X"]), s["nsg"], s["nlj"], s["L"])], precision=prec)
-        g.set_state(res["fp64"]["X"], atype=s["atype"])
-        g.build_nlist()
-        F[prec] = g.force(impact=1)["F"]
-        g.close()
-    d = np.sqrt(((F["mixed"] - F["fp64"]) ** 2).sum(1))
-    mag = np.sqrt((F["fp64"] ** 2).sum(1))
-    frms = np.sqrt((mag ** 2).mean())
-    e = d / np.maximum(mag, 1e-3 * frms)
-    rms = float(np.sqrt((e ** 2).mean()))
-    norm = float(np.linalg.norm(F["mixed"] - F["fp64"]) / np.linalg.norm(F["fp64"]))
-    print(f"force error mixed vs fp64: rms {rms:.2e}, norm {norm:.2e}, max {e.max():.2e}, "
-          f"median {np.median(e):.2e}", flush=True)
-    assert norm < 1e-5
-    assert e.max() <= 1e-5
+    _mixed_force_error_within(s, res["fp64"]["X"], impact=1)
 
 
 def test_dose_two_impacts():
