@@ -8,8 +8,8 @@ normal-incidence Ga ion fired at once.  One "step" is one MD step of every
 replica: half kick, drift, pins, removal, gated list rebuilds, SW + Moliere
 forces, second half kick, per-replica adaptive dt.  Reported value:
 atom-steps/s of the mixed-precision path (FP32 terms from FP64 pair and
-leg geometry, compensated FP32 force sums, FP64 integration and energies);
-the FP64 path runs beside it.
+leg geometry with cos + 1/3 compensated, compensated FP32 force sums, FP64
+integration and energies); the FP64 path runs beside it.
 Roofline: 240 algorithmic bytes per atom-step (read X, V, F, type and 16
 list indices, write X, V, F) over the step's HIP-event time.
 
@@ -1127,7 +1127,7 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32 terms (from f64 pair/leg geometry) + compensated f32 force sums; f64 integration and energies (mixed)",
+        "dtype": "f32 terms (from f64 pair/leg geometry, compensated cos+1/3) + compensated f32 force sums; f64 integration and energies (mixed)",
         "data": "synthetic",
         "config": {"workload": f"cumulative dose: {DOSE_REPLICAS} replica Si(100) slabs of "
                                f"{DOSE_CELLS}^3 cells in one batch, first 2 keV normal Ga impact "

@@ -48,9 +48,9 @@ enum fib_status {
 enum fib_precision {
   FIB_PREC_FP64 = 0,  /* all force arithmetic in FP64 (reference arithmetic) */
   FIB_PREC_MIXED = 1  /* FP32 pair/triplet terms from FP64 pair and leg
-                         geometry, compensated FP32 force sums, FP64
-                         positions, velocities, integration, energies and
-                         close Moliere pairs */
+                         geometry (cos + 1/3 compensated), compensated FP32
+                         force sums, FP64 positions, velocities,
+                         integration, energies and close Moliere pairs */
 };
 
 /* Physical constants and numerical knobs.  fib_default_params() fills the
