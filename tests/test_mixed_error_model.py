@@ -6,14 +6,20 @@ arithmetic in two layouts:
 
 - all-FP32 geometry: the bond vector rounded to FP32, then r, 1/r, the pair
   bracket -4 B r^-5 - (B r^-4 - 1)/(r-a)^2 and the leg values in FP32;
-- FP64 geometry (the mixed mode's layout): r, 1/r, 1/(r-a), the bracket and
-  the leg values in FP64, each rounded once to FP32, the exponentials and
-  the vertex terms in FP32.
+- FP64 geometry: r, 1/r, 1/(r-a), the bracket and the leg values in FP64,
+  each rounded once to FP32, the exponentials and the vertex terms in FP32;
+- FP64 geometry with a compensated D (the mixed mode's layout): as above,
+  and D = cos + 1/3 of the vertex terms formed from the FP32 unit vectors
+  plus their low parts (10-bit fields in units of ulp/1024), with the
+  products' exact errors and TwoSum errors carried, each rounded once.
 
 Near the bond length the bracket's two parts cancel (dphi/dr = 0 at the
 pair minimum, r = 2^(1/6) sigma = 2.35 A), so its FP32 rounding dominates
-the per-atom error.  Per-atom metric: |dF| / max(|F|, 1e-3 F_rms).  Reduced
-units (sigma = eps = 1), the SW parameters of prms.f90."""
+the per-atom error of the all-FP32 layout.  Near the tetrahedral angle D
+cancels, which dominates once the geometry is FP64, most of all where the
+forces are small (a lattice with 0.015 A noise has the forces of a 300 K
+slab).  Per-atom metric: |dF| / max(|F|, 1e-3 F_rms).  Reduced units
+(sigma = eps = 1), the SW parameters of prms.f90."""
 import numpy as np
 import pytest
 
@@ -28,8 +34,12 @@ XLEG = 1.76  # mixed leg gate (E < e^-30 beyond it)
 f4 = np.float32
 
 
-def _pairs(n):
-    X, _, Lb = configs.force_microbench(n, seed=1002)
+def _pairs(n, noise=None):
+    if noise is None:
+        X, _, Lb = configs.force_microbench(n, seed=1002)
+    else:
+        L = n * configs.A_SI
+        X, Lb = np.mod(configs.perturbed_crystal(n, noise, 1002), L), (L, L, L)
     Xs, Ls = X / SIG, Lb[0] / SIG
     tree = scipy_spatial.cKDTree(np.mod(Xs, Ls), boxsize=Ls)
     p = tree.query_pairs(CUT, output_type="ndarray")
@@ -40,8 +50,45 @@ def _pairs(n):
     return len(Xs), i, j, d
 
 
+def _ulp(h):
+    ex = np.abs(h).view(np.uint32) & np.uint32(0x7f800000)
+    return np.where(ex > (23 << 23), (ex - np.uint32(23 << 23)).view(f4), f4(0)).astype(f4)
+
+
+def _low_parts(x64, h):
+    """x - h quantized to a 10-bit field in units of ulp(h)/1024, as FP32."""
+    u = _ulp(h).astype(np.float64)
+    q = np.where(u > 0, np.rint((x64 - h.astype(np.float64)) * 1024.0 / np.where(u > 0, u, 1.0)), 0)
+    return (np.clip(q, -511, 511) * (u / 1024.0)).astype(f4)
+
+
+def _two_sum_err(a, b, s):
+    bb = (s - a).astype(f4)
+    return ((a - (s - bb).astype(f4)).astype(f4) + (b - bb).astype(f4)).astype(f4)
+
+
+def _cos_d(ea, eb, la, lb):
+    """cos and cos + 1/3 from FP32 unit vectors and their low parts, FP32
+    operations with the products' exact errors (what an FMA gives) and the
+    sums' TwoSum errors carried."""
+    p = (ea * eb).astype(f4)
+    perr = (ea.astype(np.float64) * eb.astype(np.float64) - p).astype(f4)
+    rp = ((perr[:, 0] + perr[:, 1]).astype(f4) + perr[:, 2]).astype(f4)
+    cross = ((ea * lb).astype(f4) + (la * eb).astype(f4)).astype(f4)
+    x = ((cross[:, 0] + cross[:, 1]).astype(f4) + cross[:, 2]).astype(f4)
+    s1 = (p[:, 0] + p[:, 1]).astype(f4)
+    s2 = (s1 + p[:, 2]).astype(f4)
+    lo = ((_two_sum_err(p[:, 0], p[:, 1], s1) + _two_sum_err(s1, p[:, 2], s2)).astype(f4)
+          + (rp + x).astype(f4)).astype(f4)
+    th, tl = f4(1.0 / 3.0), f4(1.0 / 3.0 - float(f4(1.0 / 3.0)))
+    s3 = (s2 + th).astype(f4)
+    D = (s3 + (_two_sum_err(s2, np.full_like(s2, th), s3) + (lo + tl).astype(f4)).astype(f4)).astype(f4)
+    return (s2 + lo).astype(f4), D
+
+
 def _forces(N, i, j, d, mode):
-    """mode: "fp64", "fp32" (all-FP32 geometry) or "geo64" (FP64 geometry)."""
+    """mode: "fp64", "fp32" (all-FP32 geometry), "geo64" (FP64 geometry)
+    or "geo64dc" (FP64 geometry, compensated D)."""
     F = np.zeros((N, 3))
     r64 = np.sqrt((d * d).sum(1))
     d32 = d.astype(f4)
@@ -86,7 +133,9 @@ def _forces(N, i, j, d, mode):
             E, T = np.exp(GAM * iraL), np.float64
         else:
             E, T = np.exp((GAM * iraL).astype(f4)), f4
+            e64 = e
             e, rinv, g, E = e.astype(f4), rinv.astype(f4), g.astype(f4), E.astype(f4)
+            elo = _low_parts(e64, e)
     starts = np.searchsorted(li, np.arange(N))
     cnt = np.searchsorted(li, np.arange(N), side="right") - starts
     ia, ib = [], []
@@ -98,8 +147,11 @@ def _forces(N, i, j, d, mode):
                 ib.append(starts[m] + k2)
     ia, ib = np.concatenate(ia), np.concatenate(ib)
     ea, eb = e[ia], e[ib]
-    cs = (ea * eb).sum(1, dtype=T).astype(T)
-    D = (cs + T(1.0 / 3.0)).astype(T)
+    if mode == "geo64dc":
+        cs, D = _cos_d(ea, eb, elo[ia], elo[ib])
+    else:
+        cs = (ea * eb).sum(1, dtype=T).astype(T)
+        D = (cs + T(1.0 / 3.0)).astype(T)
     Eab = (T(LAM) * E[ia] * E[ib]).astype(T)
     c1 = (Eab * D * D * g[ia]).astype(T)
     c2 = (T(2) * Eab * D * rinv[ia]).astype(T)
@@ -131,3 +183,19 @@ def test_fp64_geometry_cuts_the_mixed_force_error():
     # every atom within 1e-5
     assert np.median(a64) < np.median(a32) / 4
     assert e64.max() <= 1e-5
+
+
+def test_compensated_d_on_small_forces():
+    # a lattice with the forces of a 300 K slab, where D = cos + 1/3 sits
+    # near zero for most leg pairs
+    N, i, j, d = _pairs(16, noise=0.015e-10)
+    ref = _forces(N, i, j, d, "fp64")
+    e64, a64 = _err(_forces(N, i, j, d, "geo64"), ref)
+    edc, adc = _err(_forces(N, i, j, d, "geo64dc"), ref)
+    print(f"FP64 geometry: median |dF|/F_rms {np.median(a64):.2e}, max e {e64.max():.2e}; "
+          f"compensated D: {np.median(adc):.2e}, max e {edc.max():.2e}")
+    # with plain FP32 D the worst atoms exceed 1e-5; the compensated D takes
+    # the typical error down at least fourfold and every atom within 1e-5
+    assert e64.max() > 1e-5
+    assert np.median(adc) < np.median(a64) / 4
+    assert edc.max() <= 1e-5
