@@ -1058,8 +1058,10 @@ __device__ __forceinline__ uint32_t pack_lo(double x, float h, int c) {
 __device__ __forceinline__ int lo_field(uint32_t w, int c) {
   return (int)(w << (22 - 10 * c)) >> 22;
 }
+// the field times ulp(h) / 1024: for h = m 2^e with m in [0.5, 1), ulp(h)
+// = 2^(e - 24), so the field times 2^(e - 34) (h = 0 carries a zero field)
 __device__ __forceinline__ float lo_part(uint32_t w, float h, int c) {
-  return (float)lo_field(w, c) * (1.0f / 1024.0f) * ulp_of(h);
+  return __builtin_amdgcn_ldexpf((float)lo_field(w, c), __builtin_amdgcn_frexp_expf(h) - 34);
 }
 // the low parts of the reversed vector
 __device__ __forceinline__ uint32_t neg_lo(uint32_t w) {
